@@ -1,0 +1,196 @@
+/*
+ * rq.h -- C ABI of the RealQM grid solver for AMD Instinct MI355X (gfx950).
+ *
+ * One opaque handle owns the device fields of one molecular grid and one HIP
+ * stream.  All calls on a handle are issued in order on that stream; calls
+ * that return data to the host synchronise the stream.  A handle is not
+ * thread-safe; distinct handles may be used from distinct threads.
+ *
+ * Every function returns 0 (RQ_OK) on success or a negative RQ_E* code; the
+ * message of the last failure on a handle is returned by rq_last_error(h)
+ * (rq_last_error(NULL) gives the last failure of rq_create / batch create).
+ *
+ * Grid convention (cloud/solver.py:322-329): NN cells and S = NN + 1 nodes
+ * per axis, h = screen / NN, flat node index id = i*S*S + j*S + k (k fastest).
+ * Face nodes (any index 0 or NN) are Dirichlet data: kernels write interior
+ * nodes only.  Atom positions are fractional grid units (cloud/solver.py:335).
+ *
+ * Reference interfaces replaced (file:line in Claes542/RealMolecule):
+ *   rq_create            RealQMSolver.__init__          cloud/solver.py:321-400
+ *   rq_initialize        RealQMSolver.initialize        cloud/solver.py:402-419
+ *   rq_init_trial        initTrial kernel launch        cloud/solver.py:404-410, 16-59
+ *   rq_poisson           RealQMSolver._poisson_step     cloud/solver.py:434-444
+ *   rq_update            updateU launch + U/U2 swap     cloud/solver.py:567-572, 61-90
+ *   rq_normalize         RealQMSolver._normalize        cloud/solver.py:421-432
+ *   rq_boundary          boundaryEvolve launch          cloud/solver.py:579-586, 92-147
+ *                        evolveBoundary pass + copy     molecule.js:3093-3104, 426-540
+ *   rq_step              body of RealQMSolver.run       cloud/solver.py:561-586
+ *                        per-step part of doSteps       molecule.js:2829-2996
+ *   rq_energy            RealQMSolver._compute_energy   cloud/solver.py:446-460
+ *                        reduceEnergy+finalizeEnergy    molecule.js:950-1075, 3078-3091
+ *   rq_forces_hf         computeForceAll batches        cloud/solver.py:478-502, 282-315
+ *   rq_forces_gradp      gradPtotal pass                molecule.js:1199-1270, 2997-3005
+ *   rq_set_atoms         atom_data upload after moves   cloud/solver.py:538-543
+ *   rq_recompute_k       recomputeK launch              cloud/solver.py:544-548, 149-179
+ *   rq_read_field /      CuPy array .get() / assignment of U, U2, P, P2, K_buf,
+ *   rq_write_field       rho, label, bestU              cloud/solver.py:365-372;
+ *                        cloud/run_h2o_debug.py:20-57; cloud/benchmark.py:104
+ */
+#ifndef RQ_H
+#define RQ_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RQ_ABI_VERSION 1
+
+/* status codes */
+#define RQ_OK         0
+#define RQ_EINVAL    -1  /* bad argument or unsupported parameter combination */
+#define RQ_ENOMEM    -2  /* device allocation failed */
+#define RQ_EHIP      -3  /* HIP runtime error (message has the HIP error string) */
+#define RQ_ESTATE    -4  /* call out of order (e.g. step before initialise) */
+#define RQ_ERANGE    -5  /* size beyond a documented limit */
+#define RQ_EUNSTABLE -6  /* non-finite energy/field detected */
+
+/* Poisson solver for the Hartree potential, lap P = -2 pi rho */
+#define RQ_POISSON_JACOBI    0  /* cloud/solver.py: sweeps of P'=(a P + sum6 P + 2pi h^2 rho)/c */
+#define RQ_POISSON_MULTIGRID 1  /* V-cycle per Poisson step, exact FP64 coarsest solve */
+#define RQ_POISSON_TWOGRID   2  /* molecule.js: smoother sweeps + two-grid correction every
+                                   twogrid_interval steps (sign-inverted, x0.5; molecule.js:2879-2907) */
+
+/* domain-boundary rule */
+#define RQ_BOUNDARY_SCORE    0  /* cloud/solver.py:92-147 nuclear-score flip */
+#define RQ_BOUNDARY_LEVELSET 1  /* molecule.js:426-540 level set W */
+#define RQ_BOUNDARY_OFF      2
+
+/* trial initialisation / softening semantics */
+#define RQ_INIT_S1 0  /* cloud/solver.py:16-59: interior only, r~ = sqrt(r^2+h^2) or max(sqrt(r^2+.04h^2), rc) */
+#define RQ_INIT_S2 1  /* molecule.js:1377-1449: all nodes, integer positions, r~ = max(r, 2h) or
+                         max(sqrt(r^2+.04h^2), rc); initZeff = Z; faces keep trial values */
+
+/* energy semantics */
+#define RQ_ENERGY_S1 0  /* cloud/solver.py:227-280 */
+#define RQ_ENERGY_S2 1  /* molecule.js:950-1075: r < max(rc, 2h) exclusion of the own nucleus */
+
+/* field ids for rq_read_field / rq_write_field / rq_field_device_ptr */
+#define RQ_FIELD_U      0  /* float32 S^3, current amplitude */
+#define RQ_FIELD_U2     1  /* float32 S^3, ping-pong partner of U */
+#define RQ_FIELD_K      2  /* float32 S^3, nuclear potential sum Z/r~ */
+#define RQ_FIELD_P      3  /* float32 S^3, current Hartree potential */
+#define RQ_FIELD_P2     4  /* float32 S^3, ping-pong partner of P */
+#define RQ_FIELD_RHO    5  /* float32 S^3, read only: U*U (computed on demand) */
+#define RQ_FIELD_LABEL  6  /* int32 S^3 on the host (stored as uint16 on the device) */
+#define RQ_FIELD_BESTU  7  /* float32 S^3, trial maximum from initialisation */
+#define RQ_FIELD_W      8  /* float32 S^3, level-set value (RQ_BOUNDARY_LEVELSET) */
+
+typedef struct rq_solver rq_solver;
+
+/* One atom row, as cloud/solver.py:378-382 atom_data: fractional grid
+ * position (i, j, k), electron charge Z (domain Z_eff; Z <= 0 = inactive) and
+ * pseudopotential radius rc (bohr, 0 = bare). */
+typedef struct rq_atom {
+  float i, j, k, Z, rc;
+} rq_atom;
+
+typedef struct rq_params {
+  int32_t NN;             /* cells per axis (S = NN+1 nodes); even; 8 <= NN <= 1024 */
+  float   screen;         /* box edge in bohr */
+  float   dv;             /* ITP step in h^2 units; <= 0 selects cloud/solver.py:341-351 */
+  int32_t init_mode;      /* RQ_INIT_* */
+  int32_t poisson_mode;   /* RQ_POISSON_* */
+  int32_t jacobi_sweeps;  /* sweeps per Poisson step (JACOBI/TWOGRID), default 2 */
+  float   smooth_a;       /* smoother P' = a*P + (sum6 P + rhs)/c; S1 (0, 6), molecule.js (0.3333, 9) */
+  float   smooth_c;
+  int32_t mg_levels;      /* levels incl. the finest (>= 2); NN divisible by 2^(levels-1) */
+  int32_t mg_nu1;         /* pre-smoothing sweeps */
+  int32_t mg_nu2;         /* post-smoothing sweeps */
+  float   mg_a;           /* V-cycle smoother (a, c); default omega = 6/7: (1/7, 7) */
+  float   mg_c;
+  int32_t twogrid_interval;      /* RQ_POISSON_TWOGRID: correction every n steps (molecule.js:110) */
+  int32_t twogrid_coarse_sweeps; /* molecule.js: 10 */
+  int32_t boundary_mode;  /* RQ_BOUNDARY_* */
+  float   flip_ratio;     /* score rule: flip if best > ratio * current (1.1) */
+  int32_t flip_min_foreign; /* score rule: foreign neighbours needed (2) */
+  float   ls_dt_w;        /* level set: boundarySpeed dt_w (0.5, molecule.js:125) */
+  float   ls_curv_reg;    /* level set: curvature weight (0.15, molecule.js:34) */
+  int32_t energy_mode;    /* RQ_ENERGY_* */
+  int32_t rc_dirichlet;   /* 1: molecule.js rc rule in the ITP update (U'=0 inside own rc) */
+  int32_t reserved[8];    /* must be zero */
+} rq_params;
+
+/* Scheduling intervals of the run() loop (cloud/solver.py:550-586). */
+typedef struct rq_schedule {
+  int32_t poisson_interval;  /* Poisson step when step % n == 0 (2) */
+  int32_t norm_interval;     /* normalise when step % n == 0 (20) */
+  int32_t boundary_interval; /* boundary rule when step % n == 0 (5*norm_interval = 100) */
+} rq_schedule;
+
+/* --- library --- */
+int         rq_abi_version(void);
+const char* rq_last_error(const rq_solver* h);
+void        rq_default_params(rq_params* p);       /* cloud/solver.py semantics, Jacobi Poisson */
+void        rq_default_schedule(rq_schedule* s);   /* (2, 20, 100) */
+double      rq_dv_for_electrons(int32_t n_elec);   /* cloud/solver.py:341-351 */
+
+/* --- lifetime --- */
+int rq_create(const rq_params* p, const rq_atom* atoms, int32_t n_atoms, rq_solver** out);
+int rq_destroy(rq_solver* h);
+int rq_sync(rq_solver* h);
+int rq_get_info(const rq_solver* h, int32_t* NN, int32_t* S, int32_t* n_atoms, double* h_grid, double* dv);
+
+/* --- atoms --- */
+int rq_set_atoms(rq_solver* h, const rq_atom* atoms, int32_t n_atoms); /* same n as create */
+int rq_get_atoms(const rq_solver* h, rq_atom* atoms, int32_t n_atoms);
+/* Lock groups of the level-set rule (molecule.js:487-500): group[d] > 0 marks
+ * domain d; a boundary between two domains of one group does not evolve. */
+int rq_set_lock_groups(rq_solver* h, const int32_t* group, int32_t n_atoms);
+
+/* --- passes --- */
+int rq_init_trial(rq_solver* h);                 /* K, label, U, bestU (and W = 1) */
+int rq_initialize(rq_solver* h, int32_t bootstrap_poisson_steps); /* init_trial + normalise + n Poisson steps */
+int rq_poisson(rq_solver* h, int32_t n_steps);   /* n Poisson steps in the configured mode */
+int rq_update(rq_solver* h);                     /* one ITP step + swap */
+int rq_normalize(rq_solver* h);                  /* per-domain renormalisation */
+int rq_boundary(rq_solver* h);                   /* one application of the boundary rule */
+int rq_recompute_k(rq_solver* h);                /* K after rq_set_atoms */
+/* Steps first_step+1 .. first_step+n_steps of the run() loop body. */
+int rq_step(rq_solver* h, int64_t first_step, int32_t n_steps, const rq_schedule* s);
+
+/* --- reductions (FP64) --- */
+/* out[0..5] = T, V_eK, V_ee, sum rho x h^3, sum rho y h^3, sum rho z h^3 (bohr).
+ * per_domain (nullable) = n_atoms x 3 (T, V_eK, V_ee) keyed by voxel label. */
+int rq_energy(rq_solver* h, double* out6, double* per_domain);
+/* per-domain sum U^2 h^3 (n_atoms values) */
+int rq_domain_norms(rq_solver* h, double* norms);
+/* Electronic HF force on atoms [first, first+count): Z_n sum rho h^3 d/(d^2+h^2)^1.5 */
+int rq_forces_hf(rq_solver* h, int32_t first, int32_t count, double* out3);
+/* molecule.js sphere-averaged 2 Z <grad P> over radius `radius` cells (<=0: max(5, round(3/h))) */
+int rq_forces_gradp(rq_solver* h, int32_t radius, double* out3n);
+
+/* --- fields --- */
+int rq_read_field(rq_solver* h, int32_t field, void* host, size_t bytes);
+int rq_write_field(rq_solver* h, int32_t field, const void* host, size_t bytes);
+int rq_swap_u(rq_solver* h);                     /* exchange U and U2 (cloud/benchmark.py:104) */
+int rq_field_device_ptr(rq_solver* h, int32_t field, void** dptr);
+int rq_stream(rq_solver* h, void** hip_stream);
+
+/* --- timing (HIP events on the handle's stream) --- */
+/* Per-kernel-family accumulated device time in ms since the last enable, and
+ * launch counts.  Families: 0 ITP update, 1 fine smoother sweep, 2 coarse
+ * smoother sweep, 3 fine residual+restriction, 4 coarse residual+restriction,
+ * 5 fine prolongation+correction, 6 coarse prolongation+correction,
+ * 7 exact coarsest solve, 8 normalise, 9 boundary rule, 10 energy, 11 forces.
+ * Events bracket each launch on the handle's stream. */
+#define RQ_TIMING_FAMILIES 12
+int rq_timing_enable(rq_solver* h, int32_t on);
+int rq_timing_read(rq_solver* h, double* ms, int64_t* launches, int32_t n_families);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RQ_H */

@@ -1,0 +1,136 @@
+"""ctypes binding of ``librq.so`` (the C ABI declared in ``include/rq.h``).
+
+The library is loaded from this package directory.  There is no fallback: if
+it is missing or fails to load, importing the solver raises ``RuntimeError``
+naming the build step.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "librq.so")
+
+RQ_OK = 0
+RQ_EINVAL = -1
+RQ_ENOMEM = -2
+RQ_EHIP = -3
+RQ_ESTATE = -4
+RQ_ERANGE = -5
+RQ_EUNSTABLE = -6
+
+RQ_POISSON_JACOBI = 0
+RQ_POISSON_MULTIGRID = 1
+RQ_POISSON_TWOGRID = 2
+
+RQ_BOUNDARY_SCORE = 0
+RQ_BOUNDARY_LEVELSET = 1
+RQ_BOUNDARY_OFF = 2
+
+RQ_INIT_S1 = 0
+RQ_INIT_S2 = 1
+RQ_ENERGY_S1 = 0
+RQ_ENERGY_S2 = 1
+
+FIELDS = {"U": 0, "U2": 1, "K": 2, "P": 3, "P2": 4, "rho": 5, "label": 6, "bestU": 7, "W": 8}
+FAMILIES = ["itp", "smooth", "smooth_c", "restrict", "restrict_c", "prolong", "prolong_c", "coarse_solve",
+            "normalize", "boundary", "energy", "force"]
+
+
+class RqAtom(C.Structure):
+    _fields_ = [("i", C.c_float), ("j", C.c_float), ("k", C.c_float), ("Z", C.c_float), ("rc", C.c_float)]
+
+
+class RqParams(C.Structure):
+    _fields_ = [
+        ("NN", C.c_int32), ("screen", C.c_float), ("dv", C.c_float), ("init_mode", C.c_int32),
+        ("poisson_mode", C.c_int32), ("jacobi_sweeps", C.c_int32), ("smooth_a", C.c_float),
+        ("smooth_c", C.c_float), ("mg_levels", C.c_int32), ("mg_nu1", C.c_int32), ("mg_nu2", C.c_int32),
+        ("mg_a", C.c_float), ("mg_c", C.c_float), ("twogrid_interval", C.c_int32),
+        ("twogrid_coarse_sweeps", C.c_int32), ("boundary_mode", C.c_int32), ("flip_ratio", C.c_float),
+        ("flip_min_foreign", C.c_int32), ("ls_dt_w", C.c_float), ("ls_curv_reg", C.c_float),
+        ("energy_mode", C.c_int32), ("rc_dirichlet", C.c_int32), ("reserved", C.c_int32 * 8),
+    ]
+
+
+class RqSchedule(C.Structure):
+    _fields_ = [("poisson_interval", C.c_int32), ("norm_interval", C.c_int32), ("boundary_interval", C.c_int32)]
+
+
+_P = C.c_void_p
+_I = C.c_int32
+_D = C.POINTER(C.c_double)
+
+# name -> (restype, argtypes)
+PROTOTYPES = {
+    "rq_abi_version": (C.c_int, []),
+    "rq_last_error": (C.c_char_p, [_P]),
+    "rq_default_params": (None, [C.POINTER(RqParams)]),
+    "rq_default_schedule": (None, [C.POINTER(RqSchedule)]),
+    "rq_dv_for_electrons": (C.c_double, [_I]),
+    "rq_create": (C.c_int, [C.POINTER(RqParams), C.POINTER(RqAtom), _I, C.POINTER(_P)]),
+    "rq_destroy": (C.c_int, [_P]),
+    "rq_sync": (C.c_int, [_P]),
+    "rq_get_info": (C.c_int, [_P, C.POINTER(_I), C.POINTER(_I), C.POINTER(_I), _D, _D]),
+    "rq_set_atoms": (C.c_int, [_P, C.POINTER(RqAtom), _I]),
+    "rq_get_atoms": (C.c_int, [_P, C.POINTER(RqAtom), _I]),
+    "rq_set_lock_groups": (C.c_int, [_P, C.POINTER(C.c_int32), _I]),
+    "rq_init_trial": (C.c_int, [_P]),
+    "rq_initialize": (C.c_int, [_P, _I]),
+    "rq_poisson": (C.c_int, [_P, _I]),
+    "rq_update": (C.c_int, [_P]),
+    "rq_normalize": (C.c_int, [_P]),
+    "rq_boundary": (C.c_int, [_P]),
+    "rq_recompute_k": (C.c_int, [_P]),
+    "rq_step": (C.c_int, [_P, C.c_int64, _I, C.POINTER(RqSchedule)]),
+    "rq_energy": (C.c_int, [_P, _D, _D]),
+    "rq_domain_norms": (C.c_int, [_P, _D]),
+    "rq_forces_hf": (C.c_int, [_P, _I, _I, _D]),
+    "rq_forces_gradp": (C.c_int, [_P, _I, _D]),
+    "rq_read_field": (C.c_int, [_P, _I, _P, C.c_size_t]),
+    "rq_write_field": (C.c_int, [_P, _I, _P, C.c_size_t]),
+    "rq_swap_u": (C.c_int, [_P]),
+    "rq_field_device_ptr": (C.c_int, [_P, _I, C.POINTER(_P)]),
+    "rq_stream": (C.c_int, [_P, C.POINTER(_P)]),
+    "rq_timing_enable": (C.c_int, [_P, _I]),
+    "rq_timing_read": (C.c_int, [_P, _D, C.POINTER(C.c_int64), _I]),
+}
+
+_lib = None
+
+
+def load(path: str | None = None):
+    """Load librq.so once and attach prototypes; raises RuntimeError if absent."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise RuntimeError(
+            f"librq.so not found at {p}: the HIP library is not built "
+            "(run `python -c 'import __graft_entry__ as g; g.build()'` or `make -C realmolecule_amd/csrc`)")
+    try:
+        lib = C.CDLL(p)
+    except OSError as e:  # pragma: no cover - depends on the ROCm runtime
+        raise RuntimeError(f"failed to load {p}: {e}") from e
+    for name, (res, args) in PROTOTYPES.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    if path is None:
+        _lib = lib
+    return lib
+
+
+class RqError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"rq error {code}: {msg}")
+        self.code = code
+
+
+def check(lib, h, code):
+    if code != RQ_OK:
+        msg = lib.rq_last_error(h)
+        raise RqError(code, msg.decode() if msg else "")
+    return code

@@ -1,0 +1,152 @@
+"""Synthetic molecular inputs for the solver (input recipes; nothing here runs on the GPU).
+
+Atoms are rows ``(i, j, k, Z, rc)`` in fractional grid units, the layout of
+``cloud/solver.py:378-382``; the config-dict form ``{'i','j','k','Z','rc','el'}``
+is what ``RealQMSolver`` takes (``cloud/run_h2o_test.py:32-39``).
+
+Element conventions follow the reference builders: O carries Z = 2 valence
+electrons with rc = 0.4 bohr, H Z = 1 with rc = 0, C Z = 4 rc = 0.3, N Z = 3
+rc = 0.3 (``cloud/run_h2o_test.py:35-37``; ``cloud/run_hairpin.py:38``).
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+ANGSTROM = 1.8897  # bohr per angstrom, cloud/run_hairpin.py:29
+ELEMENT = {"H": (1, 0.0), "O": (2, 0.4), "C": (4, 0.3), "N": (3, 0.3)}
+
+
+def to_config_atoms(rows, els=None):
+    out = []
+    for n, r in enumerate(rows):
+        d = {"i": float(r[0]), "j": float(r[1]), "k": float(r[2]), "Z": float(r[3]), "rc": float(r[4])}
+        if els is not None:
+            d["el"] = els[n]
+        out.append(d)
+    return out
+
+
+def h2o_reference_builder(NN=100, screen=10.0, rOH=1.81, angle_deg=104.5):
+    """The reference H2O builder with integer positions (cloud/run_h2o_test.py:24-39)."""
+    h = screen / NN
+    N2 = NN // 2
+    ang = angle_deg * math.pi / 180
+    Oi, Oj = N2, N2
+    H1i = round(Oi + rOH * math.cos(math.pi / 2 + ang / 2) / h)
+    H1j = round(Oj - rOH * math.sin(math.pi / 2 + ang / 2) / h)
+    H2i = round(Oi + rOH * math.cos(math.pi / 2 - ang / 2) / h)
+    H2j = round(Oj - rOH * math.sin(math.pi / 2 - ang / 2) / h)
+    atoms = [
+        {"i": H1i, "j": H1j, "Z": 1, "el": "H", "rc": 0},
+        {"i": Oi, "j": Oj, "Z": 2, "el": "O", "rc": 0.4},
+        {"i": H2i, "j": H2j, "Z": 1, "el": "H", "rc": 0},
+    ]
+    return {"gridN": NN, "screen": screen, "atoms": atoms}
+
+
+def h2o_single(NN=96, h=0.1, rOH=1.809, angle_deg=104.5):
+    """Single H2O: O at the grid centre, H at rOH and the given angle in the
+    centre k-plane, fractional (unrounded) grid positions.  Returns
+    (rows[n,5], els, screen)."""
+    N2 = NN / 2
+    ang = math.radians(angle_deg)
+    rows = []
+    els = []
+    for s in (+1, -1):
+        th = math.pi / 2 + s * ang / 2
+        rows.append((N2 + rOH * math.cos(th) / h, N2 - rOH * math.sin(th) / h, N2, 1.0, 0.0))
+        els.append("H")
+        if s == +1:
+            rows.append((N2, N2, N2, 2.0, 0.4))
+            els.append("O")
+    return np.array(rows, np.float64), els, NN * h
+
+
+def _rand_rotation(rng):
+    q = rng.normal(size=4)
+    q /= np.linalg.norm(q)
+    a, b, c, d = q
+    return np.array([
+        [a * a + b * b - c * c - d * d, 2 * (b * c - a * d), 2 * (b * d + a * c)],
+        [2 * (b * c + a * d), a * a - b * b + c * c - d * d, 2 * (c * d - a * b)],
+        [2 * (b * d - a * c), 2 * (c * d + a * b), a * a - b * b - c * c + d * d],
+    ])
+
+
+def water_geometry(rOH=1.809, angle_deg=104.5):
+    """O at the origin, two H in the xy plane (bohr)."""
+    ang = math.radians(angle_deg)
+    return np.array([
+        [0.0, 0.0, 0.0],
+        [rOH * math.cos(ang / 2), rOH * math.sin(ang / 2), 0.0],
+        [rOH * math.cos(ang / 2), -rOH * math.sin(ang / 2), 0.0],
+    ])
+
+
+def water_box(n_mol, box_bohr, NN, seed, min_OO_angstrom=2.6, margin_bohr=3.0, max_tries=200000):
+    """n_mol waters at random positions/orientations in a cube of edge box_bohr.
+
+    Rejection sampling of O positions inside [margin, box - margin]^3 with O-O >=
+    ``min_OO_angstrom``; each molecule gets a uniformly random rotation.
+    Returns (rows[n,5] in grid units, els, screen)."""
+    rng = np.random.default_rng(seed)
+    h = box_bohr / NN
+    dmin = min_OO_angstrom * ANGSTROM
+    O = []
+    tries = 0
+    while len(O) < n_mol:
+        tries += 1
+        if tries > max_tries:
+            raise RuntimeError(f"could not place {n_mol} waters with O-O >= {min_OO_angstrom} A")
+        p = rng.uniform(margin_bohr, box_bohr - margin_bohr, size=3)
+        if all(np.linalg.norm(p - q) >= dmin for q in O):
+            O.append(p)
+    geo = water_geometry()
+    rows, els = [], []
+    for p in O:
+        R = _rand_rotation(rng)
+        for a, el in enumerate(("O", "H", "H")):
+            x = p + R @ geo[a]
+            Z, rc = ELEMENT[el]
+            rows.append((x[0] / h, x[1] / h, x[2] / h, Z, rc))
+            els.append(el)
+    return np.array(rows, np.float64), els, box_bohr
+
+
+def c1_h2o():
+    """Plain method, single H2O on 96^3, h = 0.1 bohr."""
+    rows, els, screen = h2o_single(96, 0.1)
+    return {"NN": 96, "screen": screen, "rows": rows, "els": els, "mg_levels": 4}
+
+
+def c2_waters():
+    """10 H2O, seeded orientations, O-O >= 2.6 A, 12 A cube, 192^3, 5 multigrid levels."""
+    rows, els, screen = water_box(10, 12.0 * ANGSTROM, 192, seed=1)
+    return {"NN": 192, "screen": screen, "rows": rows, "els": els, "mg_levels": 5}
+
+
+def gaussian_init(rows, NN, screen, label, seed=0, sigma_H=0.9, sigma_O=0.6, noise=1e-3):
+    """Seeded Gaussian initial amplitudes per domain: exp(-r^2 / 2 sigma^2) about the
+    domain's nucleus plus noise * N(0,1); zero on the faces.  ``label`` is the flat
+    S^3 label map.  Returns a flat float32 S^3 array."""
+    rng = np.random.default_rng(seed)
+    S = NN + 1
+    h = screen / NN
+    x = np.arange(S) * h
+    lab = np.asarray(label).reshape(S, S, S)
+    U = np.zeros((S, S, S), np.float64)
+    for n, r in enumerate(rows):
+        sig = sigma_H if r[3] <= 1 else sigma_O
+        m = lab == n
+        if not m.any():
+            continue
+        ii, jj, kk = np.nonzero(m)
+        d2 = (x[ii] - r[0] * h) ** 2 + (x[jj] - r[1] * h) ** 2 + (x[kk] - r[2] * h) ** 2
+        U[ii, jj, kk] = np.exp(-d2 / (2 * sig * sig))
+    U += noise * rng.standard_normal(U.shape)
+    U[0, :, :] = U[-1, :, :] = 0
+    U[:, 0, :] = U[:, -1, :] = 0
+    U[:, :, 0] = U[:, :, -1] = 0
+    return U.astype(np.float32).ravel()

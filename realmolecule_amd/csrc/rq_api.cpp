@@ -1,0 +1,812 @@
+// C ABI of the RealQM grid solver (declared in include/rq.h).
+//
+// One handle = one molecular grid + one HIP stream.  The handle owns all
+// device memory; host buffers are caller-owned.  Pass order of the run() loop
+// body follows cloud/solver.py:561-586.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "rq_internal.h"
+
+using namespace rq;
+
+namespace {
+thread_local std::string g_create_error;
+
+struct EventPair {
+  int fam;
+  hipEvent_t a, b;
+};
+}  // namespace
+
+struct rq_solver {
+  rq_params p{};
+  int NN = 0, S = 0;
+  size_t S3 = 0;
+  double h64 = 0, dv = 0;
+  float h = 0, h2 = 0, h3 = 0, half_dv = 0, dtv = 0, two_pi = 0;
+  int n_atoms = 0;
+  std::vector<Atom> atoms_h;
+  std::vector<int32_t> lock_h;
+  Atom* atoms_d = nullptr;
+  float* Z_d = nullptr;
+  int32_t* lock_d = nullptr;
+  float *U = nullptr, *U2 = nullptr, *K = nullptr, *P = nullptr, *P2 = nullptr, *bestU = nullptr, *W = nullptr;
+  uint16_t *lab = nullptr, *lab2 = nullptr;
+  // multigrid levels 1..L-1: S, two iterate buffers and the rhs
+  int levels = 0;
+  std::vector<int> Sl;
+  std::vector<float*> e0, e1, g;
+  double* red = nullptr;  // n_atoms*3 + 3 + n_atoms*3 doubles
+  size_t red_len = 0;
+  hipStream_t st = nullptr;
+  Tile tile;
+  bool initialized = false;
+  bool timing = false;
+  std::vector<EventPair> ev_pending;
+  std::vector<hipEvent_t> ev_free;
+  double fam_ms[F_COUNT] = {0};
+  int64_t fam_n[F_COUNT] = {0};
+  std::string err;
+};
+
+namespace {
+
+int fail(rq_solver* h, int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (h) h->err = buf;
+  else g_create_error = buf;
+  return code;
+}
+
+#define HIPCHK(h, call)                                                                          \
+  do {                                                                                           \
+    hipError_t e_ = (call);                                                                      \
+    if (e_ != hipSuccess) return fail((h), RQ_EHIP, "%s: %s", #call, hipGetErrorString(e_));     \
+  } while (0)
+
+#define LAUNCHCHK(h)                                                                             \
+  do {                                                                                           \
+    hipError_t e_ = hipGetLastError();                                                           \
+    if (e_ != hipSuccess) return fail((h), RQ_EHIP, "kernel launch: %s", hipGetErrorString(e_)); \
+  } while (0)
+
+hipEvent_t take_event(rq_solver* s) {
+  if (!s->ev_free.empty()) {
+    hipEvent_t e = s->ev_free.back();
+    s->ev_free.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  (void)hipEventCreate(&e);
+  return e;
+}
+
+void resolve_events(rq_solver* s) {
+  if (s->ev_pending.empty()) return;
+  (void)hipStreamSynchronize(s->st);
+  for (auto& ep : s->ev_pending) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, ep.a, ep.b) == hipSuccess) {
+      s->fam_ms[ep.fam] += ms;
+      s->fam_n[ep.fam] += 1;
+    }
+    s->ev_free.push_back(ep.a);
+    s->ev_free.push_back(ep.b);
+  }
+  s->ev_pending.clear();
+}
+
+// Brackets a kernel launch with events when timing is on.
+struct Timed {
+  rq_solver* s;
+  int fam;
+  hipEvent_t a = nullptr;
+  Timed(rq_solver* s_, int f) : s(s_), fam(f) {
+    if (s->timing) {
+      a = take_event(s);
+      (void)hipEventRecord(a, s->st);
+    }
+  }
+  ~Timed() {
+    if (s->timing && a) {
+      hipEvent_t b = take_event(s);
+      (void)hipEventRecord(b, s->st);
+      s->ev_pending.push_back({fam, a, b});
+      if (s->ev_pending.size() > 4096) resolve_events(s);
+    }
+  }
+};
+
+Tile tile_for(int S) {
+  Tile t;
+  const int n = S - 2;
+  t.bx = n >= 48 ? 64 : (n >= 24 ? 32 : 16);
+  t.by = 256 / t.bx;
+  const int gx = (n + t.bx - 1) / t.bx, gy = (n + t.by - 1) / t.by;
+  const int want = 2048;
+  int nz = std::max(1, std::min(n, want / std::max(1, gx * gy)));
+  t.ich = std::max(1, (n + nz - 1) / nz);
+  return t;
+}
+
+void zero_doubles(rq_solver* s, double* p, size_t n) {
+  // zero with a fill kernel (not hipMemsetAsync) so the pass is graph-capturable
+  launch_fill(s->st, reinterpret_cast<float*>(p), n * 2, 0.0f);
+}
+
+int upload_atoms(rq_solver* s) {
+  HIPCHK(s, hipMemcpyAsync(s->atoms_d, s->atoms_h.data(), sizeof(Atom) * s->n_atoms, hipMemcpyHostToDevice, s->st));
+  std::vector<float> Z(s->n_atoms);
+  for (int a = 0; a < s->n_atoms; ++a) Z[a] = s->atoms_h[a].Z;
+  HIPCHK(s, hipMemcpyAsync(s->Z_d, Z.data(), sizeof(float) * s->n_atoms, hipMemcpyHostToDevice, s->st));
+  HIPCHK(s, hipStreamSynchronize(s->st));
+  return RQ_OK;
+}
+
+// ---- Poisson ----
+int smooth_fine(rq_solver* s, float a, float c) {
+  {
+    Timed t(s, F_SMOOTH);
+    launch_smooth_fine(s->st, s->tile, s->S, s->P, s->P2, s->U, a, c, s->two_pi, s->h2);
+  }
+  LAUNCHCHK(s);
+  std::swap(s->P, s->P2);
+  return RQ_OK;
+}
+
+// V-cycle on coarse level l (x = e0[l], rhs g[l]); zero initial guess.
+int vcycle_coarse(rq_solver* s, int l) {
+  const int S = s->Sl[l];
+  const size_t n3 = (size_t)S * S * S;
+  if (l == s->levels - 1) {
+    Timed t(s, F_COARSE);
+    launch_coarse_exact(s->st, S, s->g[l], s->e0[l]);
+    LAUNCHCHK(s);
+    return RQ_OK;
+  }
+  const Tile tl = tile_for(S);
+  launch_zero(s->st, s->e0[l], n3);
+  for (int it = 0; it < s->p.mg_nu1; ++it) {
+    {
+      Timed t(s, F_SMOOTH_C);
+      launch_smooth_coarse(s->st, tl, S, s->e0[l], s->e1[l], s->g[l], s->p.mg_a, s->p.mg_c);
+    }
+    std::swap(s->e0[l], s->e1[l]);
+  }
+  {
+    Timed t(s, F_RESTRICT_C);
+    launch_residual_coarse(s->st, tl, S, s->e0[l], s->g[l], s->e1[l]);
+    launch_restrict4(s->st, S, s->e1[l], s->g[l + 1], 1.0f);
+  }
+  LAUNCHCHK(s);
+  int rc = vcycle_coarse(s, l + 1);
+  if (rc) return rc;
+  {
+    Timed t(s, F_PROLONG_C);
+    launch_prolong_correct(s->st, tl, S, s->e0[l], s->e0[l + 1], 1.0f);
+  }
+  for (int it = 0; it < s->p.mg_nu2; ++it) {
+    {
+      Timed t(s, F_SMOOTH_C);
+      launch_smooth_coarse(s->st, tl, S, s->e0[l], s->e1[l], s->g[l], s->p.mg_a, s->p.mg_c);
+    }
+    std::swap(s->e0[l], s->e1[l]);
+  }
+  LAUNCHCHK(s);
+  return RQ_OK;
+}
+
+int vcycle_fine(rq_solver* s) {
+  for (int it = 0; it < s->p.mg_nu1; ++it) {
+    int rc = smooth_fine(s, s->p.mg_a, s->p.mg_c);
+    if (rc) return rc;
+  }
+  {
+    Timed t(s, F_RESTRICT);
+    launch_residual_fine(s->st, s->tile, s->S, s->P, s->U, s->two_pi, s->h2, s->P2);
+    launch_restrict4(s->st, s->S, s->P2, s->g[1], 1.0f);
+  }
+  LAUNCHCHK(s);
+  int rc = vcycle_coarse(s, 1);
+  if (rc) return rc;
+  {
+    Timed t(s, F_PROLONG);
+    launch_prolong_correct(s->st, s->tile, s->S, s->P, s->e0[1], 1.0f);
+  }
+  LAUNCHCHK(s);
+  for (int it = 0; it < s->p.mg_nu2; ++it) {
+    rc = smooth_fine(s, s->p.mg_a, s->p.mg_c);
+    if (rc) return rc;
+  }
+  return RQ_OK;
+}
+
+// molecule.js two-grid correction (sign-inverted, 0.5 damping), molecule.js:2879-2907.
+int twogrid_correction(rq_solver* s) {
+  const int S1 = s->Sl[1];
+  const size_t n3 = (size_t)S1 * S1 * S1;
+  const Tile tl = tile_for(S1);
+  {
+    Timed t(s, F_RESTRICT);
+    launch_residual_fine(s->st, s->tile, s->S, s->P, s->U, s->two_pi, s->h2, s->P2);
+    launch_restrict4(s->st, s->S, s->P2, s->g[1], -1.0f);
+  }
+  launch_zero(s->st, s->e0[1], n3);
+  for (int it = 0; it < s->p.twogrid_coarse_sweeps; ++it) {
+    {
+      Timed t(s, F_SMOOTH_C);
+      launch_smooth_coarse(s->st, tl, S1, s->e0[1], s->e1[1], s->g[1], s->p.smooth_a, s->p.smooth_c);
+    }
+    std::swap(s->e0[1], s->e1[1]);
+  }
+  {
+    Timed t(s, F_PROLONG);
+    launch_prolong_correct(s->st, s->tile, s->S, s->P, s->e0[1], 0.5f);
+  }
+  LAUNCHCHK(s);
+  return RQ_OK;
+}
+
+int poisson_step(rq_solver* s, bool twogrid_due) {
+  if (s->p.poisson_mode == RQ_POISSON_MULTIGRID) return vcycle_fine(s);
+  for (int it = 0; it < s->p.jacobi_sweeps; ++it) {
+    int rc = smooth_fine(s, s->p.smooth_a, s->p.smooth_c);
+    if (rc) return rc;
+  }
+  if (s->p.poisson_mode == RQ_POISSON_TWOGRID && twogrid_due) return twogrid_correction(s);
+  return RQ_OK;
+}
+
+int update_step(rq_solver* s) {
+  {
+    Timed t(s, F_ITP);
+    launch_itp(s->st, s->tile, s->S, s->U, s->U2, s->K, s->P, s->lab, s->half_dv, s->dtv);
+  }
+  LAUNCHCHK(s);
+  std::swap(s->U, s->U2);
+  return RQ_OK;
+}
+
+int normalize_step(rq_solver* s) {
+  double* norms = s->red;
+  zero_doubles(s, norms, s->n_atoms);
+  {
+    Timed t(s, F_NORM);
+    launch_norm_sums(s->st, s->tile, s->S, s->h3, s->U, s->lab, s->n_atoms, norms);
+    launch_normalize(s->st, s->tile, s->S, s->U, s->lab, norms, s->Z_d, s->n_atoms);
+  }
+  LAUNCHCHK(s);
+  return RQ_OK;
+}
+
+int boundary_step(rq_solver* s) {
+  if (s->p.boundary_mode == RQ_BOUNDARY_OFF) return RQ_OK;
+  {
+    Timed t(s, F_BOUNDARY);
+    if (s->p.boundary_mode == RQ_BOUNDARY_SCORE) {
+      launch_boundary_score(s->st, s->tile, s->S, s->h, s->h2, s->atoms_d, s->n_atoms, s->lab, s->lab2,
+                            s->p.flip_ratio, s->p.flip_min_foreign);
+    } else {
+      bool any_lock = false;
+      for (int32_t g : s->lock_h) any_lock |= (g > 0);
+      launch_levelset(s->st, s->tile, s->S, s->h, s->atoms_d, s->n_atoms, any_lock ? s->lock_d : nullptr, s->lab,
+                      s->lab2, s->U, s->W, s->p.ls_dt_w, s->p.ls_curv_reg, s->p.rc_dirichlet);
+    }
+  }
+  LAUNCHCHK(s);
+  // faces are never written by the boundary kernels: keep them in the new buffer
+  std::swap(s->lab, s->lab2);
+  // copy the whole buffer back so both halves agree on the faces and interior
+  HIPCHK(s, hipMemcpyAsync(s->lab2, s->lab, s->S3 * sizeof(uint16_t), hipMemcpyDeviceToDevice, s->st));
+  return RQ_OK;
+}
+
+float* field_ptr(rq_solver* s, int f) {
+  switch (f) {
+    case RQ_FIELD_U: return s->U;
+    case RQ_FIELD_U2: return s->U2;
+    case RQ_FIELD_K: return s->K;
+    case RQ_FIELD_P: return s->P;
+    case RQ_FIELD_P2: return s->P2;
+    case RQ_FIELD_BESTU: return s->bestU;
+    case RQ_FIELD_W: return s->W;
+    default: return nullptr;
+  }
+}
+
+void free_all(rq_solver* s) {
+  auto fr = [](void* p) {
+    if (p) (void)hipFree(p);
+  };
+  fr(s->atoms_d);
+  fr(s->Z_d);
+  fr(s->lock_d);
+  fr(s->U);
+  fr(s->U2);
+  fr(s->K);
+  fr(s->P);
+  fr(s->P2);
+  fr(s->bestU);
+  fr(s->W);
+  fr(s->lab);
+  fr(s->lab2);
+  fr(s->red);
+  for (size_t l = 0; l < s->e0.size(); ++l) {
+    fr(s->e0[l]);
+    fr(s->e1[l]);
+    fr(s->g[l]);
+  }
+  for (auto& ep : s->ev_pending) {
+    (void)hipEventDestroy(ep.a);
+    (void)hipEventDestroy(ep.b);
+  }
+  for (auto e : s->ev_free) (void)hipEventDestroy(e);
+  if (s->st) (void)hipStreamDestroy(s->st);
+}
+
+}  // namespace
+
+extern "C" {
+
+int rq_abi_version(void) { return RQ_ABI_VERSION; }
+
+const char* rq_last_error(const rq_solver* h) { return h ? h->err.c_str() : g_create_error.c_str(); }
+
+double rq_dv_for_electrons(int32_t n) {
+  if (n > 500) return 0.005;
+  if (n > 200) return 0.01;
+  if (n > 100) return 0.02;
+  if (n > 30) return 0.03;
+  return 0.12;
+}
+
+void rq_default_params(rq_params* p) {
+  std::memset(p, 0, sizeof *p);
+  p->NN = 96;
+  p->screen = 9.6f;
+  p->dv = 0.0f;
+  p->init_mode = RQ_INIT_S1;
+  p->poisson_mode = RQ_POISSON_JACOBI;
+  p->jacobi_sweeps = 2;
+  p->smooth_a = 0.0f;
+  p->smooth_c = 6.0f;
+  p->mg_levels = 4;
+  p->mg_nu1 = 2;
+  p->mg_nu2 = 2;
+  p->mg_a = 1.0f / 7.0f;
+  p->mg_c = 7.0f;
+  p->twogrid_interval = 50;
+  p->twogrid_coarse_sweeps = 10;
+  p->boundary_mode = RQ_BOUNDARY_SCORE;
+  p->flip_ratio = 1.1f;
+  p->flip_min_foreign = 2;
+  p->ls_dt_w = 0.5f;
+  p->ls_curv_reg = 0.15f;
+  p->energy_mode = RQ_ENERGY_S1;
+  p->rc_dirichlet = 0;
+}
+
+void rq_default_schedule(rq_schedule* s) {
+  s->poisson_interval = 2;
+  s->norm_interval = 20;
+  s->boundary_interval = 100;
+}
+
+int rq_create(const rq_params* pp, const rq_atom* atoms, int32_t n_atoms, rq_solver** out) {
+  if (!pp || !out || (!atoms && n_atoms > 0)) return fail(nullptr, RQ_EINVAL, "null argument");
+  *out = nullptr;
+  const rq_params& p = *pp;
+  for (int r : p.reserved)
+    if (r != 0) return fail(nullptr, RQ_EINVAL, "rq_params.reserved must be zero");
+  if (p.NN < 8 || p.NN > 1024 || (p.NN & 1)) return fail(nullptr, RQ_EINVAL, "NN=%d must be even in [8, 1024]", p.NN);
+  if (!(p.screen > 0.f)) return fail(nullptr, RQ_EINVAL, "screen must be > 0");
+  if (n_atoms < 1 || n_atoms > kMaxDomains)
+    return fail(nullptr, RQ_ERANGE, "n_atoms=%d outside [1, %d]", n_atoms, kMaxDomains);
+  if (p.poisson_mode < 0 || p.poisson_mode > 2) return fail(nullptr, RQ_EINVAL, "bad poisson_mode");
+  if (p.boundary_mode < 0 || p.boundary_mode > 2) return fail(nullptr, RQ_EINVAL, "bad boundary_mode");
+  if (p.init_mode < 0 || p.init_mode > 1) return fail(nullptr, RQ_EINVAL, "bad init_mode");
+  if (p.energy_mode < 0 || p.energy_mode > 1) return fail(nullptr, RQ_EINVAL, "bad energy_mode");
+  if (!(p.smooth_c != 0.f) || !(p.mg_c != 0.f)) return fail(nullptr, RQ_EINVAL, "smoother c must be non-zero");
+  int levels = 0;
+  if (p.poisson_mode == RQ_POISSON_MULTIGRID) {
+    levels = p.mg_levels;
+    if (levels < 2 || levels > 10) return fail(nullptr, RQ_EINVAL, "mg_levels=%d outside [2, 10]", levels);
+    const int div = 1 << (levels - 1);
+    if (p.NN % div) return fail(nullptr, RQ_EINVAL, "NN=%d not divisible by 2^(levels-1)=%d", p.NN, div);
+    const int mc = p.NN / div - 1;
+    if (mc < 1 || mc > kMaxCoarseInterior)
+      return fail(nullptr, RQ_ERANGE, "coarsest interior %d outside [1, %d]; change mg_levels", mc,
+                  kMaxCoarseInterior);
+    if (p.mg_nu1 < 0 || p.mg_nu2 < 0) return fail(nullptr, RQ_EINVAL, "negative smoothing count");
+  } else if (p.poisson_mode == RQ_POISSON_TWOGRID) {
+    levels = 2;
+    if (p.NN % 2 || p.NN / 2 < 3) return fail(nullptr, RQ_EINVAL, "two-grid needs NN/2 >= 3");
+    if (p.twogrid_interval < 1 || p.twogrid_coarse_sweeps < 0) return fail(nullptr, RQ_EINVAL, "bad two-grid knobs");
+  }
+  if (p.jacobi_sweeps < 0) return fail(nullptr, RQ_EINVAL, "negative jacobi_sweeps");
+
+  rq_solver* s = new rq_solver();
+  s->p = p;
+  s->NN = p.NN;
+  s->S = p.NN + 1;
+  s->S3 = (size_t)s->S * s->S * s->S;
+  s->h64 = (double)p.screen / p.NN;
+  int n_elec = 0;
+  s->atoms_h.resize(n_atoms);
+  for (int a = 0; a < n_atoms; ++a) {
+    s->atoms_h[a] = {atoms[a].i, atoms[a].j, atoms[a].k, atoms[a].Z, atoms[a].rc};
+    if (atoms[a].Z > 0.f) ++n_elec;
+  }
+  s->lock_h.assign(n_atoms, 0);
+  s->n_atoms = n_atoms;
+  s->dv = p.dv > 0.f ? (double)p.dv : rq_dv_for_electrons(n_elec);
+  s->h = (float)s->h64;
+  s->h2 = (float)(s->h64 * s->h64);
+  s->h3 = (float)(s->h64 * s->h64 * s->h64);
+  s->half_dv = (float)(0.5 * s->dv);
+  s->dtv = (float)(s->dv * (s->h64 * s->h64));
+  s->two_pi = (float)(2.0 * M_PI);
+  s->tile = tile_for(s->S);
+  s->levels = levels;
+
+  auto alloc = [&](void** ptr, size_t bytes) -> bool { return hipMalloc(ptr, bytes) == hipSuccess; };
+  bool ok = hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking) == hipSuccess;
+  const size_t fb = s->S3 * sizeof(float), lb = s->S3 * sizeof(uint16_t);
+  ok = ok && alloc((void**)&s->atoms_d, sizeof(Atom) * n_atoms) && alloc((void**)&s->Z_d, sizeof(float) * n_atoms) &&
+       alloc((void**)&s->lock_d, sizeof(int32_t) * n_atoms) && alloc((void**)&s->U, fb) &&
+       alloc((void**)&s->U2, fb) && alloc((void**)&s->K, fb) && alloc((void**)&s->P, fb) &&
+       alloc((void**)&s->P2, fb) && alloc((void**)&s->bestU, fb) && alloc((void**)&s->W, fb) &&
+       alloc((void**)&s->lab, lb) && alloc((void**)&s->lab2, lb);
+  s->red_len = (size_t)n_atoms * 3 + 3;
+  ok = ok && alloc((void**)&s->red, sizeof(double) * s->red_len);
+  if (ok && levels > 0) {
+    s->Sl.assign(levels, 0);
+    s->e0.assign(levels, nullptr);
+    s->e1.assign(levels, nullptr);
+    s->g.assign(levels, nullptr);
+    s->Sl[0] = s->S;
+    for (int l = 1; l < levels && ok; ++l) {
+      s->Sl[l] = (s->NN >> l) + 1;
+      const size_t b = (size_t)s->Sl[l] * s->Sl[l] * s->Sl[l] * sizeof(float);
+      ok = alloc((void**)&s->e0[l], b) && alloc((void**)&s->e1[l], b) && alloc((void**)&s->g[l], b);
+      if (ok) {
+        ok = hipMemsetAsync(s->e0[l], 0, b, s->st) == hipSuccess && hipMemsetAsync(s->e1[l], 0, b, s->st) == hipSuccess &&
+             hipMemsetAsync(s->g[l], 0, b, s->st) == hipSuccess;
+      }
+    }
+  }
+  if (!ok) {
+    (void)hipGetLastError();
+    free_all(s);
+    delete s;
+    return fail(nullptr, RQ_ENOMEM, "device allocation failed for NN=%d (%zu nodes)", p.NN, (size_t)(p.NN + 1) * (p.NN + 1) * (p.NN + 1));
+  }
+  // zero every field so faces start as Dirichlet zeros
+  const void* fields[] = {s->U, s->U2, s->K, s->P, s->P2, s->bestU, s->W};
+  for (const void* f : fields) ok = ok && hipMemsetAsync((void*)f, 0, fb, s->st) == hipSuccess;
+  ok = ok && hipMemsetAsync(s->lab, 0, lb, s->st) == hipSuccess && hipMemsetAsync(s->lab2, 0, lb, s->st) == hipSuccess;
+  ok = ok && hipMemsetAsync(s->lock_d, 0, sizeof(int32_t) * n_atoms, s->st) == hipSuccess;
+  if (!ok || upload_atoms(s) != RQ_OK) {
+    std::string m = s->err;
+    free_all(s);
+    delete s;
+    return fail(nullptr, RQ_EHIP, "device initialisation failed %s", m.c_str());
+  }
+  *out = s;
+  return RQ_OK;
+}
+
+int rq_destroy(rq_solver* h) {
+  if (!h) return RQ_OK;
+  (void)hipStreamSynchronize(h->st);
+  free_all(h);
+  delete h;
+  return RQ_OK;
+}
+
+int rq_sync(rq_solver* h) {
+  if (!h) return fail(nullptr, RQ_EINVAL, "null handle");
+  HIPCHK(h, hipStreamSynchronize(h->st));
+  return RQ_OK;
+}
+
+int rq_get_info(const rq_solver* h, int32_t* NN, int32_t* S, int32_t* n_atoms, double* hg, double* dv) {
+  if (!h) return RQ_EINVAL;
+  if (NN) *NN = h->NN;
+  if (S) *S = h->S;
+  if (n_atoms) *n_atoms = h->n_atoms;
+  if (hg) *hg = h->h64;
+  if (dv) *dv = h->dv;
+  return RQ_OK;
+}
+
+int rq_set_atoms(rq_solver* h, const rq_atom* atoms, int32_t n) {
+  if (!h || !atoms) return fail(h, RQ_EINVAL, "null argument");
+  if (n != h->n_atoms) return fail(h, RQ_EINVAL, "n_atoms %d != %d", n, h->n_atoms);
+  for (int a = 0; a < n; ++a) h->atoms_h[a] = {atoms[a].i, atoms[a].j, atoms[a].k, atoms[a].Z, atoms[a].rc};
+  return upload_atoms(h);
+}
+
+int rq_get_atoms(const rq_solver* h, rq_atom* atoms, int32_t n) {
+  if (!h || !atoms || n != h->n_atoms) return RQ_EINVAL;
+  for (int a = 0; a < n; ++a) atoms[a] = {h->atoms_h[a].i, h->atoms_h[a].j, h->atoms_h[a].k, h->atoms_h[a].Z, h->atoms_h[a].rc};
+  return RQ_OK;
+}
+
+int rq_set_lock_groups(rq_solver* h, const int32_t* group, int32_t n) {
+  if (!h || !group || n != h->n_atoms) return fail(h, RQ_EINVAL, "lock groups need n_atoms entries");
+  h->lock_h.assign(group, group + n);
+  HIPCHK(h, hipMemcpyAsync(h->lock_d, group, sizeof(int32_t) * n, hipMemcpyHostToDevice, h->st));
+  HIPCHK(h, hipStreamSynchronize(h->st));
+  return RQ_OK;
+}
+
+int rq_init_trial(rq_solver* h) {
+  if (!h) return fail(nullptr, RQ_EINVAL, "null handle");
+  launch_init_trial(h->st, h->p.init_mode, h->S, h->h, h->h2, h->atoms_d, h->n_atoms, h->U, h->U2, h->lab, h->K,
+                    h->bestU, h->W);
+  LAUNCHCHK(h);
+  if (h->p.rc_dirichlet) {
+    launch_rc_flags(h->st, h->S, h->h, h->atoms_d, h->n_atoms, h->lab);
+    LAUNCHCHK(h);
+  }
+  HIPCHK(h, hipMemcpyAsync(h->lab2, h->lab, h->S3 * sizeof(uint16_t), hipMemcpyDeviceToDevice, h->st));
+  HIPCHK(h, hipMemsetAsync(h->P, 0, h->S3 * sizeof(float), h->st));
+  HIPCHK(h, hipMemsetAsync(h->P2, 0, h->S3 * sizeof(float), h->st));
+  h->initialized = true;
+  return RQ_OK;
+}
+
+int rq_initialize(rq_solver* h, int32_t bootstrap) {
+  int rc = rq_init_trial(h);
+  if (rc) return rc;
+  rc = normalize_step(h);
+  if (rc) return rc;
+  for (int i = 0; i < bootstrap; ++i) {
+    rc = poisson_step(h, false);
+    if (rc) return rc;
+  }
+  return RQ_OK;
+}
+
+int rq_poisson(rq_solver* h, int32_t n) {
+  if (!h) return fail(nullptr, RQ_EINVAL, "null handle");
+  for (int i = 0; i < n; ++i) {
+    int rc = poisson_step(h, false);
+    if (rc) return rc;
+  }
+  return RQ_OK;
+}
+
+int rq_update(rq_solver* h) {
+  if (!h) return fail(nullptr, RQ_EINVAL, "null handle");
+  return update_step(h);
+}
+
+int rq_normalize(rq_solver* h) {
+  if (!h) return fail(nullptr, RQ_EINVAL, "null handle");
+  return normalize_step(h);
+}
+
+int rq_boundary(rq_solver* h) {
+  if (!h) return fail(nullptr, RQ_EINVAL, "null handle");
+  return boundary_step(h);
+}
+
+int rq_recompute_k(rq_solver* h) {
+  if (!h) return fail(nullptr, RQ_EINVAL, "null handle");
+  launch_recompute_k(h->st, h->p.init_mode, h->S, h->h, h->h2, h->atoms_d, h->n_atoms, h->K);
+  LAUNCHCHK(h);
+  if (h->p.rc_dirichlet) {
+    launch_rc_flags(h->st, h->S, h->h, h->atoms_d, h->n_atoms, h->lab);
+    LAUNCHCHK(h);
+    HIPCHK(h, hipMemcpyAsync(h->lab2, h->lab, h->S3 * sizeof(uint16_t), hipMemcpyDeviceToDevice, h->st));
+  }
+  return RQ_OK;
+}
+
+int rq_step(rq_solver* h, int64_t first, int32_t n, const rq_schedule* sc) {
+  if (!h) return fail(nullptr, RQ_EINVAL, "null handle");
+  if (!h->initialized) return fail(h, RQ_ESTATE, "rq_step before rq_init_trial/rq_initialize");
+  rq_schedule d;
+  rq_default_schedule(&d);
+  if (!sc) sc = &d;
+  if (sc->poisson_interval < 1 || sc->norm_interval < 1 || sc->boundary_interval < 1)
+    return fail(h, RQ_EINVAL, "schedule intervals must be >= 1");
+  for (int64_t step = first + 1; step <= first + n; ++step) {
+    int rc;
+    if (step % sc->poisson_interval == 0) {
+      const bool tg = h->p.poisson_mode == RQ_POISSON_TWOGRID && step % h->p.twogrid_interval == 0;
+      if ((rc = poisson_step(h, tg))) return rc;
+    }
+    if ((rc = update_step(h))) return rc;
+    if (step % sc->norm_interval == 0 && (rc = normalize_step(h))) return rc;
+    if (h->p.boundary_mode != RQ_BOUNDARY_OFF && step % sc->boundary_interval == 0 && (rc = boundary_step(h)))
+      return rc;
+  }
+  return RQ_OK;
+}
+
+int rq_energy(rq_solver* h, double* out6, double* per_domain) {
+  if (!h || !out6) return fail(h, RQ_EINVAL, "null argument");
+  const int nd = h->n_atoms;
+  double* per = h->red;
+  double* dip = h->red + (size_t)nd * 3;
+  zero_doubles(h, h->red, h->red_len);
+  {
+    Timed t(h, F_ENERGY);
+    launch_energy(h->st, h->tile, h->p.energy_mode, h->S, h->h, h->h3, h->U, h->K, h->P, h->lab, h->atoms_d, nd, per,
+                  dip);
+  }
+  LAUNCHCHK(h);
+  std::vector<double> hostv(h->red_len);
+  HIPCHK(h, hipMemcpyAsync(hostv.data(), h->red, sizeof(double) * h->red_len, hipMemcpyDeviceToHost, h->st));
+  HIPCHK(h, hipStreamSynchronize(h->st));
+  double T = 0, K = 0, E = 0;
+  for (int d = 0; d < nd; ++d) {
+    T += hostv[d * 3 + 0];
+    K += hostv[d * 3 + 1];
+    E += hostv[d * 3 + 2];
+  }
+  out6[0] = T;
+  out6[1] = K;
+  out6[2] = E;
+  out6[3] = hostv[nd * 3 + 0];
+  out6[4] = hostv[nd * 3 + 1];
+  out6[5] = hostv[nd * 3 + 2];
+  if (per_domain) std::memcpy(per_domain, hostv.data(), sizeof(double) * nd * 3);
+  if (!std::isfinite(T + K + E)) return fail(h, RQ_EUNSTABLE, "non-finite energy (T=%g VeK=%g Vee=%g)", T, K, E);
+  return RQ_OK;
+}
+
+int rq_domain_norms(rq_solver* h, double* norms) {
+  if (!h || !norms) return fail(h, RQ_EINVAL, "null argument");
+  zero_doubles(h, h->red, h->n_atoms);
+  launch_norm_sums(h->st, h->tile, h->S, h->h3, h->U, h->lab, h->n_atoms, h->red);
+  LAUNCHCHK(h);
+  HIPCHK(h, hipMemcpyAsync(norms, h->red, sizeof(double) * h->n_atoms, hipMemcpyDeviceToHost, h->st));
+  HIPCHK(h, hipStreamSynchronize(h->st));
+  return RQ_OK;
+}
+
+int rq_forces_hf(rq_solver* h, int32_t first, int32_t count, double* out3) {
+  if (!h || !out3) return fail(h, RQ_EINVAL, "null argument");
+  if (first < 0 || count < 0 || first + count > h->n_atoms) return fail(h, RQ_ERANGE, "atom range out of bounds");
+  if (count == 0) return RQ_OK;
+  double* buf = nullptr;
+  HIPCHK(h, hipMallocAsync((void**)&buf, sizeof(double) * count * 3, h->st));
+  zero_doubles(h, buf, (size_t)count * 3);
+  {
+    Timed t(h, F_FORCE);
+    launch_forces_hf(h->st, h->S, h->h, h->h2, h->h3, h->U, h->atoms_d, first, count, buf);
+  }
+  LAUNCHCHK(h);
+  HIPCHK(h, hipMemcpyAsync(out3, buf, sizeof(double) * count * 3, hipMemcpyDeviceToHost, h->st));
+  HIPCHK(h, hipFreeAsync(buf, h->st));
+  HIPCHK(h, hipStreamSynchronize(h->st));
+  return RQ_OK;
+}
+
+int rq_forces_gradp(rq_solver* h, int32_t radius, double* out3n) {
+  if (!h || !out3n) return fail(h, RQ_EINVAL, "null argument");
+  int R = radius > 0 ? radius : std::max(5, (int)std::lround(3.0 / h->h64));
+  if (R > h->NN) return fail(h, RQ_ERANGE, "radius %d beyond grid", R);
+  double* buf = nullptr;
+  HIPCHK(h, hipMallocAsync((void**)&buf, sizeof(double) * h->n_atoms * 3, h->st));
+  {
+    Timed t(h, F_FORCE);
+    launch_forces_gradp(h->st, h->S, h->h, h->P, h->atoms_d, h->n_atoms, R, buf);
+  }
+  LAUNCHCHK(h);
+  HIPCHK(h, hipMemcpyAsync(out3n, buf, sizeof(double) * h->n_atoms * 3, hipMemcpyDeviceToHost, h->st));
+  HIPCHK(h, hipFreeAsync(buf, h->st));
+  HIPCHK(h, hipStreamSynchronize(h->st));
+  return RQ_OK;
+}
+
+int rq_read_field(rq_solver* h, int32_t f, void* host, size_t bytes) {
+  if (!h || !host) return fail(h, RQ_EINVAL, "null argument");
+  const size_t need = h->S3 * 4;
+  if (bytes != need) return fail(h, RQ_EINVAL, "field %d needs %zu bytes, got %zu", f, need, bytes);
+  if (f == RQ_FIELD_LABEL || f == RQ_FIELD_RHO) {
+    void* tmp = nullptr;
+    HIPCHK(h, hipMallocAsync(&tmp, need, h->st));
+    if (f == RQ_FIELD_LABEL) launch_label_u16_to_i32(h->st, h->S3, h->lab, (int32_t*)tmp);
+    else launch_square(h->st, h->S, h->U, (float*)tmp);
+    LAUNCHCHK(h);
+    HIPCHK(h, hipMemcpyAsync(host, tmp, need, hipMemcpyDeviceToHost, h->st));
+    HIPCHK(h, hipFreeAsync(tmp, h->st));
+  } else {
+    float* p = field_ptr(h, f);
+    if (!p) return fail(h, RQ_EINVAL, "unknown field %d", f);
+    HIPCHK(h, hipMemcpyAsync(host, p, need, hipMemcpyDeviceToHost, h->st));
+  }
+  HIPCHK(h, hipStreamSynchronize(h->st));
+  return RQ_OK;
+}
+
+int rq_write_field(rq_solver* h, int32_t f, const void* host, size_t bytes) {
+  if (!h || !host) return fail(h, RQ_EINVAL, "null argument");
+  const size_t need = h->S3 * 4;
+  if (bytes != need) return fail(h, RQ_EINVAL, "field %d needs %zu bytes, got %zu", f, need, bytes);
+  if (f == RQ_FIELD_RHO) return fail(h, RQ_EINVAL, "rho is read-only (computed from U)");
+  if (f == RQ_FIELD_LABEL) {
+    void* tmp = nullptr;
+    HIPCHK(h, hipMallocAsync(&tmp, need, h->st));
+    HIPCHK(h, hipMemcpyAsync(tmp, host, need, hipMemcpyHostToDevice, h->st));
+    launch_label_i32_to_u16(h->st, h->S3, (const int32_t*)tmp, h->lab);
+    LAUNCHCHK(h);
+    if (h->p.rc_dirichlet) {
+      launch_rc_flags(h->st, h->S, h->h, h->atoms_d, h->n_atoms, h->lab);
+      LAUNCHCHK(h);
+    }
+    HIPCHK(h, hipMemcpyAsync(h->lab2, h->lab, h->S3 * sizeof(uint16_t), hipMemcpyDeviceToDevice, h->st));
+    HIPCHK(h, hipFreeAsync(tmp, h->st));
+  } else {
+    float* p = field_ptr(h, f);
+    if (!p) return fail(h, RQ_EINVAL, "unknown field %d", f);
+    HIPCHK(h, hipMemcpyAsync(p, host, need, hipMemcpyHostToDevice, h->st));
+  }
+  HIPCHK(h, hipStreamSynchronize(h->st));
+  h->initialized = true;
+  return RQ_OK;
+}
+
+int rq_swap_u(rq_solver* h) {
+  if (!h) return fail(nullptr, RQ_EINVAL, "null handle");
+  std::swap(h->U, h->U2);
+  return RQ_OK;
+}
+
+int rq_field_device_ptr(rq_solver* h, int32_t f, void** dptr) {
+  if (!h || !dptr) return fail(h, RQ_EINVAL, "null argument");
+  if (f == RQ_FIELD_LABEL) {
+    *dptr = h->lab;
+    return RQ_OK;
+  }
+  float* p = field_ptr(h, f);
+  if (!p) return fail(h, RQ_EINVAL, "field %d has no device buffer", f);
+  *dptr = p;
+  return RQ_OK;
+}
+
+int rq_stream(rq_solver* h, void** s) {
+  if (!h || !s) return fail(h, RQ_EINVAL, "null argument");
+  *s = (void*)h->st;
+  return RQ_OK;
+}
+
+int rq_timing_enable(rq_solver* h, int32_t on) {
+  if (!h) return fail(nullptr, RQ_EINVAL, "null handle");
+  resolve_events(h);
+  h->timing = on != 0;
+  for (int f = 0; f < F_COUNT; ++f) {
+    h->fam_ms[f] = 0;
+    h->fam_n[f] = 0;
+  }
+  return RQ_OK;
+}
+
+int rq_timing_read(rq_solver* h, double* ms, int64_t* launches, int32_t n) {
+  if (!h) return fail(nullptr, RQ_EINVAL, "null handle");
+  resolve_events(h);
+  for (int f = 0; f < n && f < F_COUNT; ++f) {
+    if (ms) ms[f] = h->fam_ms[f];
+    if (launches) launches[f] = h->fam_n[f];
+  }
+  return RQ_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,75 @@
+// Internal declarations shared by the kernel and API translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/rq.h"
+
+namespace rq {
+
+// Label storage: uint16 per node; bit 15 marks "inside the pseudopotential
+// radius rc of the own domain's nucleus" (molecule.js isInsideRc rule), the
+// low 15 bits are the domain index.
+constexpr uint16_t kRcFlag = 0x8000u;
+constexpr uint16_t kLabelMask = 0x7fffu;
+constexpr int kMaxDomains = 4096;       // LDS bins of the per-domain reductions
+constexpr int kMaxCoarseInterior = 25;  // exact FP64 solve held in LDS (m^3 doubles)
+
+// Launch shape of the 2.5D stencil kernels: (bx, by) tile in (k, j), marching
+// ich planes along i.
+struct Tile {
+  int bx = 64, by = 4, ich = 8;
+};
+
+struct Atom {
+  float i, j, k, Z, rc;
+};
+
+// Kernel families timed separately by rq_timing_*: fine-grid (level 0)
+// kernels are split from the coarse-level ones so each can be priced against
+// its own algorithmic bytes.
+enum Family {
+  F_ITP = 0, F_SMOOTH, F_SMOOTH_C, F_RESTRICT, F_RESTRICT_C, F_PROLONG, F_PROLONG_C, F_COARSE, F_NORM,
+  F_BOUNDARY, F_ENERGY, F_FORCE, F_COUNT
+};
+
+// ---- launchers (rq_kernels.hip) ----
+void launch_init_trial(hipStream_t st, int mode, int S, float h, float h2, const Atom* atoms, int n_atoms,
+                       float* U, float* U2, uint16_t* lab, float* K, float* bestU, float* W);
+void launch_recompute_k(hipStream_t st, int mode, int S, float h, float h2, const Atom* atoms, int n_atoms, float* K);
+void launch_rc_flags(hipStream_t st, int S, float h, const Atom* atoms, int n_atoms, uint16_t* lab);
+void launch_itp(hipStream_t st, const Tile& t, int S, const float* U, float* Un, const float* K, const float* P,
+                const uint16_t* lab, float half_dv, float dtv);
+void launch_smooth_fine(hipStream_t st, const Tile& t, int S, const float* P, float* Pn, const float* U, float a,
+                        float c, float alpha, float beta);
+void launch_smooth_coarse(hipStream_t st, const Tile& t, int S, const float* E, float* En, const float* g, float a,
+                          float c);
+void launch_residual_fine(hipStream_t st, const Tile& t, int S, const float* P, const float* U, float alpha,
+                          float beta, float* r);
+void launch_residual_coarse(hipStream_t st, const Tile& t, int S, const float* E, const float* g, float* r);
+void launch_restrict4(hipStream_t st, int Sf, const float* r, float* gc, float sign);
+void launch_prolong_correct(hipStream_t st, const Tile& t, int Sf, float* x, const float* ec, float scale);
+void launch_coarse_exact(hipStream_t st, int Sc, const float* g, float* e);
+void launch_zero(hipStream_t st, float* p, size_t n);
+void launch_norm_sums(hipStream_t st, const Tile& t, int S, float h3, const float* U, const uint16_t* lab,
+                      int n_dom, double* norms);
+void launch_normalize(hipStream_t st, const Tile& t, int S, float* U, const uint16_t* lab, const double* norms,
+                      const float* Z, int n_dom);
+void launch_boundary_score(hipStream_t st, const Tile& t, int S, float h, float h2, const Atom* atoms, int n_atoms,
+                           const uint16_t* lin, uint16_t* lout, float ratio, int min_foreign);
+void launch_levelset(hipStream_t st, const Tile& t, int S, float h, const Atom* atoms, int n_atoms,
+                     const int32_t* lock_group, const uint16_t* lin, uint16_t* lout, const float* U, float* W,
+                     float dt_w, float curv_reg, int rc_flags);
+void launch_energy(hipStream_t st, const Tile& t, int mode, int S, float h, float h3, const float* U,
+                   const float* K, const float* P, const uint16_t* lab, const Atom* atoms, int n_dom,
+                   double* per_dom /* n_dom*3 */, double* dip /* 3 */);
+void launch_forces_hf(hipStream_t st, int S, float h, float h2, float h3, const float* U, const Atom* atoms,
+                      int first, int count, double* out3);
+void launch_forces_gradp(hipStream_t st, int S, float h, const float* P, const Atom* atoms, int n_atoms, int radius,
+                         double* out3n);
+void launch_square(hipStream_t st, int S, const float* U, float* rho);
+void launch_label_u16_to_i32(hipStream_t st, size_t n, const uint16_t* in, int32_t* out);
+void launch_label_i32_to_u16(hipStream_t st, size_t n, const int32_t* in, uint16_t* out);
+void launch_fill(hipStream_t st, float* p, size_t n, float v);
+
+}  // namespace rq

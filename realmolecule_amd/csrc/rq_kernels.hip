@@ -1,0 +1,1029 @@
+// HIP kernels of the RealQM grid solver for gfx950 (MI355X).
+//
+// All fields are FP32 on the (S, S, S) node grid, flat id = i*S*S + j*S + k.
+// Stencil kernels are 2.5D: a (bx, by) workgroup tile covers (k, j) and each
+// thread marches `ich` consecutive i-planes, carrying the centre line
+// (i-1, i, i+1) in registers so each plane is fetched once per tile; the
+// in-plane neighbours come through the L1/L2 caches.  Only interior nodes
+// (1..NN-1 on every axis) are written; faces are fixed Dirichlet data.
+//
+// Floating-point order follows the reference kernels term by term and the
+// library is built with -ffp-contract=off, so FP32 results match an IEEE
+// single-precision evaluation of the same expressions (the CPU oracle's
+// FP32-faithful mode).  Reductions accumulate FP32 per-voxel terms in FP64.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "rq_internal.h"
+
+namespace rq {
+
+namespace {
+
+__device__ __forceinline__ int nint() { return 0; }
+
+inline dim3 stencil_grid(int n, const Tile& t) {
+  return dim3((n + t.bx - 1) / t.bx, (n + t.by - 1) / t.by, (n + t.ich - 1) / t.ich);
+}
+
+// Interior coordinate of this thread in a 2.5D launch; false if outside.
+struct March {
+  int k, j, i0, i1;
+};
+
+__device__ __forceinline__ bool march_coords(int n, int ich, March& m) {
+  m.k = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+  m.j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
+  m.i0 = 1 + blockIdx.z * ich;
+  m.i1 = min(m.i0 + ich, n + 1);
+  return m.k <= n && m.j <= n;
+}
+
+// --------------------------------------------------------------------------
+// Initialisation: trial functions, labels, K (cloud/solver.py:16-59 and
+// molecule.js:1377-1449).
+// --------------------------------------------------------------------------
+__global__ void k_init_trial(int mode, int S, float h, float h2, float r_sing, const Atom* __restrict__ atoms,
+                             int n_atoms, float* __restrict__ U, float* __restrict__ U2,
+                             uint16_t* __restrict__ lab, float* __restrict__ K, float* __restrict__ bestU,
+                             float* __restrict__ W) {
+  const int S2 = S * S;
+  const int S3 = S2 * S;
+  const int id = blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= S3) return;
+  const int i = id / S2;
+  const int j = (id / S) % S;
+  const int k = id % S;
+  const int NN = S - 1;
+  const bool face = (i < 1 || i >= NN || j < 1 || j >= NN || k < 1 || k >= NN);
+  W[id] = 1.0f;
+  if (mode == RQ_INIT_S1 && face) {
+    U[id] = 0.0f;
+    U2[id] = 0.0f;
+    lab[id] = 0;
+    K[id] = 0.0f;
+    bestU[id] = 0.0f;
+    return;
+  }
+  const float xi = (float)i * h, yj = (float)j * h, zk = (float)k * h;
+  const float inv_sqrt_pi = 0.5641895835f;
+  float bU = 0.0f;
+  int bestN = 0;
+  float Kv = 0.0f;
+  for (int n = 0; n < n_atoms; ++n) {
+    const float Za = atoms[n].Z;
+    if (!(Za > 0.0f)) continue;
+    const float rc = atoms[n].rc;
+    float ax = atoms[n].i, ay = atoms[n].j, az = atoms[n].k;
+    if (mode == RQ_INIT_S2) {  // molecule.js posI/J/K are u32 (truncated)
+      ax = floorf(ax);
+      ay = floorf(ay);
+      az = floorf(az);
+    }
+    const float dx = xi - ax * h;
+    const float dy = yj - ay * h;
+    const float dz = zk - az * h;
+    const float r2 = dx * dx + dy * dy + dz * dz;
+    float r;
+    if (rc > 0.0f) {
+      r = fmaxf(sqrtf(r2 + 0.04f * h2), rc);
+    } else if (mode == RQ_INIT_S2) {
+      r = fmaxf(sqrtf(r2), r_sing);
+    } else {
+      r = sqrtf(r2 + h2);
+    }
+    Kv += Za / r;
+    const float ut = Za * Za * inv_sqrt_pi * expf(-Za * r);
+    if (ut > bU) {
+      bU = ut;
+      bestN = n;
+    }
+  }
+  K[id] = Kv;
+  bestU[id] = bU;
+  lab[id] = (uint16_t)bestN;
+  U[id] = bU;
+  U2[id] = (mode == RQ_INIT_S2) ? bU : 0.0f;
+}
+
+__global__ void k_recompute_k(int mode, int S, float h, float h2, float r_sing, const Atom* __restrict__ atoms,
+                              int n_atoms, float* __restrict__ K) {
+  const int S2 = S * S;
+  const int id = blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= S2 * S) return;
+  const int i = id / S2;
+  const int j = (id / S) % S;
+  const int k = id % S;
+  const int NN = S - 1;
+  if (mode == RQ_INIT_S1 && (i < 1 || i >= NN || j < 1 || j >= NN || k < 1 || k >= NN)) {
+    K[id] = 0.0f;
+    return;
+  }
+  const float xi = (float)i * h, yj = (float)j * h, zk = (float)k * h;
+  float Kv = 0.0f;
+  for (int n = 0; n < n_atoms; ++n) {
+    const float Za = atoms[n].Z;
+    if (!(Za > 0.0f)) continue;
+    const float rc = atoms[n].rc;
+    float ax = atoms[n].i, ay = atoms[n].j, az = atoms[n].k;
+    if (mode == RQ_INIT_S2) {
+      ax = floorf(ax);
+      ay = floorf(ay);
+      az = floorf(az);
+    }
+    const float dx = xi - ax * h;
+    const float dy = yj - ay * h;
+    const float dz = zk - az * h;
+    const float r2 = dx * dx + dy * dy + dz * dz;
+    float r;
+    if (rc > 0.0f) {
+      r = fmaxf(sqrtf(r2 + 0.04f * h2), rc);
+    } else if (mode == RQ_INIT_S2) {
+      r = fmaxf(sqrtf(r2), r_sing);
+    } else {
+      r = sqrtf(r2 + h2);
+    }
+    Kv += Za / r;
+  }
+  K[id] = Kv;
+}
+
+// rc flag (bit 15 of the label): node lies strictly inside the rc sphere of
+// the nucleus of its own domain (molecule.js:242-246 isInsideRc, integer
+// nucleus positions).
+__global__ void k_rc_flags(int S, float h, const Atom* __restrict__ atoms, int n_atoms, uint16_t* __restrict__ lab) {
+  const int S2 = S * S;
+  const int id = blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= S2 * S) return;
+  const int i = id / S2;
+  const int j = (id / S) % S;
+  const int k = id % S;
+  const uint16_t raw = lab[id];
+  const int L = raw & kLabelMask;
+  uint16_t out = (uint16_t)L;
+  if (L < n_atoms) {
+    const float rc = atoms[L].rc;
+    if (rc > 0.0f) {
+      const float dx = ((float)i - floorf(atoms[L].i)) * h;
+      const float dy = ((float)j - floorf(atoms[L].j)) * h;
+      const float dz = ((float)k - floorf(atoms[L].k)) * h;
+      if (sqrtf(dx * dx + dy * dy + dz * dz) < rc) out |= kRcFlag;
+    }
+  }
+  lab[id] = out;
+}
+
+// --------------------------------------------------------------------------
+// ITP update (cloud/solver.py:61-90; molecule.js:228-343 rc rule via flag).
+// --------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_itp(int S, int ich, const float* __restrict__ U, float* __restrict__ Un,
+                                             const float* __restrict__ K, const float* __restrict__ P,
+                                             const uint16_t* __restrict__ lab, float half_dv, float dtv) {
+  const int n = S - 2;
+  March m;
+  if (!march_coords(n, ich, m)) return;
+  const int S2 = S * S;
+  int id = m.i0 * S2 + m.j * S + m.k;
+  float um = U[id - S2], uc = U[id];
+  uint16_t lm = lab[id - S2], lc = lab[id];
+  for (int i = m.i0; i < m.i1; ++i, id += S2) {
+    const float up = U[id + S2];
+    const uint16_t lp = lab[id + S2];
+    const float ujp = U[id + S], ujm = U[id - S], ukp = U[id + 1], ukm = U[id - 1];
+    const uint16_t ljp = lab[id + S], ljm = lab[id - S], lkp = lab[id + 1], lkm = lab[id - 1];
+    float out;
+    if (lc & kRcFlag) {
+      out = 0.0f;
+    } else {
+      const float t0 = (lp == lc) ? up : uc;
+      const float t1 = (lm == lc) ? um : uc;
+      const float t2 = (ljp == lc) ? ujp : uc;
+      const float t3 = (ljm == lc) ? ujm : uc;
+      const float t4 = (lkp == lc) ? ukp : uc;
+      const float t5 = (lkm == lc) ? ukm : uc;
+      const float lap = t0 + t1 + t2 + t3 + t4 + t5 - 6.0f * uc;
+      out = uc + half_dv * lap + dtv * (K[id] - 2.0f * P[id]) * uc;
+    }
+    Un[id] = out;
+    um = uc;
+    uc = up;
+    lm = lc;
+    lc = lp;
+  }
+}
+
+// --------------------------------------------------------------------------
+// Poisson: damped-Jacobi smoother P' = a P + (sum6 P + rhs) / c.
+// FINE: rhs = (alpha * U^2) * beta (cloud/solver.py:206 ``coeff*rho*h2``).
+// COARSE: rhs = g.
+// --------------------------------------------------------------------------
+template <bool FINE>
+__global__ __launch_bounds__(256) void k_smooth(int S, int ich, const float* __restrict__ P, float* __restrict__ Pn,
+                                                const float* __restrict__ src, float a, float c, float alpha,
+                                                float beta) {
+  const int n = S - 2;
+  March m;
+  if (!march_coords(n, ich, m)) return;
+  const int S2 = S * S;
+  int id = m.i0 * S2 + m.j * S + m.k;
+  float pm = P[id - S2], pc = P[id];
+  for (int i = m.i0; i < m.i1; ++i, id += S2) {
+    const float pp = P[id + S2];
+    const float s6 = pp + pm + P[id + S] + P[id - S] + P[id + 1] + P[id - 1];
+    float rhs;
+    if (FINE) {
+      const float u = src[id];
+      rhs = (alpha * (u * u)) * beta;
+    } else {
+      rhs = src[id];
+    }
+    float v = (s6 + rhs) / c;
+    if (a != 0.0f) v = a * pc + v;
+    Pn[id] = v;
+    pm = pc;
+    pc = pp;
+  }
+}
+
+// r = rhs - (6 x - sum6 x) on the interior.
+template <bool FINE>
+__global__ __launch_bounds__(256) void k_residual(int S, int ich, const float* __restrict__ x,
+                                                  const float* __restrict__ src, float alpha, float beta,
+                                                  float* __restrict__ r) {
+  const int n = S - 2;
+  March m;
+  if (!march_coords(n, ich, m)) return;
+  const int S2 = S * S;
+  int id = m.i0 * S2 + m.j * S + m.k;
+  float xm = x[id - S2], xc = x[id];
+  for (int i = m.i0; i < m.i1; ++i, id += S2) {
+    const float xp = x[id + S2];
+    const float s6 = xp + xm + x[id + S] + x[id - S] + x[id + 1] + x[id - 1];
+    float rhs;
+    if (FINE) {
+      const float u = src[id];
+      rhs = (alpha * (u * u)) * beta;
+    } else {
+      rhs = src[id];
+    }
+    r[id] = rhs - (6.0f * xc - s6);
+    xm = xc;
+    xc = xp;
+  }
+}
+
+// Coarse rhs = sign * 4 * FW(r) at coarse interior nodes; FW = 27-point full
+// weighting (8,4,2,1)/64 summed centre, faces, edges, corners in fixed order.
+__global__ __launch_bounds__(256) void k_restrict4(int Sf, const float* __restrict__ r, float* __restrict__ gc,
+                                                   float sign) {
+  const int NC = (Sf - 1) / 2;
+  const int Sc = NC + 1;
+  const int nc = NC - 1;
+  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= nc * nc * nc) return;
+  const int K2 = tid % nc + 1;
+  const int J = (tid / nc) % nc + 1;
+  const int I = tid / (nc * nc) + 1;
+  const int S2 = Sf * Sf;
+  const int c = (2 * I) * S2 + (2 * J) * Sf + 2 * K2;
+  const int di = S2, dj = Sf, dk = 1;
+  float fs = r[c + di];
+  fs = fs + r[c - di];
+  fs = fs + r[c + dj];
+  fs = fs + r[c - dj];
+  fs = fs + r[c + dk];
+  fs = fs + r[c - dk];
+  float es = r[c + di + dj];
+  es = es + r[c + di - dj];
+  es = es + r[c - di + dj];
+  es = es + r[c - di - dj];
+  es = es + r[c + di + dk];
+  es = es + r[c + di - dk];
+  es = es + r[c - di + dk];
+  es = es + r[c - di - dk];
+  es = es + r[c + dj + dk];
+  es = es + r[c + dj - dk];
+  es = es + r[c - dj + dk];
+  es = es + r[c - dj - dk];
+  float cs = r[c + di + dj + dk];
+  cs = cs + r[c + di + dj - dk];
+  cs = cs + r[c + di - dj + dk];
+  cs = cs + r[c + di - dj - dk];
+  cs = cs + r[c - di + dj + dk];
+  cs = cs + r[c - di + dj - dk];
+  cs = cs + r[c - di - dj + dk];
+  cs = cs + r[c - di - dj - dk];
+  const float val = 8.0f * r[c] + 4.0f * fs + 2.0f * es + cs;
+  float out = val * 0.0625f;
+  if (sign < 0.0f) out = -out;
+  gc[I * Sc * Sc + J * Sc + K2] = out;
+}
+
+// Trilinear prolongation (k axis first, then j, then i) and correction
+// x += scale * P e on the fine interior.
+__device__ __forceinline__ float lerp_k(const float* __restrict__ e, int base, int k) {
+  const int kc = k >> 1;
+  if ((k & 1) == 0) return e[base + kc];
+  return 0.5f * (e[base + kc] + e[base + kc + 1]);
+}
+
+__global__ __launch_bounds__(256) void k_prolong_correct(int Sf, int ich, float* __restrict__ x,
+                                                         const float* __restrict__ e, float scale) {
+  const int n = Sf - 2;
+  March m;
+  if (!march_coords(n, ich, m)) return;
+  const int Sc = (Sf - 1) / 2 + 1;
+  const int Sc2 = Sc * Sc;
+  const int S2 = Sf * Sf;
+  const int jc = m.j >> 1;
+  const bool jodd = (m.j & 1) != 0;
+  for (int i = m.i0; i < m.i1; ++i) {
+    const int ic = i >> 1;
+    float ejk0;
+    {
+      const float a = lerp_k(e, ic * Sc2 + jc * Sc, m.k);
+      ejk0 = jodd ? 0.5f * (a + lerp_k(e, ic * Sc2 + (jc + 1) * Sc, m.k)) : a;
+    }
+    float corr;
+    if ((i & 1) == 0) {
+      corr = ejk0;
+    } else {
+      const float a = lerp_k(e, (ic + 1) * Sc2 + jc * Sc, m.k);
+      const float ejk1 = jodd ? 0.5f * (a + lerp_k(e, (ic + 1) * Sc2 + (jc + 1) * Sc, m.k)) : a;
+      corr = 0.5f * (ejk0 + ejk1);
+    }
+    const int id = i * S2 + m.j * Sf + m.k;
+    x[id] = (scale == 1.0f) ? x[id] + corr : x[id] + scale * corr;
+  }
+}
+
+// Exact coarsest-level solve of 6e - sum6 e = g (zero Dirichlet faces) in
+// FP64: e = S D^-1 S g with S the orthonormal type-I DST along each axis.
+// One workgroup; the m^3 block lives in LDS; each 1-D transform reads a line
+// into registers and writes it back in place.
+constexpr int kMaxM = kMaxCoarseInterior;
+
+__device__ void dst_lines(double* A, const double* Sm, int m, int axis) {
+  const int nlines = m * m;
+  const int stride = (axis == 2) ? 1 : (axis == 1 ? m : m * m);
+  for (int line = threadIdx.x; line < nlines; line += blockDim.x) {
+    const int a = line / m, b = line % m;
+    int base;
+    if (axis == 2) base = a * m * m + b * m;
+    else if (axis == 1) base = a * m * m + b;
+    else base = a * m + b;
+    double v[kMaxM];
+#pragma unroll
+    for (int q = 0; q < kMaxM; ++q) v[q] = (q < m) ? A[base + q * stride] : 0.0;
+    for (int p = 0; p < m; ++p) {
+      double s = 0.0;
+#pragma unroll
+      for (int q = 0; q < kMaxM; ++q)
+        if (q < m) s += Sm[p * m + q] * v[q];
+      A[base + p * stride] = s;
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_coarse_exact(int Sc, const float* __restrict__ g, float* __restrict__ e) {
+  extern __shared__ double lds[];
+  const int m = Sc - 2;
+  double* Sm = lds;             // m*m
+  double* lam = Sm + m * m;     // m
+  double* A = lam + m;          // m^3
+  const double pi = 3.14159265358979323846;
+  const double nrm = sqrt(2.0 / (double)(m + 1));
+  for (int t = threadIdx.x; t < m * m; t += blockDim.x) {
+    const int p = t / m, q = t % m;
+    Sm[t] = nrm * sin(pi * (double)((p + 1) * (q + 1)) / (double)(m + 1));
+  }
+  for (int t = threadIdx.x; t < m; t += blockDim.x) lam[t] = 2.0 - 2.0 * cos(pi * (double)(t + 1) / (double)(m + 1));
+  const int Sc2 = Sc * Sc;
+  const int m3 = m * m * m;
+  for (int t = threadIdx.x; t < m3; t += blockDim.x) {
+    const int a = t / (m * m), b = (t / m) % m, c = t % m;
+    A[t] = (double)g[(a + 1) * Sc2 + (b + 1) * Sc + (c + 1)];
+  }
+  __syncthreads();
+  dst_lines(A, Sm, m, 2);
+  __syncthreads();
+  dst_lines(A, Sm, m, 1);
+  __syncthreads();
+  dst_lines(A, Sm, m, 0);
+  __syncthreads();
+  for (int t = threadIdx.x; t < m3; t += blockDim.x) {
+    const int a = t / (m * m), b = (t / m) % m, c = t % m;
+    A[t] = A[t] / (lam[a] + lam[b] + lam[c]);
+  }
+  __syncthreads();
+  dst_lines(A, Sm, m, 2);
+  __syncthreads();
+  dst_lines(A, Sm, m, 1);
+  __syncthreads();
+  dst_lines(A, Sm, m, 0);
+  __syncthreads();
+  for (int t = threadIdx.x; t < m3; t += blockDim.x) {
+    const int a = t / (m * m), b = (t / m) % m, c = t % m;
+    e[(a + 1) * Sc2 + (b + 1) * Sc + (c + 1)] = (float)A[t];
+  }
+}
+
+__global__ void k_fill(float* __restrict__ p, size_t n, float v) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) p[t] = v;
+}
+
+__global__ void k_square(int S3, const float* __restrict__ U, float* __restrict__ rho) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < S3) {
+    const float u = U[t];
+    rho[t] = u * u;
+  }
+}
+
+__global__ void k_lab16to32(size_t n, const uint16_t* __restrict__ in, int32_t* __restrict__ out) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) out[t] = (int32_t)(in[t] & kLabelMask);
+}
+
+__global__ void k_lab32to16(size_t n, const int32_t* __restrict__ in, uint16_t* __restrict__ out) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) {
+    int v = in[t];
+    v = v < 0 ? 0 : (v > (int)kLabelMask ? (int)kLabelMask : v);
+    out[t] = (uint16_t)v;
+  }
+}
+
+// --------------------------------------------------------------------------
+// Per-domain reductions.  Each thread accumulates a run of equal labels in
+// FP64 registers, flushing to LDS bins (FP64 LDS atomics) when the label
+// changes; after a barrier each non-zero bin is added to the global result
+// with one FP64 atomic per workgroup.  Labels are clipped to [0, n_dom-1]
+// (cloud/solver.py:427 ``label.clip``).
+// --------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_norm_sums(int S, int ich, float h3, const float* __restrict__ U,
+                                                   const uint16_t* __restrict__ lab, int n_dom,
+                                                   double* __restrict__ norms) {
+  extern __shared__ double bins[];
+  const int nthr = blockDim.x * blockDim.y;
+  const int tid = threadIdx.y * blockDim.x + threadIdx.x;
+  for (int t = tid; t < n_dom; t += nthr) bins[t] = 0.0;
+  __syncthreads();
+  const int n = S - 2;
+  March m;
+  if (march_coords(n, ich, m)) {
+    const int S2 = S * S;
+    int id = m.i0 * S2 + m.j * S + m.k;
+    int cur = -1;
+    double acc = 0.0;
+    for (int i = m.i0; i < m.i1; ++i, id += S2) {
+      int L = lab[id] & kLabelMask;
+      L = L < n_dom ? L : n_dom - 1;
+      const float v = U[id];
+      const float term = (v * v) * h3;
+      if (L != cur) {
+        if (cur >= 0) atomicAdd(&bins[cur], acc);
+        cur = L;
+        acc = 0.0;
+      }
+      acc += (double)term;
+    }
+    if (cur >= 0) atomicAdd(&bins[cur], acc);
+  }
+  __syncthreads();
+  for (int t = tid; t < n_dom; t += nthr) {
+    const double b = bins[t];
+    if (b != 0.0) atomicAdd(&norms[t], b);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_normalize(int S, int ich, float* __restrict__ U,
+                                                   const uint16_t* __restrict__ lab,
+                                                   const double* __restrict__ norms, const float* __restrict__ Z,
+                                                   int n_dom) {
+  __shared__ float scale[kMaxDomains];
+  const int nthr = blockDim.x * blockDim.y;
+  const int tid = threadIdx.y * blockDim.x + threadIdx.x;
+  for (int t = tid; t < n_dom; t += nthr) {
+    const double z = (double)Z[t];
+    const double nv = norms[t];
+    scale[t] = (z > 0.0 && nv > 1e-20) ? (float)sqrt(z / nv) : 1.0f;
+  }
+  __syncthreads();
+  const int n = S - 2;
+  March m;
+  if (!march_coords(n, ich, m)) return;
+  const int S2 = S * S;
+  int id = m.i0 * S2 + m.j * S + m.k;
+  for (int i = m.i0; i < m.i1; ++i, id += S2) {
+    const int L = lab[id] & kLabelMask;
+    if (L < n_dom) U[id] = U[id] * scale[L];
+  }
+}
+
+// --------------------------------------------------------------------------
+// Boundary rules.
+// --------------------------------------------------------------------------
+// Nuclear-score flip (cloud/solver.py:92-147), reading labels from `lin`
+// and writing `lout` (double-buffered).
+__global__ __launch_bounds__(256) void k_boundary_score(int S, int ich, float h, float h2,
+                                                        const Atom* __restrict__ atoms, int n_atoms,
+                                                        const uint16_t* __restrict__ lin,
+                                                        uint16_t* __restrict__ lout, float ratio,
+                                                        int min_foreign) {
+  const int n = S - 2;
+  March m;
+  if (!march_coords(n, ich, m)) return;
+  const int S2 = S * S;
+  for (int i = m.i0; i < m.i1; ++i) {
+    const int id = i * S2 + m.j * S + m.k;
+    const uint16_t raw = lin[id];
+    const int cur = raw & kLabelMask;
+    int other = 0;
+    other += ((lin[id + S2] & kLabelMask) != cur);
+    other += ((lin[id - S2] & kLabelMask) != cur);
+    other += ((lin[id + S] & kLabelMask) != cur);
+    other += ((lin[id - S] & kLabelMask) != cur);
+    other += ((lin[id + 1] & kLabelMask) != cur);
+    other += ((lin[id - 1] & kLabelMask) != cur);
+    uint16_t out = raw;
+    if (other > 0) {
+      const float xi = (float)i * h, yj = (float)m.j * h, zk = (float)m.k * h;
+      float cur_score = 0.0f, best = 0.0f;
+      int bestN = cur;
+      for (int a = 0; a < n_atoms; ++a) {
+        const float Za = atoms[a].Z;
+        if (!(Za > 0.0f)) continue;
+        const float dx = xi - atoms[a].i * h;
+        const float dy = yj - atoms[a].j * h;
+        const float dz = zk - atoms[a].k * h;
+        const float r2 = dx * dx + dy * dy + dz * dz + h2;
+        const float sc = Za / sqrtf(r2);
+        if (a == cur) cur_score = sc;
+        if (sc > best) {
+          best = sc;
+          bestN = a;
+        }
+      }
+      if (bestN != cur && best > cur_score * ratio && other >= min_foreign) out = (uint16_t)bestN;
+    }
+    lout[id] = out;
+  }
+}
+
+// Level-set rule (molecule.js:426-540; angular-sector constraints omitted).
+__global__ __launch_bounds__(256) void k_levelset(int S, int ich, float h, const Atom* __restrict__ atoms,
+                                                  int n_atoms, const int32_t* __restrict__ lock_group,
+                                                  const uint16_t* __restrict__ lin, uint16_t* __restrict__ lout,
+                                                  const float* __restrict__ U, float* __restrict__ W, float dt_w,
+                                                  float curv_reg, int rc_flags) {
+  const int n = S - 2;
+  March m;
+  if (!march_coords(n, ich, m)) return;
+  const int S2 = S * S;
+  for (int i = m.i0; i < m.i1; ++i) {
+    const int id = i * S2 + m.j * S + m.k;
+    const uint16_t raw = lin[id];
+    const int myL = raw & kLabelMask;
+    const float uc = U[id];
+    const float myRho = uc * uc;
+    int bestL = myL;
+    float bestRho = 0.0f;
+    float cnt = 0.0f;
+    const int nb[6] = {id + S2, id - S2, id + S, id - S, id + 1, id - 1};
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      const int l = lin[nb[q]] & kLabelMask;
+      if (l == myL) {
+        cnt += 1.0f;
+      } else {
+        const float un = U[nb[q]];
+        const float r = un * un;
+        if (r > bestRho) {
+          bestRho = r;
+          bestL = l;
+        }
+      }
+    }
+    float w = W[id];
+    if (cnt >= 6.0f) {
+      W[id] = fmaxf(w, 1.0f);
+      lout[id] = raw;
+      continue;
+    }
+    if (lock_group) {
+      const int gm = myL < n_atoms ? lock_group[myL] : 0;
+      const int gb = bestL < n_atoms ? lock_group[bestL] : 0;
+      if (gm > 0 && gm == gb) {
+        W[id] = fmaxf(w, 0.5f);
+        lout[id] = raw;
+        continue;
+      }
+    }
+    const float denom = myRho + bestRho;
+    const float vel = (denom < 1e-20f) ? 0.0f : (myRho - bestRho) / denom;
+    w += dt_w * vel;
+    const float curv = (cnt - 3.0f) / 3.0f;
+    w += curv_reg * curv;
+    int newL = myL;
+    if (w < 0.0f) {
+      newL = bestL;
+      w = 0.1f;
+    }
+    W[id] = w;
+    uint16_t out = (uint16_t)newL;
+    if (newL == myL) {
+      out = raw;
+    } else if (rc_flags && newL < n_atoms) {
+      const float rc = atoms[newL].rc;
+      if (rc > 0.0f) {
+        const float dx = ((float)i - floorf(atoms[newL].i)) * h;
+        const float dy = ((float)m.j - floorf(atoms[newL].j)) * h;
+        const float dz = ((float)m.k - floorf(atoms[newL].k)) * h;
+        if (sqrtf(dx * dx + dy * dy + dz * dz) < rc) out |= kRcFlag;
+      }
+    }
+    lout[id] = out;
+  }
+}
+
+// --------------------------------------------------------------------------
+// Energy (cloud/solver.py:227-280; molecule.js:950-1075 for mode S2).
+// Per-domain T, V_eK, V_ee bins plus the three dipole sums.
+// --------------------------------------------------------------------------
+__device__ __forceinline__ bool excl_s2(int i, int j, int k, const Atom& a, float h, float r_ex) {
+  const float dx = ((float)i - floorf(a.i)) * h;
+  const float dy = ((float)j - floorf(a.j)) * h;
+  const float dz = ((float)k - floorf(a.k)) * h;
+  return sqrtf(dx * dx + dy * dy + dz * dz) < r_ex;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+__global__ __launch_bounds__(256) void k_energy(int mode, int S, int ich, float h, float h3, float r_sing,
+                                                const float* __restrict__ U, const float* __restrict__ K,
+                                                const float* __restrict__ P, const uint16_t* __restrict__ lab,
+                                                const Atom* __restrict__ atoms, int n_dom,
+                                                double* __restrict__ per_dom, double* __restrict__ dip) {
+  extern __shared__ double bins[];  // n_dom * 3, then 3 dipole slots
+  const int nthr = blockDim.x * blockDim.y;
+  const int tid = threadIdx.y * blockDim.x + threadIdx.x;
+  for (int t = tid; t < n_dom * 3 + 3; t += nthr) bins[t] = 0.0;
+  __syncthreads();
+  const int n = S - 2;
+  March m;
+  double dx_acc = 0.0, dy_acc = 0.0, dz_acc = 0.0;
+  if (march_coords(n, ich, m)) {
+    const int S2 = S * S;
+    int cur = -1;
+    double aT = 0.0, aK = 0.0, aE = 0.0;
+    const float yj = (float)m.j * h, zk = (float)m.k * h;
+    for (int i = m.i0; i < m.i1; ++i) {
+      const int id = i * S2 + m.j * S + m.k;
+      int L = lab[id] & kLabelMask;
+      L = L < n_dom ? L : n_dom - 1;
+      const float u = U[id];
+      const float rho = u * u;
+      float T, VeK, Vee;
+      if (mode == RQ_ENERGY_S2) {
+        const Atom a = atoms[L];
+        const float r_ex = fmaxf(a.rc, r_sing);
+        if (excl_s2(i, m.j, m.k, a, h, r_ex)) continue;
+        const bool sip = ((lab[id + S2] & kLabelMask) == L) && !excl_s2(i + 1, m.j, m.k, a, h, r_ex);
+        const bool sjp = ((lab[id + S] & kLabelMask) == L) && !excl_s2(i, m.j + 1, m.k, a, h, r_ex);
+        const bool skp = ((lab[id + 1] & kLabelMask) == L) && !excl_s2(i, m.j, m.k + 1, a, h, r_ex);
+        const float da = sip ? U[id + S2] - u : 0.0f;
+        const float db = sjp ? U[id + S] - u : 0.0f;
+        const float dc = skp ? U[id + 1] - u : 0.0f;
+        T = 0.5f * (da * da + db * db + dc * dc) * h;
+      } else {
+        const float da = ((lab[id + S2] & kLabelMask) == L) ? U[id + S2] - u : 0.0f;
+        const float db = ((lab[id + S] & kLabelMask) == L) ? U[id + S] - u : 0.0f;
+        const float dc = ((lab[id + 1] & kLabelMask) == L) ? U[id + 1] - u : 0.0f;
+        T = 0.5f * (da * da + db * db + dc * dc) * h;
+      }
+      VeK = -K[id] * rho * h3;
+      Vee = P[id] * rho * h3;
+      const float xi = (float)i * h;
+      dx_acc += (double)(rho * xi * h3);
+      dy_acc += (double)(rho * yj * h3);
+      dz_acc += (double)(rho * zk * h3);
+      if (L != cur) {
+        if (cur >= 0) {
+          atomicAdd(&bins[cur * 3 + 0], aT);
+          atomicAdd(&bins[cur * 3 + 1], aK);
+          atomicAdd(&bins[cur * 3 + 2], aE);
+        }
+        cur = L;
+        aT = aK = aE = 0.0;
+      }
+      aT += (double)T;
+      aK += (double)VeK;
+      aE += (double)Vee;
+    }
+    if (cur >= 0) {
+      atomicAdd(&bins[cur * 3 + 0], aT);
+      atomicAdd(&bins[cur * 3 + 1], aK);
+      atomicAdd(&bins[cur * 3 + 2], aE);
+    }
+  }
+  dx_acc = wave_sum(dx_acc);
+  dy_acc = wave_sum(dy_acc);
+  dz_acc = wave_sum(dz_acc);
+  if ((tid & 63) == 0) {
+    atomicAdd(&bins[n_dom * 3 + 0], dx_acc);
+    atomicAdd(&bins[n_dom * 3 + 1], dy_acc);
+    atomicAdd(&bins[n_dom * 3 + 2], dz_acc);
+  }
+  __syncthreads();
+  for (int t = tid; t < n_dom * 3; t += nthr) {
+    const double b = bins[t];
+    if (b != 0.0) atomicAdd(&per_dom[t], b);
+  }
+  if (tid < 3) atomicAdd(&dip[tid], bins[n_dom * 3 + tid]);
+}
+
+// --------------------------------------------------------------------------
+// Hellmann-Feynman force (cloud/solver.py:282-315).  Each thread holds a
+// column of `kCol` densities in registers and loops the atoms in chunks of
+// four; per-atom sums are FP64, reduced across the wave with shuffles and
+// across the workgroup through LDS, one FP64 atomic per atom component per
+// workgroup.
+// --------------------------------------------------------------------------
+constexpr int kCol = 16;
+constexpr int kChunk = 4;
+
+__global__ __launch_bounds__(256) void k_forces_hf(int S, float h, float h2, float h3, const float* __restrict__ U,
+                                                   const Atom* __restrict__ atoms, int first, int count,
+                                                   double* __restrict__ out) {
+  __shared__ double red[4][kChunk * 3];
+  const int n = S - 2;
+  const int tid = threadIdx.y * blockDim.x + threadIdx.x;
+  const int k = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
+  const int i0 = 1 + blockIdx.z * kCol;
+  const bool active = (k <= n && j <= n);
+  const int S2 = S * S;
+  float rho[kCol];
+#pragma unroll
+  for (int c = 0; c < kCol; ++c) {
+    const int i = i0 + c;
+    float r = 0.0f;
+    if (active && i <= n) {
+      const float u = U[i * S2 + j * S + k];
+      r = u * u;
+    }
+    rho[c] = (r < 1e-20f) ? 0.0f : r;  // skipped voxels contribute nothing
+  }
+  const float yj = (float)j * h, zk = (float)k * h;
+  for (int a0 = 0; a0 < count; a0 += kChunk) {
+    double acc[kChunk * 3];
+#pragma unroll
+    for (int q = 0; q < kChunk * 3; ++q) acc[q] = 0.0;
+#pragma unroll
+    for (int q = 0; q < kChunk; ++q) {
+      const int a = a0 + q;
+      if (a >= count) break;
+      const Atom at = atoms[first + a];
+      if (!(at.Z > 0.0f)) continue;
+      const float dy = yj - at.j * h;
+      const float dz = zk - at.k * h;
+#pragma unroll
+      for (int c = 0; c < kCol; ++c) {
+        if (rho[c] == 0.0f) continue;
+        const float xi = (float)(i0 + c) * h;
+        const float dx = xi - at.i * h;
+        const float r2 = dx * dx + dy * dy + dz * dz + h2;
+        const float inv_r3 = 1.0f / (r2 * sqrtf(r2));
+        const float w = at.Z * rho[c] * inv_r3 * h3;
+        acc[q * 3 + 0] += (double)(w * dx);
+        acc[q * 3 + 1] += (double)(w * dy);
+        acc[q * 3 + 2] += (double)(w * dz);
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < kChunk * 3; ++q) acc[q] = wave_sum(acc[q]);
+    const int wave = tid >> 6;
+    if ((tid & 63) == 0) {
+#pragma unroll
+      for (int q = 0; q < kChunk * 3; ++q) red[wave][q] = acc[q];
+    }
+    __syncthreads();
+    if (tid < kChunk * 3) {
+      const int a = a0 + tid / 3;
+      if (a < count) {
+        const double s = red[0][tid] + red[1][tid] + red[2][tid] + red[3][tid];
+        atomicAdd(&out[a * 3 + tid % 3], s);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// molecule.js:1199-1270: F = 2 Z <grad P> averaged over the ball of `radius`
+// cells around the (integer) nucleus position; central differences.  One
+// workgroup per atom; FP64 sums.
+__global__ __launch_bounds__(256) void k_forces_gradp(int S, float h, const float* __restrict__ P,
+                                                      const Atom* __restrict__ atoms, int n_atoms, int R,
+                                                      double* __restrict__ out) {
+  __shared__ double red[4][4];
+  const int a = blockIdx.x;
+  if (a >= n_atoms) return;
+  const Atom at = atoms[a];
+  const int tid = threadIdx.x;
+  if (!(at.Z > 0.0f)) {
+    if (tid < 3) out[a * 3 + tid] = 0.0;
+    return;
+  }
+  const int ci = (int)floorf(at.i), cj = (int)floorf(at.j), ck = (int)floorf(at.k);
+  const float inv2h = 0.5f * (1.0f / h);
+  const int D = 2 * R + 1;
+  const int total = D * D * D;
+  const int S2 = S * S;
+  double sx = 0.0, sy = 0.0, sz = 0.0, cnt = 0.0;
+  for (int t = tid; t < total; t += blockDim.x) {
+    const int di = t / (D * D) - R, dj = (t / D) % D - R, dk = t % D - R;
+    const int ii = ci + di, jj = cj + dj, kk = ck + dk;
+    if (ii < 1 || ii >= S - 1 || jj < 1 || jj >= S - 1 || kk < 1 || kk >= S - 1) continue;
+    if (di * di + dj * dj + dk * dk > R * R) continue;
+    const int id = ii * S2 + jj * S + kk;
+    sx += (double)((P[id + S2] - P[id - S2]) * inv2h);
+    sy += (double)((P[id + S] - P[id - S]) * inv2h);
+    sz += (double)((P[id + 1] - P[id - 1]) * inv2h);
+    cnt += 1.0;
+  }
+  sx = wave_sum(sx);
+  sy = wave_sum(sy);
+  sz = wave_sum(sz);
+  cnt = wave_sum(cnt);
+  if ((tid & 63) == 0) {
+    red[tid >> 6][0] = sx;
+    red[tid >> 6][1] = sy;
+    red[tid >> 6][2] = sz;
+    red[tid >> 6][3] = cnt;
+  }
+  __syncthreads();
+  if (tid < 3) {
+    const int nw = (blockDim.x + 63) / 64;
+    double s = 0.0, c = 0.0;
+    for (int w = 0; w < nw; ++w) {
+      s += red[w][tid];
+      c += red[w][3];
+    }
+    if (c > 0.0) s /= c;
+    out[a * 3 + tid] = 2.0 * (double)at.Z * s;
+  }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Launchers
+// ---------------------------------------------------------------------------
+static inline unsigned nblk(size_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+void launch_init_trial(hipStream_t st, int mode, int S, float h, float h2, const Atom* atoms, int n_atoms,
+                       float* U, float* U2, uint16_t* lab, float* K, float* bestU, float* W) {
+  const size_t S3 = (size_t)S * S * S;
+  const float r_sing = 2.0f * h;
+  hipLaunchKernelGGL(k_init_trial, dim3(nblk(S3, 256)), dim3(256), 0, st, mode, S, h, h2, r_sing, atoms, n_atoms, U,
+                     U2, lab, K, bestU, W);
+}
+
+void launch_recompute_k(hipStream_t st, int mode, int S, float h, float h2, const Atom* atoms, int n_atoms,
+                        float* K) {
+  const size_t S3 = (size_t)S * S * S;
+  hipLaunchKernelGGL(k_recompute_k, dim3(nblk(S3, 256)), dim3(256), 0, st, mode, S, h, h2, 2.0f * h, atoms, n_atoms,
+                     K);
+}
+
+void launch_rc_flags(hipStream_t st, int S, float h, const Atom* atoms, int n_atoms, uint16_t* lab) {
+  const size_t S3 = (size_t)S * S * S;
+  hipLaunchKernelGGL(k_rc_flags, dim3(nblk(S3, 256)), dim3(256), 0, st, S, h, atoms, n_atoms, lab);
+}
+
+void launch_itp(hipStream_t st, const Tile& t, int S, const float* U, float* Un, const float* K, const float* P,
+                const uint16_t* lab, float half_dv, float dtv) {
+  hipLaunchKernelGGL(k_itp, stencil_grid(S - 2, t), dim3(t.bx, t.by), 0, st, S, t.ich, U, Un, K, P, lab, half_dv,
+                     dtv);
+}
+
+void launch_smooth_fine(hipStream_t st, const Tile& t, int S, const float* P, float* Pn, const float* U, float a,
+                        float c, float alpha, float beta) {
+  hipLaunchKernelGGL(k_smooth<true>, stencil_grid(S - 2, t), dim3(t.bx, t.by), 0, st, S, t.ich, P, Pn, U, a, c,
+                     alpha, beta);
+}
+
+void launch_smooth_coarse(hipStream_t st, const Tile& t, int S, const float* E, float* En, const float* g, float a,
+                          float c) {
+  hipLaunchKernelGGL(k_smooth<false>, stencil_grid(S - 2, t), dim3(t.bx, t.by), 0, st, S, t.ich, E, En, g, a, c,
+                     1.0f, 1.0f);
+}
+
+void launch_residual_fine(hipStream_t st, const Tile& t, int S, const float* P, const float* U, float alpha,
+                          float beta, float* r) {
+  hipLaunchKernelGGL(k_residual<true>, stencil_grid(S - 2, t), dim3(t.bx, t.by), 0, st, S, t.ich, P, U, alpha, beta,
+                     r);
+}
+
+void launch_residual_coarse(hipStream_t st, const Tile& t, int S, const float* E, const float* g, float* r) {
+  hipLaunchKernelGGL(k_residual<false>, stencil_grid(S - 2, t), dim3(t.bx, t.by), 0, st, S, t.ich, E, g, 1.0f, 1.0f,
+                     r);
+}
+
+void launch_restrict4(hipStream_t st, int Sf, const float* r, float* gc, float sign) {
+  const int nc = (Sf - 1) / 2 - 1;
+  const size_t tot = (size_t)nc * nc * nc;
+  hipLaunchKernelGGL(k_restrict4, dim3(nblk(tot, 256)), dim3(256), 0, st, Sf, r, gc, sign);
+}
+
+void launch_prolong_correct(hipStream_t st, const Tile& t, int Sf, float* x, const float* ec, float scale) {
+  hipLaunchKernelGGL(k_prolong_correct, stencil_grid(Sf - 2, t), dim3(t.bx, t.by), 0, st, Sf, t.ich, x, ec, scale);
+}
+
+void launch_coarse_exact(hipStream_t st, int Sc, const float* g, float* e) {
+  const int m = Sc - 2;
+  const size_t lds = sizeof(double) * ((size_t)m * m + m + (size_t)m * m * m);
+  static bool attr_set = false;
+  if (!attr_set) {
+    (void)hipFuncSetAttribute((const void*)k_coarse_exact, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    attr_set = true;
+  }
+  hipLaunchKernelGGL(k_coarse_exact, dim3(1), dim3(256), lds, st, Sc, g, e);
+}
+
+void launch_zero(hipStream_t st, float* p, size_t n) { launch_fill(st, p, n, 0.0f); }
+
+void launch_fill(hipStream_t st, float* p, size_t n, float v) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_fill, dim3(nblk(n, 256)), dim3(256), 0, st, p, n, v);
+}
+
+void launch_norm_sums(hipStream_t st, const Tile& t, int S, float h3, const float* U, const uint16_t* lab, int n_dom,
+                      double* norms) {
+  hipLaunchKernelGGL(k_norm_sums, stencil_grid(S - 2, t), dim3(t.bx, t.by), sizeof(double) * n_dom, st, S, t.ich, h3,
+                     U, lab, n_dom, norms);
+}
+
+void launch_normalize(hipStream_t st, const Tile& t, int S, float* U, const uint16_t* lab, const double* norms,
+                      const float* Z, int n_dom) {
+  hipLaunchKernelGGL(k_normalize, stencil_grid(S - 2, t), dim3(t.bx, t.by), 0, st, S, t.ich, U, lab, norms, Z, n_dom);
+}
+
+void launch_boundary_score(hipStream_t st, const Tile& t, int S, float h, float h2, const Atom* atoms, int n_atoms,
+                           const uint16_t* lin, uint16_t* lout, float ratio, int min_foreign) {
+  hipLaunchKernelGGL(k_boundary_score, stencil_grid(S - 2, t), dim3(t.bx, t.by), 0, st, S, t.ich, h, h2, atoms,
+                     n_atoms, lin, lout, ratio, min_foreign);
+}
+
+void launch_levelset(hipStream_t st, const Tile& t, int S, float h, const Atom* atoms, int n_atoms,
+                     const int32_t* lock_group, const uint16_t* lin, uint16_t* lout, const float* U, float* W,
+                     float dt_w, float curv_reg, int rc_flags) {
+  hipLaunchKernelGGL(k_levelset, stencil_grid(S - 2, t), dim3(t.bx, t.by), 0, st, S, t.ich, h, atoms, n_atoms,
+                     lock_group, lin, lout, U, W, dt_w, curv_reg, rc_flags);
+}
+
+void launch_energy(hipStream_t st, const Tile& t, int mode, int S, float h, float h3, const float* U, const float* K,
+                   const float* P, const uint16_t* lab, const Atom* atoms, int n_dom, double* per_dom, double* dip) {
+  const size_t lds = sizeof(double) * ((size_t)n_dom * 3 + 3);
+  static bool attr_set = false;
+  if (!attr_set) {
+    (void)hipFuncSetAttribute((const void*)k_energy, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    attr_set = true;
+  }
+  hipLaunchKernelGGL(k_energy, stencil_grid(S - 2, t), dim3(t.bx, t.by), lds, st, mode, S, t.ich, h, h3, 2.0f * h, U,
+                     K, P, lab, atoms, n_dom, per_dom, dip);
+}
+
+void launch_forces_hf(hipStream_t st, int S, float h, float h2, float h3, const float* U, const Atom* atoms,
+                      int first, int count, double* out3) {
+  const int n = S - 2;
+  dim3 grid((n + 63) / 64, (n + 3) / 4, (n + kCol - 1) / kCol);
+  hipLaunchKernelGGL(k_forces_hf, grid, dim3(64, 4), 0, st, S, h, h2, h3, U, atoms, first, count, out3);
+}
+
+void launch_forces_gradp(hipStream_t st, int S, float h, const float* P, const Atom* atoms, int n_atoms, int radius,
+                         double* out3n) {
+  hipLaunchKernelGGL(k_forces_gradp, dim3(n_atoms), dim3(256), 0, st, S, h, P, atoms, n_atoms, radius, out3n);
+}
+
+void launch_square(hipStream_t st, int S, const float* U, float* rho) {
+  const int S3 = S * S * S;
+  hipLaunchKernelGGL(k_square, dim3(nblk(S3, 256)), dim3(256), 0, st, S3, U, rho);
+}
+
+void launch_label_u16_to_i32(hipStream_t st, size_t n, const uint16_t* in, int32_t* out) {
+  hipLaunchKernelGGL(k_lab16to32, dim3(nblk(n, 256)), dim3(256), 0, st, n, in, out);
+}
+
+void launch_label_i32_to_u16(hipStream_t st, size_t n, const int32_t* in, uint16_t* out) {
+  hipLaunchKernelGGL(k_lab32to16, dim3(nblk(n, 256)), dim3(256), 0, st, n, in, out);
+}
+
+}  // namespace rq
