@@ -98,9 +98,12 @@ typedef struct rq_atom {
 } rq_atom;
 
 typedef struct rq_params {
+  /* screen and dv are double: the reference forms h, h^2, h^3 and dv*h^2 in
+   * double precision on the host and rounds each to FP32 once
+   * (cloud/solver.py:326-328, 352-353, 408). */
+  double  screen;         /* box edge in bohr */
+  double  dv;             /* ITP step in h^2 units; <= 0 selects cloud/solver.py:341-351 */
   int32_t NN;             /* cells per axis (S = NN+1 nodes); even; 8 <= NN <= 1024 */
-  float   screen;         /* box edge in bohr */
-  float   dv;             /* ITP step in h^2 units; <= 0 selects cloud/solver.py:341-351 */
   int32_t init_mode;      /* RQ_INIT_* */
   int32_t poisson_mode;   /* RQ_POISSON_* */
   int32_t jacobi_sweeps;  /* sweeps per Poisson step (JACOBI/TWOGRID), default 2 */
@@ -178,6 +181,29 @@ int rq_write_field(rq_solver* h, int32_t field, const void* host, size_t bytes);
 int rq_swap_u(rq_solver* h);                     /* exchange U and U2 (cloud/benchmark.py:104) */
 int rq_field_device_ptr(rq_solver* h, int32_t field, void** dptr);
 int rq_stream(rq_solver* h, void** hip_stream);
+
+/* --- differentiation: frozen-label fixed-step trajectory --- */
+/* The differentiated map: K(R) and the trial U0 = Z_L^2/sqrt(pi) exp(-Z_L r~)
+ * of each voxel's CURRENT label L (labels are frozen: call rq_init_trial at
+ * the reference geometry first), per-domain normalise, `bootstrap` Poisson
+ * steps from P = 0, then steps 1..n_steps of the run() loop body without the
+ * boundary rule; E = T + V_eK + V_ee of the final state.  Positions are the
+ * handle's atoms.  Replaces the per-molecule finite-difference force path
+ * (6N+1 solves) built on RealQMSolver.run (cloud/solver.py:550-634). */
+typedef struct rq_adjoint_spec {
+  int64_t     n_steps;
+  int32_t     bootstrap;           /* Poisson steps after the initial normalise */
+  int32_t     checkpoint_interval; /* steps per stored (U, P) checkpoint; <= 0: 64 */
+  rq_schedule schedule;            /* boundary_interval is ignored (frozen labels) */
+  int32_t     reserved[4];         /* must be zero */
+} rq_adjoint_spec;
+
+/* Forward solve only; energy3 = (T, V_eK, V_ee). */
+int rq_solve_frozen(rq_solver* h, const rq_adjoint_spec* spec, double* energy3);
+/* Forward solve + reverse sweep: energy3 as above and dEdR (n_atoms x 3,
+ * Ha/bohr) of the electronic energy; the nucleus-nucleus term is the caller's.
+ * Multigrid mode requires mg_nu1 == mg_nu2 (symmetric V-cycle). */
+int rq_adjoint_gradient(rq_solver* h, const rq_adjoint_spec* spec, double* energy3, double* dEdR);
 
 /* --- timing (HIP events on the handle's stream) --- */
 /* Per-kernel-family accumulated device time in ms since the last enable, and

@@ -44,7 +44,7 @@ class RqAtom(C.Structure):
 
 class RqParams(C.Structure):
     _fields_ = [
-        ("NN", C.c_int32), ("screen", C.c_float), ("dv", C.c_float), ("init_mode", C.c_int32),
+        ("screen", C.c_double), ("dv", C.c_double), ("NN", C.c_int32), ("init_mode", C.c_int32),
         ("poisson_mode", C.c_int32), ("jacobi_sweeps", C.c_int32), ("smooth_a", C.c_float),
         ("smooth_c", C.c_float), ("mg_levels", C.c_int32), ("mg_nu1", C.c_int32), ("mg_nu2", C.c_int32),
         ("mg_a", C.c_float), ("mg_c", C.c_float), ("twogrid_interval", C.c_int32),
@@ -56,6 +56,11 @@ class RqParams(C.Structure):
 
 class RqSchedule(C.Structure):
     _fields_ = [("poisson_interval", C.c_int32), ("norm_interval", C.c_int32), ("boundary_interval", C.c_int32)]
+
+
+class RqAdjointSpec(C.Structure):
+    _fields_ = [("n_steps", C.c_int64), ("bootstrap", C.c_int32), ("checkpoint_interval", C.c_int32),
+                ("schedule", RqSchedule), ("reserved", C.c_int32 * 4)]
 
 
 _P = C.c_void_p
@@ -93,6 +98,8 @@ PROTOTYPES = {
     "rq_swap_u": (C.c_int, [_P]),
     "rq_field_device_ptr": (C.c_int, [_P, _I, C.POINTER(_P)]),
     "rq_stream": (C.c_int, [_P, C.POINTER(_P)]),
+    "rq_solve_frozen": (C.c_int, [_P, C.POINTER(RqAdjointSpec), _D]),
+    "rq_adjoint_gradient": (C.c_int, [_P, C.POINTER(RqAdjointSpec), _D, _D]),
     "rq_timing_enable": (C.c_int, [_P, _I]),
     "rq_timing_read": (C.c_int, [_P, _D, C.POINTER(C.c_int64), _I]),
 }

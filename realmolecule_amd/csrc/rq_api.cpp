@@ -24,6 +24,15 @@ struct EventPair {
   int fam;
   hipEvent_t a, b;
 };
+
+// Device buffers of the reverse sweep (allocated on first use, grown as needed).
+struct AdjState {
+  float *lamU = nullptr, *lamU2 = nullptr, *lamP = nullptr, *lamP2 = nullptr, *gK = nullptr, *acc = nullptr;
+  float *t1 = nullptr, *t2 = nullptr, *t3 = nullptr;
+  std::vector<float*> ckU, ckP;    // checkpoints (U after the step, P in use)
+  std::vector<float*> segU, segP;  // one segment of recomputed inputs
+  double* red = nullptr;           // 2 * n_dom (norms, dots) + 3 * n_atoms (dE/dR) + 3 * n_dom + 3 (energy)
+};
 }  // namespace
 
 struct rq_solver {
@@ -54,6 +63,7 @@ struct rq_solver {
   std::vector<hipEvent_t> ev_free;
   double fam_ms[F_COUNT] = {0};
   int64_t fam_n[F_COUNT] = {0};
+  AdjState adj;
   std::string err;
 };
 
@@ -353,7 +363,281 @@ void free_all(rq_solver* s) {
     (void)hipEventDestroy(ep.b);
   }
   for (auto e : s->ev_free) (void)hipEventDestroy(e);
+  AdjState& A = s->adj;
+  for (float* p : {A.lamU, A.lamU2, A.lamP, A.lamP2, A.gK, A.acc, A.t1, A.t2, A.t3}) fr(p);
+  for (float* p : A.ckU) fr(p);
+  for (float* p : A.ckP) fr(p);
+  for (float* p : A.segU) fr(p);
+  for (float* p : A.segP) fr(p);
+  fr(A.red);
   if (s->st) (void)hipStreamDestroy(s->st);
+}
+
+// ---------------------------------------------------------------------------
+// Frozen-label trajectory and its adjoint (rq_solve_frozen / rq_adjoint_gradient)
+// ---------------------------------------------------------------------------
+int adj_alloc(rq_solver* s, float** p) {
+  if (*p) return RQ_OK;
+  if (hipMalloc((void**)p, s->S3 * sizeof(float)) != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(s, RQ_ENOMEM, "adjoint: device allocation of a %zu-node field failed", s->S3);
+  }
+  HIPCHK(s, hipMemsetAsync(*p, 0, s->S3 * sizeof(float), s->st));
+  return RQ_OK;
+}
+
+int adj_alloc_vec(rq_solver* s, std::vector<float*>& v, size_t n) {
+  while (v.size() < n) {
+    float* p = nullptr;
+    int rc = adj_alloc(s, &p);
+    if (rc) return rc;
+    v.push_back(p);
+  }
+  return RQ_OK;
+}
+
+int copy_field(rq_solver* s, float* dst, const float* src) {
+  HIPCHK(s, hipMemcpyAsync(dst, src, s->S3 * sizeof(float), hipMemcpyDeviceToDevice, s->st));
+  return RQ_OK;
+}
+
+// the buffer of {t1, t2} that is not `busy`
+float* other_of(float* busy, float* t1, float* t2) { return busy == t1 ? t2 : t1; }
+
+// One Poisson step P_out = Poisson(P_in, U) with explicit buffers; t1, t2 are
+// scratch distinct from P_in, P_out and U.
+int poisson_into(rq_solver* s, const float* P_in, float* P_out, const float* U, float* t1, float* t2) {
+  const Tile& T = s->tile;
+  if (s->p.poisson_mode == RQ_POISSON_MULTIGRID) {
+    float* x;
+    const float a = s->p.mg_a, c = s->p.mg_c;
+    if (s->p.mg_nu1 == 0) {
+      if (int rc = copy_field(s, t1, P_in)) return rc;
+      x = t1;
+    } else {
+      const float* src = P_in;
+      float* dst = t1;
+      for (int it = 0; it < s->p.mg_nu1; ++it) {
+        launch_smooth_fine(s->st, T, s->S, src, dst, U, a, c, s->two_pi, s->h2);
+        src = dst;
+        dst = other_of(dst, t1, t2);
+      }
+      x = const_cast<float*>(src);
+    }
+    launch_residual_fine(s->st, T, s->S, x, U, s->two_pi, s->h2, P_out);
+    launch_restrict4(s->st, s->S, P_out, s->g[1], 1.0f);
+    LAUNCHCHK(s);
+    if (int rc = vcycle_coarse(s, 1)) return rc;
+    launch_prolong_correct(s->st, T, s->S, x, s->e0[1], 1.0f);
+    if (s->p.mg_nu2 == 0) return copy_field(s, P_out, x);
+    float* src = x;
+    for (int it = 0; it < s->p.mg_nu2; ++it) {
+      float* dst = (it == s->p.mg_nu2 - 1) ? P_out : other_of(src, t1, t2);
+      launch_smooth_fine(s->st, T, s->S, src, dst, U, a, c, s->two_pi, s->h2);
+      src = dst;
+    }
+    LAUNCHCHK(s);
+    return RQ_OK;
+  }
+  const int n = s->p.jacobi_sweeps;
+  if (n == 0) return copy_field(s, P_out, P_in);
+  const float* src = P_in;
+  for (int it = 0; it < n; ++it) {
+    float* dst = (it == n - 1) ? P_out : (src == t1 ? t2 : t1);
+    launch_smooth_fine(s->st, T, s->S, src, dst, U, s->p.smooth_a, s->p.smooth_c, s->two_pi, s->h2);
+    src = dst;
+  }
+  LAUNCHCHK(s);
+  return RQ_OK;
+}
+
+int normalize_on(rq_solver* s, float* U) {
+  zero_doubles(s, s->red, s->n_atoms);
+  launch_norm_sums(s->st, s->tile, s->S, s->h3, U, s->lab, s->n_atoms, s->red);
+  launch_normalize(s->st, s->tile, s->S, U, s->lab, s->red, s->Z_d, s->n_atoms);
+  LAUNCHCHK(s);
+  return RQ_OK;
+}
+
+// Adjoint of one Poisson step: lamP (w.r.t. P_out) -> w.r.t. P_in, in place
+// via lamP2; lamU += (rhs adjoint) * 2 alpha beta U.
+int poisson_adj(rq_solver* s, const float* U) {
+  AdjState& A = s->adj;
+  const Tile& T = s->tile;
+  const float ab2 = 2.0f * s->two_pi * s->h2;
+  if (s->p.poisson_mode == RQ_POISSON_MULTIGRID) {
+    const float a = s->p.mg_a, c = s->p.mg_c;
+    launch_fill(s->st, A.acc, s->S3, 0.0f);
+    for (int it = 0; it < s->p.mg_nu2; ++it) {
+      launch_smooth_adj(s->st, T, s->S, A.lamP, A.lamP2, nullptr, A.acc, a, c, 1.0f / c);
+      std::swap(A.lamP, A.lamP2);
+    }
+    launch_restrict4(s->st, s->S, A.lamP, s->g[1], 1.0f);
+    LAUNCHCHK(s);
+    if (int rc = vcycle_coarse(s, 1)) return rc;
+    launch_fill(s->st, A.t3, s->S3, 0.0f);
+    launch_prolong_correct(s->st, T, s->S, A.t3, s->e0[1], 1.0f);
+    launch_correct_adj(s->st, T, s->S, A.lamP, A.t3, A.lamP2, A.acc);
+    std::swap(A.lamP, A.lamP2);
+    for (int it = 0; it < s->p.mg_nu1; ++it) {
+      launch_smooth_adj(s->st, T, s->S, A.lamP, A.lamP2, nullptr, A.acc, a, c, 1.0f / c);
+      std::swap(A.lamP, A.lamP2);
+    }
+    launch_axpy_src(s->st, T, s->S, U, A.acc, A.lamU, ab2);
+    LAUNCHCHK(s);
+    return RQ_OK;
+  }
+  const float a = s->p.smooth_a, c = s->p.smooth_c;
+  for (int it = 0; it < s->p.jacobi_sweeps; ++it) {
+    launch_smooth_adj(s->st, T, s->S, A.lamP, A.lamP2, U, A.lamU, a, c, ab2 / c);
+    std::swap(A.lamP, A.lamP2);
+  }
+  LAUNCHCHK(s);
+  return RQ_OK;
+}
+
+// Adjoint of normalise (on lamU in place) given the pre-normalise field Upre.
+int normalize_adj(rq_solver* s, const float* Upre) {
+  AdjState& A = s->adj;
+  double* norms = A.red;
+  double* dots = A.red + s->n_atoms;
+  zero_doubles(s, A.red, 2 * (size_t)s->n_atoms);
+  launch_norm_sums(s->st, s->tile, s->S, s->h3, Upre, s->lab, s->n_atoms, norms);
+  launch_domain_dot(s->st, s->tile, s->S, 1.0f, Upre, A.lamU, s->lab, s->n_atoms, dots);
+  launch_normalize_adj(s->st, s->tile, s->S, s->h3, Upre, A.lamU, s->lab, norms, dots, s->Z_d, s->n_atoms);
+  LAUNCHCHK(s);
+  return RQ_OK;
+}
+
+int frozen_run(rq_solver* s, const rq_adjoint_spec* sp, bool backward, double* E3, double* dEdR) {
+  if (!sp) return fail(s, RQ_EINVAL, "null spec");
+  for (int r : sp->reserved)
+    if (r) return fail(s, RQ_EINVAL, "rq_adjoint_spec.reserved must be zero");
+  const rq_schedule& sc = sp->schedule;
+  if (sp->n_steps < 0 || sp->bootstrap < 0 || sc.poisson_interval < 1 || sc.norm_interval < 1)
+    return fail(s, RQ_EINVAL, "bad adjoint spec");
+  if (s->p.poisson_mode == RQ_POISSON_TWOGRID)
+    return fail(s, RQ_EINVAL, "adjoint supports the Jacobi and multigrid Poisson modes");
+  if (backward && s->p.poisson_mode == RQ_POISSON_MULTIGRID && s->p.mg_nu1 != s->p.mg_nu2)
+    return fail(s, RQ_EINVAL, "adjoint of the V-cycle needs mg_nu1 == mg_nu2");
+  if (s->p.rc_dirichlet || s->p.init_mode != RQ_INIT_S1 || s->p.energy_mode != RQ_ENERGY_S1)
+    return fail(s, RQ_EINVAL, "adjoint implements the cloud/solver.py semantics (init/energy S1, no rc rule)");
+  AdjState& A = s->adj;
+  const int64_t Tn = sp->n_steps;
+  const int C = sp->checkpoint_interval > 0 ? sp->checkpoint_interval : 64;
+  const int64_t nck = backward ? (Tn + C - 1) / C + 1 : 0;
+  int rc;
+  for (float** p : {&A.t1, &A.t2, &A.t3})
+    if ((rc = adj_alloc(s, p))) return rc;
+  if (!A.red) {
+    const size_t nr = 2 * (size_t)s->n_atoms + 3 * (size_t)s->n_atoms + 3 * (size_t)s->n_atoms + 3;
+    HIPCHK(s, hipMalloc((void**)&A.red, sizeof(double) * nr));
+  }
+  if (backward) {
+    for (float** p : {&A.lamU, &A.lamU2, &A.lamP, &A.lamP2, &A.gK, &A.acc})
+      if ((rc = adj_alloc(s, p))) return rc;
+    if ((rc = adj_alloc_vec(s, A.ckU, nck)) || (rc = adj_alloc_vec(s, A.ckP, nck))) return rc;
+    const size_t segn = (size_t)std::min<int64_t>(C, std::max<int64_t>(Tn, 1));
+    if ((rc = adj_alloc_vec(s, A.segU, segn + 1)) || (rc = adj_alloc_vec(s, A.segP, segn))) return rc;
+  }
+  const Tile& T = s->tile;
+  // ---- forward ----
+  launch_init_frozen(s->st, s->S, s->h, s->h2, s->atoms_d, s->n_atoms, s->lab, s->U, s->K);
+  LAUNCHCHK(s);
+  if ((rc = normalize_on(s, s->U))) return rc;
+  launch_fill(s->st, s->P, s->S3, 0.0f);
+  for (int b = 0; b < sp->bootstrap; ++b) {
+    if ((rc = poisson_into(s, s->P, s->P2, s->U, A.t1, A.t2))) return rc;
+    std::swap(s->P, s->P2);
+  }
+  if (backward) {
+    if ((rc = copy_field(s, A.ckU[0], s->U)) || (rc = copy_field(s, A.ckP[0], s->P))) return rc;
+  }
+  for (int64_t t = 1; t <= Tn; ++t) {
+    if (t % sc.poisson_interval == 0) {
+      if ((rc = poisson_into(s, s->P, s->P2, s->U, A.t1, A.t2))) return rc;
+      std::swap(s->P, s->P2);
+    }
+    launch_itp(s->st, T, s->S, s->U, s->U2, s->K, s->P, s->lab, s->half_dv, s->dtv);
+    LAUNCHCHK(s);
+    std::swap(s->U, s->U2);
+    if (t % sc.norm_interval == 0 && (rc = normalize_on(s, s->U))) return rc;
+    if (backward && t % C == 0 && t < Tn) {
+      if ((rc = copy_field(s, A.ckU[t / C], s->U)) || (rc = copy_field(s, A.ckP[t / C], s->P))) return rc;
+    }
+  }
+  // energy of the final state
+  {
+    double* per = A.red + 5 * (size_t)s->n_atoms;
+    double* dip = per + 3 * (size_t)s->n_atoms;
+    zero_doubles(s, per, 3 * (size_t)s->n_atoms + 3);
+    launch_energy(s->st, T, RQ_ENERGY_S1, s->S, s->h, s->h3, s->U, s->K, s->P, s->lab, s->atoms_d, s->n_atoms, per,
+                  dip);
+    LAUNCHCHK(s);
+    std::vector<double> hv(3 * (size_t)s->n_atoms);
+    HIPCHK(s, hipMemcpyAsync(hv.data(), per, sizeof(double) * hv.size(), hipMemcpyDeviceToHost, s->st));
+    HIPCHK(s, hipStreamSynchronize(s->st));
+    double e[3] = {0, 0, 0};
+    for (int d = 0; d < s->n_atoms; ++d)
+      for (int q = 0; q < 3; ++q) e[q] += hv[d * 3 + q];
+    if (E3)
+      for (int q = 0; q < 3; ++q) E3[q] = e[q];
+    if (!std::isfinite(e[0] + e[1] + e[2])) return fail(s, RQ_EUNSTABLE, "non-finite energy in the frozen solve");
+  }
+  if (!backward) return RQ_OK;
+  // ---- reverse sweep ----
+  launch_energy_grad(s->st, T, s->S, s->h, s->h3, s->U, s->K, s->P, s->lab, A.lamU, A.gK, A.lamP);
+  LAUNCHCHK(s);
+  std::vector<const float*> Pused(C);
+  const int64_t nseg = (Tn + C - 1) / C;
+  for (int64_t sg = nseg - 1; sg >= 0; --sg) {
+    const int64_t t0 = sg * C, t1 = std::min<int64_t>(Tn, t0 + C);
+    if ((rc = copy_field(s, A.segU[0], A.ckU[sg]))) return rc;
+    const float* Pprev = A.ckP[sg];
+    for (int64_t t = t0 + 1; t <= t1; ++t) {
+      const int idx = (int)(t - t0 - 1);
+      if (t % sc.poisson_interval == 0) {
+        if ((rc = poisson_into(s, Pprev, A.segP[idx], A.segU[idx], A.t1, A.t2))) return rc;
+        Pused[idx] = A.segP[idx];
+      } else {
+        Pused[idx] = Pprev;
+      }
+      Pprev = Pused[idx];
+      if (t < t1) {
+        launch_itp(s->st, T, s->S, A.segU[idx], A.segU[idx + 1], s->K, Pused[idx], s->lab, s->half_dv, s->dtv);
+        LAUNCHCHK(s);
+        if (t % sc.norm_interval == 0 && (rc = normalize_on(s, A.segU[idx + 1]))) return rc;
+      }
+    }
+    for (int64_t t = t1; t > t0; --t) {
+      const int idx = (int)(t - t0 - 1);
+      const float* Uin = A.segU[idx];
+      if (t % sc.norm_interval == 0) {
+        launch_itp(s->st, T, s->S, Uin, A.t1, s->K, Pused[idx], s->lab, s->half_dv, s->dtv);
+        LAUNCHCHK(s);
+        if ((rc = normalize_adj(s, A.t1))) return rc;
+      }
+      launch_itp_adj(s->st, T, s->S, A.lamU, A.lamU2, Uin, s->K, Pused[idx], s->lab, s->half_dv, s->dtv, A.gK,
+                     A.lamP);
+      LAUNCHCHK(s);
+      std::swap(A.lamU, A.lamU2);
+      if (t % sc.poisson_interval == 0 && (rc = poisson_adj(s, Uin))) return rc;
+    }
+  }
+  // bootstrap Poisson steps (P_in = 0 at the first one; its adjoint is dropped)
+  for (int b = 0; b < sp->bootstrap; ++b)
+    if ((rc = poisson_adj(s, A.ckU[0]))) return rc;
+  // initial normalise: recompute the trial field into t1
+  launch_init_frozen(s->st, s->S, s->h, s->h2, s->atoms_d, s->n_atoms, s->lab, A.t1, A.t2);
+  LAUNCHCHK(s);
+  if ((rc = normalize_adj(s, A.t1))) return rc;
+  double* gout = A.red + 2 * (size_t)s->n_atoms;
+  zero_doubles(s, gout, 3 * (size_t)s->n_atoms);
+  launch_pos_grad(s->st, s->S, s->h, s->h2, A.gK, A.lamU, s->lab, s->atoms_d, s->n_atoms, gout);
+  LAUNCHCHK(s);
+  if (dEdR) HIPCHK(s, hipMemcpyAsync(dEdR, gout, sizeof(double) * 3 * s->n_atoms, hipMemcpyDeviceToHost, s->st));
+  HIPCHK(s, hipStreamSynchronize(s->st));
+  return RQ_OK;
 }
 
 }  // namespace
@@ -375,8 +659,8 @@ double rq_dv_for_electrons(int32_t n) {
 void rq_default_params(rq_params* p) {
   std::memset(p, 0, sizeof *p);
   p->NN = 96;
-  p->screen = 9.6f;
-  p->dv = 0.0f;
+  p->screen = 9.6;
+  p->dv = 0.0;
   p->init_mode = RQ_INIT_S1;
   p->poisson_mode = RQ_POISSON_JACOBI;
   p->jacobi_sweeps = 2;
@@ -411,7 +695,7 @@ int rq_create(const rq_params* pp, const rq_atom* atoms, int32_t n_atoms, rq_sol
   for (int r : p.reserved)
     if (r != 0) return fail(nullptr, RQ_EINVAL, "rq_params.reserved must be zero");
   if (p.NN < 8 || p.NN > 1024 || (p.NN & 1)) return fail(nullptr, RQ_EINVAL, "NN=%d must be even in [8, 1024]", p.NN);
-  if (!(p.screen > 0.f)) return fail(nullptr, RQ_EINVAL, "screen must be > 0");
+  if (!(p.screen > 0.0)) return fail(nullptr, RQ_EINVAL, "screen must be > 0");
   if (n_atoms < 1 || n_atoms > kMaxDomains)
     return fail(nullptr, RQ_ERANGE, "n_atoms=%d outside [1, %d]", n_atoms, kMaxDomains);
   if (p.poisson_mode < 0 || p.poisson_mode > 2) return fail(nullptr, RQ_EINVAL, "bad poisson_mode");
@@ -442,7 +726,7 @@ int rq_create(const rq_params* pp, const rq_atom* atoms, int32_t n_atoms, rq_sol
   s->NN = p.NN;
   s->S = p.NN + 1;
   s->S3 = (size_t)s->S * s->S * s->S;
-  s->h64 = (double)p.screen / p.NN;
+  s->h64 = p.screen / p.NN;
   int n_elec = 0;
   s->atoms_h.resize(n_atoms);
   for (int a = 0; a < n_atoms; ++a) {
@@ -451,10 +735,10 @@ int rq_create(const rq_params* pp, const rq_atom* atoms, int32_t n_atoms, rq_sol
   }
   s->lock_h.assign(n_atoms, 0);
   s->n_atoms = n_atoms;
-  s->dv = p.dv > 0.f ? (double)p.dv : rq_dv_for_electrons(n_elec);
+  s->dv = p.dv > 0.0 ? p.dv : rq_dv_for_electrons(n_elec);
   s->h = (float)s->h64;
   s->h2 = (float)(s->h64 * s->h64);
-  s->h3 = (float)(s->h64 * s->h64 * s->h64);
+  s->h3 = (float)std::pow(s->h64, 3.0);  // Python h ** 3 (cloud/solver.py:328)
   s->half_dv = (float)(0.5 * s->dv);
   s->dtv = (float)(s->dv * (s->h64 * s->h64));
   s->two_pi = (float)(2.0 * M_PI);
@@ -758,6 +1042,9 @@ int rq_write_field(rq_solver* h, int32_t f, const void* host, size_t bytes) {
     float* p = field_ptr(h, f);
     if (!p) return fail(h, RQ_EINVAL, "unknown field %d", f);
     HIPCHK(h, hipMemcpyAsync(p, host, need, hipMemcpyHostToDevice, h->st));
+    // the ping-pong partner takes the same face values (kernels never write faces)
+    float* partner = (f == RQ_FIELD_U) ? h->U2 : (f == RQ_FIELD_P) ? h->P2 : nullptr;
+    if (partner) HIPCHK(h, hipMemcpyAsync(partner, p, need, hipMemcpyDeviceToDevice, h->st));
   }
   HIPCHK(h, hipStreamSynchronize(h->st));
   h->initialized = true;
@@ -786,6 +1073,16 @@ int rq_stream(rq_solver* h, void** s) {
   if (!h || !s) return fail(h, RQ_EINVAL, "null argument");
   *s = (void*)h->st;
   return RQ_OK;
+}
+
+int rq_solve_frozen(rq_solver* h, const rq_adjoint_spec* spec, double* energy3) {
+  if (!h) return fail(nullptr, RQ_EINVAL, "null handle");
+  return frozen_run(h, spec, false, energy3, nullptr);
+}
+
+int rq_adjoint_gradient(rq_solver* h, const rq_adjoint_spec* spec, double* energy3, double* dEdR) {
+  if (!h) return fail(nullptr, RQ_EINVAL, "null handle");
+  return frozen_run(h, spec, true, energy3, dEdR);
 }
 
 int rq_timing_enable(rq_solver* h, int32_t on) {

@@ -72,4 +72,24 @@ void launch_label_u16_to_i32(hipStream_t st, size_t n, const uint16_t* in, int32
 void launch_label_i32_to_u16(hipStream_t st, size_t n, const int32_t* in, uint16_t* out);
 void launch_fill(hipStream_t st, float* p, size_t n, float v);
 
+// ---- adjoint launchers (rq_adjoint.hip) ----
+void launch_init_frozen(hipStream_t st, int S, float h, float h2, const Atom* atoms, int n_atoms, const uint16_t* lab,
+                        float* U, float* K);
+void launch_energy_grad(hipStream_t st, const Tile& t, int S, float h, float h3, const float* U, const float* K,
+                        const float* P, const uint16_t* lab, float* lamU, float* gK, float* lamP);
+void launch_itp_adj(hipStream_t st, const Tile& t, int S, const float* lam, float* lam_out, const float* U,
+                    const float* K, const float* P, const uint16_t* lab, float half_dv, float dtv, float* gK,
+                    float* lamP);
+void launch_smooth_adj(hipStream_t st, const Tile& t, int S, const float* lam, float* lam_out, const float* src,
+                       float* acc, float a, float c, float coef);
+void launch_correct_adj(hipStream_t st, const Tile& t, int S, const float* lam, const float* z, float* lam_out,
+                        float* acc);
+void launch_axpy_src(hipStream_t st, const Tile& t, int S, const float* U, const float* acc, float* lamU, float coef);
+void launch_domain_dot(hipStream_t st, const Tile& t, int S, float w, const float* A, const float* B,
+                       const uint16_t* lab, int n_dom, double* out);
+void launch_normalize_adj(hipStream_t st, const Tile& t, int S, float h3, const float* Upre, float* lam,
+                          const uint16_t* lab, const double* norms, const double* dots, const float* Z, int n_dom);
+void launch_pos_grad(hipStream_t st, int S, float h, float h2, const float* gK, const float* lamU0,
+                     const uint16_t* lab, const Atom* atoms, int n_atoms, double* out);
+
 }  // namespace rq

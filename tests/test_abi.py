@@ -37,7 +37,7 @@ def test_default_params_and_struct_layout(lib):
     lib.rq_default_params(C.byref(p))
     assert (p.NN, p.jacobi_sweeps, p.smooth_a, p.smooth_c) == (96, 2, 0.0, 6.0)
     assert abs(p.flip_ratio - 1.1) < 1e-7 and p.flip_min_foreign == 2
-    assert C.sizeof(_lib.RqParams) == 22 * 4 + 8 * 4
+    assert C.sizeof(_lib.RqParams) == 2 * 8 + 20 * 4 + 8 * 4
     s = _lib.RqSchedule()
     lib.rq_default_schedule(C.byref(s))
     assert (s.poisson_interval, s.norm_interval, s.boundary_interval) == (2, 20, 100)

@@ -1,0 +1,85 @@
+"""GPU adjoint forces: hand-written reverse sweep (librq.so) against autograd of
+the FP64 torch oracle of the same frozen-label trajectory, against central
+finite differences of the GPU solve, checkpoint-interval invariance, and the
+torch.autograd.Function wrapper."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import adjoint as OA
+from oracle import s1
+from realmolecule_amd.adjoint import FrozenSolver, trajectory_energy
+from tests.helpers import three_atoms
+
+pytestmark = pytest.mark.gpu
+
+
+def _oracle_grad(fz, n_steps, **kw):
+    """Oracle dE/dR (per bohr, electronic + nuclear) at the solver's FP32 positions
+    with the solver's own frozen label map."""
+    rows = fz.rows.copy()
+    rows[:, :3] = rows[:, :3].astype(np.float32)
+    lab = fz.s.label.get()
+    E, g = OA.energy_and_gradient(rows, lab, fz.NN, fz.screen, n_steps, **kw)
+    g = g - s1.nuclear_forces(rows[:, :3], rows[:, 3], fz.h)
+    E += s1.nuclear_energy(rows[:, :3], rows[:, 3], fz.h)
+    return E, g
+
+
+@pytest.mark.parametrize("mode", ["jacobi", "multigrid"])
+def test_adjoint_matches_fp64_autograd(mode):
+    NN, screen = 16, 16 * 0.3
+    rows = three_atoms(NN)
+    n_steps = 60
+    kw = dict(bootstrap=5, checkpoint=16, poisson_interval=2 if mode == "jacobi" else 1, norm_interval=20)
+    extra = {} if mode == "jacobi" else {"poisson": "multigrid", "mg_levels": 3}
+    fz = FrozenSolver(rows, NN, screen, n_steps, **kw, **extra)
+    pos = rows[:, :3] * fz.h
+    E, g = fz.energy_and_gradient(pos)
+    okw = dict(bootstrap=5, poisson_interval=kw["poisson_interval"], norm_interval=20)
+    if mode == "multigrid":
+        okw.update(mode="multigrid", mg_levels=3)
+    Eo, go = _oracle_grad(fz, n_steps, **okw)
+    err = np.abs(g - go).max()
+    print(f"{mode}: E gpu {E:.9f} oracle {Eo:.9f}; max |dE/dR diff| {err:.2e} (|g|max {np.abs(go).max():.3f})")
+    assert abs(E - Eo) < 2e-6 * abs(Eo)
+    assert err < 1e-4 + 1e-4 * np.abs(go).max()
+    fz.close()
+
+
+def test_adjoint_matches_gpu_central_differences():
+    NN, screen = 32, 32 * 0.25
+    rows = three_atoms(NN)
+    fz = FrozenSolver(rows, NN, screen, 100, bootstrap=20, checkpoint=32)
+    pos = rows[:, :3] * fz.h
+    E, g = fz.energy_and_gradient(pos)
+    F_fd = fz.forces_fd(pos, delta=1e-2, atoms=[0, 1])
+    diff = np.abs(-g[:2] - F_fd[:2]).max()
+    print(f"adjoint vs FD(1e-2 bohr): max diff {diff:.2e} Ha/bohr; F={-g[:2]}")
+    assert diff < 2e-3
+    fz.close()
+
+
+def test_checkpoint_interval_does_not_change_gradient():
+    NN, screen = 24, 24 * 0.3
+    rows = three_atoms(NN)
+    out = []
+    for ck in (7, 64):
+        fz = FrozenSolver(rows, NN, screen, 50, bootstrap=4, checkpoint=ck)
+        out.append(fz.energy_and_gradient(rows[:, :3] * fz.h))
+        fz.close()
+    assert out[0][0] == pytest.approx(out[1][0], rel=1e-12)
+    assert np.allclose(out[0][1], out[1][1], rtol=1e-5, atol=1e-7)
+
+
+def test_autograd_function_backward():
+    NN, screen = 16, 16 * 0.3
+    rows = three_atoms(NN)
+    fz = FrozenSolver(rows, NN, screen, 30, bootstrap=3, checkpoint=8)
+    pos = torch.tensor(rows[:, :3] * fz.h, dtype=torch.float64, requires_grad=True)
+    E = trajectory_energy(pos, fz)
+    E.backward()
+    E2, g = fz.energy_and_gradient(rows[:, :3] * fz.h)
+    assert float(E) == pytest.approx(E2, rel=1e-12)
+    assert np.allclose(pos.grad.numpy(), g, rtol=1e-6, atol=1e-9)
+    fz.close()

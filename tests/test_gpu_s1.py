@@ -25,7 +25,6 @@ def _state(rows, NN, screen, seed=0, dtype=F32):
     S = NN + 1
     P = np.zeros((S, S, S), dtype)
     P[1:-1, 1:-1, 1:-1] = rng.uniform(0.0, 2.0, (S - 2,) * 3).astype(dtype)
-    U = U * dtype(1.0) + dtype(0.0)
     return g, U, lab, K, P
 
 
@@ -133,7 +132,8 @@ def test_energy_matches_oracle():
     got = np.array([s.E_T, s.E_eK, s.E_ee])
     assert np.allclose(got, tot, rtol=1e-12)
     assert np.allclose(s.E_domains, per, rtol=1e-12, atol=1e-14)
-    ekk = s1.nuclear_energy(rows[:, :3], rows[:, 3], g.h64)
+    # the solver keeps atom_pos in FP32, as cloud/solver.py:335 does
+    ekk = s1.nuclear_energy(rows[:, :3].astype(np.float32), rows[:, 3], g.h64)
     assert abs(s.E_KK - ekk) < 1e-12 * abs(ekk)
     s.close()
 

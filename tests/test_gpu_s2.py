@@ -1,0 +1,137 @@
+"""GPU parity of the molecule.js (S2) semantics against the S2 oracle."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from oracle import s1, s2, mg
+from tests.helpers import three_atoms, make_solver, cube, ulp_diff
+
+pytestmark = pytest.mark.gpu
+F32 = np.float32
+
+
+def _s2_solver(rows, NN, screen, **kw):
+    cfg = dict(init="s2", energy="s2", boundary="levelset", smooth_a=0.3333, smooth_c=9.0)
+    cfg.update(kw)
+    return make_solver(rows, NN, screen, **cfg)
+
+
+def test_init_s2_all_nodes():
+    rows = three_atoms(24)
+    NN, screen = 24, 24 * 0.3
+    s = _s2_solver(rows, NN, screen)
+    s._chk(s._lib.rq_init_trial(s._h))
+    g = s1.Grid(NN, screen, 0.12, F32)
+    U, lab, K, _ = s2.init_s2(rows, g)
+    S = NN + 1
+    assert np.allclose(cube(s.K_buf.get(), S), K, rtol=2e-6)
+    assert np.allclose(cube(s.U.get(), S), U, rtol=4e-6, atol=1e-30)
+    assert (cube(s.label.get(), S) != lab).sum() <= 1
+    assert np.all(cube(s.W.get(), S) == 1.0)
+    s.close()
+
+
+def test_levelset_bitexact():
+    rows = three_atoms(24)
+    NN, screen = 24, 24 * 0.3
+    S = NN + 1
+    g = s1.Grid(NN, screen, 0.12, F32)
+    U, lab, K, _ = s2.init_s2(rows, g)
+    rng = np.random.default_rng(2)
+    W = rng.uniform(-0.3, 1.2, (S, S, S)).astype(F32)
+    U = (U * rng.uniform(0.5, 1.5, U.shape)).astype(F32)
+    for lock in (None, np.array([1, 1, 0], np.int32)):
+        s = _s2_solver(rows, NN, screen)
+        s.U.set(U)
+        s.label.set(lab)
+        s.W.set(W)
+        if lock is not None:
+            s._chk(s._lib.rq_set_lock_groups(s._h, lock.ctypes.data_as(C.POINTER(C.c_int32)), 3))
+        s._boundary()
+        Lr, Wr = s2.levelset(U, lab, W, g, 0.5, 0.15, lock)
+        Lg, Wg = cube(s.label.get(), S), cube(s.W.get(), S)
+        assert (Lr != lab).any()
+        assert np.array_equal(Lg, Lr)
+        assert ulp_diff(Wg, Wr) == 0
+        s.close()
+
+
+def test_energy_s2_matches_oracle():
+    rows = three_atoms(24)
+    NN, screen = 24, 24 * 0.3
+    S = NN + 1
+    g = s1.Grid(NN, screen, 0.12, F32)
+    U, lab, K, _ = s2.init_s2(rows, g)
+    P = np.random.default_rng(3).uniform(0, 1, (S, S, S)).astype(F32)
+    s = _s2_solver(rows, NN, screen)
+    s.U.set(U)
+    s.K_buf.set(K)
+    s.P.set(P)
+    s.label.set(lab)
+    s._compute_energy()
+    tot, per, dip = s2.energy_s2(U, K, P, lab, rows, g)
+    assert np.allclose([s.E_T, s.E_eK, s.E_ee], tot, rtol=1e-12)
+    assert np.allclose(s.E_domains, per, rtol=1e-12, atol=1e-15)
+    assert np.allclose(s.dipole, dip, rtol=1e-12)
+    s.close()
+
+
+def test_gradp_forces_match_oracle():
+    rows = three_atoms(32)
+    NN, screen = 32, 32 * 0.25
+    S = NN + 1
+    g = s1.Grid(NN, screen, 0.12, F32)
+    P = np.random.default_rng(4).uniform(0, 1, (S, S, S)).astype(F32)
+    s = _s2_solver(rows, NN, screen)
+    s.P.set(P)
+    got = s.forces_gradp(6)
+    ref = s2.gradp_forces(P, rows, g, R=6)
+    assert np.allclose(got, ref, rtol=1e-10, atol=1e-12)
+    s.close()
+
+
+def test_itp_rc_dirichlet_bitexact():
+    rows = three_atoms(24)
+    rows[1, 4] = 0.9
+    NN, screen = 24, 24 * 0.3
+    S = NN + 1
+    g = s1.Grid(NN, screen, s1.dv_table(3), F32)
+    U, lab, K, _ = s2.init_s2(rows, g)
+    P = np.random.default_rng(5).uniform(0, 1, (S, S, S)).astype(F32)
+    s = _s2_solver(rows, NN, screen, rc_dirichlet=True)
+    s.U.set(U)
+    s.K_buf.set(K)
+    s.P.set(P)
+    s.label.set(lab)  # rc flags are derived on upload
+    s._update()
+    mask = s2.rc_mask(lab, rows, g)
+    assert mask.sum() > 0
+    ref = s1.itp_update(U, K, P, lab, g, rc_mask=mask)
+    assert ulp_diff(cube(s.U.get(), S), ref) == 0
+    s.close()
+
+
+def test_twogrid_poisson_step():
+    """Two damped sweeps + the sign-inverted two-grid correction
+    (molecule.js:2869-2907), one Poisson step at a correction step."""
+    rows = three_atoms(32)
+    NN, screen = 32, 32 * 0.25
+    S = NN + 1
+    g = s1.Grid(NN, screen, 0.12, F32)
+    rng = np.random.default_rng(6)
+    U = np.zeros((S, S, S), F32)
+    U[1:-1, 1:-1, 1:-1] = rng.uniform(0, 1, (S - 2,) * 3)
+    P = np.zeros((S, S, S), F32)
+    P[1:-1, 1:-1, 1:-1] = rng.uniform(0, 1, (S - 2,) * 3)
+    s = _s2_solver(rows, NN, screen, poisson="twogrid", twogrid_interval=1, boundary="off")
+    s.U.set(U)
+    s.P.set(P)
+    s.step(0, 1, poisson_interval=1, norm_interval=10 ** 6)
+    rhs = s1.rho_rhs(U, g.two_pi, g.h2, F32)
+    ref = s1.smooth(P, rhs, 0.3333, 9.0, F32)
+    ref = s1.smooth(ref, rhs, 0.3333, 9.0, F32)
+    ref = mg.twogrid_s2(ref, rhs, F32)
+    got = cube(s.P.get(), S)
+    assert ulp_diff(got, ref) == 0
+    s.close()
