@@ -54,12 +54,12 @@ def workload(name):
     return w
 
 
-def make_solver(w):
+def make_solver(w, graph=True):
     from realmolecule_amd.solver import RealQMSolver
     from realmolecule_amd.builders import to_config_atoms
     cfg = {"gridN": w["NN"], "screen": w["screen"], "atoms": to_config_atoms(w["rows"], w["els"]),
            "poisson": "multigrid", "mg_levels": w["mg_levels"], "mg_nu1": 2, "mg_nu2": 2,
-           "bootstrap_poisson": 10}
+           "bootstrap_poisson": 10, "graph": graph}
     return RealQMSolver(cfg)
 
 
@@ -123,10 +123,21 @@ def main():
     s._compute_energy()
     log(f"[bench] {ms_per_step:.4f} ms/step, {value:.2f} Gvoxel-updates/s, E={s.E:.6f}")
 
-    # Per-kernel device time: HIP events bracketing each launch on the solver's stream.
+    # Same steps launched eagerly (no graph replay), for comparison.
     lib = s._lib
+    s._chk(lib.rq_set_graph(s._h, 0))
+    first = args.warmup + args.steps
+    s.sync()
+    t0 = time.perf_counter()
+    s.step(first, args.steps, poisson_interval=sched_pi)
+    s.sync()
+    ms_eager = (time.perf_counter() - t0) / args.steps * 1e3
+    log(f"[bench] eager launches: {ms_eager:.4f} ms/step")
+    first += args.steps
+
+    # Per-kernel device time: HIP events bracketing each launch on the solver's stream.
     s._chk(lib.rq_timing_enable(s._h, 1))
-    s.step(args.warmup + args.steps, args.timing_steps, poisson_interval=sched_pi)
+    s.step(first, args.timing_steps, poisson_interval=sched_pi)
     nf = len(L.FAMILIES)
     ms = (C.c_double * nf)()
     cnt = (C.c_int64 * nf)()
@@ -177,6 +188,7 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": round(ms_per_step, 5),
+        "ms_per_step_eager": round(ms_eager, 5),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,

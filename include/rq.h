@@ -161,8 +161,18 @@ int rq_update(rq_solver* h);                     /* one ITP step + swap */
 int rq_normalize(rq_solver* h);                  /* per-domain renormalisation */
 int rq_boundary(rq_solver* h);                   /* one application of the boundary rule */
 int rq_recompute_k(rq_solver* h);                /* K after rq_set_atoms */
+/* One molecule.js two-grid correction of P (molecule.js:2879-2907): residual,
+ * full-weighting restriction, twogrid_coarse_sweeps damped sweeps from zero on
+ * the NN/2 grid with the sign-inverted right-hand side, P += 0.5 * prolongation.
+ * Needs poisson_mode RQ_POISSON_TWOGRID. */
+int rq_twogrid_correct(rq_solver* h);
 /* Steps first_step+1 .. first_step+n_steps of the run() loop body. */
 int rq_step(rq_solver* h, int64_t first_step, int32_t n_steps, const rq_schedule* s);
+/* on != 0: rq_step replays aligned chunks of norm_interval steps from cached
+ * HIP graphs (captured on first use; Jacobi/multigrid modes, timing off,
+ * norm_interval a multiple of poisson_interval, boundary_interval a multiple
+ * of norm_interval); other steps run eagerly.  Results are identical. */
+int rq_set_graph(rq_solver* h, int32_t on);
 
 /* --- reductions (FP64) --- */
 /* out[0..5] = T, V_eK, V_ee, sum rho x h^3, sum rho y h^3, sum rho z h^3 (bohr).

@@ -1,0 +1,190 @@
+'use strict';
+// Headless JavaScript host of the molecule.js compute-pass contract on the
+// MI355X solver (librq.so through rq_addon.node).
+//
+// Configuration keys are the molecule.js page globals (molecule.js:4-152):
+//   USER_NN, USER_SCREEN, USER_ATOMS [{i, j, k?, Z, el}], USER_Z, USER_RC,
+//   USER_DV, USER_STEPS_PER_FRAME, USER_POISSON_INTERVAL, USER_JACOBI_PER_STEP,
+//   USER_CURV_REG, USER_N_MOVE, USER_DT_NUC, USER_DAMPING, USER_FORCE_SCALE,
+//   USER_DYNAMICS, FREEZE_BOUNDARY, NO_VEE, CONVERGENCE_THRESHOLD.
+// Host API mirrored: initGPU, doSteps(n) (molecule.js:2822-3261),
+// restartWithPositions (:2127), isGpuReady (:2196), onSweepDone (:4653).
+//
+// Not reproduced (outside the grid-solver path): the per-electron direct
+// Poisson / SIC (molecule.js:572-720), angular sector splits, bare nuclei,
+// the NN-keyed hydrogen energy correction, thermostats and restraints.
+
+const path = require('path');
+const addon = require(path.join(__dirname, 'rq_addon.node'));
+
+function nucMass(z) { return ({1: 1, 2: 16, 3: 14, 4: 12}[z] || 1) * 1836; }
+
+class MoleculeHost {
+  constructor(cfg) {
+    this.cfg = cfg;
+    this.NN = cfg.USER_NN || 200;
+    this.S = this.NN + 1;
+    this.N2 = Math.round(this.NN / 2);
+    this.screen = cfg.USER_SCREEN || 10;
+    this.h = this.screen / this.NN;
+    const atoms = cfg.USER_ATOMS;
+    const Zs = cfg.USER_Z || atoms.map(a => a.Z);
+    const rcs = cfg.USER_RC || atoms.map(a => a.rc || 0);
+    this.nAtoms = atoms.length;
+    this.Z = atoms.map((a, n) => Zs[n] || 0);
+    this.rc = atoms.map((a, n) => rcs[n] || 0);
+    this.nucPos = atoms.map(a => [a.i, a.j, a.k !== undefined ? a.k : this.N2]);
+    this.NELEC = this.Z.filter(z => z > 0).length || 3;
+    this.dv = cfg.USER_DV || (this.NELEC > 500 ? 0.01 : this.NELEC > 100 ? 0.03 : 0.12);
+    this.STEPS_PER_FRAME = cfg.USER_STEPS_PER_FRAME ||
+      (this.NELEC <= 30 ? 50 : this.NELEC <= 100 ? 5 : 2);
+    this.POISSON_INTERVAL = cfg.USER_POISSON_INTERVAL || 50;
+    this.JACOBI_PER_STEP = cfg.USER_JACOBI_PER_STEP || 2;
+    this.NORM_INTERVAL = 20;
+    this.N_MOVE = cfg.USER_N_MOVE || 10;
+    this.DT_NUC = cfg.USER_DT_NUC || (this.NELEC <= 5 ? 2.0 : this.NELEC > 200 ? 0.2 : 0.8);
+    this.DAMPING = cfg.USER_DAMPING || 0.98;
+    this.MAX_VEL = this.NELEC <= 5 ? 0.3 : this.NELEC > 200 ? 0.03 : 0.1;
+    this.forceScale = cfg.USER_FORCE_SCALE || 1.0;
+    this.dynamicsEnabled = !!cfg.USER_DYNAMICS;
+    this.curvReg = cfg.USER_CURV_REG !== undefined ? cfg.USER_CURV_REG : 0.15;
+    this.threshold = cfg.CONVERGENCE_THRESHOLD || 0;
+    this.nucVel = this.nucPos.map(() => [0, 0, 0]);
+    this.nucForce = this.nucPos.map(() => [0, 0, 0]);
+    this.tStep = 0;
+    this.frameCount = 0;
+    this.E = 0; this.E_T = 0; this.E_eK = 0; this.E_ee = 0; this.E_KK = 0;
+    this.dipole = [0, 0, 0];
+    this.onSweepDone = null;
+    this._stable = 0;
+    this._lastE = Infinity;
+    this.handle = null;
+  }
+
+  _atomRows() {
+    return this.nucPos.map((p, n) => [p[0], p[1], p[2], this.Z[n], this.rc[n]]);
+  }
+
+  initGPU() {
+    this.handle = addon.create({
+      NN: this.NN, screen: this.screen, dv: this.dv,
+      poisson: 'twogrid', jacobiSweeps: this.JACOBI_PER_STEP, smoothA: 0.3333, smoothC: 9,
+      twogridInterval: this.POISSON_INTERVAL, twogridCoarseSweeps: 10,
+      boundary: this.cfg.FREEZE_BOUNDARY ? 'off' : 'levelset', curvReg: this.curvReg, dtW: 0.5,
+      init: 's2', energy: 's2', rcDirichlet: 1,
+    }, this._atomRows());
+    addon.initTrial(this.handle);
+    return this.handle;
+  }
+
+  isGpuReady() { return this.handle !== null; }
+
+  // One frame of molecule.js doSteps (total-Poisson path).
+  doSteps(n) {
+    const h = this.handle;
+    const t0 = Date.now();
+    let forcesDue = false;
+    for (let s = 0; s < n; s++) {
+      if (!this.cfg.NO_VEE) {
+        addon.poisson(h, 1);                          // rho + JACOBI_PER_STEP damped sweeps
+        if (s > 0 && s % this.POISSON_INTERVAL === 0) addon.twogridCorrect(h);
+      }
+      addon.update(h);
+      if ((s + 1) % this.NORM_INTERVAL === 0 || s === n - 1) addon.normalize(h);
+      if ((this.tStep + s + 1) % this.N_MOVE === 0) forcesDue = true;
+    }
+    const e = addon.energy(h);
+    if (!this.cfg.FREEZE_BOUNDARY) {
+      for (let w = 0; w < Math.max(1, this.STEPS_PER_FRAME); w++) addon.boundary(h);
+    }
+    this.E_T = e.T; this.E_eK = e.VeK; this.E_ee = e.Vee;
+    this.dipole = e.dipole;
+    this.E_KK = this.nuclearEnergy();
+    this.E = this.E_T + this.E_eK + this.E_ee + this.E_KK;
+    if (!Number.isFinite(this.E)) throw new Error('Numerical instability at step ' + this.tStep);
+    this.tStep += n;
+    this.frameCount++;
+    this.msPerStep = (Date.now() - t0) / n;
+    if (forcesDue && this.dynamicsEnabled) this.moveNuclei(addon.forcesGradP(h, 0));
+    this._checkConvergence();
+    return this.E;
+  }
+
+  // E_KK over nuclei with the 0.04 h^2 softening of molecule.js:3172.
+  nuclearEnergy() {
+    let e = 0;
+    const h = this.h;
+    for (let a = 0; a < this.nAtoms; a++) {
+      if (this.Z[a] <= 0) continue;
+      for (let b = a + 1; b < this.nAtoms; b++) {
+        if (this.Z[b] <= 0) continue;
+        const dx = (this.nucPos[a][0] - this.nucPos[b][0]) * h;
+        const dy = (this.nucPos[a][1] - this.nucPos[b][1]) * h;
+        const dz = (this.nucPos[a][2] - this.nucPos[b][2]) * h;
+        e += this.Z[a] * this.Z[b] / Math.sqrt(dx * dx + dy * dy + dz * dz + 0.04 * h * h);
+      }
+    }
+    return e;
+  }
+
+  // Damped explicit nuclear step (molecule.js:4486-4492) with nucleus-nucleus
+  // forces (h^2 softening, molecule.js:3232) added to the electronic force.
+  moveNuclei(felec) {
+    const h = this.h;
+    for (let a = 0; a < this.nAtoms; a++) {
+      if (this.Z[a] <= 0) continue;
+      const f = [felec[3 * a], felec[3 * a + 1], felec[3 * a + 2]];
+      for (let b = 0; b < this.nAtoms; b++) {
+        if (b === a || this.Z[b] <= 0) continue;
+        const d = [0, 1, 2].map(c => (this.nucPos[a][c] - this.nucPos[b][c]) * h);
+        const r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2] + h * h;
+        const w = this.Z[a] * this.Z[b] / (r2 * Math.sqrt(r2));
+        for (let c = 0; c < 3; c++) f[c] += w * d[c];
+      }
+      this.nucForce[a] = f;
+      const m = nucMass(this.Z[a]);
+      for (let c = 0; c < 3; c++) {
+        let v = this.nucVel[a][c] + f[c] / m * this.DT_NUC * this.forceScale;
+        v *= this.DAMPING;
+        v = Math.max(-this.MAX_VEL, Math.min(this.MAX_VEL, v));
+        this.nucVel[a][c] = v;
+        this.nucPos[a][c] = Math.max(5, Math.min(this.NN - 5, this.nucPos[a][c] + v * this.DT_NUC / h));
+      }
+    }
+    addon.setAtoms(this.handle, this._atomRows());
+    addon.recomputeK(this.handle);
+  }
+
+  restartWithPositions(pos) {
+    this.nucPos = pos.map(p => [p[0], p[1], p[2]]);
+    this.nucVel = this.nucPos.map(() => [0, 0, 0]);
+    addon.setAtoms(this.handle, this._atomRows());
+    addon.initTrial(this.handle);
+    this.tStep = 0;
+    this._stable = 0;
+    this._lastE = Infinity;
+  }
+
+  // Convergence rule of molecule.js:4644-4659: three consecutive frames with
+  // |dE| below the threshold fire onSweepDone(E, steps, E_T, E_eK, E_ee, E_KK).
+  _checkConvergence() {
+    if (!this.threshold) return;
+    if (Math.abs(this.E - this._lastE) < this.threshold) this._stable++;
+    else this._stable = 0;
+    this._lastE = this.E;
+    if (this._stable >= 3 && this.onSweepDone) {
+      const cb = this.onSweepDone;
+      this.onSweepDone = null;
+      cb(this.E, this.tStep, this.E_T, this.E_eK, this.E_ee, this.E_KK);
+    }
+  }
+
+  readField(name) { return addon.readField(this.handle, name); }
+
+  destroy() {
+    if (this.handle) addon.destroy(this.handle);
+    this.handle = null;
+  }
+}
+
+module.exports = { MoleculeHost, addon };

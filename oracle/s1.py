@@ -238,12 +238,16 @@ def jacobi(P, U, g: Grid, sweeps=2):
 
 
 def domain_norms(U, label, n_atoms, g: Grid):
-    """n_d = sum over nodes with clip(label, 0, n-1) == d of U*U*h^3 (solver.py:421-427).
+    """n_d = sum over interior nodes with clip(label, 0, n-1) == d of U*U*h^3.
 
+    cloud/solver.py:421-427 sums all nodes, but its face values of U are
+    always zero, so the interior sum is the same; molecule.js sums interior
+    nodes only (molecule.js:912-935), where the faces hold trial values.
     The per-node term is formed in the working dtype; the sum is FP64.
     """
-    t = (U * U) * g.h3
-    lab = np.clip(label, 0, n_atoms - 1).ravel()
+    I = _interior(U.shape[0])
+    t = (U[I] * U[I]) * g.h3
+    lab = np.clip(label[I], 0, n_atoms - 1).ravel()
     return np.bincount(lab, weights=t.ravel().astype(np.float64), minlength=n_atoms)
 
 
@@ -257,13 +261,17 @@ def norm_scales(norms, Z, dtype):
 
 
 def normalize(U, label, Z, g: Grid):
-    """Per-domain renormalisation U *= sqrt(Z_d / n_d) (solver.py:421-432)."""
+    """Per-domain renormalisation U *= sqrt(Z_d / n_d) on interior nodes
+    (solver.py:421-432; molecule.js:1077-1105 for the interior-only rule)."""
     n = len(Z)
     norms = domain_norms(U, label, n, g)
     sc = norm_scales(norms, Z, g.dtype)
-    ok = (label >= 0) & (label < n)
-    lab = np.clip(label, 0, n - 1)
-    return np.where(ok, U * sc[lab], U), norms
+    I = _interior(U.shape[0])
+    L = label[I]
+    ok = (L >= 0) & (L < n)
+    out = U.copy()
+    out[I] = np.where(ok, U[I] * sc[np.clip(L, 0, n - 1)], U[I])
+    return out, norms
 
 
 def boundary_score(U, label, atoms, g: Grid, ratio=1.1, min_foreign=2):

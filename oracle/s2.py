@@ -214,3 +214,44 @@ def gradp_forces(P, atoms, g: Grid, R=None):
         out[n] = [2.0 * Z * gi.astype(np.float64).sum() / cnt, 2.0 * Z * gj.astype(np.float64).sum() / cnt,
                   2.0 * Z * gk.astype(np.float64).sum() / cnt]
     return out
+
+
+class S2State:
+    """molecule.js frame schedule on the total-Poisson path (molecule.js:2822-3104):
+    per step rho + ``jacobi`` damped sweeps (0.3333, 9), the two-grid correction
+    when s > 0 and s % poisson_interval == 0, the rc-rule ITP update, normalise at
+    (s+1) % 20 == 0 or the frame's last step; per frame the energy and
+    ``ls_steps`` level-set steps.  FP32-faithful when dtype is float32."""
+
+    def __init__(self, rows, NN, screen, dtype=np.float32, dv=0.12, jacobi=2, poisson_interval=50,
+                 curv_reg=0.15, ls_steps=None):
+        from . import s1
+        self.rows = np.asarray(rows, np.float64)
+        self.g = s1.Grid(NN, screen, dv, dtype)
+        self.U, self.label, self.K, _ = init_s2(self.rows, self.g)
+        self.P = np.zeros_like(self.U)
+        self.W = np.ones_like(self.U)
+        self.jacobi = jacobi
+        self.poisson_interval = poisson_interval
+        self.curv_reg = curv_reg
+        self.ls_steps = ls_steps
+
+    def frame(self, n):
+        from . import s1, mg
+        g = self.g
+        dt = g.dtype
+        Z = self.rows[:, 3]
+        for s in range(n):
+            rhs = s1.rho_rhs(self.U, g.two_pi, g.h2, dt)
+            for _ in range(self.jacobi):
+                self.P = s1.smooth(self.P, rhs, 0.3333, 9.0, dt)
+            if s > 0 and s % self.poisson_interval == 0:
+                self.P = mg.twogrid_s2(self.P, rhs, dt)
+            mask = rc_mask(self.label, self.rows, g)
+            self.U = s1.itp_update(self.U, self.K, self.P, self.label, g, rc_mask=mask)
+            if (s + 1) % 20 == 0 or s == n - 1:
+                self.U, _ = s1.normalize(self.U, self.label, Z, g)
+        tot, per, dip = energy_s2(self.U, self.K, self.P, self.label, self.rows, g)
+        for _ in range(self.ls_steps if self.ls_steps is not None else max(1, n)):
+            self.label, self.W = levelset(self.U, self.label, self.W, g, 0.5, self.curv_reg)
+        return tot, per, dip

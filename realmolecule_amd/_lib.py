@@ -88,7 +88,9 @@ PROTOTYPES = {
     "rq_normalize": (C.c_int, [_P]),
     "rq_boundary": (C.c_int, [_P]),
     "rq_recompute_k": (C.c_int, [_P]),
+    "rq_twogrid_correct": (C.c_int, [_P]),
     "rq_step": (C.c_int, [_P, C.c_int64, _I, C.POINTER(RqSchedule)]),
+    "rq_set_graph": (C.c_int, [_P, _I]),
     "rq_energy": (C.c_int, [_P, _D, _D]),
     "rq_domain_norms": (C.c_int, [_P, _D]),
     "rq_forces_hf": (C.c_int, [_P, _I, _I, _D]),

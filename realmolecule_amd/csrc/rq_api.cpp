@@ -25,6 +25,24 @@ struct EventPair {
   hipEvent_t a, b;
 };
 
+// Pointer state of the ping-pong buffers; a captured step chunk is valid for
+// the state it was captured in and leaves the solver in `after`.
+struct PtrState {
+  float *U, *U2, *P, *P2;
+  uint16_t* lab;
+  std::vector<float*> e0, e1;
+  bool operator==(const PtrState& o) const {
+    return U == o.U && U2 == o.U2 && P == o.P && P2 == o.P2 && lab == o.lab && e0 == o.e0 && e1 == o.e1;
+  }
+};
+
+struct GraphEntry {
+  PtrState before, after;
+  int pi, ni;
+  hipGraph_t graph;
+  hipGraphExec_t exec;
+};
+
 // Device buffers of the reverse sweep (allocated on first use, grown as needed).
 struct AdjState {
   float *lamU = nullptr, *lamU2 = nullptr, *lamP = nullptr, *lamP2 = nullptr, *gK = nullptr, *acc = nullptr;
@@ -64,6 +82,8 @@ struct rq_solver {
   double fam_ms[F_COUNT] = {0};
   int64_t fam_n[F_COUNT] = {0};
   AdjState adj;
+  bool use_graph = false;
+  std::vector<GraphEntry> graphs;
   std::string err;
 };
 
@@ -363,6 +383,11 @@ void free_all(rq_solver* s) {
     (void)hipEventDestroy(ep.b);
   }
   for (auto e : s->ev_free) (void)hipEventDestroy(e);
+  for (auto& ge : s->graphs) {
+    (void)hipGraphExecDestroy(ge.exec);
+    (void)hipGraphDestroy(ge.graph);
+  }
+  s->graphs.clear();
   AdjState& A = s->adj;
   for (float* p : {A.lamU, A.lamU2, A.lamP, A.lamP2, A.gK, A.acc, A.t1, A.t2, A.t3}) fr(p);
   for (float* p : A.ckU) fr(p);
@@ -901,6 +926,82 @@ int rq_recompute_k(rq_solver* h) {
   return RQ_OK;
 }
 
+int rq_twogrid_correct(rq_solver* h) {
+  if (!h) return fail(nullptr, RQ_EINVAL, "null handle");
+  if (h->p.poisson_mode != RQ_POISSON_TWOGRID) return fail(h, RQ_EINVAL, "two-grid correction needs RQ_POISSON_TWOGRID");
+  return twogrid_correction(h);
+}
+
+static int eager_step(rq_solver* h, int64_t step, const rq_schedule* sc, bool with_boundary) {
+  int rc;
+  if (step % sc->poisson_interval == 0) {
+    const bool tg = h->p.poisson_mode == RQ_POISSON_TWOGRID && step % h->p.twogrid_interval == 0;
+    if ((rc = poisson_step(h, tg))) return rc;
+  }
+  if ((rc = update_step(h))) return rc;
+  if (step % sc->norm_interval == 0 && (rc = normalize_step(h))) return rc;
+  if (with_boundary && h->p.boundary_mode != RQ_BOUNDARY_OFF && step % sc->boundary_interval == 0 &&
+      (rc = boundary_step(h)))
+    return rc;
+  return RQ_OK;
+}
+
+static PtrState ptr_state(const rq_solver* h) { return PtrState{h->U, h->U2, h->P, h->P2, h->lab, h->e0, h->e1}; }
+
+static void set_ptr_state(rq_solver* h, const PtrState& p) {
+  h->U = p.U;
+  h->U2 = p.U2;
+  h->P = p.P;
+  h->P2 = p.P2;
+  h->lab = p.lab;
+  h->e0 = p.e0;
+  h->e1 = p.e1;
+}
+
+// One aligned chunk of norm_interval steps (no boundary rule inside) from a
+// cached HIP graph, capturing it on first use for the current buffer state.
+static int graph_chunk(rq_solver* h, int64_t c0, const rq_schedule* sc) {
+  const PtrState cur = ptr_state(h);
+  for (auto& ge : h->graphs) {
+    if (ge.pi == sc->poisson_interval && ge.ni == sc->norm_interval && ge.before == cur) {
+      HIPCHK(h, hipGraphLaunch(ge.exec, h->st));
+      set_ptr_state(h, ge.after);
+      return RQ_OK;
+    }
+  }
+  if (h->graphs.size() >= 8) {
+    for (auto& ge : h->graphs) {
+      (void)hipGraphExecDestroy(ge.exec);
+      (void)hipGraphDestroy(ge.graph);
+    }
+    h->graphs.clear();
+  }
+  GraphEntry ge;
+  ge.before = cur;
+  ge.pi = sc->poisson_interval;
+  ge.ni = sc->norm_interval;
+  HIPCHK(h, hipStreamBeginCapture(h->st, hipStreamCaptureModeThreadLocal));
+  int rc = RQ_OK;
+  for (int64_t step = c0 + 1; step <= c0 + sc->norm_interval && rc == RQ_OK; ++step) rc = eager_step(h, step, sc, false);
+  hipGraph_t g = nullptr;
+  hipError_t ee = hipStreamEndCapture(h->st, &g);
+  if (rc != RQ_OK) {
+    if (g) (void)hipGraphDestroy(g);
+    set_ptr_state(h, cur);
+    return rc;
+  }
+  if (ee != hipSuccess) {
+    set_ptr_state(h, cur);
+    return fail(h, RQ_EHIP, "hipStreamEndCapture: %s", hipGetErrorString(ee));
+  }
+  ge.graph = g;
+  HIPCHK(h, hipGraphInstantiate(&ge.exec, g, nullptr, nullptr, 0));
+  ge.after = ptr_state(h);
+  h->graphs.push_back(ge);
+  HIPCHK(h, hipGraphLaunch(ge.exec, h->st));
+  return RQ_OK;
+}
+
 int rq_step(rq_solver* h, int64_t first, int32_t n, const rq_schedule* sc) {
   if (!h) return fail(nullptr, RQ_EINVAL, "null handle");
   if (!h->initialized) return fail(h, RQ_ESTATE, "rq_step before rq_init_trial/rq_initialize");
@@ -909,17 +1010,31 @@ int rq_step(rq_solver* h, int64_t first, int32_t n, const rq_schedule* sc) {
   if (!sc) sc = &d;
   if (sc->poisson_interval < 1 || sc->norm_interval < 1 || sc->boundary_interval < 1)
     return fail(h, RQ_EINVAL, "schedule intervals must be >= 1");
-  for (int64_t step = first + 1; step <= first + n; ++step) {
+  const bool graphable = h->use_graph && !h->timing && h->p.poisson_mode != RQ_POISSON_TWOGRID &&
+                         sc->norm_interval % sc->poisson_interval == 0 &&
+                         sc->boundary_interval % sc->norm_interval == 0;
+  int64_t step = first + 1;
+  const int64_t last = first + n;
+  while (step <= last) {
     int rc;
-    if (step % sc->poisson_interval == 0) {
-      const bool tg = h->p.poisson_mode == RQ_POISSON_TWOGRID && step % h->p.twogrid_interval == 0;
-      if ((rc = poisson_step(h, tg))) return rc;
+    const int64_t c0 = step - 1;
+    if (graphable && c0 % sc->norm_interval == 0 && c0 + sc->norm_interval <= last) {
+      if ((rc = graph_chunk(h, c0, sc))) return rc;
+      step = c0 + sc->norm_interval;
+      if (h->p.boundary_mode != RQ_BOUNDARY_OFF && step % sc->boundary_interval == 0 && (rc = boundary_step(h)))
+        return rc;
+      ++step;
+      continue;
     }
-    if ((rc = update_step(h))) return rc;
-    if (step % sc->norm_interval == 0 && (rc = normalize_step(h))) return rc;
-    if (h->p.boundary_mode != RQ_BOUNDARY_OFF && step % sc->boundary_interval == 0 && (rc = boundary_step(h)))
-      return rc;
+    if ((rc = eager_step(h, step, sc, true))) return rc;
+    ++step;
   }
+  return RQ_OK;
+}
+
+int rq_set_graph(rq_solver* h, int32_t on) {
+  if (!h) return fail(nullptr, RQ_EINVAL, "null handle");
+  h->use_graph = on != 0;
   return RQ_OK;
 }
 

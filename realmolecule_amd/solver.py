@@ -19,7 +19,8 @@ Extra constructor keys (all optional; defaults reproduce cloud/solver.py):
 ``poisson`` ('jacobi' | 'multigrid' | 'twogrid'), ``mg_levels``, ``mg_nu1``,
 ``mg_nu2``, ``mg_omega``, ``jacobi_sweeps``, ``smooth_a``, ``smooth_c``,
 ``boundary`` ('score' | 'levelset' | 'off'), ``init`` ('s1' | 's2'),
-``energy`` ('s1' | 's2'), ``rc_dirichlet`` (bool), ``dv``.
+``energy`` ('s1' | 's2'), ``rc_dirichlet`` (bool), ``dv``, ``graph`` (bool: replay
+step chunks from HIP graphs), ``bootstrap_poisson`` (Poisson steps in initialize, 100).
 """
 from __future__ import annotations
 
@@ -174,6 +175,8 @@ class RealQMSolver:
         self._h = h
         self._sched = L.RqSchedule()
         self._lib.rq_default_schedule(C.byref(self._sched))
+        if config.get('graph'):
+            self._chk(self._lib.rq_set_graph(self._h, 1))
 
         self.E_T = 0.0
         self.E_eK = 0.0

@@ -1,0 +1,75 @@
+"""Node.js N-API addon: exports and error handling (CPU), and the JS host's
+molecule.js frame schedule against the S2 oracle schedule (GPU)."""
+import json
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ADDON = os.path.join(ROOT, "node", "rq_addon.node")
+NODE = shutil.which("node")
+
+needs_node = pytest.mark.skipif(NODE is None or not os.path.exists(ADDON), reason="node or rq_addon.node missing")
+
+EXPORTS = ["abiVersion", "dvForElectrons", "create", "destroy", "info", "initTrial", "initialize", "poisson",
+           "twogridCorrect", "update", "normalize", "boundary", "recomputeK", "step", "sync", "energy", "forcesHF",
+           "forcesGradP", "readField", "writeField", "setAtoms", "setLockGroups", "adjointGradient"]
+
+
+def run_node(code, timeout=60):
+    return subprocess.run([NODE, "-e", code], capture_output=True, text=True, timeout=timeout, cwd=ROOT)
+
+
+@needs_node
+def test_addon_exports_and_errors():
+    code = """
+const a = require('./node/rq_addon.node');
+const out = {keys: Object.keys(a), abi: a.abiVersion(), dv: [a.dvForElectrons(3), a.dvForElectrons(150)]};
+try { a.create({NN: 9, screen: 3}, [[4, 4, 4, 1, 0]]); out.err = 'none'; } catch (e) { out.err = e.message; }
+try { a.create({NN: 32, screen: 3}, 'x'); } catch (e) { out.err2 = e.message; }
+console.log(JSON.stringify(out));
+"""
+    r = run_node(code)
+    assert r.returncode == 0, r.stderr
+    out = json.loads(r.stdout.strip().splitlines()[-1])
+    assert sorted(out["keys"]) == sorted(EXPORTS)
+    assert out["abi"] == 1 and out["dv"] == [0.12, 0.02]
+    assert "NN=9" in out["err"]
+    assert "atoms must be an array" in out["err2"]
+
+
+@needs_node
+@pytest.mark.gpu
+def test_js_host_frames_match_s2_oracle(tmp_path):
+    """Three 20-step frames of the JS host (two-grid correction every 8 steps,
+    rc rule, level set) against the S2 oracle schedule in FP32."""
+    from oracle import s2
+    from tests.helpers import three_atoms, rel_l2
+    NN, screen = 24, 24 * 0.3
+    rows = three_atoms(NN)
+    rows[1, 4] = 0.7
+    cfg = {"USER_NN": NN, "USER_SCREEN": screen,
+           "USER_ATOMS": [{"i": float(r[0]), "j": float(r[1]), "k": float(r[2]), "Z": float(r[3])} for r in rows],
+           "USER_Z": [float(z) for z in rows[:, 3]], "USER_RC": [float(x) for x in rows[:, 4]],
+           "USER_STEPS_PER_FRAME": 20, "USER_POISSON_INTERVAL": 8}
+    spec = {"cfg": cfg, "frames": 3, "steps": 20}
+    print("running node host", flush=True)
+    r = subprocess.run([NODE, os.path.join(ROOT, "node", "test_host_gpu.js"), str(tmp_path), json.dumps(spec)],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    frames = json.loads((tmp_path / "node_frames.json").read_text())
+    S = NN + 1
+    U = np.fromfile(tmp_path / "node_U.f32", np.float32).reshape(S, S, S)
+    L = np.fromfile(tmp_path / "node_label.i32", np.int32).reshape(S, S, S)
+    st = s2.S2State(rows, NN, screen, np.float32, dv=0.12, poisson_interval=8)
+    for f in range(3):
+        tot, per, dip = st.frame(20)
+        E_el = tot.sum()
+        node_el = frames[f]["E_T"] + frames[f]["E_eK"] + frames[f]["E_ee"]
+        print(f"frame {f}: node {node_el:.9f} oracle {E_el:.9f}")
+        assert abs(node_el - E_el) <= 1e-6 * abs(E_el)
+    assert rel_l2(U.astype(np.float64) ** 2, st.U.astype(np.float64) ** 2) < 1e-5
+    assert (L != st.label).sum() <= 2
