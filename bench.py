@@ -28,13 +28,18 @@ sys.path.insert(0, ROOT)
 METRIC = "Gvoxel-updates/s & %HBM roofline; batched adjoint forces/sec vs plain at ΔE≤1e-6"
 HBM_PEAK_GBS = 8000.0
 
-# Algorithmic bytes per interior voxel and launch (each array counted once).
+# Algorithmic bytes per fine interior voxel and launch (each array counted
+# once) of the fine-level passes.  On grids above 65^3 nodes the solver runs
+# residual + restriction and prolongation + correction as separate passes;
+# the last post-sweep is always fused with the ITP update.
 BYTES_PER_VOXEL = {
-    "itp": 18.0,        # U, K, P (f32) + label (u16) in; U' out
-    "smooth": 12.0,     # P, U in; P' out
-    "restrict": 16.5,   # residual: P, U in, r out (12) + restriction: r in, coarse rhs out (4.5)
-    "prolong": 8.5,     # P in + out, coarse correction in (1/8)
+    "itp": 18.0,         # U, K, P (f32) + label (u16) in; U' out  (steps without a V-cycle)
+    "smooth": 12.0,      # P, U in; P' out
+    "restrict": 16.5,    # residual (P, U in; r out) + restriction (r in; coarse rhs out, 1/8 of 4 B)
+    "prolong": 8.5,      # P in and out, coarse correction in (1/8)
+    "smooth_itp": 22.0,  # last post-sweep + ITP: P, U, K, label in; P', U' out
 }
+BYTES_FUSED_SMALL = {"restrict": 8.5, "prolong": 12.5}  # one-pass variants on grids <= 65^3 nodes
 
 
 def log(msg):
@@ -143,20 +148,23 @@ def main():
     cnt = (C.c_int64 * nf)()
     s._chk(lib.rq_timing_read(s._h, ms, cnt, nf))
     s._chk(lib.rq_timing_enable(s._h, 0))
+    bpv = dict(BYTES_PER_VOXEL)
+    if NN + 1 <= 65:
+        bpv.update(BYTES_FUSED_SMALL)
     kernels = {}
     for f, name in enumerate(L.FAMILIES):
         if cnt[f] == 0:
             continue
         avg_us = ms[f] / cnt[f] * 1e3
         ent = {"launches": int(cnt[f]), "avg_us": round(avg_us, 3), "total_ms": round(ms[f], 4)}
-        if name in BYTES_PER_VOXEL:
-            gbs = BYTES_PER_VOXEL[name] * n3 / (avg_us * 1e-6) / 1e9
-            ent["alg_bytes_per_launch"] = BYTES_PER_VOXEL[name] * n3
+        if name in bpv:
+            gbs = bpv[name] * n3 / (avg_us * 1e-6) / 1e9
+            ent["alg_bytes_per_launch"] = bpv[name] * n3
             ent["achieved_GBs"] = round(gbs, 1)
         kernels[name] = ent
-    fine = {k: v for k, v in kernels.items() if k in BYTES_PER_VOXEL}
+    fine = {k: v for k, v in kernels.items() if k in bpv}
     dom = max(fine, key=lambda k: fine[k]["total_ms"])
-    step_bytes = sum(BYTES_PER_VOXEL[k] * n3 * kernels[k]["launches"] for k in fine) / args.timing_steps
+    step_bytes = sum(bpv[k] * n3 * kernels[k]["launches"] for k in fine) / args.timing_steps
     roof = {
         "kernel": dom,
         "bound": "hbm",
@@ -165,7 +173,7 @@ def main():
         "unit": "GB/s",
         "frac": round(fine[dom]["achieved_GBs"] / HBM_PEAK_GBS, 4),
         "traffic": None,
-        "alg_bytes_per_voxel": BYTES_PER_VOXEL[dom],
+        "alg_bytes_per_voxel": bpv[dom],
         "step_alg_bytes_fine": step_bytes,
         "step_frac": round(step_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
     }

@@ -173,6 +173,17 @@ int rq_step(rq_solver* h, int64_t first_step, int32_t n_steps, const rq_schedule
  * norm_interval a multiple of poisson_interval, boundary_interval a multiple
  * of norm_interval); other steps run eagerly.  Results are identical. */
 int rq_set_graph(rq_solver* h, int32_t on);
+/* on != 0 (default): multigrid passes run fused -- two pre-sweeps from the zero
+ * coarse guess in one pass, residual+restriction without the residual array,
+ * prolongation+correction with the first post-sweep, and the last fine
+ * post-sweep with the ITP update.  Results are bit-identical to on == 0. */
+int rq_set_fused(rq_solver* h, int32_t on);
+/* Launch shape of the 2.5D stencil kernels on multigrid level `level` (0 =
+ * finest, -1 = all): a (bx, by) workgroup tile in (k, j), bx*by a multiple of
+ * 64 and <= 256, each thread marching `ich` planes along i.  bx <= 0 restores
+ * the built-in choice.  Used by the autotuner (realmolecule_amd/autotune.py). */
+int rq_set_tile(rq_solver* h, int32_t level, int32_t bx, int32_t by, int32_t ich);
+int rq_get_tile(const rq_solver* h, int32_t level, int32_t* bx, int32_t* by, int32_t* ich);
 
 /* --- reductions (FP64) --- */
 /* out[0..5] = T, V_eK, V_ee, sum rho x h^3, sum rho y h^3, sum rho z h^3 (bohr).
@@ -215,14 +226,52 @@ int rq_solve_frozen(rq_solver* h, const rq_adjoint_spec* spec, double* energy3);
  * Multigrid mode requires mg_nu1 == mg_nu2 (symmetric V-cycle). */
 int rq_adjoint_gradient(rq_solver* h, const rq_adjoint_spec* spec, double* energy3, double* dEdR);
 
+/* --- batches: many molecules with uneven grids in one launch per pass --- */
+/* A batch advances n molecules together: every pass (each V-cycle level,
+ * the ITP update, normalise, energy, init) is ONE persistent launch over the
+ * concatenated tiles of all molecules, and step chunks replay from HIP
+ * graphs.  Semantics per molecule: cloud/solver.py trial init, normalise,
+ * multigrid Poisson (rq_params: mg_levels, mg_nu1, mg_nu2, mg_a, mg_c; every
+ * NN divisible by 2^(mg_levels-1), coarsest interior <= 19), ITP, normalise
+ * every norm_interval; labels frozen (no boundary rule).  Replaces running the
+ * reference's per-molecule solver once per conformer in a geometry scan. */
+typedef struct rq_batch rq_batch;
+typedef struct rq_molecule {
+  int32_t        NN;       /* cells per axis of this molecule's grid */
+  double         screen;   /* box edge (bohr) */
+  int32_t        n_atoms;  /* 1..256 */
+  const rq_atom* atoms;
+} rq_molecule;
+
+const char* rq_batch_last_error(const rq_batch* b);
+int rq_batch_create(const rq_params* p, const rq_molecule* mols, int32_t n_mols, rq_batch** out);
+int rq_batch_destroy(rq_batch* b);
+int rq_batch_info(const rq_batch* b, int32_t* n_mols, int64_t* interior_voxels);
+/* (bx, by) workgroup tile for all stencil passes; ich[m] planes marched per
+ * thread on molecule m's fine grid (nullable: built-in choice). */
+int rq_batch_set_tiles(rq_batch* b, int32_t bx, int32_t by, const int32_t* ich);
+int rq_batch_set_graph(rq_batch* b, int32_t on);
+int rq_batch_initialize(rq_batch* b, int32_t bootstrap_vcycles);
+int rq_batch_step(rq_batch* b, int64_t first_step, int32_t n_steps, const rq_schedule* s);
+int rq_batch_sync(rq_batch* b);
+/* out3[3m..3m+2] = (T, V_eK, V_ee) of molecule m, FP64 */
+int rq_batch_energy(rq_batch* b, double* out3);
+/* fields U, K, P (float32) or label (int32) of molecule m, (NN_m+1)^3 values */
+int rq_batch_read_field(rq_batch* b, int32_t mol, int32_t field, void* host, size_t bytes);
+int rq_batch_timing_enable(rq_batch* b, int32_t on);
+int rq_batch_timing_read(rq_batch* b, double* ms, int64_t* launches, int32_t n_families);
+
 /* --- timing (HIP events on the handle's stream) --- */
 /* Per-kernel-family accumulated device time in ms since the last enable, and
  * launch counts.  Families: 0 ITP update, 1 fine smoother sweep, 2 coarse
  * smoother sweep, 3 fine residual+restriction, 4 coarse residual+restriction,
  * 5 fine prolongation+correction, 6 coarse prolongation+correction,
- * 7 exact coarsest solve, 8 normalise, 9 boundary rule, 10 energy, 11 forces.
- * Events bracket each launch on the handle's stream. */
-#define RQ_TIMING_FAMILIES 12
+ * 7 exact coarsest solve, 8 normalise, 9 boundary rule, 10 energy, 11 forces,
+ * 12 last fine smoother sweep fused with the ITP update.  With fused passes
+ * (the default) family 3 is the fused residual+restriction and family 5 the
+ * fused prolongation+first post-smoothing sweep.  Events bracket each launch
+ * on the handle's stream. */
+#define RQ_TIMING_FAMILIES 13
 int rq_timing_enable(rq_solver* h, int32_t on);
 int rq_timing_read(rq_solver* h, double* ms, int64_t* launches, int32_t n_families);
 

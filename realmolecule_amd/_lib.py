@@ -35,7 +35,7 @@ RQ_ENERGY_S2 = 1
 
 FIELDS = {"U": 0, "U2": 1, "K": 2, "P": 3, "P2": 4, "rho": 5, "label": 6, "bestU": 7, "W": 8}
 FAMILIES = ["itp", "smooth", "smooth_c", "restrict", "restrict_c", "prolong", "prolong_c", "coarse_solve",
-            "normalize", "boundary", "energy", "force"]
+            "normalize", "boundary", "energy", "force", "smooth_itp"]
 
 
 class RqAtom(C.Structure):
@@ -56,6 +56,10 @@ class RqParams(C.Structure):
 
 class RqSchedule(C.Structure):
     _fields_ = [("poisson_interval", C.c_int32), ("norm_interval", C.c_int32), ("boundary_interval", C.c_int32)]
+
+
+class RqMolecule(C.Structure):
+    _fields_ = [("NN", C.c_int32), ("screen", C.c_double), ("n_atoms", C.c_int32), ("atoms", C.POINTER(RqAtom))]
 
 
 class RqAdjointSpec(C.Structure):
@@ -91,6 +95,9 @@ PROTOTYPES = {
     "rq_twogrid_correct": (C.c_int, [_P]),
     "rq_step": (C.c_int, [_P, C.c_int64, _I, C.POINTER(RqSchedule)]),
     "rq_set_graph": (C.c_int, [_P, _I]),
+    "rq_set_fused": (C.c_int, [_P, _I]),
+    "rq_set_tile": (C.c_int, [_P, _I, _I, _I, _I]),
+    "rq_get_tile": (C.c_int, [_P, _I, C.POINTER(_I), C.POINTER(_I), C.POINTER(_I)]),
     "rq_energy": (C.c_int, [_P, _D, _D]),
     "rq_domain_norms": (C.c_int, [_P, _D]),
     "rq_forces_hf": (C.c_int, [_P, _I, _I, _D]),
@@ -102,6 +109,19 @@ PROTOTYPES = {
     "rq_stream": (C.c_int, [_P, C.POINTER(_P)]),
     "rq_solve_frozen": (C.c_int, [_P, C.POINTER(RqAdjointSpec), _D]),
     "rq_adjoint_gradient": (C.c_int, [_P, C.POINTER(RqAdjointSpec), _D, _D]),
+    "rq_batch_last_error": (C.c_char_p, [_P]),
+    "rq_batch_create": (C.c_int, [C.POINTER(RqParams), C.POINTER(RqMolecule), _I, C.POINTER(_P)]),
+    "rq_batch_destroy": (C.c_int, [_P]),
+    "rq_batch_info": (C.c_int, [_P, C.POINTER(_I), C.POINTER(C.c_int64)]),
+    "rq_batch_set_tiles": (C.c_int, [_P, _I, _I, C.POINTER(C.c_int32)]),
+    "rq_batch_set_graph": (C.c_int, [_P, _I]),
+    "rq_batch_initialize": (C.c_int, [_P, _I]),
+    "rq_batch_step": (C.c_int, [_P, C.c_int64, _I, C.POINTER(RqSchedule)]),
+    "rq_batch_sync": (C.c_int, [_P]),
+    "rq_batch_energy": (C.c_int, [_P, _D]),
+    "rq_batch_read_field": (C.c_int, [_P, _I, _I, _P, C.c_size_t]),
+    "rq_batch_timing_enable": (C.c_int, [_P, _I]),
+    "rq_batch_timing_read": (C.c_int, [_P, _D, C.POINTER(C.c_int64), _I]),
     "rq_timing_enable": (C.c_int, [_P, _I]),
     "rq_timing_read": (C.c_int, [_P, _D, C.POINTER(C.c_int64), _I]),
 }

@@ -127,6 +127,75 @@ def c2_waters():
     return {"NN": 192, "screen": screen, "rows": rows, "els": els, "mg_levels": 5}
 
 
+# Molecule templates (angstrom) of the reference's grow pages, solvent waters
+# dropped: formamide_grow.html:23-29, ethanol_grow.html:23-34,
+# glycine_grow.html:21-33.  Each entry: element, x, y, z.  Torsion groups are
+# (axis atom a, axis atom b, atoms rotated about the a-b bond).
+TEMPLATES = {
+    "h2o": {"atoms": [("O", 0.0, 0.0, 0.0), ("H", 0.9572 * math.cos(math.radians(52.25)),
+                                             0.9572 * math.sin(math.radians(52.25)), 0.0),
+                      ("H", 0.9572 * math.cos(math.radians(52.25)), -0.9572 * math.sin(math.radians(52.25)), 0.0)],
+            "torsions": []},
+    "formamide": {"atoms": [("C", 0.0, 0.0, 0.0), ("O", -0.6, 1.2, 0.0), ("N", 1.3, -0.3, 0.0),
+                            ("H", -0.6, -0.9, 0.0), ("H", 1.8, -1.2, 0.0), ("H", 1.9, 0.5, 0.0)],
+                  "torsions": [(0, 2, [4, 5])]},
+    "ethanol": {"atoms": [("C", 0.0, 0.0, 0.0), ("C", 1.52, 0.0, 0.0), ("O", 2.1, 1.2, 0.5),
+                          ("H", -0.4, 0.9, 0.4), ("H", -0.4, -0.5, 0.9), ("H", -0.4, -0.4, -0.9),
+                          ("H", 1.9, -0.5, 0.9), ("H", 1.9, -0.5, -0.9), ("H", 3.0, 1.3, 0.5)],
+                "torsions": [(0, 1, [3, 4, 5]), (1, 2, [8])]},
+    "glycine": {"atoms": [("N", 0.0, 0.0, 0.0), ("C", 1.47, 0.0, 0.0), ("C", 2.0, 1.4, 0.0), ("O", 1.3, 2.4, 0.0),
+                          ("O", 3.2, 1.5, 0.0), ("H", -0.4, 0.8, 0.3), ("H", -0.4, -0.8, -0.3),
+                          ("H", 1.8, -0.5, 0.9), ("H", 1.8, -0.5, -0.9), ("H", 3.6, 2.3, 0.0)],
+                "torsions": [(1, 2, [3, 4, 9]), (0, 1, [5, 6])]},
+}
+
+
+def _rotate_about(points, a, b, angle):
+    """Rotate points about the axis through a and b by angle (Rodrigues)."""
+    k = (b - a) / np.linalg.norm(b - a)
+    p = points - a
+    c, s = math.cos(angle), math.sin(angle)
+    return a + p * c + np.cross(k, p) * s + np.outer(p @ k, k) * (1 - c)
+
+
+def conformer(name, rng, max_torsion_deg=60.0):
+    """Random conformer: template, every torsion group rotated by U(-60, 60)
+    degrees, then a uniformly random overall rotation.  Bohr, centred."""
+    t = TEMPLATES[name]
+    els = [a[0] for a in t["atoms"]]
+    X = np.array([[a[1], a[2], a[3]] for a in t["atoms"]]) * ANGSTROM
+    for ia, ib, group in t["torsions"]:
+        ang = math.radians(rng.uniform(-max_torsion_deg, max_torsion_deg))
+        X[group] = _rotate_about(X[group], X[ia], X[ib], ang)
+    X = X @ _rand_rotation(rng).T
+    X -= X.mean(0)
+    return els, X
+
+
+def c3_conformers(n=64, seed=2, h=0.1, sizes=(64, 80, 96, 112, 128, 144, 160), margin_bohr=2.5):
+    """Batched geometry scan: n seeded conformers of H2O, formamide, ethanol and
+    glycine; each gets a grid NN drawn from ``sizes`` among those whose box
+    (NN h) holds the molecule with ``margin_bohr`` on every side.  Returns a
+    list of (rows[n,5], NN, screen, name)."""
+    rng = np.random.default_rng(seed)
+    names = ["h2o", "formamide", "ethanol", "glycine"]
+    out = []
+    for c in range(n):
+        name = names[c % 4]
+        els, X = conformer(name, rng)
+        extent = (X.max(0) - X.min(0)).max()
+        fits = [s for s in sizes if s * h >= extent + 2 * margin_bohr]
+        NN = int(rng.choice(fits))
+        screen = NN * h
+        centre = screen / 2
+        rows = []
+        for el, x in zip(els, X):
+            Z, rc = ELEMENT[el]
+            rows.append(((centre + x[0]) / h, (centre + x[1]) / h, (centre + x[2]) / h, Z, rc))
+        out.append((np.array(rows), NN, screen, name))
+    return out
+
+
 def gaussian_init(rows, NN, screen, label, seed=0, sigma_H=0.9, sigma_O=0.6, noise=1e-3):
     """Seeded Gaussian initial amplitudes per domain: exp(-r^2 / 2 sigma^2) about the
     domain's nucleus plus noise * N(0,1); zero on the faces.  ``label`` is the flat

@@ -72,9 +72,11 @@ struct rq_solver {
   std::vector<int> Sl;
   std::vector<float*> e0, e1, g;
   double* red = nullptr;  // n_atoms*3 + 3 + n_atoms*3 doubles
+  double *dst_S = nullptr, *dst_rden = nullptr;  // coarsest-level DST tables
   size_t red_len = 0;
   hipStream_t st = nullptr;
-  Tile tile;
+  Tile tile;                 // fine-level stencil launch shape
+  std::vector<Tile> ctile;   // coarse levels (index = level)
   bool initialized = false;
   bool timing = false;
   std::vector<EventPair> ev_pending;
@@ -83,6 +85,7 @@ struct rq_solver {
   int64_t fam_n[F_COUNT] = {0};
   AdjState adj;
   bool use_graph = false;
+  bool fused = true;  // fused multigrid passes (bit-identical to the unfused sequence)
   std::vector<GraphEntry> graphs;
   std::string err;
 };
@@ -202,35 +205,60 @@ int vcycle_coarse(rq_solver* s, int l) {
   const size_t n3 = (size_t)S * S * S;
   if (l == s->levels - 1) {
     Timed t(s, F_COARSE);
-    launch_coarse_exact(s->st, S, s->g[l], s->e0[l]);
+    launch_coarse_exact(s->st, S, s->g[l], s->e0[l], s->dst_S, s->dst_rden);
     LAUNCHCHK(s);
     return RQ_OK;
   }
-  const Tile tl = tile_for(S);
-  launch_zero(s->st, s->e0[l], n3);
-  for (int it = 0; it < s->p.mg_nu1; ++it) {
+  const Tile tl = s->ctile[l];
+  const float a = s->p.mg_a, c = s->p.mg_c;
+  int pre = s->p.mg_nu1;
+  if (s->fused && pre >= 2) {
     {
       Timed t(s, F_SMOOTH_C);
-      launch_smooth_coarse(s->st, tl, S, s->e0[l], s->e1[l], s->g[l], s->p.mg_a, s->p.mg_c);
+      launch_smooth2_from_zero(s->st, tl, S, s->g[l], s->e0[l], a, c);
+    }
+    pre -= 2;
+  } else {
+    launch_zero(s->st, s->e0[l], n3);
+  }
+  for (int it = 0; it < pre; ++it) {
+    {
+      Timed t(s, F_SMOOTH_C);
+      launch_smooth_coarse(s->st, tl, S, s->e0[l], s->e1[l], s->g[l], a, c);
     }
     std::swap(s->e0[l], s->e1[l]);
   }
+  // the one-pass residual+restriction and prolongation+sweep trade extra cache
+  // traffic for fewer launches: they pay off only on small, latency-bound levels
+  const bool small = s->fused && S <= kFuseMaxS;
   {
     Timed t(s, F_RESTRICT_C);
-    launch_residual_coarse(s->st, tl, S, s->e0[l], s->g[l], s->e1[l]);
-    launch_restrict4(s->st, S, s->e1[l], s->g[l + 1], 1.0f);
+    if (small) {
+      launch_residual_restrict(s->st, false, S, s->e0[l], s->g[l], 1.0f, 1.0f, s->g[l + 1], 1.0f);
+    } else {
+      launch_residual_coarse(s->st, tl, S, s->e0[l], s->g[l], s->e1[l]);
+      launch_restrict4(s->st, S, s->e1[l], s->g[l + 1], 1.0f);
+    }
   }
   LAUNCHCHK(s);
   int rc = vcycle_coarse(s, l + 1);
   if (rc) return rc;
-  {
+  int post = s->p.mg_nu2;
+  if (small && post >= 1) {
+    {
+      Timed t(s, F_PROLONG_C);
+      launch_prolong_smooth(s->st, tl, false, S, s->e0[l], s->e0[l + 1], s->e1[l], s->g[l], a, c, 1.0f, 1.0f);
+    }
+    std::swap(s->e0[l], s->e1[l]);
+    --post;
+  } else {
     Timed t(s, F_PROLONG_C);
     launch_prolong_correct(s->st, tl, S, s->e0[l], s->e0[l + 1], 1.0f);
   }
-  for (int it = 0; it < s->p.mg_nu2; ++it) {
+  for (int it = 0; it < post; ++it) {
     {
       Timed t(s, F_SMOOTH_C);
-      launch_smooth_coarse(s->st, tl, S, s->e0[l], s->e1[l], s->g[l], s->p.mg_a, s->p.mg_c);
+      launch_smooth_coarse(s->st, tl, S, s->e0[l], s->e1[l], s->g[l], a, c);
     }
     std::swap(s->e0[l], s->e1[l]);
   }
@@ -238,26 +266,55 @@ int vcycle_coarse(rq_solver* s, int l) {
   return RQ_OK;
 }
 
-int vcycle_fine(rq_solver* s) {
+// One V-cycle on the fine level.  With fuse_itp the last post-smoothing sweep
+// and the ITP update run as one kernel and *did_itp is set.
+int vcycle_fine(rq_solver* s, bool fuse_itp, bool* did_itp) {
+  if (did_itp) *did_itp = false;
+  const float a = s->p.mg_a, c = s->p.mg_c;
   for (int it = 0; it < s->p.mg_nu1; ++it) {
-    int rc = smooth_fine(s, s->p.mg_a, s->p.mg_c);
+    int rc = smooth_fine(s, a, c);
     if (rc) return rc;
   }
+  const bool small = s->fused && s->S <= kFuseMaxS;
   {
     Timed t(s, F_RESTRICT);
-    launch_residual_fine(s->st, s->tile, s->S, s->P, s->U, s->two_pi, s->h2, s->P2);
-    launch_restrict4(s->st, s->S, s->P2, s->g[1], 1.0f);
+    if (small) {
+      launch_residual_restrict(s->st, true, s->S, s->P, s->U, s->two_pi, s->h2, s->g[1], 1.0f);
+    } else {
+      launch_residual_fine(s->st, s->tile, s->S, s->P, s->U, s->two_pi, s->h2, s->P2);
+      launch_restrict4(s->st, s->S, s->P2, s->g[1], 1.0f);
+    }
   }
   LAUNCHCHK(s);
   int rc = vcycle_coarse(s, 1);
   if (rc) return rc;
-  {
+  int post = s->p.mg_nu2;
+  if (small && post >= 1) {
+    {
+      Timed t(s, F_PROLONG);
+      launch_prolong_smooth(s->st, s->tile, true, s->S, s->P, s->e0[1], s->P2, s->U, a, c, s->two_pi, s->h2);
+    }
+    std::swap(s->P, s->P2);
+    --post;
+  } else {
     Timed t(s, F_PROLONG);
     launch_prolong_correct(s->st, s->tile, s->S, s->P, s->e0[1], 1.0f);
   }
   LAUNCHCHK(s);
-  for (int it = 0; it < s->p.mg_nu2; ++it) {
-    rc = smooth_fine(s, s->p.mg_a, s->p.mg_c);
+  for (int it = 0; it < post; ++it) {
+    if (s->fused && fuse_itp && it == post - 1) {
+      {
+        Timed t(s, F_SMOOTH_ITP);
+        launch_smooth_itp(s->st, s->tile, s->S, s->P, s->P2, s->U, s->U2, s->K, s->lab, a, c, s->two_pi, s->h2,
+                          s->half_dv, s->dtv);
+      }
+      LAUNCHCHK(s);
+      std::swap(s->P, s->P2);
+      std::swap(s->U, s->U2);
+      if (did_itp) *did_itp = true;
+      break;
+    }
+    rc = smooth_fine(s, a, c);
     if (rc) return rc;
   }
   return RQ_OK;
@@ -267,7 +324,7 @@ int vcycle_fine(rq_solver* s) {
 int twogrid_correction(rq_solver* s) {
   const int S1 = s->Sl[1];
   const size_t n3 = (size_t)S1 * S1 * S1;
-  const Tile tl = tile_for(S1);
+  const Tile tl = s->ctile[1];
   {
     Timed t(s, F_RESTRICT);
     launch_residual_fine(s->st, s->tile, s->S, s->P, s->U, s->two_pi, s->h2, s->P2);
@@ -289,8 +346,9 @@ int twogrid_correction(rq_solver* s) {
   return RQ_OK;
 }
 
-int poisson_step(rq_solver* s, bool twogrid_due) {
-  if (s->p.poisson_mode == RQ_POISSON_MULTIGRID) return vcycle_fine(s);
+int poisson_step(rq_solver* s, bool twogrid_due, bool fuse_itp = false, bool* did_itp = nullptr) {
+  if (did_itp) *did_itp = false;
+  if (s->p.poisson_mode == RQ_POISSON_MULTIGRID) return vcycle_fine(s, fuse_itp, did_itp);
   for (int it = 0; it < s->p.jacobi_sweeps; ++it) {
     int rc = smooth_fine(s, s->p.smooth_a, s->p.smooth_c);
     if (rc) return rc;
@@ -373,6 +431,8 @@ void free_all(rq_solver* s) {
   fr(s->lab);
   fr(s->lab2);
   fr(s->red);
+  fr(s->dst_S);
+  fr(s->dst_rden);
   for (size_t l = 0; l < s->e0.size(); ++l) {
     fr(s->e0[l]);
     fr(s->e1[l]);
@@ -786,8 +846,10 @@ int rq_create(const rq_params* pp, const rq_atom* atoms, int32_t n_atoms, rq_sol
     s->e1.assign(levels, nullptr);
     s->g.assign(levels, nullptr);
     s->Sl[0] = s->S;
+    s->ctile.assign(levels, Tile());
     for (int l = 1; l < levels && ok; ++l) {
       s->Sl[l] = (s->NN >> l) + 1;
+      s->ctile[l] = tile_for(s->Sl[l]);
       const size_t b = (size_t)s->Sl[l] * s->Sl[l] * s->Sl[l] * sizeof(float);
       ok = alloc((void**)&s->e0[l], b) && alloc((void**)&s->e1[l], b) && alloc((void**)&s->g[l], b);
       if (ok) {
@@ -795,6 +857,22 @@ int rq_create(const rq_params* pp, const rq_atom* atoms, int32_t n_atoms, rq_sol
              hipMemsetAsync(s->g[l], 0, b, s->st) == hipSuccess;
       }
     }
+  }
+  if (ok && levels > 0) {
+    // orthonormal type-I DST matrix and 1/(l_a + l_b + l_c) of the coarsest level
+    const int m = s->Sl[levels - 1] - 2;
+    std::vector<double> Sm((size_t)m * m), rd((size_t)m * m * m), lam(m);
+    const double nrm = std::sqrt(2.0 / (m + 1));
+    for (int p = 0; p < m; ++p) {
+      lam[p] = 2.0 - 2.0 * std::cos(M_PI * (p + 1) / (m + 1));
+      for (int q = 0; q < m; ++q) Sm[(size_t)p * m + q] = nrm * std::sin(M_PI * (double)((p + 1) * (q + 1)) / (m + 1));
+    }
+    for (int a = 0; a < m; ++a)
+      for (int b = 0; b < m; ++b)
+        for (int c = 0; c < m; ++c) rd[((size_t)a * m + b) * m + c] = 1.0 / (lam[a] + lam[b] + lam[c]);
+    ok = alloc((void**)&s->dst_S, sizeof(double) * Sm.size()) && alloc((void**)&s->dst_rden, sizeof(double) * rd.size());
+    ok = ok && hipMemcpy(s->dst_S, Sm.data(), sizeof(double) * Sm.size(), hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(s->dst_rden, rd.data(), sizeof(double) * rd.size(), hipMemcpyHostToDevice) == hipSuccess;
   }
   if (!ok) {
     (void)hipGetLastError();
@@ -934,11 +1012,12 @@ int rq_twogrid_correct(rq_solver* h) {
 
 static int eager_step(rq_solver* h, int64_t step, const rq_schedule* sc, bool with_boundary) {
   int rc;
+  bool did_itp = false;
   if (step % sc->poisson_interval == 0) {
     const bool tg = h->p.poisson_mode == RQ_POISSON_TWOGRID && step % h->p.twogrid_interval == 0;
-    if ((rc = poisson_step(h, tg))) return rc;
+    if ((rc = poisson_step(h, tg, true, &did_itp))) return rc;
   }
-  if ((rc = update_step(h))) return rc;
+  if (!did_itp && (rc = update_step(h))) return rc;
   if (step % sc->norm_interval == 0 && (rc = normalize_step(h))) return rc;
   if (with_boundary && h->p.boundary_mode != RQ_BOUNDARY_OFF && step % sc->boundary_interval == 0 &&
       (rc = boundary_step(h)))
@@ -1032,9 +1111,62 @@ int rq_step(rq_solver* h, int64_t first, int32_t n, const rq_schedule* sc) {
   return RQ_OK;
 }
 
+int rq_set_fused(rq_solver* h, int32_t on) {
+  if (!h) return fail(nullptr, RQ_EINVAL, "null handle");
+  h->fused = on != 0;
+  for (auto& ge : h->graphs) {
+    (void)hipGraphExecDestroy(ge.exec);
+    (void)hipGraphDestroy(ge.graph);
+  }
+  h->graphs.clear();
+  return RQ_OK;
+}
+
 int rq_set_graph(rq_solver* h, int32_t on) {
   if (!h) return fail(nullptr, RQ_EINVAL, "null handle");
   h->use_graph = on != 0;
+  return RQ_OK;
+}
+
+static void drop_graphs(rq_solver* h) {
+  for (auto& ge : h->graphs) {
+    (void)hipGraphExecDestroy(ge.exec);
+    (void)hipGraphDestroy(ge.graph);
+  }
+  h->graphs.clear();
+}
+
+int rq_set_tile(rq_solver* h, int32_t level, int32_t bx, int32_t by, int32_t ich) {
+  if (!h) return fail(nullptr, RQ_EINVAL, "null handle");
+  const int nl = std::max(1, h->levels);
+  if (level < -1 || level >= nl) return fail(h, RQ_ERANGE, "level %d outside [-1, %d)", level, nl);
+  const bool dflt = bx <= 0;
+  if (!dflt && (by < 1 || ich < 1 || bx * by > 256 || (bx * by) % 64 != 0))
+    return fail(h, RQ_EINVAL, "tile %dx%d (ich %d): need bx*by a multiple of 64 and <= 256", bx, by, ich);
+  for (int l = 0; l < nl; ++l) {
+    if (level != -1 && level != l) continue;
+    const int S = (l == 0) ? h->S : h->Sl[l];
+    Tile t = tile_for(S);
+    if (!dflt) {
+      t.bx = bx;
+      t.by = by;
+      t.ich = ich;
+    }
+    if (l == 0) h->tile = t;
+    else h->ctile[l] = t;
+  }
+  drop_graphs(h);
+  return RQ_OK;
+}
+
+int rq_get_tile(const rq_solver* h, int32_t level, int32_t* bx, int32_t* by, int32_t* ich) {
+  if (!h) return RQ_EINVAL;
+  const int nl = std::max(1, h->levels);
+  if (level < 0 || level >= nl) return RQ_ERANGE;
+  const Tile& t = (level == 0) ? h->tile : h->ctile[level];
+  if (bx) *bx = t.bx;
+  if (by) *by = t.by;
+  if (ich) *ich = t.ich;
   return RQ_OK;
 }
 

@@ -1,0 +1,1056 @@
+// Batched solver: many molecules with uneven grids advanced together.
+//
+// Every pass of the step (V-cycle levels, ITP, normalise, energy, init) is
+// ONE persistent launch over the concatenated tile space of all molecules:
+// a fixed grid of workgroups strides over work items, each item decodes its
+// molecule by binary search over per-level descriptors and runs the same
+// per-column body as the single-grid kernels (rq_stencil.h).  The per-step
+// launch count is that of one molecule, independent of the batch size, and
+// step chunks replay from HIP graphs.  Labels are frozen (no boundary rule),
+// the schedule is the cloud/solver.py loop body with a multigrid Poisson step.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "rq_internal.h"
+#include "rq_stencil.h"
+
+namespace rq {
+namespace {
+
+constexpr int kMaxBatchDom = 256;  // domains per molecule (LDS bins per work item)
+
+struct BLev {
+  int S, ich, gx, gy, gz;
+  int tile0;     // first tile of this molecule in the level's tile space
+  int cnode0;    // first coarse interior node (restriction items from this level)
+  long long off; // element offset of this molecule in the level's arrays
+};
+
+struct BMol {
+  int atom0, n_atoms, dom0, node0;
+  float half_dv, dtv, h, h2, h3, two_pi;
+  int dst0, rden0;  // offsets into the coarsest-level DST tables
+};
+
+__device__ __forceinline__ int find_mol(const BLev* __restrict__ L, int nm, int t) {
+  int lo = 0, hi = nm - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (L[mid].tile0 <= t) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ int find_cnode(const BLev* __restrict__ L, int nm, int t) {
+  int lo = 0, hi = nm - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (L[mid].cnode0 <= t) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ int find_node(const BMol* __restrict__ M, int nm, int t) {
+  int lo = 0, hi = nm - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (M[mid].node0 <= t) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// Work item t -> XCD-contiguous item (the persistent grid is a multiple of 8,
+// so item t runs on XCD t mod 8).
+__device__ __forceinline__ int xcd_item(int t, int n) {
+  const int per = n >> 3, rem = n & 7;
+  const int xcd = t & 7, local = t >> 3;
+  return (xcd < rem) ? xcd * (per + 1) + local : rem * (per + 1) + (xcd - rem) * per + local;
+}
+
+struct Col {
+  int m, k, j, i0, i1;
+};
+
+__device__ __forceinline__ bool item_col(const BLev* __restrict__ L, int nm, int ntiles, int t, Col& c) {
+  const int u = xcd_item(t, ntiles);
+  const int m = find_mol(L, nm, u);
+  const BLev d = L[m];
+  const int local = u - d.tile0;
+  const int tx = local % d.gx, ty = (local / d.gx) % d.gy, tz = local / (d.gx * d.gy);
+  const int n = d.S - 2;
+  c.m = m;
+  c.k = 1 + tx * blockDim.x + threadIdx.x;
+  c.j = 1 + ty * blockDim.y + threadIdx.y;
+  c.i0 = 1 + tz * d.ich;
+  c.i1 = min(c.i0 + d.ich, n + 1);
+  return c.k <= n && c.j <= n;
+}
+
+template <bool FINE>
+__global__ __launch_bounds__(256) void kb_smooth(const BLev* __restrict__ L, const BMol* __restrict__ M, int nm,
+                                                 int ntiles, const float* __restrict__ P, float* __restrict__ Pn,
+                                                 const float* __restrict__ src, float a, float c) {
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    Col q;
+    if (!item_col(L, nm, ntiles, t, q)) continue;
+    const long long o = L[q.m].off;
+    const float al = FINE ? M[q.m].two_pi : 1.0f, be = FINE ? M[q.m].h2 : 1.0f;
+    smooth_col<FINE>(L[q.m].S, q.k, q.j, q.i0, q.i1, P + o, Pn + o, src + o, a, c, al, be);
+  }
+}
+
+__global__ __launch_bounds__(256) void kb_smooth2z(const BLev* __restrict__ L, int nm, int ntiles,
+                                                   const float* __restrict__ g, float* __restrict__ e, float a,
+                                                   float c) {
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    Col q;
+    if (!item_col(L, nm, ntiles, t, q)) continue;
+    const long long o = L[q.m].off;
+    smooth2z_col(L[q.m].S, q.k, q.j, q.i0, q.i1, g + o, e + o, a, c);
+  }
+}
+
+template <bool FINE>
+__global__ __launch_bounds__(256) void kb_prolong_smooth(const BLev* __restrict__ Lf, const BLev* __restrict__ Lc,
+                                                         const BMol* __restrict__ M, int nm, int ntiles,
+                                                         const float* __restrict__ x, const float* __restrict__ e,
+                                                         float* __restrict__ x_out, const float* __restrict__ src,
+                                                         float a, float c) {
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    Col q;
+    if (!item_col(Lf, nm, ntiles, t, q)) continue;
+    const long long o = Lf[q.m].off, oc = Lc[q.m].off;
+    const float al = FINE ? M[q.m].two_pi : 1.0f, be = FINE ? M[q.m].h2 : 1.0f;
+    prolong_smooth_col<FINE>(Lf[q.m].S, q.k, q.j, q.i0, q.i1, x + o, e + oc, x_out + o, src + o, a, c, al, be);
+  }
+}
+
+__global__ __launch_bounds__(256) void kb_smooth_itp(const BLev* __restrict__ L, const BMol* __restrict__ M, int nm,
+                                                     int ntiles, const float* __restrict__ P, float* __restrict__ Pn,
+                                                     const float* __restrict__ U, float* __restrict__ Un,
+                                                     const float* __restrict__ K, const uint16_t* __restrict__ lab,
+                                                     float a, float c) {
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    Col q;
+    if (!item_col(L, nm, ntiles, t, q)) continue;
+    const long long o = L[q.m].off;
+    const BMol mm = M[q.m];
+    smooth_itp_col(L[q.m].S, q.k, q.j, q.i0, q.i1, P + o, Pn + o, U + o, Un + o, K + o, lab + o, a, c, mm.two_pi,
+                   mm.h2, mm.half_dv, mm.dtv);
+  }
+}
+
+__global__ __launch_bounds__(256) void kb_itp(const BLev* __restrict__ L, const BMol* __restrict__ M, int nm,
+                                              int ntiles, const float* __restrict__ U, float* __restrict__ Un,
+                                              const float* __restrict__ K, const float* __restrict__ P,
+                                              const uint16_t* __restrict__ lab) {
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    Col q;
+    if (!item_col(L, nm, ntiles, t, q)) continue;
+    const long long o = L[q.m].off;
+    itp_col(L[q.m].S, q.k, q.j, q.i0, q.i1, U + o, Un + o, K + o, P + o, lab + o, M[q.m].half_dv, M[q.m].dtv);
+  }
+}
+
+// items = coarse interior nodes of level l+1 over all molecules
+template <bool FINE>
+__global__ __launch_bounds__(256) void kb_residual_restrict(const BLev* __restrict__ Lf, const BLev* __restrict__ Lc,
+                                                            const BMol* __restrict__ M, int nm, int nitems,
+                                                            const float* __restrict__ x,
+                                                            const float* __restrict__ src, float* __restrict__ gc) {
+  const int stride = gridDim.x * blockDim.x * blockDim.y;
+  for (int t = blockIdx.x * blockDim.x * blockDim.y + threadIdx.y * blockDim.x + threadIdx.x; t < nitems;
+       t += stride) {
+    const int m = find_cnode(Lf, nm, t);
+    const int Sf = Lf[m].S;
+    const int nc = (Sf - 1) / 2 - 1;
+    const int local = t - Lf[m].cnode0;
+    const float al = FINE ? M[m].two_pi : 1.0f, be = FINE ? M[m].h2 : 1.0f;
+    residual_restrict_pt<FINE>(Sf, local / (nc * nc) + 1, (local / nc) % nc + 1, local % nc + 1, x + Lf[m].off,
+                               src + Lf[m].off, al, be, gc + Lc[m].off, 1.0f);
+  }
+}
+
+template <bool FINE>
+__global__ __launch_bounds__(256) void kb_residual(const BLev* __restrict__ L, const BMol* __restrict__ M, int nm,
+                                                   int ntiles, const float* __restrict__ x,
+                                                   const float* __restrict__ src, float* __restrict__ r) {
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    Col q;
+    if (!item_col(L, nm, ntiles, t, q)) continue;
+    const long long o = L[q.m].off;
+    const float al = FINE ? M[q.m].two_pi : 1.0f, be = FINE ? M[q.m].h2 : 1.0f;
+    residual_col<FINE>(L[q.m].S, q.k, q.j, q.i0, q.i1, x + o, src + o, al, be, r + o);
+  }
+}
+
+__global__ __launch_bounds__(256) void kb_restrict4(const BLev* __restrict__ Lf, const BLev* __restrict__ Lc, int nm,
+                                                    int nitems, const float* __restrict__ r, float* __restrict__ gc) {
+  const int stride = gridDim.x * blockDim.x;
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nitems; t += stride) {
+    const int m = find_cnode(Lf, nm, t);
+    const int Sf = Lf[m].S;
+    const int nc = (Sf - 1) / 2 - 1;
+    const int local = t - Lf[m].cnode0;
+    restrict4_pt(Sf, local / (nc * nc) + 1, (local / nc) % nc + 1, local % nc + 1, r + Lf[m].off, gc + Lc[m].off,
+                 1.0f);
+  }
+}
+
+__global__ __launch_bounds__(256) void kb_prolong_correct(const BLev* __restrict__ Lf, const BLev* __restrict__ Lc,
+                                                          int nm, int ntiles, float* __restrict__ x,
+                                                          const float* __restrict__ e) {
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    Col q;
+    if (!item_col(Lf, nm, ntiles, t, q)) continue;
+    prolong_correct_col(Lf[q.m].S, q.k, q.j, q.i0, q.i1, x + Lf[q.m].off, e + Lc[q.m].off, 1.0f);
+  }
+}
+
+// exact coarsest solve, one workgroup per molecule (ping-pong DST as in the single-grid kernel)
+__device__ __forceinline__ void dst_axis_b(const double* __restrict__ in, double* __restrict__ out,
+                                           const double* __restrict__ Sm, int m, int axis) {
+  const int m2 = m * m, m3 = m2 * m;
+  for (int t = threadIdx.x; t < m3; t += blockDim.x) {
+    const int a = t / m2, b = (t / m) % m, c = t % m;
+    int p, base, stride;
+    if (axis == 2) {
+      p = c;
+      base = a * m2 + b * m;
+      stride = 1;
+    } else if (axis == 1) {
+      p = b;
+      base = a * m2 + c;
+      stride = m;
+    } else {
+      p = a;
+      base = b * m + c;
+      stride = m2;
+    }
+    const double* row = Sm + p * m;
+    double s0 = 0.0, s1 = 0.0;
+    int q = 0;
+    for (; q + 1 < m; q += 2) {
+      s0 += row[q] * in[base + q * stride];
+      s1 += row[q + 1] * in[base + (q + 1) * stride];
+    }
+    if (q < m) s0 += row[q] * in[base + q * stride];
+    out[t] = s0 + s1;
+  }
+}
+
+__global__ __launch_bounds__(1024) void kb_exact(const BLev* __restrict__ L, const BMol* __restrict__ M,
+                                                 const float* __restrict__ g, float* __restrict__ e,
+                                                 const double* __restrict__ Stab, const double* __restrict__ Rtab) {
+  extern __shared__ double lds[];
+  const int mol = blockIdx.x;
+  const int Sc = L[mol].S;
+  const int m = Sc - 2, m3 = m * m * m;
+  const long long o = L[mol].off;
+  const double* Smat = Stab + M[mol].dst0;
+  const double* rden = Rtab + M[mol].rden0;
+  double* Sm = lds;
+  double* A = Sm + m * m;
+  double* B = A + m3;
+  const int Sc2 = Sc * Sc;
+  for (int t = threadIdx.x; t < m * m; t += blockDim.x) Sm[t] = Smat[t];
+  for (int t = threadIdx.x; t < m3; t += blockDim.x) {
+    const int a = t / (m * m), b = (t / m) % m, c = t % m;
+    A[t] = (double)g[o + (a + 1) * Sc2 + (b + 1) * Sc + (c + 1)];
+  }
+  __syncthreads();
+  dst_axis_b(A, B, Sm, m, 2);
+  __syncthreads();
+  dst_axis_b(B, A, Sm, m, 1);
+  __syncthreads();
+  dst_axis_b(A, B, Sm, m, 0);
+  __syncthreads();
+  for (int t = threadIdx.x; t < m3; t += blockDim.x) B[t] *= rden[t];
+  __syncthreads();
+  dst_axis_b(B, A, Sm, m, 2);
+  __syncthreads();
+  dst_axis_b(A, B, Sm, m, 1);
+  __syncthreads();
+  dst_axis_b(B, A, Sm, m, 0);
+  __syncthreads();
+  for (int t = threadIdx.x; t < m3; t += blockDim.x) {
+    const int a = t / (m * m), b = (t / m) % m, c = t % m;
+    e[o + (a + 1) * Sc2 + (b + 1) * Sc + (c + 1)] = (float)A[t];
+  }
+}
+
+__global__ __launch_bounds__(256) void kb_norm_sums(const BLev* __restrict__ L, const BMol* __restrict__ M, int nm,
+                                                    int ntiles, const float* __restrict__ U,
+                                                    const uint16_t* __restrict__ lab, double* __restrict__ norms) {
+  __shared__ double bins[kMaxBatchDom];
+  const int nthr = blockDim.x * blockDim.y;
+  const int tid = threadIdx.y * blockDim.x + threadIdx.x;
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const int u = xcd_item(t, ntiles);
+    const int m = find_mol(L, nm, u);  // uniform per workgroup
+    const int nd = M[m].n_atoms;
+    for (int d = tid; d < nd; d += nthr) bins[d] = 0.0;
+    __syncthreads();
+    Col q;
+    if (item_col(L, nm, ntiles, t, q))
+      norm_sums_col(L[m].S, q.k, q.j, q.i0, q.i1, M[m].h3, U + L[m].off, lab + L[m].off, nd, bins);
+    __syncthreads();
+    for (int d = tid; d < nd; d += nthr) {
+      const double b = bins[d];
+      if (b != 0.0) atomicAdd(&norms[M[m].dom0 + d], b);
+    }
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(256) void kb_normalize(const BLev* __restrict__ L, const BMol* __restrict__ M, int nm,
+                                                    int ntiles, float* __restrict__ U,
+                                                    const uint16_t* __restrict__ lab,
+                                                    const double* __restrict__ norms, const float* __restrict__ Z) {
+  __shared__ float scale[kMaxBatchDom];
+  const int nthr = blockDim.x * blockDim.y;
+  const int tid = threadIdx.y * blockDim.x + threadIdx.x;
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const int u = xcd_item(t, ntiles);
+    const int m = find_mol(L, nm, u);
+    const int nd = M[m].n_atoms, d0 = M[m].dom0;
+    __syncthreads();
+    for (int d = tid; d < nd; d += nthr) {
+      const double z = (double)Z[d0 + d], nv = norms[d0 + d];
+      scale[d] = (z > 0.0 && nv > 1e-20) ? (float)sqrt(z / nv) : 1.0f;
+    }
+    __syncthreads();
+    Col q;
+    if (item_col(L, nm, ntiles, t, q)) normalize_col(L[m].S, q.k, q.j, q.i0, q.i1, U + L[m].off, lab + L[m].off, scale, nd);
+  }
+}
+
+__global__ __launch_bounds__(256) void kb_energy(const BLev* __restrict__ L, const BMol* __restrict__ M, int nm,
+                                                 int ntiles, const float* __restrict__ U, const float* __restrict__ K,
+                                                 const float* __restrict__ P, const uint16_t* __restrict__ lab,
+                                                 double* __restrict__ out) {
+  __shared__ double bins[3 * kMaxBatchDom];
+  const int nthr = blockDim.x * blockDim.y;
+  const int tid = threadIdx.y * blockDim.x + threadIdx.x;
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const int u = xcd_item(t, ntiles);
+    const int m = find_mol(L, nm, u);
+    const int nd = M[m].n_atoms;
+    for (int d = tid; d < 3 * nd; d += nthr) bins[d] = 0.0;
+    __syncthreads();
+    Col q;
+    if (item_col(L, nm, ntiles, t, q)) {
+      const long long o = L[m].off;
+      energy_s1_col(L[m].S, q.k, q.j, q.i0, q.i1, M[m].h, M[m].h3, U + o, K + o, P + o, lab + o, nd, bins);
+    }
+    __syncthreads();
+    for (int d = tid; d < 3 * nd; d += nthr) {
+      const double b = bins[d];
+      if (b != 0.0) atomicAdd(&out[3 * M[m].dom0 + d], b);
+    }
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(256) void kb_init(const BMol* __restrict__ M, const BLev* __restrict__ L, int nm,
+                                               int nnodes, const Atom* __restrict__ atoms, float* __restrict__ U,
+                                               uint16_t* __restrict__ lab, float* __restrict__ K) {
+  const int stride = gridDim.x * blockDim.x;
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nnodes; t += stride) {
+    const int m = find_node(M, nm, t);
+    const long long o = L[m].off;
+    init_trial_s1_node(L[m].S, t - M[m].node0, M[m].h, M[m].h2, atoms + M[m].atom0, M[m].n_atoms, U + o, lab + o,
+                       K + o);
+  }
+}
+
+}  // namespace
+}  // namespace rq
+
+using namespace rq;
+
+namespace {
+thread_local std::string g_batch_error;
+
+struct BatchPtrs {
+  float *U, *U2, *P, *P2;
+  std::vector<float*> e0, e1;
+  bool operator==(const BatchPtrs& o) const {
+    return U == o.U && U2 == o.U2 && P == o.P && P2 == o.P2 && e0 == o.e0 && e1 == o.e1;
+  }
+};
+
+struct BatchGraph {
+  BatchPtrs before, after;
+  int pi, ni;
+  hipGraph_t graph;
+  hipGraphExec_t exec;
+};
+}  // namespace
+
+struct rq_batch {
+  rq_params p{};
+  int nm = 0, L = 0;
+  int bx = 64, by = 4;
+  std::vector<int> NN;
+  std::vector<double> hh, dv;
+  std::vector<std::vector<BLev>> lev;  // [level][mol]
+  std::vector<BMol> mol;
+  std::vector<int> ntiles, ncnodes;    // per level
+  std::vector<size_t> len;             // elements per level array
+  std::vector<int> maxS;               // largest node count per axis on each level
+  int nnodes = 0, ndom = 0;
+  int grid_stencil = 2048;
+  BLev* lev_d = nullptr;  // L * nm
+  BMol* mol_d = nullptr;
+  Atom* atoms_d = nullptr;
+  float* Z_d = nullptr;
+  double *Stab = nullptr, *Rtab = nullptr;
+  size_t lds_exact = 0;
+  float *U = nullptr, *U2 = nullptr, *K = nullptr, *P = nullptr, *P2 = nullptr;
+  uint16_t* lab = nullptr;
+  std::vector<float*> e0, e1, g;  // per level (index 0 unused)
+  double* red = nullptr;          // norms (ndom) + energies (3 ndom)
+  hipStream_t st = nullptr;
+  bool initialized = false, use_graph = false, timing = false;
+  std::vector<BatchGraph> graphs;
+  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev;
+  double fam_ms[F_COUNT] = {0};
+  int64_t fam_n[F_COUNT] = {0};
+  std::string err;
+};
+
+namespace {
+
+int bfail(rq_batch* b, int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (b) b->err = buf;
+  else g_batch_error = buf;
+  return code;
+}
+
+#define BCHK(b, call)                                                                         \
+  do {                                                                                        \
+    hipError_t e_ = (call);                                                                   \
+    if (e_ != hipSuccess) return bfail((b), RQ_EHIP, "%s: %s", #call, hipGetErrorString(e_)); \
+  } while (0)
+
+#define BLAUNCH(b)                                                                                  \
+  do {                                                                                              \
+    hipError_t e_ = hipGetLastError();                                                              \
+    if (e_ != hipSuccess) return bfail((b), RQ_EHIP, "kernel launch: %s", hipGetErrorString(e_)); \
+  } while (0)
+
+struct BTimed {
+  rq_batch* b;
+  int fam;
+  hipEvent_t a = nullptr;
+  BTimed(rq_batch* b_, int f) : b(b_), fam(f) {
+    if (b->timing) {
+      (void)hipEventCreate(&a);
+      (void)hipEventRecord(a, b->st);
+    }
+  }
+  ~BTimed() {
+    if (b->timing && a) {
+      hipEvent_t e = nullptr;
+      (void)hipEventCreate(&e);
+      (void)hipEventRecord(e, b->st);
+      b->ev.push_back({fam, {a, e}});
+    }
+  }
+};
+
+void resolve(rq_batch* b) {
+  if (b->ev.empty()) return;
+  (void)hipStreamSynchronize(b->st);
+  for (auto& x : b->ev) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, x.second.first, x.second.second) == hipSuccess) {
+      b->fam_ms[x.first] += ms;
+      b->fam_n[x.first] += 1;
+    }
+    (void)hipEventDestroy(x.second.first);
+    (void)hipEventDestroy(x.second.second);
+  }
+  b->ev.clear();
+}
+
+const BLev* LD(rq_batch* b, int l) { return b->lev_d + (size_t)l * b->nm; }
+
+int grid_for(int items) { return std::max(8, std::min(((items + 7) / 8) * 8, 2048)); }
+
+int b_vcycle_coarse(rq_batch* b, int l) {
+  const float a = b->p.mg_a, c = b->p.mg_c;
+  if (l == b->L - 1) {
+    BTimed t(b, F_COARSE);
+    hipLaunchKernelGGL(kb_exact, dim3(b->nm), dim3(1024), b->lds_exact, b->st, LD(b, l), b->mol_d, b->g[l], b->e0[l],
+                       b->Stab, b->Rtab);
+    BLAUNCH(b);
+    return RQ_OK;
+  }
+  const dim3 blk(b->bx, b->by);
+  const int nt = b->ntiles[l];
+  int pre = b->p.mg_nu1;
+  if (pre >= 2) {
+    BTimed t(b, F_SMOOTH_C);
+    hipLaunchKernelGGL(kb_smooth2z, dim3(grid_for(nt)), blk, 0, b->st, LD(b, l), b->nm, nt, b->g[l], b->e0[l], a, c);
+    pre -= 2;
+  } else {
+    launch_fill(b->st, b->e0[l], b->len[l], 0.0f);
+  }
+  for (int it = 0; it < pre; ++it) {
+    {
+      BTimed t(b, F_SMOOTH_C);
+      hipLaunchKernelGGL(kb_smooth<false>, dim3(grid_for(nt)), blk, 0, b->st, LD(b, l), b->mol_d, b->nm, nt, b->e0[l],
+                         b->e1[l], b->g[l], a, c);
+    }
+    std::swap(b->e0[l], b->e1[l]);
+  }
+  const bool small = b->maxS[l] <= kFuseMaxS;
+  {
+    BTimed t(b, F_RESTRICT_C);
+    const int ni = b->ncnodes[l];
+    if (small) {
+      hipLaunchKernelGGL(kb_residual_restrict<false>, dim3(std::min((ni + 255) / 256, 4096)), dim3(256), 0, b->st,
+                         LD(b, l), LD(b, l + 1), b->mol_d, b->nm, ni, b->e0[l], b->g[l], b->g[l + 1]);
+    } else {
+      hipLaunchKernelGGL(kb_residual<false>, dim3(grid_for(nt)), blk, 0, b->st, LD(b, l), b->mol_d, b->nm, nt,
+                         b->e0[l], b->g[l], b->e1[l]);
+      hipLaunchKernelGGL(kb_restrict4, dim3(std::min((ni + 255) / 256, 4096)), dim3(256), 0, b->st, LD(b, l),
+                         LD(b, l + 1), b->nm, ni, b->e1[l], b->g[l + 1]);
+    }
+  }
+  BLAUNCH(b);
+  if (int rc = b_vcycle_coarse(b, l + 1)) return rc;
+  int post = b->p.mg_nu2;
+  if (small && post >= 1) {
+    {
+      BTimed t(b, F_PROLONG_C);
+      hipLaunchKernelGGL(kb_prolong_smooth<false>, dim3(grid_for(nt)), blk, 0, b->st, LD(b, l), LD(b, l + 1),
+                         b->mol_d, b->nm, nt, b->e0[l], b->e0[l + 1], b->e1[l], b->g[l], a, c);
+    }
+    std::swap(b->e0[l], b->e1[l]);
+    --post;
+  } else {
+    BTimed t(b, F_PROLONG_C);
+    hipLaunchKernelGGL(kb_prolong_correct, dim3(grid_for(nt)), blk, 0, b->st, LD(b, l), LD(b, l + 1), b->nm, nt,
+                       b->e0[l], b->e0[l + 1]);
+  }
+  for (int it = 0; it < post; ++it) {
+    {
+      BTimed t(b, F_SMOOTH_C);
+      hipLaunchKernelGGL(kb_smooth<false>, dim3(grid_for(nt)), blk, 0, b->st, LD(b, l), b->mol_d, b->nm, nt, b->e0[l],
+                         b->e1[l], b->g[l], a, c);
+    }
+    std::swap(b->e0[l], b->e1[l]);
+  }
+  BLAUNCH(b);
+  return RQ_OK;
+}
+
+// one V-cycle on all molecules; the last post-sweep carries the ITP update when fuse_itp
+int b_vcycle(rq_batch* b, bool fuse_itp, bool* did_itp) {
+  *did_itp = false;
+  const float a = b->p.mg_a, c = b->p.mg_c;
+  const dim3 blk(b->bx, b->by);
+  const int nt = b->ntiles[0];
+  const dim3 grd(grid_for(nt));
+  for (int it = 0; it < b->p.mg_nu1; ++it) {
+    {
+      BTimed t(b, F_SMOOTH);
+      hipLaunchKernelGGL(kb_smooth<true>, grd, blk, 0, b->st, LD(b, 0), b->mol_d, b->nm, nt, b->P, b->P2, b->U, a, c);
+    }
+    std::swap(b->P, b->P2);
+  }
+  const bool small = b->maxS[0] <= kFuseMaxS;
+  {
+    BTimed t(b, F_RESTRICT);
+    const int ni = b->ncnodes[0];
+    if (small) {
+      hipLaunchKernelGGL(kb_residual_restrict<true>, dim3(std::min((ni + 255) / 256, 4096)), dim3(256), 0, b->st,
+                         LD(b, 0), LD(b, 1), b->mol_d, b->nm, ni, b->P, b->U, b->g[1]);
+    } else {
+      hipLaunchKernelGGL(kb_residual<true>, grd, blk, 0, b->st, LD(b, 0), b->mol_d, b->nm, nt, b->P, b->U, b->P2);
+      hipLaunchKernelGGL(kb_restrict4, dim3(std::min((ni + 255) / 256, 4096)), dim3(256), 0, b->st, LD(b, 0),
+                         LD(b, 1), b->nm, ni, b->P2, b->g[1]);
+    }
+  }
+  BLAUNCH(b);
+  if (int rc = b_vcycle_coarse(b, 1)) return rc;
+  int post = b->p.mg_nu2;
+  if (small) {
+    {
+      BTimed t(b, F_PROLONG);
+      hipLaunchKernelGGL(kb_prolong_smooth<true>, grd, blk, 0, b->st, LD(b, 0), LD(b, 1), b->mol_d, b->nm, nt, b->P,
+                         b->e0[1], b->P2, b->U, a, c);
+    }
+    std::swap(b->P, b->P2);
+    --post;
+  } else {
+    BTimed t(b, F_PROLONG);
+    hipLaunchKernelGGL(kb_prolong_correct, grd, blk, 0, b->st, LD(b, 0), LD(b, 1), b->nm, nt, b->P, b->e0[1]);
+  }
+  for (int it = 0; it < post; ++it) {
+    if (fuse_itp && it == post - 1) {
+      {
+        BTimed t(b, F_SMOOTH_ITP);
+        hipLaunchKernelGGL(kb_smooth_itp, grd, blk, 0, b->st, LD(b, 0), b->mol_d, b->nm, nt, b->P, b->P2, b->U, b->U2,
+                           b->K, b->lab, a, c);
+      }
+      std::swap(b->P, b->P2);
+      std::swap(b->U, b->U2);
+      *did_itp = true;
+      break;
+    }
+    {
+      BTimed t(b, F_SMOOTH);
+      hipLaunchKernelGGL(kb_smooth<true>, grd, blk, 0, b->st, LD(b, 0), b->mol_d, b->nm, nt, b->P, b->P2, b->U, a, c);
+    }
+    std::swap(b->P, b->P2);
+  }
+  BLAUNCH(b);
+  return RQ_OK;
+}
+
+int b_normalize(rq_batch* b) {
+  const dim3 blk(b->bx, b->by);
+  const int nt = b->ntiles[0];
+  launch_fill(b->st, reinterpret_cast<float*>(b->red), 2 * (size_t)b->ndom, 0.0f);
+  BTimed t(b, F_NORM);
+  hipLaunchKernelGGL(kb_norm_sums, dim3(grid_for(nt)), blk, 0, b->st, LD(b, 0), b->mol_d, b->nm, nt, b->U, b->lab,
+                     b->red);
+  hipLaunchKernelGGL(kb_normalize, dim3(grid_for(nt)), blk, 0, b->st, LD(b, 0), b->mol_d, b->nm, nt, b->U, b->lab,
+                     b->red, b->Z_d);
+  BLAUNCH(b);
+  return RQ_OK;
+}
+
+int b_step(rq_batch* b, int64_t step, const rq_schedule* sc) {
+  bool did = false;
+  if (step % sc->poisson_interval == 0)
+    if (int rc = b_vcycle(b, true, &did)) return rc;
+  if (!did) {
+    BTimed t(b, F_ITP);
+    const int nt = b->ntiles[0];
+    hipLaunchKernelGGL(kb_itp, dim3(grid_for(nt)), dim3(b->bx, b->by), 0, b->st, LD(b, 0), b->mol_d, b->nm, nt, b->U,
+                       b->U2, b->K, b->P, b->lab);
+    BLAUNCH(b);
+    std::swap(b->U, b->U2);
+  }
+  if (step % sc->norm_interval == 0)
+    if (int rc = b_normalize(b)) return rc;
+  return RQ_OK;
+}
+
+BatchPtrs bptrs(const rq_batch* b) { return BatchPtrs{b->U, b->U2, b->P, b->P2, b->e0, b->e1}; }
+
+void set_bptrs(rq_batch* b, const BatchPtrs& p) {
+  b->U = p.U;
+  b->U2 = p.U2;
+  b->P = p.P;
+  b->P2 = p.P2;
+  b->e0 = p.e0;
+  b->e1 = p.e1;
+}
+
+void drop_bgraphs(rq_batch* b) {
+  for (auto& g : b->graphs) {
+    (void)hipGraphExecDestroy(g.exec);
+    (void)hipGraphDestroy(g.graph);
+  }
+  b->graphs.clear();
+}
+
+int b_graph_chunk(rq_batch* b, int64_t c0, const rq_schedule* sc) {
+  const BatchPtrs cur = bptrs(b);
+  for (auto& ge : b->graphs)
+    if (ge.pi == sc->poisson_interval && ge.ni == sc->norm_interval && ge.before == cur) {
+      BCHK(b, hipGraphLaunch(ge.exec, b->st));
+      set_bptrs(b, ge.after);
+      return RQ_OK;
+    }
+  if (b->graphs.size() >= 8) drop_bgraphs(b);
+  BatchGraph ge;
+  ge.before = cur;
+  ge.pi = sc->poisson_interval;
+  ge.ni = sc->norm_interval;
+  BCHK(b, hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
+  int rc = RQ_OK;
+  for (int64_t s = c0 + 1; s <= c0 + sc->norm_interval && rc == RQ_OK; ++s) rc = b_step(b, s, sc);
+  hipGraph_t g = nullptr;
+  hipError_t ee = hipStreamEndCapture(b->st, &g);
+  if (rc != RQ_OK || ee != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    set_bptrs(b, cur);
+    return rc != RQ_OK ? rc : bfail(b, RQ_EHIP, "hipStreamEndCapture: %s", hipGetErrorString(ee));
+  }
+  ge.graph = g;
+  BCHK(b, hipGraphInstantiate(&ge.exec, g, nullptr, nullptr, 0));
+  ge.after = bptrs(b);
+  b->graphs.push_back(ge);
+  BCHK(b, hipGraphLaunch(ge.exec, b->st));
+  return RQ_OK;
+}
+
+void bfree(rq_batch* b) {
+  auto fr = [](void* p) {
+    if (p) (void)hipFree(p);
+  };
+  drop_bgraphs(b);
+  for (auto& x : b->ev) {
+    (void)hipEventDestroy(x.second.first);
+    (void)hipEventDestroy(x.second.second);
+  }
+  fr(b->lev_d);
+  fr(b->mol_d);
+  fr(b->atoms_d);
+  fr(b->Z_d);
+  fr(b->Stab);
+  fr(b->Rtab);
+  fr(b->U);
+  fr(b->U2);
+  fr(b->K);
+  fr(b->P);
+  fr(b->P2);
+  fr(b->lab);
+  fr(b->red);
+  for (size_t l = 0; l < b->e0.size(); ++l) {
+    fr(b->e0[l]);
+    fr(b->e1[l]);
+    fr(b->g[l]);
+  }
+  if (b->st) (void)hipStreamDestroy(b->st);
+}
+
+int upload_descriptors(rq_batch* b) {
+  std::vector<BLev> flat;
+  for (int l = 0; l < b->L; ++l) flat.insert(flat.end(), b->lev[l].begin(), b->lev[l].end());
+  BCHK(b, hipMemcpy(b->lev_d, flat.data(), sizeof(BLev) * flat.size(), hipMemcpyHostToDevice));
+  return RQ_OK;
+}
+
+// tile counts and offsets of every level for the current (bx, by, ich)
+void layout(rq_batch* b, const std::vector<int>* ich_fine) {
+  b->ntiles.assign(b->L, 0);
+  b->ncnodes.assign(b->L, 0);
+  for (int l = 0; l < b->L; ++l) {
+    int tile = 0, cnode = 0;
+    for (int m = 0; m < b->nm; ++m) {
+      BLev& d = b->lev[l][m];
+      const int n = d.S - 2;
+      if (l == 0 && ich_fine && (*ich_fine)[m] > 0) d.ich = (*ich_fine)[m];
+      else if (d.ich <= 0) d.ich = std::max(1, std::min(8, n / 8));
+      d.gx = (n + b->bx - 1) / b->bx;
+      d.gy = (n + b->by - 1) / b->by;
+      d.gz = (n + d.ich - 1) / d.ich;
+      d.tile0 = tile;
+      tile += d.gx * d.gy * d.gz;
+      d.cnode0 = cnode;
+      const int nc = (d.S - 1) / 2 - 1;
+      if (l + 1 < b->L) cnode += nc * nc * nc;
+    }
+    b->ntiles[l] = tile;
+    b->ncnodes[l] = cnode;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* rq_batch_last_error(const rq_batch* b) { return b ? b->err.c_str() : g_batch_error.c_str(); }
+
+int rq_batch_create(const rq_params* pp, const rq_molecule* mols, int32_t n_mols, rq_batch** out) {
+  if (!pp || !mols || !out || n_mols < 1) return bfail(nullptr, RQ_EINVAL, "null argument or empty batch");
+  *out = nullptr;
+  const rq_params& p = *pp;
+  if (p.poisson_mode != RQ_POISSON_MULTIGRID) return bfail(nullptr, RQ_EINVAL, "batches run the multigrid Poisson mode");
+  if (p.mg_nu2 < 1) return bfail(nullptr, RQ_EINVAL, "batches need mg_nu2 >= 1");
+  const int L = p.mg_levels;
+  if (L < 2 || L > 10) return bfail(nullptr, RQ_EINVAL, "mg_levels %d outside [2, 10]", L);
+  rq_batch* b = new rq_batch();
+  b->p = p;
+  b->nm = n_mols;
+  b->L = L;
+  std::vector<Atom> atoms;
+  std::vector<float> Z;
+  b->lev.assign(L, std::vector<BLev>(n_mols));
+  b->len.assign(L, 0);
+  int node0 = 0;
+  for (int m = 0; m < n_mols; ++m) {
+    const rq_molecule& mo = mols[m];
+    const int div = 1 << (L - 1);
+    if (mo.NN < 8 || mo.NN > 1024 || mo.NN % div || !(mo.screen > 0) || mo.n_atoms < 1 ||
+        mo.n_atoms > kMaxBatchDom || !mo.atoms) {
+      delete b;
+      return bfail(nullptr, RQ_EINVAL, "molecule %d: NN=%d (divisible by %d), n_atoms=%d (1..%d)", m, mo.NN, div,
+                   mo.n_atoms, kMaxBatchDom);
+    }
+    const int mc = mo.NN / div - 1;
+    if (mc < 1 || mc > kMaxCoarsePP) {
+      delete b;
+      return bfail(nullptr, RQ_ERANGE, "molecule %d: coarsest interior %d outside [1, %d]", m, mc, kMaxCoarsePP);
+    }
+    const double h = mo.screen / mo.NN;
+    int n_elec = 0;
+    for (int a = 0; a < mo.n_atoms; ++a) {
+      atoms.push_back({mo.atoms[a].i, mo.atoms[a].j, mo.atoms[a].k, mo.atoms[a].Z, mo.atoms[a].rc});
+      Z.push_back(mo.atoms[a].Z);
+      if (mo.atoms[a].Z > 0.f) ++n_elec;
+    }
+    const double dv = p.dv > 0.0 ? p.dv : rq_dv_for_electrons(n_elec);
+    BMol bm;
+    bm.atom0 = (int)atoms.size() - mo.n_atoms;
+    bm.n_atoms = mo.n_atoms;
+    bm.dom0 = bm.atom0;
+    bm.node0 = node0;
+    node0 += (mo.NN + 1) * (mo.NN + 1) * (mo.NN + 1);
+    bm.half_dv = (float)(0.5 * dv);
+    bm.dtv = (float)(dv * (h * h));
+    bm.h = (float)h;
+    bm.h2 = (float)(h * h);
+    bm.h3 = (float)std::pow(h, 3.0);
+    bm.two_pi = (float)(2.0 * M_PI);
+    bm.dst0 = bm.rden0 = 0;
+    b->mol.push_back(bm);
+    for (int l = 0; l < L; ++l) {
+      BLev& d = b->lev[l][m];
+      d.S = (mo.NN >> l) + 1;
+      d.ich = 0;
+      d.off = (long long)b->len[l];
+      const size_t n3 = (size_t)d.S * d.S * d.S;
+      b->len[l] += (n3 + 63) / 64 * 64;
+    }
+  }
+  b->nnodes = node0;
+  b->ndom = (int)atoms.size();
+  b->maxS.assign(L, 0);
+  for (int l = 0; l < L; ++l)
+    for (int m = 0; m < n_mols; ++m) b->maxS[l] = std::max(b->maxS[l], b->lev[l][m].S);
+  layout(b, nullptr);
+  // DST tables of every molecule's coarsest level
+  std::vector<double> Stab, Rtab;
+  int mmax = 0;
+  for (int m = 0; m < n_mols; ++m) {
+    const int mc = b->lev[L - 1][m].S - 2;
+    mmax = std::max(mmax, mc);
+    b->mol[m].dst0 = (int)Stab.size();
+    b->mol[m].rden0 = (int)Rtab.size();
+    std::vector<double> lam(mc);
+    const double nrm = std::sqrt(2.0 / (mc + 1));
+    for (int pi = 0; pi < mc; ++pi) {
+      lam[pi] = 2.0 - 2.0 * std::cos(M_PI * (pi + 1) / (mc + 1));
+      for (int q = 0; q < mc; ++q) Stab.push_back(nrm * std::sin(M_PI * (double)((pi + 1) * (q + 1)) / (mc + 1)));
+    }
+    for (int a = 0; a < mc; ++a)
+      for (int c = 0; c < mc; ++c)
+        for (int e = 0; e < mc; ++e) Rtab.push_back(1.0 / (lam[a] + lam[c] + lam[e]));
+  }
+  b->lds_exact = sizeof(double) * ((size_t)mmax * mmax + 2 * (size_t)mmax * mmax * mmax);
+  auto alloc = [&](void** q, size_t bytes) { return hipMalloc(q, bytes) == hipSuccess; };
+  bool ok = hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking) == hipSuccess;
+  const size_t n0 = b->len[0];
+  ok = ok && alloc((void**)&b->lev_d, sizeof(BLev) * L * n_mols) && alloc((void**)&b->mol_d, sizeof(BMol) * n_mols) &&
+       alloc((void**)&b->atoms_d, sizeof(Atom) * atoms.size()) && alloc((void**)&b->Z_d, sizeof(float) * Z.size()) &&
+       alloc((void**)&b->Stab, sizeof(double) * Stab.size()) && alloc((void**)&b->Rtab, sizeof(double) * Rtab.size()) &&
+       alloc((void**)&b->U, n0 * 4) && alloc((void**)&b->U2, n0 * 4) && alloc((void**)&b->K, n0 * 4) &&
+       alloc((void**)&b->P, n0 * 4) && alloc((void**)&b->P2, n0 * 4) && alloc((void**)&b->lab, n0 * 2) &&
+       alloc((void**)&b->red, sizeof(double) * 4 * (size_t)b->ndom);
+  b->e0.assign(L, nullptr);
+  b->e1.assign(L, nullptr);
+  b->g.assign(L, nullptr);
+  for (int l = 1; l < L && ok; ++l)
+    ok = alloc((void**)&b->e0[l], b->len[l] * 4) && alloc((void**)&b->e1[l], b->len[l] * 4) &&
+         alloc((void**)&b->g[l], b->len[l] * 4);
+  if (ok) {
+    ok = hipMemcpy(b->mol_d, b->mol.data(), sizeof(BMol) * n_mols, hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(b->atoms_d, atoms.data(), sizeof(Atom) * atoms.size(), hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(b->Z_d, Z.data(), sizeof(float) * Z.size(), hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(b->Stab, Stab.data(), sizeof(double) * Stab.size(), hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(b->Rtab, Rtab.data(), sizeof(double) * Rtab.size(), hipMemcpyHostToDevice) == hipSuccess;
+    for (float* q : {b->U, b->U2, b->K, b->P, b->P2}) ok = ok && hipMemset(q, 0, n0 * 4) == hipSuccess;
+    ok = ok && hipMemset(b->lab, 0, n0 * 2) == hipSuccess;
+    for (int l = 1; l < L && ok; ++l)
+      for (float* q : {b->e0[l], b->e1[l], b->g[l]}) ok = ok && hipMemset(q, 0, b->len[l] * 4) == hipSuccess;
+  }
+  if (ok) ok = upload_descriptors(b) == RQ_OK;
+  if (ok)
+    ok = hipFuncSetAttribute((const void*)kb_exact, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024) ==
+         hipSuccess;
+  if (!ok) {
+    (void)hipGetLastError();
+    bfree(b);
+    delete b;
+    return bfail(nullptr, RQ_ENOMEM, "batch allocation/upload failed (%d molecules, %zu fine nodes)", n_mols, n0);
+  }
+  *out = b;
+  return RQ_OK;
+}
+
+int rq_batch_destroy(rq_batch* b) {
+  if (!b) return RQ_OK;
+  (void)hipStreamSynchronize(b->st);
+  bfree(b);
+  delete b;
+  return RQ_OK;
+}
+
+int rq_batch_info(const rq_batch* b, int32_t* n_mols, int64_t* interior_voxels) {
+  if (!b) return RQ_EINVAL;
+  if (n_mols) *n_mols = b->nm;
+  if (interior_voxels) {
+    int64_t v = 0;
+    for (int m = 0; m < b->nm; ++m) {
+      const int64_t n = b->lev[0][m].S - 2;
+      v += n * n * n;
+    }
+    *interior_voxels = v;
+  }
+  return RQ_OK;
+}
+
+int rq_batch_set_tiles(rq_batch* b, int32_t bx, int32_t by, const int32_t* ich) {
+  if (!b) return bfail(nullptr, RQ_EINVAL, "null batch");
+  if (bx < 1 || by < 1 || bx * by > 256 || (bx * by) % 64) return bfail(b, RQ_EINVAL, "tile %dx%d invalid", bx, by);
+  b->bx = bx;
+  b->by = by;
+  std::vector<int> ichv;
+  if (ich) ichv.assign(ich, ich + b->nm);
+  for (int l = 0; l < b->L; ++l)
+    for (auto& d : b->lev[l]) d.ich = 0;
+  layout(b, ich ? &ichv : nullptr);
+  drop_bgraphs(b);
+  return upload_descriptors(b);
+}
+
+int rq_batch_set_graph(rq_batch* b, int32_t on) {
+  if (!b) return bfail(nullptr, RQ_EINVAL, "null batch");
+  b->use_graph = on != 0;
+  return RQ_OK;
+}
+
+int rq_batch_initialize(rq_batch* b, int32_t bootstrap) {
+  if (!b) return bfail(nullptr, RQ_EINVAL, "null batch");
+  hipLaunchKernelGGL(kb_init, dim3(std::min((b->nnodes + 255) / 256, 65535)), dim3(256), 0, b->st, b->mol_d,
+                     LD(b, 0), b->nm, b->nnodes, b->atoms_d, b->U, b->lab, b->K);
+  BLAUNCH(b);
+  launch_fill(b->st, b->P, b->len[0], 0.0f);
+  if (int rc = b_normalize(b)) return rc;
+  for (int i = 0; i < bootstrap; ++i) {
+    bool did = false;
+    if (int rc = b_vcycle(b, false, &did)) return rc;
+  }
+  b->initialized = true;
+  return RQ_OK;
+}
+
+int rq_batch_step(rq_batch* b, int64_t first, int32_t n, const rq_schedule* sc) {
+  if (!b) return bfail(nullptr, RQ_EINVAL, "null batch");
+  if (!b->initialized) return bfail(b, RQ_ESTATE, "rq_batch_step before rq_batch_initialize");
+  rq_schedule d;
+  rq_default_schedule(&d);
+  if (!sc) sc = &d;
+  if (sc->poisson_interval < 1 || sc->norm_interval < 1) return bfail(b, RQ_EINVAL, "bad schedule");
+  const bool graphable = b->use_graph && !b->timing && sc->norm_interval % sc->poisson_interval == 0;
+  int64_t s = first + 1;
+  const int64_t last = first + n;
+  while (s <= last) {
+    const int64_t c0 = s - 1;
+    if (graphable && c0 % sc->norm_interval == 0 && c0 + sc->norm_interval <= last) {
+      if (int rc = b_graph_chunk(b, c0, sc)) return rc;
+      s = c0 + sc->norm_interval + 1;
+      continue;
+    }
+    if (int rc = b_step(b, s, sc)) return rc;
+    ++s;
+  }
+  return RQ_OK;
+}
+
+int rq_batch_sync(rq_batch* b) {
+  if (!b) return bfail(nullptr, RQ_EINVAL, "null batch");
+  BCHK(b, hipStreamSynchronize(b->st));
+  return RQ_OK;
+}
+
+int rq_batch_energy(rq_batch* b, double* out3) {
+  if (!b || !out3) return bfail(b, RQ_EINVAL, "null argument");
+  double* per = b->red + b->ndom;
+  launch_fill(b->st, reinterpret_cast<float*>(per), 6 * (size_t)b->ndom, 0.0f);
+  {
+    BTimed t(b, F_ENERGY);
+    const int nt = b->ntiles[0];
+    hipLaunchKernelGGL(kb_energy, dim3(grid_for(nt)), dim3(b->bx, b->by), 0, b->st, LD(b, 0), b->mol_d, b->nm, nt,
+                       b->U, b->K, b->P, b->lab, per);
+  }
+  BLAUNCH(b);
+  std::vector<double> hv(3 * (size_t)b->ndom);
+  BCHK(b, hipMemcpyAsync(hv.data(), per, sizeof(double) * hv.size(), hipMemcpyDeviceToHost, b->st));
+  BCHK(b, hipStreamSynchronize(b->st));
+  for (int m = 0; m < b->nm; ++m) {
+    double e[3] = {0, 0, 0};
+    for (int d = 0; d < b->mol[m].n_atoms; ++d)
+      for (int q = 0; q < 3; ++q) e[q] += hv[3 * (size_t)(b->mol[m].dom0 + d) + q];
+    for (int q = 0; q < 3; ++q) out3[3 * m + q] = e[q];
+  }
+  return RQ_OK;
+}
+
+int rq_batch_read_field(rq_batch* b, int32_t m, int32_t field, void* host, size_t bytes) {
+  if (!b || !host) return bfail(b, RQ_EINVAL, "null argument");
+  if (m < 0 || m >= b->nm) return bfail(b, RQ_ERANGE, "molecule %d outside [0, %d)", m, b->nm);
+  const BLev& d = b->lev[0][m];
+  const size_t n3 = (size_t)d.S * d.S * d.S;
+  if (bytes != n3 * 4) return bfail(b, RQ_EINVAL, "molecule %d needs %zu bytes", m, n3 * 4);
+  const float* src = field == RQ_FIELD_U ? b->U : field == RQ_FIELD_K ? b->K : field == RQ_FIELD_P ? b->P : nullptr;
+  if (field == RQ_FIELD_LABEL) {
+    std::vector<uint16_t> tmp(n3);
+    BCHK(b, hipMemcpyAsync(tmp.data(), b->lab + d.off, n3 * 2, hipMemcpyDeviceToHost, b->st));
+    BCHK(b, hipStreamSynchronize(b->st));
+    int32_t* o = (int32_t*)host;
+    for (size_t i = 0; i < n3; ++i) o[i] = tmp[i] & kLabelMask;
+    return RQ_OK;
+  }
+  if (!src) return bfail(b, RQ_EINVAL, "field %d not available in batches (U, K, P, label)", field);
+  BCHK(b, hipMemcpyAsync(host, src + d.off, n3 * 4, hipMemcpyDeviceToHost, b->st));
+  BCHK(b, hipStreamSynchronize(b->st));
+  return RQ_OK;
+}
+
+int rq_batch_timing_enable(rq_batch* b, int32_t on) {
+  if (!b) return bfail(nullptr, RQ_EINVAL, "null batch");
+  resolve(b);
+  b->timing = on != 0;
+  for (int f = 0; f < F_COUNT; ++f) {
+    b->fam_ms[f] = 0;
+    b->fam_n[f] = 0;
+  }
+  return RQ_OK;
+}
+
+int rq_batch_timing_read(rq_batch* b, double* ms, int64_t* launches, int32_t n) {
+  if (!b) return bfail(nullptr, RQ_EINVAL, "null batch");
+  resolve(b);
+  for (int f = 0; f < n && f < F_COUNT; ++f) {
+    if (ms) ms[f] = b->fam_ms[f];
+    if (launches) launches[f] = b->fam_n[f];
+  }
+  return RQ_OK;
+}
+
+}  // extern "C"

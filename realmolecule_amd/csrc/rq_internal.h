@@ -14,6 +14,8 @@ constexpr uint16_t kRcFlag = 0x8000u;
 constexpr uint16_t kLabelMask = 0x7fffu;
 constexpr int kMaxDomains = 4096;       // LDS bins of the per-domain reductions
 constexpr int kMaxCoarseInterior = 25;  // exact FP64 solve held in LDS (m^3 doubles)
+constexpr int kMaxCoarsePP = 19;        // ping-pong variant: 2 m^3 + m^2 doubles of LDS
+constexpr int kFuseMaxS = 65;           // one-pass residual+restriction / prolongation+sweep up to 65^3 nodes
 
 // Launch shape of the 2.5D stencil kernels: (bx, by) tile in (k, j), marching
 // ich planes along i.
@@ -30,7 +32,7 @@ struct Atom {
 // its own algorithmic bytes.
 enum Family {
   F_ITP = 0, F_SMOOTH, F_SMOOTH_C, F_RESTRICT, F_RESTRICT_C, F_PROLONG, F_PROLONG_C, F_COARSE, F_NORM,
-  F_BOUNDARY, F_ENERGY, F_FORCE, F_COUNT
+  F_BOUNDARY, F_ENERGY, F_FORCE, F_SMOOTH_ITP, F_COUNT
 };
 
 // ---- launchers (rq_kernels.hip) ----
@@ -49,7 +51,18 @@ void launch_residual_fine(hipStream_t st, const Tile& t, int S, const float* P, 
 void launch_residual_coarse(hipStream_t st, const Tile& t, int S, const float* E, const float* g, float* r);
 void launch_restrict4(hipStream_t st, int Sf, const float* r, float* gc, float sign);
 void launch_prolong_correct(hipStream_t st, const Tile& t, int Sf, float* x, const float* ec, float scale);
-void launch_coarse_exact(hipStream_t st, int Sc, const float* g, float* e);
+// Smat (m x m DST matrix) and rden (m^3 reciprocal eigenvalue sums) may be
+// null, which selects the line-transform variant that computes them itself.
+void launch_coarse_exact(hipStream_t st, int Sc, const float* g, float* e, const double* Smat, const double* rden);
+// fused multigrid passes (bit-identical to the unfused sequences)
+void launch_smooth2_from_zero(hipStream_t st, const Tile& t, int S, const float* g, float* e_out, float a, float c);
+void launch_residual_restrict(hipStream_t st, bool fine, int Sf, const float* x, const float* src, float alpha,
+                              float beta, float* gc, float sign);
+void launch_prolong_smooth(hipStream_t st, const Tile& t, bool fine, int Sf, const float* x, const float* e,
+                           float* x_out, const float* src, float a, float c, float alpha, float beta);
+void launch_smooth_itp(hipStream_t st, const Tile& t, int S, const float* P, float* Pn, const float* U, float* Un,
+                       const float* K, const uint16_t* lab, float a, float c, float alpha, float beta, float half_dv,
+                       float dtv);
 void launch_zero(hipStream_t st, float* p, size_t n);
 void launch_norm_sums(hipStream_t st, const Tile& t, int S, float h3, const float* U, const uint16_t* lab,
                       int n_dom, double* norms);

@@ -16,12 +16,11 @@
 #include <cmath>
 
 #include "rq_internal.h"
+#include "rq_stencil.h"
 
 namespace rq {
 
 namespace {
-
-__device__ __forceinline__ int nint() { return 0; }
 
 inline dim3 stencil_grid(int n, const Tile& t) {
   return dim3((n + t.bx - 1) / t.bx, (n + t.by - 1) / t.by, (n + t.ich - 1) / t.ich);
@@ -32,10 +31,21 @@ struct March {
   int k, j, i0, i1;
 };
 
+// Tile of this workgroup.  Workgroups are dispatched round-robin over the 8
+// XCDs (linear id mod 8), each with its own L2; the linear id is remapped so
+// that every XCD sweeps one contiguous range of tiles (neighbouring j-rows
+// and i-chunks, whose halos overlap, then share an L2).
 __device__ __forceinline__ bool march_coords(int n, int ich, March& m) {
-  m.k = 1 + blockIdx.x * blockDim.x + threadIdx.x;
-  m.j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
-  m.i0 = 1 + blockIdx.z * ich;
+  const int gx = gridDim.x, gy = gridDim.y;
+  const int nblk = gx * gy * gridDim.z;
+  const int b = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+  const int per = nblk >> 3, rem = nblk & 7;
+  const int xcd = b & 7, local = b >> 3;
+  const int t = (xcd < rem) ? xcd * (per + 1) + local : rem * (per + 1) + (xcd - rem) * per + local;
+  const int tx = t % gx, ty = (t / gx) % gy, tz = t / (gx * gy);
+  m.k = 1 + tx * blockDim.x + threadIdx.x;
+  m.j = 1 + ty * blockDim.y + threadIdx.y;
+  m.i0 = 1 + tz * ich;
   m.i1 = min(m.i0 + ich, n + 1);
   return m.k <= n && m.j <= n;
 }
@@ -180,37 +190,9 @@ __global__ void k_rc_flags(int S, float h, const Atom* __restrict__ atoms, int n
 __global__ __launch_bounds__(256) void k_itp(int S, int ich, const float* __restrict__ U, float* __restrict__ Un,
                                              const float* __restrict__ K, const float* __restrict__ P,
                                              const uint16_t* __restrict__ lab, float half_dv, float dtv) {
-  const int n = S - 2;
   March m;
-  if (!march_coords(n, ich, m)) return;
-  const int S2 = S * S;
-  int id = m.i0 * S2 + m.j * S + m.k;
-  float um = U[id - S2], uc = U[id];
-  uint16_t lm = lab[id - S2], lc = lab[id];
-  for (int i = m.i0; i < m.i1; ++i, id += S2) {
-    const float up = U[id + S2];
-    const uint16_t lp = lab[id + S2];
-    const float ujp = U[id + S], ujm = U[id - S], ukp = U[id + 1], ukm = U[id - 1];
-    const uint16_t ljp = lab[id + S], ljm = lab[id - S], lkp = lab[id + 1], lkm = lab[id - 1];
-    float out;
-    if (lc & kRcFlag) {
-      out = 0.0f;
-    } else {
-      const float t0 = (lp == lc) ? up : uc;
-      const float t1 = (lm == lc) ? um : uc;
-      const float t2 = (ljp == lc) ? ujp : uc;
-      const float t3 = (ljm == lc) ? ujm : uc;
-      const float t4 = (lkp == lc) ? ukp : uc;
-      const float t5 = (lkm == lc) ? ukm : uc;
-      const float lap = t0 + t1 + t2 + t3 + t4 + t5 - 6.0f * uc;
-      out = uc + half_dv * lap + dtv * (K[id] - 2.0f * P[id]) * uc;
-    }
-    Un[id] = out;
-    um = uc;
-    uc = up;
-    lm = lc;
-    lc = lp;
-  }
+  if (!march_coords(S - 2, ich, m)) return;
+  itp_col(S, m.k, m.j, m.i0, m.i1, U, Un, K, P, lab, half_dv, dtv);
 }
 
 // --------------------------------------------------------------------------
@@ -222,28 +204,9 @@ template <bool FINE>
 __global__ __launch_bounds__(256) void k_smooth(int S, int ich, const float* __restrict__ P, float* __restrict__ Pn,
                                                 const float* __restrict__ src, float a, float c, float alpha,
                                                 float beta) {
-  const int n = S - 2;
   March m;
-  if (!march_coords(n, ich, m)) return;
-  const int S2 = S * S;
-  int id = m.i0 * S2 + m.j * S + m.k;
-  float pm = P[id - S2], pc = P[id];
-  for (int i = m.i0; i < m.i1; ++i, id += S2) {
-    const float pp = P[id + S2];
-    const float s6 = pp + pm + P[id + S] + P[id - S] + P[id + 1] + P[id - 1];
-    float rhs;
-    if (FINE) {
-      const float u = src[id];
-      rhs = (alpha * (u * u)) * beta;
-    } else {
-      rhs = src[id];
-    }
-    float v = (s6 + rhs) / c;
-    if (a != 0.0f) v = a * pc + v;
-    Pn[id] = v;
-    pm = pc;
-    pc = pp;
-  }
+  if (!march_coords(S - 2, ich, m)) return;
+  smooth_col<FINE>(S, m.k, m.j, m.i0, m.i1, P, Pn, src, a, c, alpha, beta);
 }
 
 // r = rhs - (6 x - sum6 x) on the interior.
@@ -251,111 +214,81 @@ template <bool FINE>
 __global__ __launch_bounds__(256) void k_residual(int S, int ich, const float* __restrict__ x,
                                                   const float* __restrict__ src, float alpha, float beta,
                                                   float* __restrict__ r) {
-  const int n = S - 2;
   March m;
-  if (!march_coords(n, ich, m)) return;
-  const int S2 = S * S;
-  int id = m.i0 * S2 + m.j * S + m.k;
-  float xm = x[id - S2], xc = x[id];
-  for (int i = m.i0; i < m.i1; ++i, id += S2) {
-    const float xp = x[id + S2];
-    const float s6 = xp + xm + x[id + S] + x[id - S] + x[id + 1] + x[id - 1];
-    float rhs;
-    if (FINE) {
-      const float u = src[id];
-      rhs = (alpha * (u * u)) * beta;
-    } else {
-      rhs = src[id];
-    }
-    r[id] = rhs - (6.0f * xc - s6);
-    xm = xc;
-    xc = xp;
-  }
+  if (!march_coords(S - 2, ich, m)) return;
+  residual_col<FINE>(S, m.k, m.j, m.i0, m.i1, x, src, alpha, beta, r);
 }
 
 // Coarse rhs = sign * 4 * FW(r) at coarse interior nodes; FW = 27-point full
 // weighting (8,4,2,1)/64 summed centre, faces, edges, corners in fixed order.
 __global__ __launch_bounds__(256) void k_restrict4(int Sf, const float* __restrict__ r, float* __restrict__ gc,
                                                    float sign) {
-  const int NC = (Sf - 1) / 2;
-  const int Sc = NC + 1;
-  const int nc = NC - 1;
+  const int nc = (Sf - 1) / 2 - 1;
   const int tid = blockIdx.x * blockDim.x + threadIdx.x;
   if (tid >= nc * nc * nc) return;
-  const int K2 = tid % nc + 1;
-  const int J = (tid / nc) % nc + 1;
-  const int I = tid / (nc * nc) + 1;
-  const int S2 = Sf * Sf;
-  const int c = (2 * I) * S2 + (2 * J) * Sf + 2 * K2;
-  const int di = S2, dj = Sf, dk = 1;
-  float fs = r[c + di];
-  fs = fs + r[c - di];
-  fs = fs + r[c + dj];
-  fs = fs + r[c - dj];
-  fs = fs + r[c + dk];
-  fs = fs + r[c - dk];
-  float es = r[c + di + dj];
-  es = es + r[c + di - dj];
-  es = es + r[c - di + dj];
-  es = es + r[c - di - dj];
-  es = es + r[c + di + dk];
-  es = es + r[c + di - dk];
-  es = es + r[c - di + dk];
-  es = es + r[c - di - dk];
-  es = es + r[c + dj + dk];
-  es = es + r[c + dj - dk];
-  es = es + r[c - dj + dk];
-  es = es + r[c - dj - dk];
-  float cs = r[c + di + dj + dk];
-  cs = cs + r[c + di + dj - dk];
-  cs = cs + r[c + di - dj + dk];
-  cs = cs + r[c + di - dj - dk];
-  cs = cs + r[c - di + dj + dk];
-  cs = cs + r[c - di + dj - dk];
-  cs = cs + r[c - di - dj + dk];
-  cs = cs + r[c - di - dj - dk];
-  const float val = 8.0f * r[c] + 4.0f * fs + 2.0f * es + cs;
-  float out = val * 0.0625f;
-  if (sign < 0.0f) out = -out;
-  gc[I * Sc * Sc + J * Sc + K2] = out;
+  restrict4_pt(Sf, tid / (nc * nc) + 1, (tid / nc) % nc + 1, tid % nc + 1, r, gc, sign);
 }
 
 // Trilinear prolongation (k axis first, then j, then i) and correction
 // x += scale * P e on the fine interior.
-__device__ __forceinline__ float lerp_k(const float* __restrict__ e, int base, int k) {
-  const int kc = k >> 1;
-  if ((k & 1) == 0) return e[base + kc];
-  return 0.5f * (e[base + kc] + e[base + kc + 1]);
+
+
+// ---- fused multigrid passes (bit-identical to the unfused sequences) ----
+
+// Two smoother sweeps from a zero initial guess: e1 = g / c, then
+// e2 = a e1 + (sum6 e1 + g) / c, with e1 formed on the fly (g is zero on faces).
+__global__ __launch_bounds__(256) void k_smooth2_from_zero(int S, int ich, const float* __restrict__ g,
+                                                           float* __restrict__ e_out, float a, float c) {
+  March m;
+  if (!march_coords(S - 2, ich, m)) return;
+  smooth2z_col(S, m.k, m.j, m.i0, m.i1, g, e_out, a, c);
+}
+
+
+// Coarse rhs = 4 FW(r) with the 27 fine residuals formed on the fly (same
+// sums and order as k_residual + k_restrict4), no fine residual array.
+template <bool FINE>
+__global__ __launch_bounds__(256) void k_residual_restrict(int Sf, const float* __restrict__ x,
+                                                           const float* __restrict__ src, float alpha, float beta,
+                                                           float* __restrict__ gc, float sign) {
+  const int nc = (Sf - 1) / 2 - 1;
+  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= nc * nc * nc) return;
+  residual_restrict_pt<FINE>(Sf, tid / (nc * nc) + 1, (tid / nc) % nc + 1, tid % nc + 1, x, src, alpha, beta, gc,
+                             sign);
+}
+
+// Prolongation + correction followed by one smoother sweep:
+// x_out = a xc + (sum6 xc + rhs) / c with xc = x + P e on interior nodes
+// (faces keep x).  Same values as k_prolong_correct then k_smooth.
+template <bool FINE>
+__global__ __launch_bounds__(256) void k_prolong_smooth(int Sf, int ich, const float* __restrict__ x,
+                                                        const float* __restrict__ e, float* __restrict__ x_out,
+                                                        const float* __restrict__ src, float a, float c, float alpha,
+                                                        float beta) {
+  March m;
+  if (!march_coords(Sf - 2, ich, m)) return;
+  prolong_smooth_col<FINE>(Sf, m.k, m.j, m.i0, m.i1, x, e, x_out, src, a, c, alpha, beta);
+}
+
+// Last fine smoother sweep fused with the ITP update that consumes it:
+// P' = a P + (sum6 P + (alpha U^2) beta) / c, then U' = ITP(U, K, P', label)
+// with P' taken at the centre (the only place the update reads P).
+__global__ __launch_bounds__(256) void k_smooth_itp(int S, int ich, const float* __restrict__ P,
+                                                    float* __restrict__ Pn, const float* __restrict__ U,
+                                                    float* __restrict__ Un, const float* __restrict__ K,
+                                                    const uint16_t* __restrict__ lab, float a, float c, float alpha,
+                                                    float beta, float half_dv, float dtv) {
+  March m;
+  if (!march_coords(S - 2, ich, m)) return;
+  smooth_itp_col(S, m.k, m.j, m.i0, m.i1, P, Pn, U, Un, K, lab, a, c, alpha, beta, half_dv, dtv);
 }
 
 __global__ __launch_bounds__(256) void k_prolong_correct(int Sf, int ich, float* __restrict__ x,
                                                          const float* __restrict__ e, float scale) {
-  const int n = Sf - 2;
   March m;
-  if (!march_coords(n, ich, m)) return;
-  const int Sc = (Sf - 1) / 2 + 1;
-  const int Sc2 = Sc * Sc;
-  const int S2 = Sf * Sf;
-  const int jc = m.j >> 1;
-  const bool jodd = (m.j & 1) != 0;
-  for (int i = m.i0; i < m.i1; ++i) {
-    const int ic = i >> 1;
-    float ejk0;
-    {
-      const float a = lerp_k(e, ic * Sc2 + jc * Sc, m.k);
-      ejk0 = jodd ? 0.5f * (a + lerp_k(e, ic * Sc2 + (jc + 1) * Sc, m.k)) : a;
-    }
-    float corr;
-    if ((i & 1) == 0) {
-      corr = ejk0;
-    } else {
-      const float a = lerp_k(e, (ic + 1) * Sc2 + jc * Sc, m.k);
-      const float ejk1 = jodd ? 0.5f * (a + lerp_k(e, (ic + 1) * Sc2 + (jc + 1) * Sc, m.k)) : a;
-      corr = 0.5f * (ejk0 + ejk1);
-    }
-    const int id = i * S2 + m.j * Sf + m.k;
-    x[id] = (scale == 1.0f) ? x[id] + corr : x[id] + scale * corr;
-  }
+  if (!march_coords(Sf - 2, ich, m)) return;
+  prolong_correct_col(Sf, m.k, m.j, m.i0, m.i1, x, e, scale);
 }
 
 // Exact coarsest-level solve of 6e - sum6 e = g (zero Dirichlet faces) in
@@ -429,6 +362,78 @@ __global__ __launch_bounds__(256) void k_coarse_exact(int Sc, const float* __res
   }
 }
 
+// Exact coarsest solve with the DST matrix Sm (m x m, orthonormal, symmetric)
+// and the reciprocal eigenvalue sums rden (m^3) precomputed on the host.
+// 1024 threads each produce transform outputs directly (ping-pong LDS
+// buffers A/B), so no thread runs a long dependent chain.  Needs
+// 2 m^3 + m^2 doubles of LDS (m <= 19).
+__device__ __forceinline__ void dst_axis(const double* __restrict__ in, double* __restrict__ out,
+                                         const double* __restrict__ Sm, int m, int axis) {
+  const int m2 = m * m, m3 = m2 * m;
+  for (int t = threadIdx.x; t < m3; t += blockDim.x) {
+    const int a = t / m2, b = (t / m) % m, c = t % m;
+    int p, base, stride;
+    if (axis == 2) {
+      p = c;
+      base = a * m2 + b * m;
+      stride = 1;
+    } else if (axis == 1) {
+      p = b;
+      base = a * m2 + c;
+      stride = m;
+    } else {
+      p = a;
+      base = b * m + c;
+      stride = m2;
+    }
+    const double* row = Sm + p * m;
+    double s0 = 0.0, s1 = 0.0;
+    int q = 0;
+    for (; q + 1 < m; q += 2) {
+      s0 += row[q] * in[base + q * stride];
+      s1 += row[q + 1] * in[base + (q + 1) * stride];
+    }
+    if (q < m) s0 += row[q] * in[base + q * stride];
+    out[t] = s0 + s1;
+  }
+}
+
+__global__ __launch_bounds__(1024) void k_coarse_exact_pp(int Sc, const float* __restrict__ g, float* __restrict__ e,
+                                                          const double* __restrict__ Smat,
+                                                          const double* __restrict__ rden) {
+  extern __shared__ double lds[];
+  const int m = Sc - 2;
+  const int m3 = m * m * m;
+  double* Sm = lds;
+  double* A = Sm + m * m;
+  double* B = A + m3;
+  const int Sc2 = Sc * Sc;
+  for (int t = threadIdx.x; t < m * m; t += blockDim.x) Sm[t] = Smat[t];
+  for (int t = threadIdx.x; t < m3; t += blockDim.x) {
+    const int a = t / (m * m), b = (t / m) % m, c = t % m;
+    A[t] = (double)g[(a + 1) * Sc2 + (b + 1) * Sc + (c + 1)];
+  }
+  __syncthreads();
+  dst_axis(A, B, Sm, m, 2);
+  __syncthreads();
+  dst_axis(B, A, Sm, m, 1);
+  __syncthreads();
+  dst_axis(A, B, Sm, m, 0);
+  __syncthreads();
+  for (int t = threadIdx.x; t < m3; t += blockDim.x) B[t] *= rden[t];
+  __syncthreads();
+  dst_axis(B, A, Sm, m, 2);
+  __syncthreads();
+  dst_axis(A, B, Sm, m, 1);
+  __syncthreads();
+  dst_axis(B, A, Sm, m, 0);
+  __syncthreads();
+  for (int t = threadIdx.x; t < m3; t += blockDim.x) {
+    const int a = t / (m * m), b = (t / m) % m, c = t % m;
+    e[(a + 1) * Sc2 + (b + 1) * Sc + (c + 1)] = (float)A[t];
+  }
+}
+
 __global__ void k_fill(float* __restrict__ p, size_t n, float v) {
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t < n) p[t] = v;
@@ -471,27 +476,8 @@ __global__ __launch_bounds__(256) void k_norm_sums(int S, int ich, float h3, con
   const int tid = threadIdx.y * blockDim.x + threadIdx.x;
   for (int t = tid; t < n_dom; t += nthr) bins[t] = 0.0;
   __syncthreads();
-  const int n = S - 2;
   March m;
-  if (march_coords(n, ich, m)) {
-    const int S2 = S * S;
-    int id = m.i0 * S2 + m.j * S + m.k;
-    int cur = -1;
-    double acc = 0.0;
-    for (int i = m.i0; i < m.i1; ++i, id += S2) {
-      int L = lab[id] & kLabelMask;
-      L = L < n_dom ? L : n_dom - 1;
-      const float v = U[id];
-      const float term = (v * v) * h3;
-      if (L != cur) {
-        if (cur >= 0) atomicAdd(&bins[cur], acc);
-        cur = L;
-        acc = 0.0;
-      }
-      acc += (double)term;
-    }
-    if (cur >= 0) atomicAdd(&bins[cur], acc);
-  }
+  if (march_coords(S - 2, ich, m)) norm_sums_col(S, m.k, m.j, m.i0, m.i1, h3, U, lab, n_dom, bins);
   __syncthreads();
   for (int t = tid; t < n_dom; t += nthr) {
     const double b = bins[t];
@@ -512,15 +498,9 @@ __global__ __launch_bounds__(256) void k_normalize(int S, int ich, float* __rest
     scale[t] = (z > 0.0 && nv > 1e-20) ? (float)sqrt(z / nv) : 1.0f;
   }
   __syncthreads();
-  const int n = S - 2;
   March m;
-  if (!march_coords(n, ich, m)) return;
-  const int S2 = S * S;
-  int id = m.i0 * S2 + m.j * S + m.k;
-  for (int i = m.i0; i < m.i1; ++i, id += S2) {
-    const int L = lab[id] & kLabelMask;
-    if (L < n_dom) U[id] = U[id] * scale[L];
-  }
+  if (!march_coords(S - 2, ich, m)) return;
+  normalize_col(S, m.k, m.j, m.i0, m.i1, U, lab, scale, n_dom);
 }
 
 // --------------------------------------------------------------------------
@@ -947,8 +927,19 @@ void launch_prolong_correct(hipStream_t st, const Tile& t, int Sf, float* x, con
   hipLaunchKernelGGL(k_prolong_correct, stencil_grid(Sf - 2, t), dim3(t.bx, t.by), 0, st, Sf, t.ich, x, ec, scale);
 }
 
-void launch_coarse_exact(hipStream_t st, int Sc, const float* g, float* e) {
+void launch_coarse_exact(hipStream_t st, int Sc, const float* g, float* e, const double* Smat, const double* rden) {
   const int m = Sc - 2;
+  if (Smat && rden && m <= kMaxCoarsePP) {
+    const size_t lds = sizeof(double) * ((size_t)m * m + 2 * (size_t)m * m * m);
+    static bool attr_pp = false;
+    if (!attr_pp) {
+      (void)hipFuncSetAttribute((const void*)k_coarse_exact_pp, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                160 * 1024);
+      attr_pp = true;
+    }
+    hipLaunchKernelGGL(k_coarse_exact_pp, dim3(1), dim3(1024), lds, st, Sc, g, e, Smat, rden);
+    return;
+  }
   const size_t lds = sizeof(double) * ((size_t)m * m + m + (size_t)m * m * m);
   static bool attr_set = false;
   if (!attr_set) {
@@ -956,6 +947,39 @@ void launch_coarse_exact(hipStream_t st, int Sc, const float* g, float* e) {
     attr_set = true;
   }
   hipLaunchKernelGGL(k_coarse_exact, dim3(1), dim3(256), lds, st, Sc, g, e);
+}
+
+void launch_smooth2_from_zero(hipStream_t st, const Tile& t, int S, const float* g, float* e_out, float a, float c) {
+  hipLaunchKernelGGL(k_smooth2_from_zero, stencil_grid(S - 2, t), dim3(t.bx, t.by), 0, st, S, t.ich, g, e_out, a, c);
+}
+
+void launch_residual_restrict(hipStream_t st, bool fine, int Sf, const float* x, const float* src, float alpha,
+                              float beta, float* gc, float sign) {
+  const int nc = (Sf - 1) / 2 - 1;
+  const size_t tot = (size_t)nc * nc * nc;
+  if (fine)
+    hipLaunchKernelGGL(k_residual_restrict<true>, dim3(nblk(tot, 256)), dim3(256), 0, st, Sf, x, src, alpha, beta, gc,
+                       sign);
+  else
+    hipLaunchKernelGGL(k_residual_restrict<false>, dim3(nblk(tot, 256)), dim3(256), 0, st, Sf, x, src, 1.0f, 1.0f,
+                       gc, sign);
+}
+
+void launch_prolong_smooth(hipStream_t st, const Tile& t, bool fine, int Sf, const float* x, const float* e,
+                           float* x_out, const float* src, float a, float c, float alpha, float beta) {
+  if (fine)
+    hipLaunchKernelGGL(k_prolong_smooth<true>, stencil_grid(Sf - 2, t), dim3(t.bx, t.by), 0, st, Sf, t.ich, x, e,
+                       x_out, src, a, c, alpha, beta);
+  else
+    hipLaunchKernelGGL(k_prolong_smooth<false>, stencil_grid(Sf - 2, t), dim3(t.bx, t.by), 0, st, Sf, t.ich, x, e,
+                       x_out, src, a, c, 1.0f, 1.0f);
+}
+
+void launch_smooth_itp(hipStream_t st, const Tile& t, int S, const float* P, float* Pn, const float* U, float* Un,
+                       const float* K, const uint16_t* lab, float a, float c, float alpha, float beta, float half_dv,
+                       float dtv) {
+  hipLaunchKernelGGL(k_smooth_itp, stencil_grid(S - 2, t), dim3(t.bx, t.by), 0, st, S, t.ich, P, Pn, U, Un, K, lab, a,
+                     c, alpha, beta, half_dv, dtv);
 }
 
 void launch_zero(hipStream_t st, float* p, size_t n) { launch_fill(st, p, n, 0.0f); }

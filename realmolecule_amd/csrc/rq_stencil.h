@@ -1,0 +1,432 @@
+// Per-column stencil bodies shared by the single-grid kernels
+// (rq_kernels.hip) and the batched persistent kernels (rq_batch.hip), so both
+// evaluate identical FP32 expressions in identical order.
+//
+// A "column" is one (k, j) interior position and the i-planes [i0, i1) a
+// thread marches through; S is the node count per axis of that grid.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "rq_internal.h"
+
+namespace rq {
+
+// label-Neumann ITP update (cloud/solver.py:61-90; rc rule via label bit 15)
+__device__ __forceinline__ void itp_col(int S, int k, int j, int i0, int i1, const float* __restrict__ U,
+                                        float* __restrict__ Un, const float* __restrict__ K,
+                                        const float* __restrict__ P, const uint16_t* __restrict__ lab, float half_dv,
+                                        float dtv) {
+  const int S2 = S * S;
+  int id = i0 * S2 + j * S + k;
+  float um = U[id - S2], uc = U[id];
+  uint16_t lm = lab[id - S2], lc = lab[id];
+  for (int i = i0; i < i1; ++i, id += S2) {
+    const float up = U[id + S2];
+    const uint16_t lp = lab[id + S2];
+    const float ujp = U[id + S], ujm = U[id - S], ukp = U[id + 1], ukm = U[id - 1];
+    const uint16_t ljp = lab[id + S], ljm = lab[id - S], lkp = lab[id + 1], lkm = lab[id - 1];
+    float out;
+    if (lc & kRcFlag) {
+      out = 0.0f;
+    } else {
+      const float t0 = (lp == lc) ? up : uc;
+      const float t1 = (lm == lc) ? um : uc;
+      const float t2 = (ljp == lc) ? ujp : uc;
+      const float t3 = (ljm == lc) ? ujm : uc;
+      const float t4 = (lkp == lc) ? ukp : uc;
+      const float t5 = (lkm == lc) ? ukm : uc;
+      const float lap = t0 + t1 + t2 + t3 + t4 + t5 - 6.0f * uc;
+      out = uc + half_dv * lap + dtv * (K[id] - 2.0f * P[id]) * uc;
+    }
+    Un[id] = out;
+    um = uc;
+    uc = up;
+    lm = lc;
+    lc = lp;
+  }
+}
+
+// damped Jacobi P' = a P + (sum6 P + rhs) / c; FINE: rhs = (alpha U^2) beta, else rhs = g
+template <bool FINE>
+__device__ __forceinline__ void smooth_col(int S, int k, int j, int i0, int i1, const float* __restrict__ P,
+                                           float* __restrict__ Pn, const float* __restrict__ src, float a, float c,
+                                           float alpha, float beta) {
+  const int S2 = S * S;
+  int id = i0 * S2 + j * S + k;
+  float pm = P[id - S2], pc = P[id];
+  for (int i = i0; i < i1; ++i, id += S2) {
+    const float pp = P[id + S2];
+    const float s6 = pp + pm + P[id + S] + P[id - S] + P[id + 1] + P[id - 1];
+    float rhs;
+    if (FINE) {
+      const float u = src[id];
+      rhs = (alpha * (u * u)) * beta;
+    } else {
+      rhs = src[id];
+    }
+    float v = (s6 + rhs) / c;
+    if (a != 0.0f) v = a * pc + v;
+    Pn[id] = v;
+    pm = pc;
+    pc = pp;
+  }
+}
+
+// two sweeps from a zero guess: e1 = g / c, e2 = a e1 + (sum6 e1 + g) / c
+__device__ __forceinline__ void smooth2z_col(int S, int k, int j, int i0, int i1, const float* __restrict__ g,
+                                             float* __restrict__ e_out, float a, float c) {
+  const int S2 = S * S;
+  for (int i = i0; i < i1; ++i) {
+    const int id = i * S2 + j * S + k;
+    const float e_ip = g[id + S2] / c, e_im = g[id - S2] / c, e_jp = g[id + S] / c, e_jm = g[id - S] / c;
+    const float e_kp = g[id + 1] / c, e_km = g[id - 1] / c;
+    const float gc = g[id];
+    const float s6 = e_ip + e_im + e_jp + e_jm + e_kp + e_km;
+    float v = (s6 + gc) / c;
+    if (a != 0.0f) v = a * (gc / c) + v;
+    e_out[id] = v;
+  }
+}
+
+__device__ __forceinline__ float lerp_k(const float* __restrict__ e, int base, int k) {
+  const int kc = k >> 1;
+  if ((k & 1) == 0) return e[base + kc];
+  return 0.5f * (e[base + kc] + e[base + kc + 1]);
+}
+
+// trilinear prolongation of coarse e at fine interior node (i, j, k): k axis, then j, then i
+__device__ __forceinline__ float prolong_at(const float* __restrict__ e, int Sc, int i, int j, int k) {
+  const int Sc2 = Sc * Sc;
+  const int ic = i >> 1, jc = j >> 1;
+  const bool jodd = (j & 1) != 0;
+  float a = lerp_k(e, ic * Sc2 + jc * Sc, k);
+  const float ejk0 = jodd ? 0.5f * (a + lerp_k(e, ic * Sc2 + (jc + 1) * Sc, k)) : a;
+  if ((i & 1) == 0) return ejk0;
+  a = lerp_k(e, (ic + 1) * Sc2 + jc * Sc, k);
+  const float ejk1 = jodd ? 0.5f * (a + lerp_k(e, (ic + 1) * Sc2 + (jc + 1) * Sc, k)) : a;
+  return 0.5f * (ejk0 + ejk1);
+}
+
+// x_out = smoother applied to xc = x + P e (interior; faces keep x)
+template <bool FINE>
+__device__ __forceinline__ void prolong_smooth_col(int Sf, int k, int j, int i0, int i1, const float* __restrict__ x,
+                                                   const float* __restrict__ e, float* __restrict__ x_out,
+                                                   const float* __restrict__ src, float a, float c, float alpha,
+                                                   float beta) {
+  const int n = Sf - 2;
+  const int Sc = (Sf - 1) / 2 + 1;
+  const int S2 = Sf * Sf;
+  auto xc = [&](int ii, int jj, int kk) -> float {
+    const int id = ii * S2 + jj * Sf + kk;
+    if (ii < 1 || ii > n || jj < 1 || jj > n || kk < 1 || kk > n) return x[id];
+    return x[id] + prolong_at(e, Sc, ii, jj, kk);
+  };
+  for (int i = i0; i < i1; ++i) {
+    const int id = i * S2 + j * Sf + k;
+    const float s6 = xc(i + 1, j, k) + xc(i - 1, j, k) + xc(i, j + 1, k) + xc(i, j - 1, k) + xc(i, j, k + 1) +
+                     xc(i, j, k - 1);
+    float rhs;
+    if (FINE) {
+      const float u = src[id];
+      rhs = (alpha * (u * u)) * beta;
+    } else {
+      rhs = src[id];
+    }
+    float v = (s6 + rhs) / c;
+    if (a != 0.0f) v = a * xc(i, j, k) + v;
+    x_out[id] = v;
+  }
+}
+
+// last fine sweep fused with the ITP update reading the new P at the centre
+__device__ __forceinline__ void smooth_itp_col(int S, int k, int j, int i0, int i1, const float* __restrict__ P,
+                                               float* __restrict__ Pn, const float* __restrict__ U,
+                                               float* __restrict__ Un, const float* __restrict__ K,
+                                               const uint16_t* __restrict__ lab, float a, float c, float alpha,
+                                               float beta, float half_dv, float dtv) {
+  const int S2 = S * S;
+  int id = i0 * S2 + j * S + k;
+  float pm = P[id - S2], pc = P[id];
+  float um = U[id - S2], uc = U[id];
+  uint16_t lm = lab[id - S2], lc = lab[id];
+  for (int i = i0; i < i1; ++i, id += S2) {
+    const float pp = P[id + S2];
+    const float s6 = pp + pm + P[id + S] + P[id - S] + P[id + 1] + P[id - 1];
+    const float rhs = (alpha * (uc * uc)) * beta;
+    float v = (s6 + rhs) / c;
+    if (a != 0.0f) v = a * pc + v;
+    Pn[id] = v;
+    const float up = U[id + S2];
+    const uint16_t lp = lab[id + S2];
+    float out;
+    if (lc & kRcFlag) {
+      out = 0.0f;
+    } else {
+      const float ujp = U[id + S], ujm = U[id - S], ukp = U[id + 1], ukm = U[id - 1];
+      const uint16_t ljp = lab[id + S], ljm = lab[id - S], lkp = lab[id + 1], lkm = lab[id - 1];
+      const float t0 = (lp == lc) ? up : uc;
+      const float t1 = (lm == lc) ? um : uc;
+      const float t2 = (ljp == lc) ? ujp : uc;
+      const float t3 = (ljm == lc) ? ujm : uc;
+      const float t4 = (lkp == lc) ? ukp : uc;
+      const float t5 = (lkm == lc) ? ukm : uc;
+      const float lap = t0 + t1 + t2 + t3 + t4 + t5 - 6.0f * uc;
+      out = uc + half_dv * lap + dtv * (K[id] - 2.0f * v) * uc;
+    }
+    Un[id] = out;
+    pm = pc;
+    pc = pp;
+    um = uc;
+    uc = up;
+    lm = lc;
+    lc = lp;
+  }
+}
+
+// residual r = rhs - (6 x - sum6 x) at one fine node
+template <bool FINE>
+__device__ __forceinline__ float residual_at(const float* __restrict__ x, const float* __restrict__ src, int id, int S2,
+                                             int S, float alpha, float beta) {
+  const float s6 = x[id + S2] + x[id - S2] + x[id + S] + x[id - S] + x[id + 1] + x[id - 1];
+  float rhs;
+  if (FINE) {
+    const float u = src[id];
+    rhs = (alpha * (u * u)) * beta;
+  } else {
+    rhs = src[id];
+  }
+  return rhs - (6.0f * x[id] - s6);
+}
+
+// coarse rhs at coarse interior node (I, J, K2) = sign * 4 FW(r), 27 residuals on the fly
+template <bool FINE>
+__device__ __forceinline__ void residual_restrict_pt(int Sf, int I, int J, int K2, const float* __restrict__ x,
+                                                     const float* __restrict__ src, float alpha, float beta,
+                                                     float* __restrict__ gc, float sign) {
+  const int Sc = (Sf - 1) / 2 + 1;
+  const int S2 = Sf * Sf;
+  const int c = (2 * I) * S2 + (2 * J) * Sf + 2 * K2;
+  const int di = S2, dj = Sf, dk = 1;
+#define RQ_R(off) residual_at<FINE>(x, src, c + (off), S2, Sf, alpha, beta)
+  float fs = RQ_R(di);
+  fs = fs + RQ_R(-di);
+  fs = fs + RQ_R(dj);
+  fs = fs + RQ_R(-dj);
+  fs = fs + RQ_R(dk);
+  fs = fs + RQ_R(-dk);
+  float es = RQ_R(di + dj);
+  es = es + RQ_R(di - dj);
+  es = es + RQ_R(-di + dj);
+  es = es + RQ_R(-di - dj);
+  es = es + RQ_R(di + dk);
+  es = es + RQ_R(di - dk);
+  es = es + RQ_R(-di + dk);
+  es = es + RQ_R(-di - dk);
+  es = es + RQ_R(dj + dk);
+  es = es + RQ_R(dj - dk);
+  es = es + RQ_R(-dj + dk);
+  es = es + RQ_R(-dj - dk);
+  float cs = RQ_R(di + dj + dk);
+  cs = cs + RQ_R(di + dj - dk);
+  cs = cs + RQ_R(di - dj + dk);
+  cs = cs + RQ_R(di - dj - dk);
+  cs = cs + RQ_R(-di + dj + dk);
+  cs = cs + RQ_R(-di + dj - dk);
+  cs = cs + RQ_R(-di - dj + dk);
+  cs = cs + RQ_R(-di - dj - dk);
+  const float val = 8.0f * RQ_R(0) + 4.0f * fs + 2.0f * es + cs;
+#undef RQ_R
+  float out = val * 0.0625f;
+  if (sign < 0.0f) out = -out;
+  gc[I * Sc * Sc + J * Sc + K2] = out;
+}
+
+// residual r = rhs - (6 x - sum6 x) along one column (written to r)
+template <bool FINE>
+__device__ __forceinline__ void residual_col(int S, int k, int j, int i0, int i1, const float* __restrict__ x,
+                                             const float* __restrict__ src, float alpha, float beta,
+                                             float* __restrict__ r) {
+  const int S2 = S * S;
+  int id = i0 * S2 + j * S + k;
+  float xm = x[id - S2], xc = x[id];
+  for (int i = i0; i < i1; ++i, id += S2) {
+    const float xp = x[id + S2];
+    const float s6 = xp + xm + x[id + S] + x[id - S] + x[id + 1] + x[id - 1];
+    float rhs;
+    if (FINE) {
+      const float u = src[id];
+      rhs = (alpha * (u * u)) * beta;
+    } else {
+      rhs = src[id];
+    }
+    r[id] = rhs - (6.0f * xc - s6);
+    xm = xc;
+    xc = xp;
+  }
+}
+
+// coarse rhs = sign * 4 FW(r) at coarse interior node (I, J, K2) from a stored residual
+__device__ __forceinline__ void restrict4_pt(int Sf, int I, int J, int K2, const float* __restrict__ r,
+                                             float* __restrict__ gc, float sign) {
+  const int Sc = (Sf - 1) / 2 + 1;
+  const int S2 = Sf * Sf;
+  const int c = (2 * I) * S2 + (2 * J) * Sf + 2 * K2;
+  const int di = S2, dj = Sf, dk = 1;
+  float fs = r[c + di];
+  fs = fs + r[c - di];
+  fs = fs + r[c + dj];
+  fs = fs + r[c - dj];
+  fs = fs + r[c + dk];
+  fs = fs + r[c - dk];
+  float es = r[c + di + dj];
+  es = es + r[c + di - dj];
+  es = es + r[c - di + dj];
+  es = es + r[c - di - dj];
+  es = es + r[c + di + dk];
+  es = es + r[c + di - dk];
+  es = es + r[c - di + dk];
+  es = es + r[c - di - dk];
+  es = es + r[c + dj + dk];
+  es = es + r[c + dj - dk];
+  es = es + r[c - dj + dk];
+  es = es + r[c - dj - dk];
+  float cs = r[c + di + dj + dk];
+  cs = cs + r[c + di + dj - dk];
+  cs = cs + r[c + di - dj + dk];
+  cs = cs + r[c + di - dj - dk];
+  cs = cs + r[c - di + dj + dk];
+  cs = cs + r[c - di + dj - dk];
+  cs = cs + r[c - di - dj + dk];
+  cs = cs + r[c - di - dj - dk];
+  const float val = 8.0f * r[c] + 4.0f * fs + 2.0f * es + cs;
+  float out = val * 0.0625f;
+  if (sign < 0.0f) out = -out;
+  gc[I * Sc * Sc + J * Sc + K2] = out;
+}
+
+// x += scale * P e along one fine column
+__device__ __forceinline__ void prolong_correct_col(int Sf, int k, int j, int i0, int i1, float* __restrict__ x,
+                                                    const float* __restrict__ e, float scale) {
+  const int Sc = (Sf - 1) / 2 + 1;
+  const int S2 = Sf * Sf;
+  for (int i = i0; i < i1; ++i) {
+    const float corr = prolong_at(e, Sc, i, j, k);
+    const int id = i * S2 + j * Sf + k;
+    x[id] = (scale == 1.0f) ? x[id] + corr : x[id] + scale * corr;
+  }
+}
+
+// run-length FP64 accumulation of (U*U)*h3 per clipped label into LDS bins
+__device__ __forceinline__ void norm_sums_col(int S, int k, int j, int i0, int i1, float h3,
+                                              const float* __restrict__ U, const uint16_t* __restrict__ lab,
+                                              int n_dom, double* bins) {
+  const int S2 = S * S;
+  int id = i0 * S2 + j * S + k;
+  int cur = -1;
+  double acc = 0.0;
+  for (int i = i0; i < i1; ++i, id += S2) {
+    int L = lab[id] & kLabelMask;
+    L = L < n_dom ? L : n_dom - 1;
+    const float v = U[id];
+    const float term = (v * v) * h3;
+    if (L != cur) {
+      if (cur >= 0) atomicAdd(&bins[cur], acc);
+      cur = L;
+      acc = 0.0;
+    }
+    acc += (double)term;
+  }
+  if (cur >= 0) atomicAdd(&bins[cur], acc);
+}
+
+// U *= scale[label] for labels inside [0, n_dom)
+__device__ __forceinline__ void normalize_col(int S, int k, int j, int i0, int i1, float* __restrict__ U,
+                                              const uint16_t* __restrict__ lab, const float* scale, int n_dom) {
+  const int S2 = S * S;
+  int id = i0 * S2 + j * S + k;
+  for (int i = i0; i < i1; ++i, id += S2) {
+    const int L = lab[id] & kLabelMask;
+    if (L < n_dom) U[id] = U[id] * scale[L];
+  }
+}
+
+// S1 energy terms of one column into per-domain LDS bins (3 per domain)
+__device__ __forceinline__ void energy_s1_col(int S, int k, int j, int i0, int i1, float h, float h3,
+                                              const float* __restrict__ U, const float* __restrict__ K,
+                                              const float* __restrict__ P, const uint16_t* __restrict__ lab,
+                                              int n_dom, double* bins) {
+  const int S2 = S * S;
+  int cur = -1;
+  double aT = 0.0, aK = 0.0, aE = 0.0;
+  for (int i = i0; i < i1; ++i) {
+    const int id = i * S2 + j * S + k;
+    int L = lab[id] & kLabelMask;
+    L = L < n_dom ? L : n_dom - 1;
+    const float u = U[id];
+    const float rho = u * u;
+    const float da = ((lab[id + S2] & kLabelMask) == L) ? U[id + S2] - u : 0.0f;
+    const float db = ((lab[id + S] & kLabelMask) == L) ? U[id + S] - u : 0.0f;
+    const float dc = ((lab[id + 1] & kLabelMask) == L) ? U[id + 1] - u : 0.0f;
+    const float T = 0.5f * (da * da + db * db + dc * dc) * h;
+    const float VeK = -K[id] * rho * h3;
+    const float Vee = P[id] * rho * h3;
+    if (L != cur) {
+      if (cur >= 0) {
+        atomicAdd(&bins[cur * 3 + 0], aT);
+        atomicAdd(&bins[cur * 3 + 1], aK);
+        atomicAdd(&bins[cur * 3 + 2], aE);
+      }
+      cur = L;
+      aT = aK = aE = 0.0;
+    }
+    aT += (double)T;
+    aK += (double)VeK;
+    aE += (double)Vee;
+  }
+  if (cur >= 0) {
+    atomicAdd(&bins[cur * 3 + 0], aT);
+    atomicAdd(&bins[cur * 3 + 1], aK);
+    atomicAdd(&bins[cur * 3 + 2], aE);
+  }
+}
+
+// trial init (S1) at one node: K, trial argmax label, best trial
+__device__ __forceinline__ void init_trial_s1_node(int S, int id, float h, float h2, const Atom* __restrict__ atoms,
+                                                   int n_atoms, float* __restrict__ U, uint16_t* __restrict__ lab,
+                                                   float* __restrict__ K) {
+  const int S2 = S * S;
+  const int i = id / S2, j = (id / S) % S, k = id % S;
+  const int NN = S - 1;
+  if (i < 1 || i >= NN || j < 1 || j >= NN || k < 1 || k >= NN) {
+    U[id] = 0.0f;
+    lab[id] = 0;
+    K[id] = 0.0f;
+    return;
+  }
+  const float xi = (float)i * h, yj = (float)j * h, zk = (float)k * h;
+  const float inv_sqrt_pi = 0.5641895835f;
+  float bU = 0.0f, Kv = 0.0f;
+  int bestN = 0;
+  for (int n = 0; n < n_atoms; ++n) {
+    const float Za = atoms[n].Z;
+    if (!(Za > 0.0f)) continue;
+    const float rc = atoms[n].rc;
+    const float dx = xi - atoms[n].i * h;
+    const float dy = yj - atoms[n].j * h;
+    const float dz = zk - atoms[n].k * h;
+    const float r2 = dx * dx + dy * dy + dz * dz;
+    const float r = (rc > 0.0f) ? fmaxf(sqrtf(r2 + 0.04f * h2), rc) : sqrtf(r2 + h2);
+    Kv += Za / r;
+    const float ut = Za * Za * inv_sqrt_pi * expf(-Za * r);
+    if (ut > bU) {
+      bU = ut;
+      bestN = n;
+    }
+  }
+  K[id] = Kv;
+  lab[id] = (uint16_t)bestN;
+  U[id] = bU;
+}
+
+}  // namespace rq

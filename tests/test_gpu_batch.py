@@ -1,0 +1,50 @@
+"""Batched solver: one persistent launch per pass over several molecules with
+different grid sizes gives the same fields and energies as solving each
+molecule alone with the single-grid solver (identical column bodies)."""
+import numpy as np
+import pytest
+
+from realmolecule_amd.batch import Batch
+from realmolecule_amd.builders import c3_conformers
+from tests.helpers import three_atoms, make_solver, ulp_diff
+
+pytestmark = pytest.mark.gpu
+
+
+def _mols():
+    out = []
+    for NN, h in ((16, 0.3), (24, 0.3), (32, 0.25)):
+        out.append((three_atoms(NN), NN, NN * h))
+    return out
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_batch_matches_single_solves(graph):
+    mols = _mols()
+    b = Batch(mols, mg_levels=3, mg_nu=2, graph=graph)
+    b.initialize(bootstrap=4)
+    b.step(0, 45, poisson_interval=1, norm_interval=20)
+    E = b.energies()
+    for m, (rows, NN, screen) in enumerate(mols):
+        s = make_solver(rows, NN, screen, poisson="multigrid", mg_levels=3, boundary="off", bootstrap_poisson=4)
+        s.initialize()
+        s.step(0, 45, poisson_interval=1, norm_interval=20)
+        s._compute_energy()
+        assert ulp_diff(b.read(m, "U"), s.U.get()) <= 1, m
+        assert np.array_equal(b.read(m, "label"), s.label.get())
+        assert np.allclose(E[m], [s.E_T, s.E_eK, s.E_ee], rtol=1e-9), m
+        s.close()
+    b.close()
+
+
+def test_c3_batch_runs_and_stays_finite():
+    """The 64-conformer scan (grids 64^3..160^3) in one batch: a few chunks of
+    graph-replayed steps, finite energies of the expected sign."""
+    mols = [(r, NN, sc) for (r, NN, sc, _name) in c3_conformers()]
+    b = Batch(mols, mg_levels=4, mg_nu=2, graph=True)
+    b.initialize(bootstrap=5)
+    b.step(0, 40)
+    E = b.energies()
+    assert np.isfinite(E).all()
+    assert (E[:, 1] < 0).all() and (E[:, 0] > 0).all()
+    b.close()
