@@ -41,6 +41,11 @@ BYTES_PER_VOXEL = {
 }
 BYTES_FUSED_SMALL = {"restrict": 8.5, "prolong": 12.5}  # one-pass variants on grids <= 65^3 nodes
 
+# Fine-level kernel of each family, as rocprofv3 names it, for the HBM
+# traffic counters (FETCH_SIZE / WRITE_SIZE per dispatch).
+PMC_KERNEL = {"smooth": "k_smooth<true>", "smooth_itp": "k_smooth_itp", "itp": "k_itp("}
+COPY_MIB = 512  # plain device copy of known size that calibrates the counters
+
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
@@ -89,6 +94,166 @@ def cpu_baseline(w, steps):
     return n3 * steps / dt / 1e9, dt
 
 
+def forces_bench(w, steps, bootstrap, n_batch, fd_delta, log_fn):
+    """Forces/sec: plain path = central differences of full frozen-label solves
+    (6N+1 solves per force vector, each molecule in turn) against the adjoint
+    (one forward solve + one reverse sweep per force vector), single and with
+    ``n_batch`` geometries advanced concurrently on their own HIP streams.
+    Both paths evaluate the same E(R) (same steps, same labels), so their
+    energies match exactly."""
+    import threading
+    from realmolecule_amd.adjoint import FrozenSolver
+    NN, screen, rows = w["NN"], w["screen"], w["rows"]
+    kw = dict(bootstrap=bootstrap, checkpoint=64, poisson_interval=1, norm_interval=20, poisson="multigrid",
+              mg_levels=w["mg_levels"])
+    fz = FrozenSolver(rows, NN, screen, steps, **kw)
+    pos = rows[:, :3] * fz.h
+    n = len(rows)
+    fz.energy(pos)  # warm
+    fz.s.sync()
+    t0 = time.perf_counter()
+    E_adj, g = fz.energy_and_gradient(pos)
+    t_adj = time.perf_counter() - t0
+    log_fn(f"[forces] adjoint: {t_adj:.3f} s for {3 * n} components")
+    t0 = time.perf_counter()
+    E0 = fz.energy(pos)
+    F_fd = np.zeros((n, 3))
+    for a in range(n):
+        for c in range(3):
+            p = pos.copy()
+            p[a, c] += fd_delta
+            ep = fz.energy(p)
+            p[a, c] -= 2 * fd_delta
+            em = fz.energy(p)
+            F_fd[a, c] = -(ep - em) / (2 * fd_delta)
+    t_fd = time.perf_counter() - t0
+    log_fn(f"[forces] central differences: {t_fd:.2f} s for {6 * n + 1} solves")
+    err = float(np.abs(-g - F_fd).max())
+    fz.close()
+    # batched: n_batch geometries (seeded 0.05 bohr jitter) on concurrent handles
+    rng = np.random.default_rng(7)
+    solvers = [FrozenSolver(rows, NN, screen, steps, **kw) for _ in range(n_batch)]
+    geos = [pos + rng.normal(0, 0.05, pos.shape) for _ in range(n_batch)]
+    for s_, p_ in zip(solvers, geos):
+        s_.energy(p_)
+    out = [None] * n_batch
+
+    def run(i):
+        out[i] = solvers[i].energy_and_gradient(geos[i])
+
+    t0 = time.perf_counter()
+    th = [threading.Thread(target=run, args=(i,)) for i in range(n_batch)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    t_batch = time.perf_counter() - t0
+    for s_ in solvers:
+        s_.close()
+    log_fn(f"[forces] batched adjoint: {n_batch} force vectors in {t_batch:.3f} s")
+    return {
+        "workload": f"{w['desc']}; frozen-label solve of {bootstrap} bootstrap V-cycles + {steps} steps",
+        "n_atoms": n,
+        "energy_Ha": E0,
+        "energy_rel_diff_adjoint_vs_plain": abs(E_adj - E0) / abs(E0),
+        "plain_fd_forces_per_s": 1.0 / t_fd,
+        "plain_fd_solves": 6 * n + 1,
+        "adjoint_forces_per_s": 1.0 / t_adj,
+        "batched_adjoint_forces_per_s": n_batch / t_batch,
+        "batch": n_batch,
+        "speedup_adjoint": t_fd / t_adj,
+        "speedup_batched_adjoint": (n_batch / t_batch) * t_fd,
+        "max_abs_force_diff_vs_fd_Ha_per_bohr": err,
+        "fd_delta_bohr": fd_delta,
+    }
+
+
+def pmc_child(kind, w, steps):
+    """Body of one counter pass: either a few eager solver steps or a plain
+    device copy of COPY_MIB MiB, three times (the calibration kernel)."""
+    import torch
+    if kind == "copy":
+        n = COPY_MIB * (1 << 20) // 4
+        a = torch.ones(n, device="cuda", dtype=torch.float32)
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        return
+    s = make_solver(w, graph=False)
+    s.initialize()
+    s.step(0, steps, poisson_interval=1)
+    s.sync()
+    s.close()
+
+
+def _pmc_pass(counter, kind, workload_name, outdir, timeout_s=120):
+    """Run rocprofv3 --pmc <counter> over a child bench.py process; returns
+    {kernel name: [KiB per dispatch]} or None if the pass did not complete."""
+    import csv
+    import subprocess
+    d = os.path.join(outdir, f"{kind}_{counter}")
+    cmd = ["rocprofv3", "--pmc", counter, "--output-format", "csv", "-d", d, "-o", "run", "--",
+           sys.executable, os.path.join(ROOT, "bench.py"), "--pmc-child", kind, "--workload", workload_name]
+    env = dict(os.environ, TMPDIR="/tmp")
+    try:
+        r = subprocess.run(cmd, cwd="/tmp", env=env, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                           timeout=timeout_s)
+    except (OSError, subprocess.TimeoutExpired) as e:
+        log(f"[pmc] {counter} {kind}: {type(e).__name__}")
+        return None
+    path = None
+    for root, _, files in os.walk(d):
+        for f in files:
+            if f.endswith("counter_collection.csv"):
+                path = os.path.join(root, f)
+    if path is None:
+        log(f"[pmc] {counter} {kind}: no counter file (rc {r.returncode}): {r.stdout[-400:]!r}")
+        return None
+    out = {}
+    with open(path) as f:
+        for row in csv.DictReader(f):
+            out.setdefault(row["Kernel_Name"], []).append(float(row["Counter_Value"]))
+    return out
+
+
+def pmc_traffic(workload_name, kernel_key):
+    """HBM bytes per dispatch of the fine-level kernel ``kernel_key``: FETCH_SIZE
+    and WRITE_SIZE (KiB per dispatch, one rocprofv3 pass each), each scaled by
+    the bytes a plain copy of known size moves over what the same counter
+    reads for it."""
+    import shutil
+    import tempfile
+    if shutil.which("rocprofv3") is None:
+        return None, "rocprofv3 not on PATH"
+    outdir = tempfile.mkdtemp(prefix="rq_pmc_", dir="/tmp")
+    res = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        for kind in ("solver", "copy"):
+            log(f"[pmc] {counter} over {kind}")
+            r = _pmc_pass(counter, kind, workload_name, outdir)
+            if r is None:
+                return None, f"{counter} pass over {kind} failed"
+            res[(counter, kind)] = r
+    shutil.rmtree(outdir, ignore_errors=True)
+    copy_bytes = COPY_MIB * (1 << 20)
+    info = {"copy_MiB": COPY_MIB}
+    total = 0.0
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        copies = [v for k, vs in res[(counter, "copy")].items() if "copy" in k.lower() for v in vs
+                  if v * 1024 > 0.25 * copy_bytes]
+        kern = [v for k, vs in res[(counter, "solver")].items() if kernel_key in k for v in vs]
+        if not copies or not kern:
+            return None, f"{counter}: no copy or no {kernel_key} dispatches"
+        scale = copy_bytes / (np.median(copies) * 1024)
+        per = float(np.mean(kern)) * 1024 * scale
+        info[counter] = {"KiB_per_dispatch": round(float(np.mean(kern)), 1), "dispatches": len(kern),
+                         "copy_KiB": round(float(np.median(copies)), 1), "scale": round(float(scale), 4),
+                         "bytes": per}
+        total += per
+    return total, info
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -98,11 +263,20 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--timing-steps", type=int, default=100)
+    ap.add_argument("--no-forces", action="store_true")
+    ap.add_argument("--force-steps", type=int, default=100)
+    ap.add_argument("--force-batch", type=int, default=4)
+    ap.add_argument("--no-pmc", action="store_true", help="skip the rocprofv3 FETCH/WRITE_SIZE passes")
+    ap.add_argument("--pmc-child", choices=("solver", "copy"), help=argparse.SUPPRESS)
+    ap.add_argument("--pmc-steps", type=int, default=4)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the solver is single-GPU; "
               "multi-GPU sharding is design-only)", file=sys.stderr)
         sys.exit(2)
+    if args.pmc_child:
+        pmc_child(args.pmc_child, workload(args.workload), args.pmc_steps)
+        return
 
     import ctypes as C
     from realmolecule_amd import _lib as L
@@ -173,10 +347,26 @@ def main():
         "unit": "GB/s",
         "frac": round(fine[dom]["achieved_GBs"] / HBM_PEAK_GBS, 4),
         "traffic": None,
+        "traffic_unit": "bytes per launch",
         "alg_bytes_per_voxel": bpv[dom],
+        "alg_bytes_per_launch": bpv[dom] * n3,
         "step_alg_bytes_fine": step_bytes,
         "step_frac": round(step_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
     }
+
+    s.close()
+    if not args.no_pmc and dom in PMC_KERNEL:
+        log("[bench] HBM traffic of the dominant kernel: rocprofv3 FETCH_SIZE / WRITE_SIZE passes")
+        traffic, info = pmc_traffic(args.workload, PMC_KERNEL[dom])
+        if traffic is not None:
+            roof["traffic"] = round(traffic)
+            roof["traffic_over_alg"] = round(traffic / (bpv[dom] * n3), 4)
+        roof["traffic_pmc"] = info
+
+    forces = None
+    if not args.no_forces:
+        log(f"[bench] forces: adjoint vs central differences, {args.force_steps} steps per solve")
+        forces = forces_bench(w, args.force_steps, 20, args.force_batch, 1e-2, log)
 
     cpu = None
     if not args.no_cpu:
@@ -208,10 +398,10 @@ def main():
         "roofline": roof,
         "kernels": kernels,
         "cpu_baseline": cpu,
+        "forces": forces,
         "energy_Ha": s.E,
     }
     print(json.dumps(out))
-    s.close()
 
 
 if __name__ == "__main__":

@@ -461,14 +461,14 @@ struct BTimed {
   hipEvent_t a = nullptr;
   BTimed(rq_batch* b_, int f) : b(b_), fam(f) {
     if (b->timing) {
-      (void)hipEventCreate(&a);
+      (void)hipEventCreateWithFlags(&a, hipEventDisableSystemFence);
       (void)hipEventRecord(a, b->st);
     }
   }
   ~BTimed() {
     if (b->timing && a) {
       hipEvent_t e = nullptr;
-      (void)hipEventCreate(&e);
+      (void)hipEventCreateWithFlags(&e, hipEventDisableSystemFence);
       (void)hipEventRecord(e, b->st);
       b->ev.push_back({fam, {a, e}});
     }

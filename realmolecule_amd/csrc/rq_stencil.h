@@ -89,22 +89,24 @@ __device__ __forceinline__ void smooth2z_col(int S, int k, int j, int i0, int i1
   }
 }
 
+// Linear interpolation along one axis written without a branch: for an even
+// fine index the second sample is the first one again and 0.5f * (a + a) == a
+// exactly in IEEE arithmetic, so the result equals the two-case rule
+// (copy at even, average at odd) bit for bit while every lane of a wave runs
+// the same instructions.
 __device__ __forceinline__ float lerp_k(const float* __restrict__ e, int base, int k) {
   const int kc = k >> 1;
-  if ((k & 1) == 0) return e[base + kc];
-  return 0.5f * (e[base + kc] + e[base + kc + 1]);
+  return 0.5f * (e[base + kc] + e[base + kc + (k & 1)]);
 }
 
 // trilinear prolongation of coarse e at fine interior node (i, j, k): k axis, then j, then i
 __device__ __forceinline__ float prolong_at(const float* __restrict__ e, int Sc, int i, int j, int k) {
   const int Sc2 = Sc * Sc;
   const int ic = i >> 1, jc = j >> 1;
-  const bool jodd = (j & 1) != 0;
-  float a = lerp_k(e, ic * Sc2 + jc * Sc, k);
-  const float ejk0 = jodd ? 0.5f * (a + lerp_k(e, ic * Sc2 + (jc + 1) * Sc, k)) : a;
-  if ((i & 1) == 0) return ejk0;
-  a = lerp_k(e, (ic + 1) * Sc2 + jc * Sc, k);
-  const float ejk1 = jodd ? 0.5f * (a + lerp_k(e, (ic + 1) * Sc2 + (jc + 1) * Sc, k)) : a;
+  const int jo = (j & 1) * Sc, io = (i & 1) * Sc2;
+  const int b0 = ic * Sc2 + jc * Sc;
+  const float ejk0 = 0.5f * (lerp_k(e, b0, k) + lerp_k(e, b0 + jo, k));
+  const float ejk1 = 0.5f * (lerp_k(e, b0 + io, k) + lerp_k(e, b0 + io + jo, k));
   return 0.5f * (ejk0 + ejk1);
 }
 
