@@ -176,7 +176,10 @@ int rq_set_graph(rq_solver* h, int32_t on);
 /* on != 0 (default): multigrid passes run fused -- two pre-sweeps from the zero
  * coarse guess in one pass, residual+restriction without the residual array,
  * prolongation+correction with the first post-sweep, and the last fine
- * post-sweep with the ITP update.  Results are bit-identical to on == 0. */
+ * post-sweep with the ITP update.  on == 3: in addition the last two levels
+ * of the V-cycle (up to 25^3 nodes) run in one LDS-resident workgroup (one
+ * launch instead of five, but on a single compute unit).  Results are
+ * bit-identical to on == 0. */
 int rq_set_fused(rq_solver* h, int32_t on);
 /* Launch shape of the 2.5D stencil kernels on multigrid level `level` (0 =
  * finest, -1 = all): a (bx, by) workgroup tile in (k, j), bx*by a multiple of
@@ -267,11 +270,12 @@ int rq_batch_timing_read(rq_batch* b, double* ms, int64_t* launches, int32_t n_f
  * smoother sweep, 3 fine residual+restriction, 4 coarse residual+restriction,
  * 5 fine prolongation+correction, 6 coarse prolongation+correction,
  * 7 exact coarsest solve, 8 normalise, 9 boundary rule, 10 energy, 11 forces,
- * 12 last fine smoother sweep fused with the ITP update.  With fused passes
+ * 12 last fine smoother sweep fused with the ITP update, 13 the last two
+ * levels of the V-cycle in one single-workgroup launch.  With fused passes
  * (the default) family 3 is the fused residual+restriction and family 5 the
  * fused prolongation+first post-smoothing sweep.  Events bracket each launch
  * on the handle's stream. */
-#define RQ_TIMING_FAMILIES 13
+#define RQ_TIMING_FAMILIES 14
 int rq_timing_enable(rq_solver* h, int32_t on);
 int rq_timing_read(rq_solver* h, double* ms, int64_t* launches, int32_t n_families);
 

@@ -35,7 +35,7 @@ RQ_ENERGY_S2 = 1
 
 FIELDS = {"U": 0, "U2": 1, "K": 2, "P": 3, "P2": 4, "rho": 5, "label": 6, "bestU": 7, "W": 8}
 FAMILIES = ["itp", "smooth", "smooth_c", "restrict", "restrict_c", "prolong", "prolong_c", "coarse_solve",
-            "normalize", "boundary", "energy", "force", "smooth_itp"]
+            "normalize", "boundary", "energy", "force", "smooth_itp", "bottom"]
 
 
 class RqAtom(C.Structure):

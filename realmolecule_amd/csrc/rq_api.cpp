@@ -9,6 +9,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -86,6 +87,10 @@ struct rq_solver {
   AdjState adj;
   bool use_graph = false;
   bool fused = true;  // fused multigrid passes (bit-identical to the unfused sequence)
+  bool bottom = false;  // last two levels in one LDS-resident workgroup (with fused)
+  // largest level (nodes per axis) that runs the one-pass residual+restriction
+  // and prolongation+sweep; RQ_FUSE_MAX_S overrides it for tuning
+  int fuse_max_s = kFuseMaxS;
   std::vector<GraphEntry> graphs;
   std::string err;
 };
@@ -214,6 +219,15 @@ int vcycle_coarse(rq_solver* s, int l) {
     LAUNCHCHK(s);
     return RQ_OK;
   }
+  if (s->fused && s->bottom && l == s->levels - 2 && s->dst_S && bottom_fits(S)) {
+    {
+      Timed t(s, F_BOTTOM);
+      launch_bottom(s->st, S, s->p.mg_nu1, s->p.mg_nu2, s->p.mg_a, s->p.mg_c, s->g[l], s->e0[l], s->dst_S,
+                    s->dst_rden);
+    }
+    LAUNCHCHK(s);
+    return RQ_OK;
+  }
   const Tile tl = s->ctile[l];
   const float a = s->p.mg_a, c = s->p.mg_c;
   int pre = s->p.mg_nu1;
@@ -235,7 +249,7 @@ int vcycle_coarse(rq_solver* s, int l) {
   }
   // the one-pass residual+restriction and prolongation+sweep trade extra cache
   // traffic for fewer launches: they pay off only on small, latency-bound levels
-  const bool small = s->fused && S <= kFuseMaxS;
+  const bool small = s->fused && S <= s->fuse_max_s;
   {
     Timed t(s, F_RESTRICT_C);
     if (small) {
@@ -280,7 +294,7 @@ int vcycle_fine(rq_solver* s, bool fuse_itp, bool* did_itp) {
     int rc = smooth_fine(s, a, c);
     if (rc) return rc;
   }
-  const bool small = s->fused && s->S <= kFuseMaxS;
+  const bool small = s->fused && s->S <= s->fuse_max_s;
   {
     Timed t(s, F_RESTRICT);
     if (small) {
@@ -813,6 +827,7 @@ int rq_create(const rq_params* pp, const rq_atom* atoms, int32_t n_atoms, rq_sol
 
   rq_solver* s = new rq_solver();
   s->p = p;
+  if (const char* fm = std::getenv("RQ_FUSE_MAX_S")) s->fuse_max_s = std::atoi(fm);
   s->NN = p.NN;
   s->S = p.NN + 1;
   s->S3 = (size_t)s->S * s->S * s->S;
@@ -1119,6 +1134,7 @@ int rq_step(rq_solver* h, int64_t first, int32_t n, const rq_schedule* sc) {
 int rq_set_fused(rq_solver* h, int32_t on) {
   if (!h) return fail(nullptr, RQ_EINVAL, "null handle");
   h->fused = on != 0;
+  h->bottom = on == 3;
   for (auto& ge : h->graphs) {
     (void)hipGraphExecDestroy(ge.exec);
     (void)hipGraphDestroy(ge.graph);

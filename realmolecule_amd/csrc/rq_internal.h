@@ -16,6 +16,7 @@ constexpr int kMaxDomains = 4096;       // LDS bins of the per-domain reductions
 constexpr int kMaxCoarseInterior = 25;  // exact FP64 solve held in LDS (m^3 doubles)
 constexpr int kMaxCoarsePP = 19;        // ping-pong variant: 2 m^3 + m^2 doubles of LDS
 constexpr int kFuseMaxS = 65;           // one-pass residual+restriction / prolongation+sweep up to 65^3 nodes
+constexpr int kBottomMaxS = 25;         // last two V-cycle levels in one LDS-resident workgroup up to 25^3 nodes
 
 // Launch shape of the 2.5D stencil kernels: (bx, by) tile in (k, j), marching
 // ich planes along i.
@@ -32,7 +33,7 @@ struct Atom {
 // its own algorithmic bytes.
 enum Family {
   F_ITP = 0, F_SMOOTH, F_SMOOTH_C, F_RESTRICT, F_RESTRICT_C, F_PROLONG, F_PROLONG_C, F_COARSE, F_NORM,
-  F_BOUNDARY, F_ENERGY, F_FORCE, F_SMOOTH_ITP, F_COUNT
+  F_BOUNDARY, F_ENERGY, F_FORCE, F_SMOOTH_ITP, F_BOTTOM, F_COUNT
 };
 
 // ---- launchers (rq_kernels.hip) ----
@@ -54,6 +55,12 @@ void launch_prolong_correct(hipStream_t st, const Tile& t, int Sf, float* x, con
 // Smat (m x m DST matrix) and rden (m^3 reciprocal eigenvalue sums) may be
 // null, which selects the line-transform variant that computes them itself.
 void launch_coarse_exact(hipStream_t st, int Sc, const float* g, float* e, const double* Smat, const double* rden);
+// Levels L (S nodes per axis) and L+1 of a V-cycle in one workgroup: nu1 / nu2
+// sweeps around the exact DST solve (tables of level L+1); e gets the level-L
+// correction.  bottom_fits(S): S is odd, <= kBottomMaxS and the LDS fits.
+bool bottom_fits(int S);
+void launch_bottom(hipStream_t st, int S, int nu1, int nu2, float a, float c, const float* g, float* e,
+                   const double* Smat, const double* rden);
 // fused multigrid passes (bit-identical to the unfused sequences)
 void launch_smooth2_from_zero(hipStream_t st, const Tile& t, int S, const float* g, float* e_out, float a, float c);
 void launch_residual_restrict(hipStream_t st, bool fine, int Sf, const float* x, const float* src, float alpha,

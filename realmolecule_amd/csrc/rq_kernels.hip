@@ -434,6 +434,155 @@ __global__ __launch_bounds__(1024) void k_coarse_exact_pp(int Sc, const float* _
   }
 }
 
+// The last two V-cycle levels in one workgroup, everything in LDS: on level L
+// (S <= kBottomMaxS nodes per axis) pre-smoothing, residual, 4 FW restriction,
+// the exact FP64 DST solve of level L+1, prolongation + correction and
+// post-smoothing; only the level-L correction is written back.  Each step is
+// the expression of the multi-launch sequence (k_smooth2_from_zero / k_smooth
+// / k_residual / k_restrict4 / k_coarse_exact_pp / k_prolong_correct) in the
+// same order, so the result is bit-identical to it; Jacobi sweeps run in
+// place (new values held in registers across a barrier).
+// LDS: A, B (m^3 f64), Sm (m^2 f64), E, R (S^3 f32), G (Sc^3 f32).
+constexpr int kBotThreads = 1024;
+constexpr int kBotPer = ((kBottomMaxS - 2) * (kBottomMaxS - 2) * (kBottomMaxS - 2) + kBotThreads - 1) / kBotThreads;
+constexpr int kBotGroup = 4;  // nodes whose loads the scheduler may interleave
+
+// Every thread owns kBotPer interior nodes (packed i | j << 8 | k << 16); a
+// thread with fewer real nodes repeats node (1, 1, 1) for the rest and masks
+// their stores, so the per-node loops carry no branches.
+__device__ __forceinline__ int bot_id(int p, int S) {
+  return (p & 0xff) * S * S + ((p >> 8) & 0xff) * S + (p >> 16);
+}
+
+__global__ __launch_bounds__(kBotThreads) void k_bottom(int S, int nu1, int nu2, float a, float c,
+                                                        const float* __restrict__ g, float* __restrict__ e_out,
+                                                        const double* __restrict__ Smat,
+                                                        const double* __restrict__ rden) {
+  extern __shared__ double lds[];
+  const int Sc = (S - 1) / 2 + 1;
+  const int m = Sc - 2, m2 = m * m, m3 = m2 * m;
+  const int S2 = S * S, S3 = S2 * S, Sc2 = Sc * Sc;
+  const int n = S - 2, n2 = n * n, n3 = n2 * n;
+  double* A = lds;
+  double* B = A + m3;
+  double* Sm = B + m3;
+  float* cur = reinterpret_cast<float*>(Sm + m2);  // current iterate (faces 0)
+  float* oth = cur + S3;                            // sweep target / residual (faces 0)
+  float* G = oth + S3;                              // level L+1: rhs, then solution (faces 0)
+  const int tid = threadIdx.x;
+  int pk[kBotPer];
+  float gr[kBotPer];
+  unsigned live = 0;
+#pragma unroll
+  for (int q = 0; q < kBotPer; ++q) {
+    const int p = tid + q * kBotThreads;
+    if (p < n3) {
+      pk[q] = (p / n2 + 1) | (((p / n) % n + 1) << 8) | ((p % n + 1) << 16);
+      live |= 1u << q;
+    } else {
+      pk[q] = 1 | (1 << 8) | (1 << 16);
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < kBotPer; ++q) gr[q] = g[bot_id(pk[q], S)];
+  // reciprocal eigenvalue sums of this thread's transform outputs, loaded early
+  const double rd0 = tid < m3 ? rden[tid] : 0.0;
+  const double rd1 = tid + kBotThreads < m3 ? rden[tid + kBotThreads] : 0.0;
+  for (int t = tid; t < 2 * S3; t += kBotThreads) cur[t] = 0.0f;
+  for (int t = tid; t < Sc * Sc2; t += kBotThreads) G[t] = 0.0f;
+  for (int t = tid; t < m2; t += kBotThreads) Sm[t] = Smat[t];
+  __syncthreads();
+  // pre-smoothing from a zero guess (k_smooth2_from_zero, then k_smooth)
+  int pre = nu1;
+  if (pre >= 2) {
+#pragma unroll
+    for (int q = 0; q < kBotPer; ++q) {
+      const int id = bot_id(pk[q], S);
+      const float e_ip = g[id + S2] / c, e_im = g[id - S2] / c, e_jp = g[id + S] / c, e_jm = g[id - S] / c;
+      const float e_kp = g[id + 1] / c, e_km = g[id - 1] / c;
+      const float gc = gr[q];
+      const float s6 = e_ip + e_im + e_jp + e_jm + e_kp + e_km;
+      float w = (s6 + gc) / c;
+      if (a != 0.0f) w = a * (gc / c) + w;
+      if (live & (1u << q)) cur[id] = w;
+      if (q % kBotGroup == kBotGroup - 1) __builtin_amdgcn_sched_barrier(0);
+    }
+    pre -= 2;
+    __syncthreads();
+  }
+  auto sweep = [&]() {
+#pragma unroll
+    for (int q = 0; q < kBotPer; ++q) {
+      const int id = bot_id(pk[q], S);
+      const float s6 = cur[id + S2] + cur[id - S2] + cur[id + S] + cur[id - S] + cur[id + 1] + cur[id - 1];
+      float w = (s6 + gr[q]) / c;
+      if (a != 0.0f) w = a * cur[id] + w;
+      if (live & (1u << q)) oth[id] = w;
+      if (q % kBotGroup == kBotGroup - 1) __builtin_amdgcn_sched_barrier(0);
+    }
+    float* t = cur;
+    cur = oth;
+    oth = t;
+    __syncthreads();
+  };
+  for (int it = 0; it < pre; ++it) sweep();
+  // residual (k_residual), then 4 FW onto the coarse interior (k_restrict4)
+#pragma unroll
+  for (int q = 0; q < kBotPer; ++q) {
+    const int id = bot_id(pk[q], S);
+    const float s6 = cur[id + S2] + cur[id - S2] + cur[id + S] + cur[id - S] + cur[id + 1] + cur[id - 1];
+    const float r = gr[q] - (6.0f * cur[id] - s6);
+    if (live & (1u << q)) oth[id] = r;
+    if (q % kBotGroup == kBotGroup - 1) __builtin_amdgcn_sched_barrier(0);
+  }
+  __syncthreads();
+  for (int t = tid; t < m3; t += kBotThreads)
+    restrict4_pt(S, t / m2 + 1, (t / m) % m + 1, t % m + 1, oth, G, 1.0f);
+  __syncthreads();
+  // exact solve on level L+1 (the transforms of k_coarse_exact_pp)
+  for (int t = tid; t < m3; t += kBotThreads) {
+    const int a3 = t / m2, b3 = (t / m) % m, c3 = t % m;
+    A[t] = (double)G[(a3 + 1) * Sc2 + (b3 + 1) * Sc + (c3 + 1)];
+  }
+  __syncthreads();
+  dst_axis(A, B, Sm, m, 2);
+  __syncthreads();
+  dst_axis(B, A, Sm, m, 1);
+  __syncthreads();
+  dst_axis(A, B, Sm, m, 0);
+  __syncthreads();
+  if (tid < m3) B[tid] *= rd0;
+  if (tid + kBotThreads < m3) B[tid + kBotThreads] *= rd1;
+  __syncthreads();
+  dst_axis(B, A, Sm, m, 2);
+  __syncthreads();
+  dst_axis(A, B, Sm, m, 1);
+  __syncthreads();
+  dst_axis(B, A, Sm, m, 0);
+  __syncthreads();
+  for (int t = tid; t < m3; t += kBotThreads) {
+    const int a3 = t / m2, b3 = (t / m) % m, c3 = t % m;
+    G[(a3 + 1) * Sc2 + (b3 + 1) * Sc + (c3 + 1)] = (float)A[t];
+  }
+  __syncthreads();
+  // prolongation + correction in place (k_prolong_correct, scale 1; faces keep 0)
+#pragma unroll
+  for (int q = 0; q < kBotPer; ++q) {
+    const int i = pk[q] & 0xff, j = (pk[q] >> 8) & 0xff, k = pk[q] >> 16;
+    const int id = bot_id(pk[q], S);
+    const float x = cur[id] + prolong_at(G, Sc, i, j, k);
+    if (live & (1u << q)) cur[id] = x;
+    if (q % kBotGroup == kBotGroup - 1) __builtin_amdgcn_sched_barrier(0);
+  }
+  __syncthreads();
+  for (int it = 0; it < nu2; ++it) sweep();
+#pragma unroll
+  for (int q = 0; q < kBotPer; ++q) {
+    const int id = bot_id(pk[q], S);
+    if (live & (1u << q)) e_out[id] = cur[id];
+  }
+}
+
 __global__ void k_fill(float* __restrict__ p, size_t n, float v) {
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t < n) p[t] = v;
@@ -947,6 +1096,26 @@ void launch_coarse_exact(hipStream_t st, int Sc, const float* g, float* e, const
     attr_set = true;
   }
   hipLaunchKernelGGL(k_coarse_exact, dim3(1), dim3(256), lds, st, Sc, g, e);
+}
+
+size_t bottom_lds_bytes(int S) {
+  const size_t Sc = (S - 1) / 2 + 1, m = Sc - 2;
+  return sizeof(double) * (2 * m * m * m + m * m) + sizeof(float) * (2 * (size_t)S * S * S + Sc * Sc * Sc);
+}
+
+bool bottom_fits(int S) {
+  return S >= 7 && S <= kBottomMaxS && (S - 1) % 2 == 0 && bottom_lds_bytes(S) <= 160 * 1024;
+}
+
+void launch_bottom(hipStream_t st, int S, int nu1, int nu2, float a, float c, const float* g, float* e,
+                   const double* Smat, const double* rden) {
+  static bool attr = false;
+  if (!attr) {
+    (void)hipFuncSetAttribute((const void*)k_bottom, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    attr = true;
+  }
+  hipLaunchKernelGGL(k_bottom, dim3(1), dim3(kBotThreads), bottom_lds_bytes(S), st, S, nu1, nu2, a, c, g, e, Smat,
+                     rden);
 }
 
 void launch_smooth2_from_zero(hipStream_t st, const Tile& t, int S, const float* g, float* e_out, float a, float c) {

@@ -23,3 +23,24 @@ def test_fused_vcycle_steps_bitexact(nu):
         s.close()
     assert ulp_diff(out[0][0], out[1][0]) == 0
     assert ulp_diff(out[0][1], out[1][1]) == 0
+
+
+@pytest.mark.parametrize("NN,levels,nu", [(96, 4, 2), (48, 3, 1), (64, 4, 3)])
+def test_bottom_levels_bitexact(NN, levels, nu):
+    """The single-workgroup launch of the last two levels (up to 25^3 nodes)
+    against the same cycle with separate launches, fused (mode 1) and
+    unfused (mode 0)."""
+    screen = NN * 0.2
+    rows = three_atoms(NN)
+    out = []
+    for mode in (0, 1, 3):
+        s = make_solver(rows, NN, screen, poisson="multigrid", mg_levels=levels, mg_nu1=nu, mg_nu2=nu,
+                        bootstrap_poisson=2)
+        s._chk(s._lib.rq_set_fused(s._h, mode))
+        s.initialize()
+        s.step(0, 25, poisson_interval=1)
+        out.append((s.U.get(), s.P.get()))
+        s.close()
+    for u, p in out[1:]:
+        assert ulp_diff(out[0][0], u) == 0
+        assert ulp_diff(out[0][1], p) == 0
