@@ -88,9 +88,10 @@ struct rq_solver {
   bool use_graph = false;
   bool fused = true;  // fused multigrid passes (bit-identical to the unfused sequence)
   bool bottom = false;  // last two levels in one LDS-resident workgroup (with fused)
-  // largest level (nodes per axis) that runs the one-pass residual+restriction
-  // and prolongation+sweep; RQ_FUSE_MAX_S overrides it for tuning
-  int fuse_max_s = kFuseMaxS;
+  // largest level (nodes per axis) that runs the one-pass prolongation+sweep;
+  // 0: none -- on gfx950 the separate correction and sweep launches measured
+  // faster on every level of the 192^3 workload.  RQ_FUSE_MAX_S overrides it.
+  int fuse_max_s = 0;
   std::vector<GraphEntry> graphs;
   std::string err;
 };
@@ -170,13 +171,20 @@ struct Timed {
   }
 };
 
+// Built-in launch shape per level, from sweeps on gfx950 (192^3 workload):
+// large levels 64x4 tiles marching 2..8 planes for about 4096 workgroups;
+// mid levels 32x8; levels of <= 64 interior nodes per axis one plane per
+// thread, since there a launch is latency-bound and a march only lengthens
+// each thread's dependent chain.
 Tile tile_for(int S) {
   Tile t;
   const int n = S - 2;
-  t.bx = n >= 48 ? 64 : (n >= 24 ? 32 : 16);
+  t.bx = n >= 128 ? 64 : (n >= 24 ? 32 : 16);
   t.by = 256 / t.bx;
-  // about 4096 workgroups (16 per CU), marching 2..8 planes per thread: the
-  // measured optimum on gfx950 for 192^3 (64x4 tiles, 4-8 planes)
+  if (n <= 64) {
+    t.ich = 1;
+    return t;
+  }
   const int gx = (n + t.bx - 1) / t.bx, gy = (n + t.by - 1) / t.by;
   const int want = 4096;
   int nz = std::max(1, std::min(n, want / std::max(1, gx * gy)));
@@ -247,12 +255,12 @@ int vcycle_coarse(rq_solver* s, int l) {
     }
     std::swap(s->e0[l], s->e1[l]);
   }
-  // the one-pass residual+restriction and prolongation+sweep trade extra cache
-  // traffic for fewer launches: they pay off only on small, latency-bound levels
+  // the one-pass prolongation+sweep trades extra cache traffic for a launch:
+  // it pays off only on small, latency-bound levels
   const bool small = s->fused && S <= s->fuse_max_s;
   {
     Timed t(s, F_RESTRICT_C);
-    if (small) {
+    if (s->fused && S <= kResResMaxS) {
       launch_residual_restrict(s->st, false, S, s->e0[l], s->g[l], 1.0f, 1.0f, s->g[l + 1], 1.0f);
     } else {
       launch_residual_coarse(s->st, tl, S, s->e0[l], s->g[l], s->e1[l]);
@@ -297,7 +305,9 @@ int vcycle_fine(rq_solver* s, bool fuse_itp, bool* did_itp) {
   const bool small = s->fused && s->S <= s->fuse_max_s;
   {
     Timed t(s, F_RESTRICT);
-    if (small) {
+    // the LDS-tiled one-pass kernel re-reads its halos through L1 less
+    // efficiently than two streaming passes on the largest grids
+    if (s->fused && s->S <= kResResMaxS) {
       launch_residual_restrict(s->st, true, s->S, s->P, s->U, s->two_pi, s->h2, s->g[1], 1.0f);
     } else {
       launch_residual_fine(s->st, s->tile, s->S, s->P, s->U, s->two_pi, s->h2, s->P2);

@@ -15,7 +15,8 @@ constexpr uint16_t kLabelMask = 0x7fffu;
 constexpr int kMaxDomains = 4096;       // LDS bins of the per-domain reductions
 constexpr int kMaxCoarseInterior = 25;  // exact FP64 solve held in LDS (m^3 doubles)
 constexpr int kMaxCoarsePP = 19;        // ping-pong variant: 2 m^3 + m^2 doubles of LDS
-constexpr int kFuseMaxS = 65;           // one-pass residual+restriction / prolongation+sweep up to 65^3 nodes
+constexpr int kFuseMaxS = 65;           // one-pass prolongation+sweep up to 65^3 nodes
+constexpr int kResResMaxS = 129;        // LDS-tiled residual+restriction up to 129^3 nodes
 constexpr int kBottomMaxS = 25;         // last two V-cycle levels in one LDS-resident workgroup up to 25^3 nodes
 
 // Launch shape of the 2.5D stencil kernels: (bx, by) tile in (k, j), marching

@@ -245,17 +245,57 @@ __global__ __launch_bounds__(256) void k_smooth2_from_zero(int S, int ich, const
 }
 
 
-// Coarse rhs = 4 FW(r) with the 27 fine residuals formed on the fly (same
-// sums and order as k_residual + k_restrict4), no fine residual array.
-template <bool FINE>
-__global__ __launch_bounds__(256) void k_residual_restrict(int Sf, const float* __restrict__ x,
-                                                           const float* __restrict__ src, float alpha, float beta,
-                                                           float* __restrict__ gc, float sign) {
-  const int nc = (Sf - 1) / 2 - 1;
-  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
-  if (tid >= nc * nc * nc) return;
-  residual_restrict_pt<FINE>(Sf, tid / (nc * nc) + 1, (tid / nc) % nc + 1, tid % nc + 1, x, src, alpha, beta, gc,
-                             sign);
+// Linear workgroup index remapped so each XCD (linear id mod 8) takes one
+// contiguous range of tiles (the remap of march_coords).
+__device__ __forceinline__ int xcd_block() {
+  const int nblk = gridDim.x * gridDim.y * gridDim.z;
+  const int b = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+  const int per = nblk >> 3, rem = nblk & 7;
+  const int xcd = b & 7, local = b >> 3;
+  return (xcd < rem) ? xcd * (per + 1) + local : rem * (per + 1) + (xcd - rem) * per + local;
+}
+
+// Residual + 4 FW restriction through LDS.  A workgroup owns a CK x CJ x CI
+// tile of coarse interior nodes: it first evaluates the residual once at each
+// of the (2CK+1)(2CJ+1)(2CI+1) fine nodes the tile's restriction stencils
+// cover (residual_at, the expression of k_residual), stores them in LDS, then
+// every thread forms one coarse value with restrict27 (the sums of
+// k_restrict4).  Same values as k_residual + k_restrict4, without the fine
+// residual array in HBM and without recomputing each residual 27 times.
+template <bool FINE, int CK, int CJ, int CI>
+__global__ __launch_bounds__(CK * CJ * CI) void k_resres_tiled(int Sf, const float* __restrict__ x,
+                                                                const float* __restrict__ src, float alpha,
+                                                                float beta, float* __restrict__ gc, float sign) {
+  constexpr int FK = 2 * CK + 1, FJ = 2 * CJ + 1, FI = 2 * CI + 1;
+  constexpr int NT = CK * CJ * CI, NF = FK * FJ * FI;
+  __shared__ float r[NF];
+  const int nc = (Sf - 1) / 2 - 1, n = Sf - 2;
+  const int Sc = nc + 2, S2 = Sf * Sf;
+  const int gx = (nc + CK - 1) / CK, gy = (nc + CJ - 1) / CJ;
+  const int t = xcd_block();
+  const int K0 = 1 + (t % gx) * CK, J0 = 1 + ((t / gx) % gy) * CJ, I0 = 1 + (t / (gx * gy)) * CI;
+  const int fk0 = 2 * K0 - 1, fj0 = 2 * J0 - 1, fi0 = 2 * I0 - 1;
+  const int tid = threadIdx.x;
+  // all residual loads of a thread are independent and (indices clamped to
+  // the interior) unconditional, so the unrolled loop has them in flight
+  // together; nodes past the interior store 0 (no valid coarse node reads them)
+#pragma unroll
+  for (int it = 0; it < (NF + NT - 1) / NT; ++it) {
+    const int q = tid + it * NT;
+    const int qq = q < NF ? q : NF - 1;
+    const int li = qq / (FJ * FK), lj = (qq / FK) % FJ, lk = qq % FK;
+    const int fi = fi0 + li, fj = fj0 + lj, fk = fk0 + lk;
+    const bool in = fi <= n && fj <= n && fk <= n;
+    const float v = residual_at<FINE>(x, src, min(fi, n) * S2 + min(fj, n) * Sf + min(fk, n), S2, Sf, alpha, beta);
+    if (q < NF) r[q] = in ? v : 0.0f;
+  }
+  __syncthreads();
+  const int tk = tid % CK, tj = (tid / CK) % CJ, ti = tid / (CK * CJ);
+  const int K = K0 + tk, J = J0 + tj, I = I0 + ti;
+  if (K > nc || J > nc || I > nc) return;
+  float out = restrict27(r, (2 * ti + 1) * (FJ * FK) + (2 * tj + 1) * FK + (2 * tk + 1), FJ * FK, FK, 1);
+  if (sign < 0.0f) out = -out;
+  gc[I * Sc * Sc + J * Sc + K] = out;
 }
 
 // Prolongation + correction followed by one smoother sweep:
@@ -1122,16 +1162,28 @@ void launch_smooth2_from_zero(hipStream_t st, const Tile& t, int S, const float*
   hipLaunchKernelGGL(k_smooth2_from_zero, stencil_grid(S - 2, t), dim3(t.bx, t.by), 0, st, S, t.ich, g, e_out, a, c);
 }
 
+template <int CK, int CJ, int CI>
+void resres_tiled(hipStream_t st, bool fine, int Sf, const float* x, const float* src, float alpha, float beta,
+                  float* gc, float sign) {
+  const int nc = (Sf - 1) / 2 - 1;
+  const dim3 grid((nc + CK - 1) / CK, (nc + CJ - 1) / CJ, (nc + CI - 1) / CI);
+  if (fine)
+    hipLaunchKernelGGL((k_resres_tiled<true, CK, CJ, CI>), grid, dim3(CK * CJ * CI), 0, st, Sf, x, src, alpha, beta,
+                       gc, sign);
+  else
+    hipLaunchKernelGGL((k_resres_tiled<false, CK, CJ, CI>), grid, dim3(CK * CJ * CI), 0, st, Sf, x, src, 1.0f, 1.0f,
+                       gc, sign);
+}
+
 void launch_residual_restrict(hipStream_t st, bool fine, int Sf, const float* x, const float* src, float alpha,
                               float beta, float* gc, float sign) {
   const int nc = (Sf - 1) / 2 - 1;
-  const size_t tot = (size_t)nc * nc * nc;
-  if (fine)
-    hipLaunchKernelGGL(k_residual_restrict<true>, dim3(nblk(tot, 256)), dim3(256), 0, st, Sf, x, src, alpha, beta, gc,
-                       sign);
+  // 8x8x4 coarse tiles (1.27 residuals per fine node) where there are enough
+  // of them to fill the chip, 4x4x4 (1.42) on the small levels
+  if (nc >= 40)
+    resres_tiled<8, 8, 4>(st, fine, Sf, x, src, alpha, beta, gc, sign);
   else
-    hipLaunchKernelGGL(k_residual_restrict<false>, dim3(nblk(tot, 256)), dim3(256), 0, st, Sf, x, src, 1.0f, 1.0f,
-                       gc, sign);
+    resres_tiled<4, 4, 4>(st, fine, Sf, x, src, alpha, beta, gc, sign);
 }
 
 void launch_prolong_smooth(hipStream_t st, const Tile& t, bool fine, int Sf, const float* x, const float* e,

@@ -268,13 +268,9 @@ __device__ __forceinline__ void residual_col(int S, int k, int j, int i0, int i1
   }
 }
 
-// coarse rhs = sign * 4 FW(r) at coarse interior node (I, J, K2) from a stored residual
-__device__ __forceinline__ void restrict4_pt(int Sf, int I, int J, int K2, const float* __restrict__ r,
-                                             float* __restrict__ gc, float sign) {
-  const int Sc = (Sf - 1) / 2 + 1;
-  const int S2 = Sf * Sf;
-  const int c = (2 * I) * S2 + (2 * J) * Sf + 2 * K2;
-  const int di = S2, dj = Sf, dk = 1;
+// 4 FW(r) / 4 = (8 centre + 4 faces + 2 edges + corners) / 16 about r[c],
+// strides di, dj, dk; faces, edges and corners each summed in a fixed order
+__device__ __forceinline__ float restrict27(const float* __restrict__ r, int c, int di, int dj, int dk) {
   float fs = r[c + di];
   fs = fs + r[c - di];
   fs = fs + r[c + dj];
@@ -302,7 +298,15 @@ __device__ __forceinline__ void restrict4_pt(int Sf, int I, int J, int K2, const
   cs = cs + r[c - di - dj + dk];
   cs = cs + r[c - di - dj - dk];
   const float val = 8.0f * r[c] + 4.0f * fs + 2.0f * es + cs;
-  float out = val * 0.0625f;
+  return val * 0.0625f;
+}
+
+// coarse rhs = sign * 4 FW(r) at coarse interior node (I, J, K2) from a stored residual
+__device__ __forceinline__ void restrict4_pt(int Sf, int I, int J, int K2, const float* __restrict__ r,
+                                             float* __restrict__ gc, float sign) {
+  const int Sc = (Sf - 1) / 2 + 1;
+  const int S2 = Sf * Sf;
+  float out = restrict27(r, (2 * I) * S2 + (2 * J) * Sf + 2 * K2, S2, Sf, 1);
   if (sign < 0.0f) out = -out;
   gc[I * Sc * Sc + J * Sc + K2] = out;
 }
