@@ -176,10 +176,12 @@ int rq_set_graph(rq_solver* h, int32_t on);
 /* on != 0 (default): multigrid passes run fused -- two pre-sweeps from the zero
  * coarse guess in one pass, residual+restriction without the residual array,
  * prolongation+correction with the first post-sweep, and the last fine
- * post-sweep with the ITP update.  on == 3: in addition the last two levels
- * of the V-cycle (up to 25^3 nodes) run in one LDS-resident workgroup (one
- * launch instead of five, but on a single compute unit).  Results are
- * bit-identical to on == 0. */
+ * post-sweep with the ITP update; with two pre-sweeps and 1..3 post-sweeps
+ * every coarse level runs as one down-leg launch (sweeps, residual,
+ * restriction) and one up-leg launch (prolongation, sweeps).  on == 2: fused
+ * passes without the per-level legs.  on == 3: as 1, and the last two levels
+ * of the V-cycle (up to 25^3 nodes) run in one LDS-resident workgroup.
+ * Results are bit-identical to on == 0. */
 int rq_set_fused(rq_solver* h, int32_t on);
 /* Launch shape of the 2.5D stencil kernels on multigrid level `level` (0 =
  * finest, -1 = all): a (bx, by) workgroup tile in (k, j), bx*by a multiple of

@@ -88,6 +88,7 @@ struct rq_solver {
   bool use_graph = false;
   bool fused = true;  // fused multigrid passes (bit-identical to the unfused sequence)
   bool bottom = false;  // last two levels in one LDS-resident workgroup (with fused)
+  bool legs = true;     // one launch per coarse level and direction (with fused)
   // largest level (nodes per axis) that runs the one-pass prolongation+sweep;
   // 0: none -- on gfx950 the separate correction and sweep launches measured
   // faster on every level of the 192^3 workload.  RQ_FUSE_MAX_S overrides it.
@@ -238,6 +239,24 @@ int vcycle_coarse(rq_solver* s, int l) {
   }
   const Tile tl = s->ctile[l];
   const float a = s->p.mg_a, c = s->p.mg_c;
+  // per-level legs pay off where launches are latency-bound (<= 65^3 nodes);
+  // above, their halo recomputation costs more than the launches they save
+  if (s->fused && s->legs && S <= kLegsMaxS && s->p.mg_nu1 == 2 && s->p.mg_nu2 >= 1 && s->p.mg_nu2 <= 3) {
+    {
+      Timed t(s, F_RESTRICT_C);
+      launch_down2(s->st, S, s->g[l], s->e0[l], s->g[l + 1], a, c);
+    }
+    LAUNCHCHK(s);
+    int rc = vcycle_coarse(s, l + 1);
+    if (rc) return rc;
+    {
+      Timed t(s, F_PROLONG_C);
+      launch_up(s->st, S, s->p.mg_nu2, s->e0[l], s->e0[l + 1], s->e1[l], s->g[l], a, c);
+    }
+    LAUNCHCHK(s);
+    std::swap(s->e0[l], s->e1[l]);
+    return RQ_OK;
+  }
   int pre = s->p.mg_nu1;
   if (s->fused && pre >= 2) {
     {
@@ -260,7 +279,7 @@ int vcycle_coarse(rq_solver* s, int l) {
   const bool small = s->fused && S <= s->fuse_max_s;
   {
     Timed t(s, F_RESTRICT_C);
-    if (s->fused && S <= kResResMaxS) {
+    if (s->fused) {
       launch_residual_restrict(s->st, false, S, s->e0[l], s->g[l], 1.0f, 1.0f, s->g[l + 1], 1.0f);
     } else {
       launch_residual_coarse(s->st, tl, S, s->e0[l], s->g[l], s->e1[l]);
@@ -305,9 +324,7 @@ int vcycle_fine(rq_solver* s, bool fuse_itp, bool* did_itp) {
   const bool small = s->fused && s->S <= s->fuse_max_s;
   {
     Timed t(s, F_RESTRICT);
-    // the LDS-tiled one-pass kernel re-reads its halos through L1 less
-    // efficiently than two streaming passes on the largest grids
-    if (s->fused && s->S <= kResResMaxS) {
+    if (s->fused) {
       launch_residual_restrict(s->st, true, s->S, s->P, s->U, s->two_pi, s->h2, s->g[1], 1.0f);
     } else {
       launch_residual_fine(s->st, s->tile, s->S, s->P, s->U, s->two_pi, s->h2, s->P2);
@@ -1145,6 +1162,7 @@ int rq_set_fused(rq_solver* h, int32_t on) {
   if (!h) return fail(nullptr, RQ_EINVAL, "null handle");
   h->fused = on != 0;
   h->bottom = on == 3;
+  h->legs = on != 2;
   for (auto& ge : h->graphs) {
     (void)hipGraphExecDestroy(ge.exec);
     (void)hipGraphDestroy(ge.graph);

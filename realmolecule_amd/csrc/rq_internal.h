@@ -16,7 +16,8 @@ constexpr int kMaxDomains = 4096;       // LDS bins of the per-domain reductions
 constexpr int kMaxCoarseInterior = 25;  // exact FP64 solve held in LDS (m^3 doubles)
 constexpr int kMaxCoarsePP = 19;        // ping-pong variant: 2 m^3 + m^2 doubles of LDS
 constexpr int kFuseMaxS = 65;           // one-pass prolongation+sweep up to 65^3 nodes
-constexpr int kResResMaxS = 129;        // LDS-tiled residual+restriction up to 129^3 nodes
+constexpr int kLegsMaxS = 65;           // one down-leg + one up-leg launch per coarse level up to 65^3 nodes
+constexpr int kResResTiledMaxNc = 63;   // 3D-tiled residual+restriction up to 63 coarse interior nodes, march above
 constexpr int kBottomMaxS = 25;         // last two V-cycle levels in one LDS-resident workgroup up to 25^3 nodes
 
 // Launch shape of the 2.5D stencil kernels: (bx, by) tile in (k, j), marching
@@ -60,6 +61,13 @@ void launch_coarse_exact(hipStream_t st, int Sc, const float* g, float* e, const
 // sweeps around the exact DST solve (tables of level L+1); e gets the level-L
 // correction.  bottom_fits(S): S is odd, <= kBottomMaxS and the LDS fits.
 bool bottom_fits(int S);
+// One launch per coarse level and direction: down leg (two pre-sweeps from a
+// zero guess, residual, 4 FW restriction into gc; e_out gets the iterate) and
+// up leg (prolongation + correction and nu post-sweeps, nu in 1..3: returns
+// false for other nu without launching).
+void launch_down2(hipStream_t st, int S, const float* g, float* e_out, float* gc, float a, float c);
+bool launch_up(hipStream_t st, int S, int nu, const float* x, const float* e, float* x_out, const float* g, float a,
+               float c);
 void launch_bottom(hipStream_t st, int S, int nu1, int nu2, float a, float c, const float* g, float* e,
                    const double* Smat, const double* rden);
 // fused multigrid passes (bit-identical to the unfused sequences)

@@ -13,6 +13,7 @@
 // FP32-faithful mode).  Reductions accumulate FP32 per-voxel terms in FP64.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 
 #include "rq_internal.h"
@@ -296,6 +297,279 @@ __global__ __launch_bounds__(CK * CJ * CI) void k_resres_tiled(int Sf, const flo
   float out = restrict27(r, (2 * ti + 1) * (FJ * FK) + (2 * tj + 1) * FK + (2 * tk + 1), FJ * FK, FK, 1);
   if (sign < 0.0f) out = -out;
   gc[I * Sc * Sc + J * Sc + K] = out;
+}
+
+// Workgroup barrier that orders LDS only: global loads issued before it stay
+// in flight (a full __syncthreads would also wait for this wave's global
+// stores, which on gfx9 share one counter with its loads).
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// Residual + 4 FW restriction as a march along i for the large levels.  A
+// workgroup owns 32 (k) x CJ (j) coarse columns and CI coarse planes; per
+// fine plane it evaluates the residual on the (65 x 2CJ+1) fine region those
+// columns' stencils cover (each thread keeps the i-1 / i values of its nodes
+// in registers, so a fine plane is fetched once) into a 3-plane LDS ring, and
+// after every odd fine plane 2I+1 it restricts coarse plane I.  The loads of
+// the next two fine planes are issued before the restriction's barriers.
+// Same residual_at / restrict27 expressions as k_residual + k_restrict4.
+template <bool FINE, int CJ>
+__global__ __launch_bounds__(128 * CJ) void k_resres_march(int Sf, int CI, const float* __restrict__ x,
+                                                            const float* __restrict__ src, float alpha, float beta,
+                                                            float* __restrict__ gc, float sign) {
+  constexpr int CK = 32, RK = 2 * CK + 1, RJ = 2 * CJ + 1, PL = RK * RJ;
+  constexpr int NT = 128 * CJ, PTS = (PL + NT - 1) / NT;
+  __shared__ float ring[3 * PL];
+  const int nc = (Sf - 1) / 2 - 1, n = Sf - 2;
+  const int Sc = nc + 2, S2 = Sf * Sf;
+  const int gx = (nc + CK - 1) / CK, gy = (nc + CJ - 1) / CJ;
+  const int t = xcd_block();
+  const int K0 = 1 + (t % gx) * CK, J0 = 1 + ((t / gx) % gy) * CJ, I0 = 1 + (t / (gx * gy)) * CI;
+  const int I1 = min(I0 + CI, nc + 1);
+  const int tid = threadIdx.x;
+  // this thread's region nodes (clamped into the interior for the loads)
+  int col[PTS];
+  bool in[PTS];
+  float xm[PTS], xc[PTS];
+  const int f0 = 2 * I0 - 1;
+#pragma unroll
+  for (int p = 0; p < PTS; ++p) {
+    const int q = min(tid + p * NT, PL - 1);
+    const int j = 2 * J0 - 1 + q / RK, k = 2 * K0 - 1 + q % RK;
+    in[p] = tid + p * NT < PL && j <= n && k <= n;
+    col[p] = min(j, n) * Sf + min(k, n);
+    xm[p] = x[(f0 - 1) * S2 + col[p]];
+    xc[p] = x[f0 * S2 + col[p]];
+  }
+  auto rhs_of = [&](int id) -> float {
+    if (FINE) {
+      const float u = src[id];
+      return (alpha * (u * u)) * beta;
+    }
+    return src[id];
+  };
+  // residual at one region node from its centre line and in-plane neighbours
+  auto store = [&](int f, int p, float xmv, float xcv, float xpv, const float* nb, float rhs) {
+    const float s6 = xpv + xmv + nb[0] + nb[1] + nb[2] + nb[3];
+    const float r = rhs - (6.0f * xcv - s6);
+    if (tid + p * NT < PL) ring[(f % 3) * PL + tid + p * NT] = in[p] ? r : 0.0f;
+  };
+  // plane f0 (the low halo plane of the first coarse plane)
+#pragma unroll
+  for (int p = 0; p < PTS; ++p) {
+    const int id = f0 * S2 + col[p];
+    const float nb[4] = {x[id + Sf], x[id - Sf], x[id + 1], x[id - 1]};
+    const float xp = x[id + S2];
+    store(f0, p, xm[p], xc[p], xp, nb, rhs_of(id));
+    xm[p] = xc[p];
+    xc[p] = xp;
+  }
+  // loads of fine planes fa = 2I and fa + 1
+  float pa[PTS], pb[PTS], na[PTS][4], nb2[PTS][4], ra[PTS], rb[PTS];
+  auto fetch = [&](int fa) {
+#pragma unroll
+    for (int p = 0; p < PTS; ++p) {
+      const int ida = fa * S2 + col[p], idb = ida + S2;
+      pa[p] = x[idb];
+      pb[p] = x[idb + S2];
+      na[p][0] = x[ida + Sf];
+      na[p][1] = x[ida - Sf];
+      na[p][2] = x[ida + 1];
+      na[p][3] = x[ida - 1];
+      nb2[p][0] = x[idb + Sf];
+      nb2[p][1] = x[idb - Sf];
+      nb2[p][2] = x[idb + 1];
+      nb2[p][3] = x[idb - 1];
+      ra[p] = rhs_of(ida);
+      rb[p] = rhs_of(idb);
+    }
+  };
+  fetch(2 * I0);
+  const int tk = tid % CK, tj = tid / CK;
+  for (int I = I0; I < I1; ++I) {
+#pragma unroll
+    for (int p = 0; p < PTS; ++p) {
+      store(2 * I, p, xm[p], xc[p], pa[p], na[p], ra[p]);
+      store(2 * I + 1, p, xc[p], pa[p], pb[p], nb2[p], rb[p]);
+      xm[p] = pa[p];
+      xc[p] = pb[p];
+    }
+    if (I + 1 < I1) fetch(2 * I + 2);
+    lds_barrier();
+    if (tj < CJ) {
+      const int K = K0 + tk, J = J0 + tj;
+      if (K <= nc && J <= nc) {
+        const int c = (2 * tj + 1) * RK + (2 * tk + 1);
+        float out = restrict27_planes(ring, ((2 * I - 1) % 3) * PL + c, ((2 * I) % 3) * PL + c,
+                                      ((2 * I + 1) % 3) * PL + c, RK, 1);
+        if (sign < 0.0f) out = -out;
+        gc[I * Sc * Sc + J * Sc + K] = out;
+      }
+    }
+    lds_barrier();
+  }
+}
+
+// ---- one launch per coarse level and direction (temporal blocking in LDS) ----
+//
+// The global loads that stage a tile are unrolled with clamped (always valid)
+// indices, so a thread has all of them in flight at once instead of paying
+// one memory latency per loop trip.
+//
+// Down leg of coarse level l (S nodes, zero initial guess, two pre-sweeps):
+// e = two sweeps from zero (smooth2z expression), r = g - A e, coarse rhs =
+// 4 FW(r).  A workgroup owns CK x CJ x CI coarse interior nodes; the fine
+// region R0 their restriction reads, R1 = R0 + 1 (where e is needed) and
+// R2 = R0 + 2 (where g is needed) live in LDS.  e is also written to e_out
+// on R0 (neighbouring tiles write the same values on their shared plane).
+template <int CK, int CJ, int CI>
+__global__ __launch_bounds__(256) void k_down2(int S, const float* __restrict__ g, float* __restrict__ e_out,
+                                               float* __restrict__ gc, float a, float c) {
+  constexpr int NT = 256;
+  static_assert(CK * CJ * CI <= NT, "one thread per coarse node of the tile");
+  constexpr int K0n = 2 * CK + 1, J0n = 2 * CJ + 1, I0n = 2 * CI + 1;  // R0
+  constexpr int K1n = K0n + 2, J1n = J0n + 2, I1n = I0n + 2;           // R1
+  constexpr int K2n = K0n + 4, J2n = J0n + 4, I2n = I0n + 4;           // R2
+  constexpr int N0 = K0n * J0n * I0n, N1 = K1n * J1n * I1n, N2 = K2n * J2n * I2n;
+  constexpr int IT2 = (N2 + NT - 1) / NT;
+  // G (and g / c in Gc, one division per node) padded to whole loop trips so
+  // the staging stores need no branch
+  __shared__ float G[IT2 * NT], Gc[IT2 * NT], E[N1], R[N0];
+  const int n = S - 2, nc = (S - 1) / 2 - 1, Sc = nc + 2, S2 = S * S;
+  const int gx = (nc + CK - 1) / CK, gy = (nc + CJ - 1) / CJ;
+  const int t = xcd_block();
+  const int K0 = 1 + (t % gx) * CK, J0 = 1 + ((t / gx) % gy) * CJ, I0 = 1 + (t / (gx * gy)) * CI;
+  // fine coordinates of R0's first node; R1 / R2 start 1 / 2 lower
+  const int fk = 2 * K0 - 1, fj = 2 * J0 - 1, fi = 2 * I0 - 1;
+  const int tid = threadIdx.x;
+  {
+    float v[IT2];
+    unsigned long long inb = 0;
+#pragma unroll
+    for (int it = 0; it < IT2; ++it) {
+      const int q = tid + it * NT;
+      const int qq = q < N2 ? q : N2 - 1;
+      const int i = fi - 2 + qq / (J2n * K2n), j = fj - 2 + (qq / K2n) % J2n, k = fk - 2 + qq % K2n;
+      if (i >= 0 && i < S && j >= 0 && j < S && k >= 0 && k < S) inb |= 1ull << it;
+      v[it] = g[min(max(i, 0), S - 1) * S2 + min(max(j, 0), S - 1) * S + min(max(k, 0), S - 1)];
+    }
+    // keep every load above the first store
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int it = 0; it < IT2; ++it) {
+      const float gv = (inb >> it) & 1 ? v[it] : 0.0f;
+      G[tid + it * NT] = gv;
+      Gc[tid + it * NT] = gv / c;
+    }
+  }
+  __syncthreads();
+  constexpr int dI2 = J2n * K2n, dJ2 = K2n;
+  for (int q = tid; q < N1; q += NT) {
+    const int li = q / (J1n * K1n), lj = (q / K1n) % J1n, lk = q % K1n;
+    const int i = fi - 1 + li, j = fj - 1 + lj, k = fk - 1 + lk;
+    float v = 0.0f;
+    if (i >= 1 && i <= n && j >= 1 && j <= n && k >= 1 && k <= n) {
+      const int q2 = (li + 1) * dI2 + (lj + 1) * dJ2 + (lk + 1);
+      const float s6 = Gc[q2 + dI2] + Gc[q2 - dI2] + Gc[q2 + dJ2] + Gc[q2 - dJ2] + Gc[q2 + 1] + Gc[q2 - 1];
+      v = (s6 + G[q2]) / c;
+      if (a != 0.0f) v = a * Gc[q2] + v;
+      if (li >= 1 && li <= I0n && lj >= 1 && lj <= J0n && lk >= 1 && lk <= K0n) e_out[i * S2 + j * S + k] = v;
+    }
+    E[q] = v;
+  }
+  __syncthreads();
+  constexpr int dI1 = J1n * K1n, dJ1 = K1n;
+  for (int q = tid; q < N0; q += NT) {
+    const int li = q / (J0n * K0n), lj = (q / K0n) % J0n, lk = q % K0n;
+    const int i = fi + li, j = fj + lj, k = fk + lk;
+    float r = 0.0f;
+    if (i <= n && j <= n && k <= n) {
+      const int q1 = (li + 1) * dI1 + (lj + 1) * dJ1 + (lk + 1);
+      const int q2 = (li + 2) * dI2 + (lj + 2) * dJ2 + (lk + 2);
+      const float s6 = E[q1 + dI1] + E[q1 - dI1] + E[q1 + dJ1] + E[q1 - dJ1] + E[q1 + 1] + E[q1 - 1];
+      r = G[q2] - (6.0f * E[q1] - s6);
+    }
+    R[q] = r;
+  }
+  __syncthreads();
+  const int tk = tid % CK, tj = (tid / CK) % CJ, ti = tid / (CK * CJ);
+  const int K = K0 + tk, J = J0 + tj, I = I0 + ti;
+  if (tid >= CK * CJ * CI || K > nc || J > nc || I > nc) return;
+  gc[I * Sc * Sc + J * Sc + K] =
+      restrict27(R, (2 * ti + 1) * (J0n * K0n) + (2 * tj + 1) * K0n + (2 * tk + 1), J0n * K0n, K0n, 1);
+}
+
+// Up leg of coarse level l: xc = x + P e (faces keep x), then NU sweeps
+// (the k_prolong_correct + k_smooth expressions).  A workgroup owns a
+// TK x TJ x TI output tile; xc and g are staged on the tile plus a halo of
+// NU and each sweep shrinks the valid region by one node.
+template <int TK, int TJ, int TI, int NU>
+__global__ __launch_bounds__(256) void k_up(int S, const float* __restrict__ x, const float* __restrict__ e,
+                                            float* __restrict__ x_out, const float* __restrict__ g, float a,
+                                            float c) {
+  constexpr int RK = TK + 2 * NU, RJ = TJ + 2 * NU, RI = TI + 2 * NU, NR = RK * RJ * RI;
+  constexpr int dI = RJ * RK, dJ = RK, ITR = (NR + 255) / 256;
+  // padded to whole loop trips so the staging stores need no branch
+  __shared__ float buf[2][ITR * 256], gl[ITR * 256];
+  const int n = S - 2, Sc = (S - 1) / 2 + 1, S2 = S * S;
+  const int gx = (n + TK - 1) / TK, gy = (n + TJ - 1) / TJ;
+  const int t = xcd_block();
+  const int k0 = 1 + (t % gx) * TK - NU, j0 = 1 + ((t / gx) % gy) * TJ - NU, i0 = 1 + (t / (gx * gy)) * TI - NU;
+  const int tid = threadIdx.x;
+  // staged in groups of 4 trips: the loads of a group are issued before any
+  // of its stores
+#pragma unroll
+  for (int it0 = 0; it0 < ITR; it0 += 4) {
+    float xv[4], gv[4], pe[4];
+    unsigned flags = 0;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int q = tid + (it0 + u) * 256;
+      const int qq = q < NR ? q : NR - 1;
+      const int i = i0 + qq / dI, j = j0 + (qq / dJ) % RJ, k = k0 + qq % RK;
+      if (i >= 0 && i < S && j >= 0 && j < S && k >= 0 && k < S) flags |= 1u << (2 * u);
+      if (i >= 1 && i <= n && j >= 1 && j <= n && k >= 1 && k <= n) flags |= 2u << (2 * u);
+      const int id = min(max(i, 0), S - 1) * S2 + min(max(j, 0), S - 1) * S + min(max(k, 0), S - 1);
+      xv[u] = x[id];
+      gv[u] = g[id];
+      pe[u] = prolong_at(e, Sc, min(max(i, 1), n), min(max(j, 1), n), min(max(k, 1), n));
+    }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      if (it0 + u >= ITR) break;
+      const int q = tid + (it0 + u) * 256;
+      const bool grid = (flags >> (2 * u)) & 1, inner = (flags >> (2 * u)) & 2;
+      buf[0][q] = grid ? (inner ? xv[u] + pe[u] : xv[u]) : 0.0f;
+      gl[q] = gv[u];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int s = 1; s <= NU; ++s) {
+    const float* A = buf[(s - 1) & 1];
+    float* B = buf[s & 1];
+    for (int q = tid; q < NR; q += 256) {
+      const int li = q / dI, lj = (q / dJ) % RJ, lk = q % RK;
+      if (li < s || li >= RI - s || lj < s || lj >= RJ - s || lk < s || lk >= RK - s) continue;
+      const int i = i0 + li, j = j0 + lj, k = k0 + lk;
+      if (i < 1 || i > n || j < 1 || j > n || k < 1 || k > n) {
+        B[q] = A[q];
+        continue;
+      }
+      const float s6 = A[q + dI] + A[q - dI] + A[q + dJ] + A[q - dJ] + A[q + 1] + A[q - 1];
+      float v = (s6 + gl[q]) / c;
+      if (a != 0.0f) v = a * A[q] + v;
+      if (s == NU)
+        x_out[i * S2 + j * S + k] = v;
+      else
+        B[q] = v;
+    }
+    if (s < NU) __syncthreads();
+  }
 }
 
 // Prolongation + correction followed by one smoother sweep:
@@ -1178,12 +1452,72 @@ void resres_tiled(hipStream_t st, bool fine, int Sf, const float* x, const float
 void launch_residual_restrict(hipStream_t st, bool fine, int Sf, const float* x, const float* src, float alpha,
                               float beta, float* gc, float sign) {
   const int nc = (Sf - 1) / 2 - 1;
+  if (nc > kResResTiledMaxNc) {
+    // march: 32 x 2 coarse columns per workgroup, CI planes for ~2048 workgroups
+    constexpr int CJ = 2;
+    const int gx = (nc + 31) / 32, gy = (nc + CJ - 1) / CJ;
+    const int gz = std::max(1, std::min(nc, 2048 / (gx * gy)));
+    const int CI = std::max(4, (nc + gz - 1) / gz);
+    const dim3 grid(gx, gy, (nc + CI - 1) / CI);
+    if (fine)
+      hipLaunchKernelGGL((k_resres_march<true, CJ>), grid, dim3(128 * CJ), 0, st, Sf, CI, x, src, alpha, beta, gc,
+                         sign);
+    else
+      hipLaunchKernelGGL((k_resres_march<false, CJ>), grid, dim3(128 * CJ), 0, st, Sf, CI, x, src, 1.0f, 1.0f, gc,
+                         sign);
+    return;
+  }
   // 8x8x4 coarse tiles (1.27 residuals per fine node) where there are enough
   // of them to fill the chip, 4x4x4 (1.42) on the small levels
   if (nc >= 40)
     resres_tiled<8, 8, 4>(st, fine, Sf, x, src, alpha, beta, gc, sign);
   else
     resres_tiled<4, 4, 4>(st, fine, Sf, x, src, alpha, beta, gc, sign);
+}
+
+template <int CK, int CJ, int CI>
+static void down2_tiled(hipStream_t st, int S, const float* g, float* e_out, float* gc, float a, float c) {
+  const int nc = (S - 1) / 2 - 1;
+  const dim3 grid((nc + CK - 1) / CK, (nc + CJ - 1) / CJ, (nc + CI - 1) / CI);
+  hipLaunchKernelGGL((k_down2<CK, CJ, CI>), grid, dim3(256), 0, st, S, g, e_out, gc, a, c);
+}
+
+// 256 threads per workgroup on every level; smaller coarse tiles on smaller
+// levels keep a few nodes per thread (these launches are latency-bound)
+void launch_down2(hipStream_t st, int S, const float* g, float* e_out, float* gc, float a, float c) {
+  const int nc = (S - 1) / 2 - 1;
+  if (nc >= 32)
+    down2_tiled<8, 8, 4>(st, S, g, e_out, gc, a, c);
+  else if (nc >= 16)
+    down2_tiled<4, 4, 4>(st, S, g, e_out, gc, a, c);
+  else
+    down2_tiled<4, 4, 2>(st, S, g, e_out, gc, a, c);
+}
+
+template <int TK, int TJ, int TI>
+static bool up_tiled(hipStream_t st, int S, int nu, const float* x, const float* e, float* x_out, const float* g,
+                     float a, float c) {
+  const int n = S - 2;
+  const dim3 grid((n + TK - 1) / TK, (n + TJ - 1) / TJ, (n + TI - 1) / TI);
+  switch (nu) {
+    case 1:
+      hipLaunchKernelGGL((k_up<TK, TJ, TI, 1>), grid, dim3(256), 0, st, S, x, e, x_out, g, a, c);
+      return true;
+    case 2:
+      hipLaunchKernelGGL((k_up<TK, TJ, TI, 2>), grid, dim3(256), 0, st, S, x, e, x_out, g, a, c);
+      return true;
+    case 3:
+      hipLaunchKernelGGL((k_up<TK, TJ, TI, 3>), grid, dim3(256), 0, st, S, x, e, x_out, g, a, c);
+      return true;
+    default:
+      return false;
+  }
+}
+
+bool launch_up(hipStream_t st, int S, int nu, const float* x, const float* e, float* x_out, const float* g, float a,
+               float c) {
+  if (S - 2 >= 64) return up_tiled<16, 8, 4>(st, S, nu, x, e, x_out, g, a, c);
+  return up_tiled<8, 8, 4>(st, S, nu, x, e, x_out, g, a, c);
 }
 
 void launch_prolong_smooth(hipStream_t st, const Tile& t, bool fine, int Sf, const float* x, const float* e,

@@ -268,37 +268,43 @@ __device__ __forceinline__ void residual_col(int S, int k, int j, int i0, int i1
   }
 }
 
-// 4 FW(r) / 4 = (8 centre + 4 faces + 2 edges + corners) / 16 about r[c],
-// strides di, dj, dk; faces, edges and corners each summed in a fixed order
-__device__ __forceinline__ float restrict27(const float* __restrict__ r, int c, int di, int dj, int dk) {
-  float fs = r[c + di];
-  fs = fs + r[c - di];
-  fs = fs + r[c + dj];
-  fs = fs + r[c - dj];
-  fs = fs + r[c + dk];
-  fs = fs + r[c - dk];
-  float es = r[c + di + dj];
-  es = es + r[c + di - dj];
-  es = es + r[c - di + dj];
-  es = es + r[c - di - dj];
-  es = es + r[c + di + dk];
-  es = es + r[c + di - dk];
-  es = es + r[c - di + dk];
-  es = es + r[c - di - dk];
-  es = es + r[c + dj + dk];
-  es = es + r[c + dj - dk];
-  es = es + r[c - dj + dk];
-  es = es + r[c - dj - dk];
-  float cs = r[c + di + dj + dk];
-  cs = cs + r[c + di + dj - dk];
-  cs = cs + r[c + di - dj + dk];
-  cs = cs + r[c + di - dj - dk];
-  cs = cs + r[c - di + dj + dk];
-  cs = cs + r[c - di + dj - dk];
-  cs = cs + r[c - di - dj + dk];
-  cs = cs + r[c - di - dj - dk];
-  const float val = 8.0f * r[c] + 4.0f * fs + 2.0f * es + cs;
+// 4 FW(r) / 4 = (8 centre + 4 faces + 2 edges + corners) / 16 about r[c0],
+// where r[cm] / r[cp] are its i-1 / i+1 neighbours and dj, dk the in-plane
+// strides; faces, edges and corners each summed in a fixed order
+__device__ __forceinline__ float restrict27_planes(const float* r, int cm, int c0, int cp, int dj, int dk) {
+  float fs = r[cp];
+  fs = fs + r[cm];
+  fs = fs + r[c0 + dj];
+  fs = fs + r[c0 - dj];
+  fs = fs + r[c0 + dk];
+  fs = fs + r[c0 - dk];
+  float es = r[cp + dj];
+  es = es + r[cp - dj];
+  es = es + r[cm + dj];
+  es = es + r[cm - dj];
+  es = es + r[cp + dk];
+  es = es + r[cp - dk];
+  es = es + r[cm + dk];
+  es = es + r[cm - dk];
+  es = es + r[c0 + dj + dk];
+  es = es + r[c0 + dj - dk];
+  es = es + r[c0 - dj + dk];
+  es = es + r[c0 - dj - dk];
+  float cs = r[cp + dj + dk];
+  cs = cs + r[cp + dj - dk];
+  cs = cs + r[cp - dj + dk];
+  cs = cs + r[cp - dj - dk];
+  cs = cs + r[cm + dj + dk];
+  cs = cs + r[cm + dj - dk];
+  cs = cs + r[cm - dj + dk];
+  cs = cs + r[cm - dj - dk];
+  const float val = 8.0f * r[c0] + 4.0f * fs + 2.0f * es + cs;
   return val * 0.0625f;
+}
+
+// the same about r[c] with i-stride di
+__device__ __forceinline__ float restrict27(const float* r, int c, int di, int dj, int dk) {
+  return restrict27_planes(r, c - di, c, c + di, dj, dk);
 }
 
 // coarse rhs = sign * 4 FW(r) at coarse interior node (I, J, K2) from a stored residual

@@ -44,3 +44,41 @@ def test_bottom_levels_bitexact(NN, levels, nu):
     for u, p in out[1:]:
         assert ulp_diff(out[0][0], u) == 0
         assert ulp_diff(out[0][1], p) == 0
+
+
+def test_fused_large_grid_bitexact():
+    """Levels above 63 coarse interior nodes per axis restrict with the
+    i-march kernel (NN = 160: 79 coarse nodes), against the unfused passes."""
+    NN, levels = 160, 4
+    rows = three_atoms(NN)
+    out = []
+    for mode in (0, 1):
+        s = make_solver(rows, NN, NN * 0.1, poisson="multigrid", mg_levels=levels, mg_nu1=2, mg_nu2=2,
+                        bootstrap_poisson=2)
+        s._chk(s._lib.rq_set_fused(s._h, mode))
+        s.initialize()
+        s.step(0, 12, poisson_interval=1)
+        out.append((s.U.get(), s.P.get()))
+        s.close()
+    assert ulp_diff(out[0][0], out[1][0]) == 0
+    assert ulp_diff(out[0][1], out[1][1]) == 0
+
+
+@pytest.mark.parametrize("NN,levels,nu2", [(96, 4, 2), (64, 4, 1), (160, 4, 3), (48, 3, 2)])
+def test_level_legs_bitexact(NN, levels, nu2):
+    """One down-leg and one up-leg launch per coarse level (two pre-sweeps,
+    nu2 post-sweeps) against the separate fused passes (mode 2) and the
+    unfused sequence (mode 0)."""
+    rows = three_atoms(NN)
+    out = []
+    for mode in (0, 2, 1):
+        s = make_solver(rows, NN, NN * 0.15, poisson="multigrid", mg_levels=levels, mg_nu1=2, mg_nu2=nu2,
+                        bootstrap_poisson=2)
+        s._chk(s._lib.rq_set_fused(s._h, mode))
+        s.initialize()
+        s.step(0, 15, poisson_interval=1)
+        out.append((s.U.get(), s.P.get()))
+        s.close()
+    for u, p in out[1:]:
+        assert ulp_diff(out[0][0], u) == 0
+        assert ulp_diff(out[0][1], p) == 0
