@@ -14,7 +14,7 @@ constexpr uint16_t kRcFlag = 0x8000u;
 constexpr uint16_t kLabelMask = 0x7fffu;
 constexpr int kMaxDomains = 4096;       // LDS bins of the per-domain reductions
 constexpr int kMaxCoarseInterior = 25;  // exact FP64 solve held in LDS (m^3 doubles)
-constexpr int kMaxCoarsePP = 19;        // ping-pong variant: 2 m^3 + m^2 doubles of LDS
+constexpr int kMaxCoarsePP = 19;        // table-driven exact solve: 2 m^3 doubles of LDS, matrix row in registers
 constexpr int kFuseMaxS = 65;           // one-pass prolongation+sweep up to 65^3 nodes
 constexpr int kLegsMaxS = 65;           // one down-leg + one up-leg launch per coarse level up to 65^3 nodes
 constexpr int kResResTiledMaxNc = 63;   // 3D-tiled residual+restriction up to 63 coarse interior nodes, march above

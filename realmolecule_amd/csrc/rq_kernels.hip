@@ -676,11 +676,11 @@ __global__ __launch_bounds__(256) void k_coarse_exact(int Sc, const float* __res
   }
 }
 
-// Exact coarsest solve with the DST matrix Sm (m x m, orthonormal, symmetric)
-// and the reciprocal eigenvalue sums rden (m^3) precomputed on the host.
-// 1024 threads each produce transform outputs directly (ping-pong LDS
-// buffers A/B), so no thread runs a long dependent chain.  Needs
-// 2 m^3 + m^2 doubles of LDS (m <= 19).
+// One orthonormal type-I DST along `axis` of the m^3 block `in` (LDS), all
+// threads of the workgroup producing outputs directly: out[t] = sum_q
+// Sm[p][q] in[line(t) + q stride] with even / odd q in two partial sums.
+// Used by the single-workgroup bottom; k_coarse_exact_rows evaluates the same
+// sums with the matrix row in registers.
 __device__ __forceinline__ void dst_axis(const double* __restrict__ in, double* __restrict__ out,
                                          const double* __restrict__ Sm, int m, int axis) {
   const int m2 = m * m, m3 = m2 * m;
@@ -712,48 +712,12 @@ __device__ __forceinline__ void dst_axis(const double* __restrict__ in, double* 
   }
 }
 
-__global__ __launch_bounds__(1024) void k_coarse_exact_pp(int Sc, const float* __restrict__ g, float* __restrict__ e,
-                                                          const double* __restrict__ Smat,
-                                                          const double* __restrict__ rden) {
-  extern __shared__ double lds[];
-  const int m = Sc - 2;
-  const int m3 = m * m * m;
-  double* Sm = lds;
-  double* A = Sm + m * m;
-  double* B = A + m3;
-  const int Sc2 = Sc * Sc;
-  for (int t = threadIdx.x; t < m * m; t += blockDim.x) Sm[t] = Smat[t];
-  for (int t = threadIdx.x; t < m3; t += blockDim.x) {
-    const int a = t / (m * m), b = (t / m) % m, c = t % m;
-    A[t] = (double)g[(a + 1) * Sc2 + (b + 1) * Sc + (c + 1)];
-  }
-  __syncthreads();
-  dst_axis(A, B, Sm, m, 2);
-  __syncthreads();
-  dst_axis(B, A, Sm, m, 1);
-  __syncthreads();
-  dst_axis(A, B, Sm, m, 0);
-  __syncthreads();
-  for (int t = threadIdx.x; t < m3; t += blockDim.x) B[t] *= rden[t];
-  __syncthreads();
-  dst_axis(B, A, Sm, m, 2);
-  __syncthreads();
-  dst_axis(A, B, Sm, m, 1);
-  __syncthreads();
-  dst_axis(B, A, Sm, m, 0);
-  __syncthreads();
-  for (int t = threadIdx.x; t < m3; t += blockDim.x) {
-    const int a = t / (m * m), b = (t / m) % m, c = t % m;
-    e[(a + 1) * Sc2 + (b + 1) * Sc + (c + 1)] = (float)A[t];
-  }
-}
-
 // The last two V-cycle levels in one workgroup, everything in LDS: on level L
 // (S <= kBottomMaxS nodes per axis) pre-smoothing, residual, 4 FW restriction,
 // the exact FP64 DST solve of level L+1, prolongation + correction and
 // post-smoothing; only the level-L correction is written back.  Each step is
 // the expression of the multi-launch sequence (k_smooth2_from_zero / k_smooth
-// / k_residual / k_restrict4 / k_coarse_exact_pp / k_prolong_correct) in the
+// / k_residual / k_restrict4 / k_coarse_exact_rows / k_prolong_correct) in the
 // same order, so the result is bit-identical to it; Jacobi sweeps run in
 // place (new values held in registers across a barrier).
 // LDS: A, B (m^3 f64), Sm (m^2 f64), E, R (S^3 f32), G (Sc^3 f32).
@@ -853,7 +817,7 @@ __global__ __launch_bounds__(kBotThreads) void k_bottom(int S, int nu1, int nu2,
   for (int t = tid; t < m3; t += kBotThreads)
     restrict4_pt(S, t / m2 + 1, (t / m) % m + 1, t % m + 1, oth, G, 1.0f);
   __syncthreads();
-  // exact solve on level L+1 (the transforms of k_coarse_exact_pp)
+  // exact solve on level L+1 (the transforms of k_coarse_exact_rows)
   for (int t = tid; t < m3; t += kBotThreads) {
     const int a3 = t / m2, b3 = (t / m) % m, c3 = t % m;
     A[t] = (double)G[(a3 + 1) * Sc2 + (b3 + 1) * Sc + (c3 + 1)];
@@ -894,6 +858,104 @@ __global__ __launch_bounds__(kBotThreads) void k_bottom(int S, int nu1, int nu2,
   for (int q = 0; q < kBotPer; ++q) {
     const int id = bot_id(pk[q], S);
     if (live & (1u << q)) e_out[id] = cur[id];
+  }
+}
+
+// Exact coarsest solve with the orthonormal type-I DST matrix Sm (m x m,
+// symmetric, m <= kMaxCoarsePP) and the reciprocal eigenvalue sums rden (m^3)
+// precomputed on the host: e = S D^-1 S g along the three axes.  The
+// workgroup is (m, L) threads and thread (p, l) produces output p of lines l,
+// l + L, ... on every axis, so its DST matrix row Sm[p][*] is loaded into
+// registers once for all six transforms and only the input lines are read
+// from LDS.  The 1 / eigenvalue-sum scaling (prefetched at the start) is
+// applied as the third transform stores.  M > 0 fixes m at compile time so
+// each line's loads are issued together; M = 0 takes m at run time.  Same
+// products and sums, in the same order, as dst_axis.
+constexpr int kExactLines = 8;  // lines per thread: ceil(m^2 / floor(1024 / m)) for m <= 19
+
+template <int M>
+__global__ __launch_bounds__(1024) void k_coarse_exact_rows(int Sc, const float* __restrict__ g,
+                                                            float* __restrict__ e, const double* __restrict__ Smat,
+                                                            const double* __restrict__ rden) {
+  extern __shared__ double lds[];
+  constexpr int QM = M > 0 ? M : kMaxCoarsePP;
+  const int m = M > 0 ? M : Sc - 2;
+  const int m2 = m * m, m3 = m2 * m;
+  double* A = lds;
+  double* B = A + m3;
+  const int p = threadIdx.x, L = blockDim.y;
+  const int tid = threadIdx.y * blockDim.x + threadIdx.x, nt = blockDim.x * blockDim.y;
+  double row[QM];
+#pragma unroll
+  for (int q = 0; q < QM; ++q) row[q] = (M > 0 || q < m) ? Smat[p * m + q] : 0.0;
+  double rd[kExactLines];
+#pragma unroll
+  for (int it = 0; it < kExactLines; ++it) {
+    const int l = threadIdx.y + it * L;
+    rd[it] = l < m2 ? rden[(l / m) * m + (l % m) + p * m2] : 0.0;
+  }
+  const int Sc2 = Sc * Sc;
+  for (int t = tid; t < m3; t += nt) {
+    const int a = t / m2, b = (t / m) % m, c = t % m;
+    A[t] = (double)g[(a + 1) * Sc2 + (b + 1) * Sc + (c + 1)];
+  }
+  __syncthreads();
+  auto pass = [&](const double* __restrict__ in, double* __restrict__ out, int axis, bool scale) {
+#pragma unroll
+    for (int it = 0; it < kExactLines; ++it) {
+      const int l = threadIdx.y + it * L;
+      if (l >= m2) break;
+      const int u = l / m, w = l % m;
+      int base, stride, o;
+      if (axis == 2) {
+        base = u * m2 + w * m;
+        stride = 1;
+        o = base + p;
+      } else if (axis == 1) {
+        base = u * m2 + w;
+        stride = m;
+        o = base + p * m;
+      } else {
+        base = u * m + w;
+        stride = m2;
+        o = base + p * m2;
+      }
+      double s0 = 0.0, s1 = 0.0;
+      if (M > 0) {
+#pragma unroll
+        for (int q = 0; q + 1 < QM; q += 2) {
+          s0 += row[q] * in[base + q * stride];
+          s1 += row[q + 1] * in[base + (q + 1) * stride];
+        }
+        if (QM & 1) s0 += row[QM - 1] * in[base + (QM - 1) * stride];
+      } else {
+        int q = 0;
+        for (; q + 1 < m; q += 2) {
+          s0 += row[q] * in[base + q * stride];
+          s1 += row[q + 1] * in[base + (q + 1) * stride];
+        }
+        if (q < m) s0 += row[q] * in[base + q * stride];
+      }
+      double v = s0 + s1;
+      if (scale) v *= rd[it];
+      out[o] = v;
+    }
+  };
+  pass(A, B, 2, false);
+  __syncthreads();
+  pass(B, A, 1, false);
+  __syncthreads();
+  pass(A, B, 0, true);
+  __syncthreads();
+  pass(B, A, 2, false);
+  __syncthreads();
+  pass(A, B, 1, false);
+  __syncthreads();
+  pass(B, A, 0, false);
+  __syncthreads();
+  for (int t = tid; t < m3; t += nt) {
+    const int a = t / m2, b = (t / m) % m, c = t % m;
+    e[(a + 1) * Sc2 + (b + 1) * Sc + (c + 1)] = (float)A[t];
   }
 }
 
@@ -1390,18 +1452,36 @@ void launch_prolong_correct(hipStream_t st, const Tile& t, int Sf, float* x, con
   hipLaunchKernelGGL(k_prolong_correct, stencil_grid(Sf - 2, t), dim3(t.bx, t.by), 0, st, Sf, t.ich, x, ec, scale);
 }
 
+template <int M>
+static void exact_rows(hipStream_t st, int Sc, const float* g, float* e, const double* Smat, const double* rden,
+                       size_t lds) {
+  static bool attr = false;
+  if (!attr) {
+    (void)hipFuncSetAttribute((const void*)k_coarse_exact_rows<M>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              160 * 1024);
+    attr = true;
+  }
+  const int m = Sc - 2;
+  hipLaunchKernelGGL((k_coarse_exact_rows<M>), dim3(1), dim3(m, 1024 / m), lds, st, Sc, g, e, Smat, rden);
+}
+
 void launch_coarse_exact(hipStream_t st, int Sc, const float* g, float* e, const double* Smat, const double* rden) {
   const int m = Sc - 2;
   if (Smat && rden && m <= kMaxCoarsePP) {
-    const size_t lds = sizeof(double) * ((size_t)m * m + 2 * (size_t)m * m * m);
-    static bool attr_pp = false;
-    if (!attr_pp) {
-      (void)hipFuncSetAttribute((const void*)k_coarse_exact_pp, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                160 * 1024);
-      attr_pp = true;
+    const size_t lds = sizeof(double) * 2 * (size_t)m * m * m;
+    // rows variants with m fixed at compile time for the odd m that 2^k-cell
+    // grids produce; m^2 * 8 * 2 bytes of LDS never exceeds 110 KB (m <= 19)
+    switch (m) {
+      case 3: exact_rows<3>(st, Sc, g, e, Smat, rden, lds); return;
+      case 5: exact_rows<5>(st, Sc, g, e, Smat, rden, lds); return;
+      case 7: exact_rows<7>(st, Sc, g, e, Smat, rden, lds); return;
+      case 9: exact_rows<9>(st, Sc, g, e, Smat, rden, lds); return;
+      case 11: exact_rows<11>(st, Sc, g, e, Smat, rden, lds); return;
+      case 13: exact_rows<13>(st, Sc, g, e, Smat, rden, lds); return;
+      case 15: exact_rows<15>(st, Sc, g, e, Smat, rden, lds); return;
+      case 19: exact_rows<19>(st, Sc, g, e, Smat, rden, lds); return;
+      default: exact_rows<0>(st, Sc, g, e, Smat, rden, lds); return;
     }
-    hipLaunchKernelGGL(k_coarse_exact_pp, dim3(1), dim3(1024), lds, st, Sc, g, e, Smat, rden);
-    return;
   }
   const size_t lds = sizeof(double) * ((size_t)m * m + m + (size_t)m * m * m);
   static bool attr_set = false;
