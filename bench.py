@@ -29,10 +29,9 @@ METRIC = "Gvoxel-updates/s & %HBM roofline; batched adjoint forces/sec vs plain 
 HBM_PEAK_GBS = 8000.0
 
 # Algorithmic bytes per fine interior voxel and launch (each array counted
-# once) of the fine-level passes.  Residual + restriction is one pass through
-# LDS up to 129^3 nodes and two passes (residual array in HBM) above,
-# prolongation + correction a pass of its own, and the last post-sweep is
-# fused with the ITP update.
+# once) of the fine-level passes.  Residual + restriction is one pass,
+# prolongation + correction one, and the last post-sweep is fused with the
+# ITP update.
 BYTES_PER_VOXEL = {
     "itp": 18.0,         # U, K, P (f32) + label (u16) in; U' out  (steps without a V-cycle)
     "smooth": 12.0,      # P, U in; P' out
@@ -40,7 +39,6 @@ BYTES_PER_VOXEL = {
     "prolong": 8.5,      # P in and out, coarse correction in (1/8)
     "smooth_itp": 22.0,  # last post-sweep + ITP: P, U, K, label in; P', U' out
 }
-BYTES_TWO_PASS = {"restrict": 16.5}  # residual (P, U in; r out) + restriction (r in; rhs out), > 129^3 nodes
 
 # Fine-level kernel of each family, as rocprofv3 names it, for the HBM
 # traffic counters (FETCH_SIZE / WRITE_SIZE per dispatch).
@@ -330,8 +328,6 @@ def main():
     s._chk(lib.rq_timing_read(s._h, ms, cnt, nf))
     s._chk(lib.rq_timing_enable(s._h, 0))
     bpv = dict(BYTES_PER_VOXEL)
-    if NN + 1 > 129:
-        bpv.update(BYTES_TWO_PASS)
     kernels = {}
     for f, name in enumerate(L.FAMILIES):
         if cnt[f] == 0:

@@ -110,6 +110,13 @@ __device__ __forceinline__ float prolong_at(const float* __restrict__ e, int Sc,
   return 0.5f * (ejk0 + ejk1);
 }
 
+// bilinear (k, then j) interpolation of coarse plane ic at fine column (j, k):
+// the per-plane term of prolong_at
+__device__ __forceinline__ float prolong_plane(const float* __restrict__ e, int Sc, int ic, int j, int k) {
+  const int b0 = ic * Sc * Sc + (j >> 1) * Sc;
+  return 0.5f * (lerp_k(e, b0, k) + lerp_k(e, b0 + (j & 1) * Sc, k));
+}
+
 // x_out = smoother applied to xc = x + P e (interior; faces keep x)
 template <bool FINE>
 __device__ __forceinline__ void prolong_smooth_col(int Sf, int k, int j, int i0, int i1, const float* __restrict__ x,
@@ -140,6 +147,31 @@ __device__ __forceinline__ void prolong_smooth_col(int Sf, int k, int j, int i0,
     x_out[id] = v;
   }
 }
+
+// Coarse-plane interpolant cache of one fine column for a march along i:
+// prolong_at(i) = 0.5 (T(i>>1) + T((i>>1) + (i&1))) with T = prolong_plane,
+// each T formed once.  i advances uniformly across a workgroup, so the reuse
+// branches do not diverge.
+struct PlaneCache {
+  int ca = -1, cb = -1;
+  float ta = 0.0f, tb = 0.0f;
+  __device__ __forceinline__ float corr(const float* __restrict__ e, int Sc, int i, int j, int k) {
+    const int ic = i >> 1;
+    if (ic != ca) {
+      ta = (ic == cb) ? tb : prolong_plane(e, Sc, ic, j, k);
+      ca = ic;
+    }
+    float t1 = ta;
+    if (i & 1) {
+      if (cb != ic + 1) {
+        tb = prolong_plane(e, Sc, ic + 1, j, k);
+        cb = ic + 1;
+      }
+      t1 = tb;
+    }
+    return 0.5f * (ta + t1);
+  }
+};
 
 // last fine sweep fused with the ITP update reading the new P at the centre
 __device__ __forceinline__ void smooth_itp_col(int S, int k, int j, int i0, int i1, const float* __restrict__ P,
@@ -317,39 +349,15 @@ __device__ __forceinline__ void restrict4_pt(int Sf, int I, int J, int K2, const
   gc[I * Sc * Sc + J * Sc + K2] = out;
 }
 
-// bilinear (k, then j) interpolation of coarse plane ic at fine column (j, k):
-// the per-plane term of prolong_at
-__device__ __forceinline__ float prolong_plane(const float* __restrict__ e, int Sc, int ic, int j, int k) {
-  const int b0 = ic * Sc * Sc + (j >> 1) * Sc;
-  return 0.5f * (lerp_k(e, b0, k) + lerp_k(e, b0 + (j & 1) * Sc, k));
-}
-
-// x += scale * P e along one fine column.  prolong_at(i) = 0.5 (T(i>>1) +
-// T((i>>1) + (i&1))) with T the plane term, so marching along i each coarse
-// plane's T is formed once and reused by the fine planes that need it (the
-// same values and operations as prolong_at).  i is uniform across a
-// workgroup, so the reuse branches do not diverge.
+// x += scale * P e along one fine column, each coarse-plane interpolant
+// formed once (PlaneCache; same values and operations as prolong_at).
 __device__ __forceinline__ void prolong_correct_col(int Sf, int k, int j, int i0, int i1, float* __restrict__ x,
                                                     const float* __restrict__ e, float scale) {
   const int Sc = (Sf - 1) / 2 + 1;
   const int S2 = Sf * Sf;
-  int ca = i0 >> 1, cb = -1;
-  float ta = prolong_plane(e, Sc, ca, j, k), tb = 0.0f;
+  PlaneCache pc;
   for (int i = i0; i < i1; ++i) {
-    const int ic = i >> 1;
-    if (ic != ca) {
-      ta = (ic == cb) ? tb : prolong_plane(e, Sc, ic, j, k);
-      ca = ic;
-    }
-    float t1 = ta;
-    if (i & 1) {
-      if (cb != ic + 1) {
-        tb = prolong_plane(e, Sc, ic + 1, j, k);
-        cb = ic + 1;
-      }
-      t1 = tb;
-    }
-    const float corr = 0.5f * (ta + t1);
+    const float corr = pc.corr(e, Sc, i, j, k);
     const int id = i * S2 + j * Sf + k;
     x[id] = (scale == 1.0f) ? x[id] + corr : x[id] + scale * corr;
   }
