@@ -58,6 +58,12 @@ def workload(name):
     elif name == "c1":
         w = B.c1_h2o()
         w["desc"] = "single H2O, 96^3 FP32 (h=0.1), 4-level V-cycle/step"
+    elif name == "c4":
+        w = B.c4_peptide()
+        w["desc"] = "tri-glycine + 9 H2O (seed 3), 256^3 FP32, 5-level V-cycle/step"
+    elif name == "c5":
+        w = B.c5_waters()
+        w["desc"] = "1000 H2O (seed 4, O-O>=2.5A) in 58.58 bohr cube, 512^3 FP32, 7-level V-cycle/step"
     else:
         raise SystemExit(f"unknown workload {name}")
     return w
@@ -116,18 +122,12 @@ def forces_bench(w, steps, bootstrap, n_batch, fd_delta, log_fn):
     log_fn(f"[forces] adjoint: {t_adj:.3f} s for {3 * n} components")
     t0 = time.perf_counter()
     E0 = fz.energy(pos)
-    F_fd = np.zeros((n, 3))
-    for a in range(n):
-        for c in range(3):
-            p = pos.copy()
-            p[a, c] += fd_delta
-            ep = fz.energy(p)
-            p[a, c] -= 2 * fd_delta
-            em = fz.energy(p)
-            F_fd[a, c] = -(ep - em) / (2 * fd_delta)
+    F_fd = fz.forces_fd(pos, delta=fd_delta)
     t_fd = time.perf_counter() - t0
     log_fn(f"[forces] central differences: {t_fd:.2f} s for {6 * n + 1} solves")
-    err = float(np.abs(-g - F_fd).max())
+    dF = np.abs(-g - F_fd)
+    err = float(dF.max())
+    bare = rows[:, 4] <= 0  # rc = 0: E is smooth in R; rc > 0 atoms have the clamp kink
     fz.close()
     # batched: n_batch geometries (seeded 0.05 bohr jitter) on concurrent handles
     rng = np.random.default_rng(7)
@@ -163,7 +163,67 @@ def forces_bench(w, steps, bootstrap, n_batch, fd_delta, log_fn):
         "speedup_adjoint": t_fd / t_adj,
         "speedup_batched_adjoint": (n_batch / t_batch) * t_fd,
         "max_abs_force_diff_vs_fd_Ha_per_bohr": err,
+        "max_abs_force_diff_rc0_atoms": float(dF[bare].max()) if bare.any() else None,
+        "max_abs_force_diff_rc_atoms": float(dF[~bare].max()) if (~bare).any() else None,
+        "max_abs_force_Ha_per_bohr": float(np.abs(F_fd).max()),
         "fd_delta_bohr": fd_delta,
+    }
+
+
+def bench_c3(steps, warmup):
+    """Batched geometry scan: 64 seeded conformers (H2O, formamide, ethanol,
+    glycine) on grids of 64..160 cells, 4-level V(2,2) + ITP per step.  The
+    batch engine advances all of them with one persistent launch per pass;
+    the comparison runs the same molecules one after another, each on its own
+    single-grid handle (the per-molecule path a scan would take)."""
+    from realmolecule_amd import builders as B
+    from realmolecule_amd.batch import Batch
+    from realmolecule_amd.solver import RealQMSolver
+    mols = B.c3_conformers()
+    vox = sum((nn - 1) ** 3 for _, nn, _, _ in mols)
+    b = Batch([(r, nn, sc) for r, nn, sc, _ in mols], mg_levels=4, graph=True)
+    b.initialize(bootstrap=10)
+    b.step(0, warmup)
+    b.sync()
+    t0 = time.perf_counter()
+    b.step(warmup, steps)
+    b.sync()
+    dt_b = time.perf_counter() - t0
+    b.close()
+    log(f"[c3] batch: {dt_b / steps * 1e3:.3f} ms/step for {len(mols)} molecules, {vox * steps / dt_b / 1e9:.2f} Gvox/s")
+    seq_steps = max(20, steps // 5)
+    dt_s = 0.0
+    for r, nn, sc, name in mols:
+        from realmolecule_amd.builders import to_config_atoms
+        s = RealQMSolver({"gridN": nn, "screen": sc, "atoms": to_config_atoms(r), "poisson": "multigrid",
+                          "mg_levels": 4, "bootstrap_poisson": 10, "graph": True})
+        s.initialize()
+        s.step(0, 20, poisson_interval=1)
+        s.sync()
+        t0 = time.perf_counter()
+        s.step(20, seq_steps, poisson_interval=1)
+        s.sync()
+        dt_s += time.perf_counter() - t0
+        s.close()
+    log(f"[c3] one handle per molecule, in turn: {dt_s / seq_steps * 1e3:.3f} ms per step of all molecules")
+    return {
+        "metric": "Gvoxel-updates/s (64-conformer batch)",
+        "value": round(vox * steps / dt_b / 1e9, 3),
+        "unit": "Gvoxel-updates/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": warmup,
+        "ms_per_step": round(dt_b / steps * 1e3, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": "64 conformers (seed 2) of H2O/formamide/ethanol/glycine, grids 64..160 cells, "
+                               "4-level V(2,2) + ITP per step", "molecules": len(mols), "interior_voxels": vox,
+                   "parallelism": "single-gpu"},
+        "sequential_ms_per_step": round(dt_s / seq_steps * 1e3, 5),
+        "batch_speedup": round((dt_s / seq_steps) / (dt_b / steps), 3),
     }
 
 
@@ -258,7 +318,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=50)
-    ap.add_argument("--workload", default="c2")
+    ap.add_argument("--workload", default="c2", choices=("c1", "c2", "c3", "c4", "c5"))
     ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--timing-steps", type=int, default=100)
@@ -275,6 +335,9 @@ def main():
         sys.exit(2)
     if args.pmc_child:
         pmc_child(args.pmc_child, workload(args.workload), args.pmc_steps)
+        return
+    if args.workload == "c3":
+        print(json.dumps(bench_c3(args.steps, args.warmup)))
         return
 
     import ctypes as C

@@ -77,19 +77,30 @@ class FrozenSolver:
     def _ekk(self, rows):
         return nuclear_energy(rows[:, :3].astype(np.float32), rows[:, 3], self.h)
 
+    def stored_position(self, x_bohr):
+        """The position the solver actually uses: grid units held as FP32."""
+        return float(np.float32(x_bohr / self.h)) * self.h
+
     def forces_fd(self, pos_bohr, delta=1e-3, atoms=None):
-        """Plain path: central differences of full solves (2 per component)."""
+        """Plain path: central differences of full solves (2 per component).
+
+        The step is divided by the difference of the stored (FP32 grid-unit)
+        positions, not by 2 delta: at |R| ~ 20 bohr an FP32 grid coordinate
+        moves in steps of ~2e-6 bohr, a 1e-4 relative error on a 1e-2 bohr
+        step."""
         pos = np.asarray(pos_bohr, np.float64)
         idx = range(self.n) if atoms is None else atoms
         F = np.zeros((self.n, 3))
         for a in idx:
             for c in range(3):
                 p = pos.copy()
-                p[a, c] += delta
+                p[a, c] = pos[a, c] + delta
+                xp = self.stored_position(p[a, c])
                 ep = self.energy(p)
-                p[a, c] -= 2 * delta
+                p[a, c] = pos[a, c] - delta
+                xm = self.stored_position(p[a, c])
                 em = self.energy(p)
-                F[a, c] = -(ep - em) / (2 * delta)
+                F[a, c] = -(ep - em) / (xp - xm)
         return F
 
     def close(self):

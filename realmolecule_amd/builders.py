@@ -94,15 +94,17 @@ def water_box(n_mol, box_bohr, NN, seed, min_OO_angstrom=2.6, margin_bohr=3.0, m
     rng = np.random.default_rng(seed)
     h = box_bohr / NN
     dmin = min_OO_angstrom * ANGSTROM
-    O = []
+    O = np.empty((n_mol, 3))
+    placed = 0
     tries = 0
-    while len(O) < n_mol:
+    while placed < n_mol:
         tries += 1
         if tries > max_tries:
             raise RuntimeError(f"could not place {n_mol} waters with O-O >= {min_OO_angstrom} A")
         p = rng.uniform(margin_bohr, box_bohr - margin_bohr, size=3)
-        if all(np.linalg.norm(p - q) >= dmin for q in O):
-            O.append(p)
+        if placed == 0 or ((O[:placed] - p) ** 2).sum(1).min() >= dmin * dmin:
+            O[placed] = p
+            placed += 1
     geo = water_geometry()
     rows, els = [], []
     for p in O:
@@ -125,6 +127,67 @@ def c2_waters():
     """10 H2O, seeded orientations, O-O >= 2.6 A, 12 A cube, 192^3, 5 multigrid levels."""
     rows, els, screen = water_box(10, 12.0 * ANGSTROM, 192, seed=1)
     return {"NN": 192, "screen": screen, "rows": rows, "els": els, "mg_levels": 5}
+
+
+def c5_waters():
+    """1000 H2O at 1 g/cm^3 in a 58.58 bohr cube (O-O >= 2.5 A, seed 4), 512^3
+    (h = 0.1144 bohr), 7 multigrid levels: 3000 domains."""
+    rows, els, screen = water_box(1000, 58.58, 512, seed=4, min_OO_angstrom=2.5, margin_bohr=2.0,
+                                  max_tries=2000000)
+    return {"NN": 512, "screen": screen, "rows": rows, "els": els, "mg_levels": 7}
+
+
+# Tri-glycine NH2-Gly-Gly-Gly-COOH with one water (angstrom, element), the
+# coordinate table of peptide_grow.html:22-56.
+TRIGLYCINE = [
+    ("N", 0.0, 0.0, 0.0), ("C", 1.47, 0.0, 0.0), ("C", 2.1, 1.23, 0.0), ("O", 1.6, 2.3, 0.0),
+    ("N", 3.4, 1.2, 0.0), ("C", 4.1, 2.4, 0.0), ("C", 5.6, 2.3, 0.0), ("O", 6.2, 1.2, 0.0),
+    ("N", 6.3, 3.4, 0.0), ("C", 7.7, 3.5, 0.0), ("C", 8.4, 4.7, 0.0), ("O", 7.9, 5.8, 0.0),
+    ("O", 9.6, 4.6, 0.0), ("H", 10.2, 5.3, 0.0),
+    ("H", -0.4, 0.8, 0.0), ("H", -0.4, -0.8, 0.0), ("H", 1.5, -0.5, 0.9), ("H", 1.5, -0.5, -0.9),
+    ("H", 3.8, 0.3, 0.0), ("H", 3.8, 3.0, 0.9), ("H", 3.8, 3.0, -0.9), ("H", 5.8, 4.1, 0.0),
+    ("H", 8.0, 2.6, 0.9), ("H", 8.0, 2.6, -0.9),
+    ("O", 2.5, -1.5, 0.0), ("H", 1.8, -1.9, 0.0), ("H", 3.1, -2.0, 0.0),
+]
+
+
+def c4_peptide(NN=256, n_water=8, seed=3, jitter_angstrom=0.1, min_dist_angstrom=2.6):
+    """Tri-glycine + its water (27 atoms) with a seeded +-0.1 A jitter, plus
+    n_water waters placed at random orientations with every heavy atom >= 2.6 A
+    from theirs; screen from peptide_grow.html:73-75 (extent + 15 A, clamped
+    to [35, 90] bohr); NN = 256, 5 multigrid levels (coarsest 15^3)."""
+    rng = np.random.default_rng(seed)
+    els = [a[0] for a in TRIGLYCINE]
+    X = np.array([a[1:] for a in TRIGLYCINE], np.float64)
+    X += rng.uniform(-jitter_angstrom, jitter_angstrom, X.shape)
+    X = (X - X.mean(0)) * ANGSTROM
+    extent = (X.max(0) - X.min(0)).max()
+    screen = float(min(90.0, max(35.0, extent + 15 * ANGSTROM)))
+    centre = screen / 2
+    heavy = [x for x, el in zip(X, els) if el != "H"]
+    geo = water_geometry()
+    dmin = min_dist_angstrom * ANGSTROM
+    placed = 0
+    tries = 0
+    while placed < n_water:
+        tries += 1
+        if tries > 200000:
+            raise RuntimeError("could not place the waters")
+        p = rng.uniform(-extent / 2 - 3.0, extent / 2 + 3.0, size=3)
+        if np.abs(p).max() > centre - 3.0 or any(np.linalg.norm(p - q) < dmin for q in heavy):
+            continue
+        R = _rand_rotation(rng)
+        for a, el in enumerate(("O", "H", "H")):
+            X = np.vstack([X, p + R @ geo[a]])
+            els.append(el)
+        heavy.append(p)
+        placed += 1
+    h = screen / NN
+    rows = []
+    for el, x in zip(els, X):
+        Z, rc = ELEMENT[el]
+        rows.append(((centre + x[0]) / h, (centre + x[1]) / h, (centre + x[2]) / h, Z, rc))
+    return {"NN": NN, "screen": screen, "rows": np.array(rows), "els": els, "mg_levels": 5}
 
 
 # Molecule templates (angstrom) of the reference's grow pages, solvent waters

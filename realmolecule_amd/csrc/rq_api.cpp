@@ -67,6 +67,7 @@ struct rq_solver {
   float* Z_d = nullptr;
   int32_t* lock_d = nullptr;
   float *U = nullptr, *U2 = nullptr, *K = nullptr, *P = nullptr, *P2 = nullptr, *bestU = nullptr, *W = nullptr;
+  void* arena = nullptr;  // one allocation holding the fine fields and labels
   uint16_t *lab = nullptr, *lab2 = nullptr;
   // multigrid levels 1..L-1: S, two iterate buffers and the rhs
   int levels = 0;
@@ -467,15 +468,7 @@ void free_all(rq_solver* s) {
   fr(s->atoms_d);
   fr(s->Z_d);
   fr(s->lock_d);
-  fr(s->U);
-  fr(s->U2);
-  fr(s->K);
-  fr(s->P);
-  fr(s->P2);
-  fr(s->bestU);
-  fr(s->W);
-  fr(s->lab);
-  fr(s->lab2);
+  fr(s->arena);
   fr(s->red);
   fr(s->dst_S);
   fr(s->dst_rden);
@@ -881,10 +874,30 @@ int rq_create(const rq_params* pp, const rq_atom* atoms, int32_t n_atoms, rq_sol
   bool ok = hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking) == hipSuccess;
   const size_t fb = s->S3 * sizeof(float), lb = s->S3 * sizeof(uint16_t);
   ok = ok && alloc((void**)&s->atoms_d, sizeof(Atom) * n_atoms) && alloc((void**)&s->Z_d, sizeof(float) * n_atoms) &&
-       alloc((void**)&s->lock_d, sizeof(int32_t) * n_atoms) && alloc((void**)&s->U, fb) &&
-       alloc((void**)&s->U2, fb) && alloc((void**)&s->K, fb) && alloc((void**)&s->P, fb) &&
-       alloc((void**)&s->P2, fb) && alloc((void**)&s->bestU, fb) && alloc((void**)&s->W, fb) &&
-       alloc((void**)&s->lab, lb) && alloc((void**)&s->lab2, lb);
+       alloc((void**)&s->lock_d, sizeof(int32_t) * n_atoms);
+  {
+    // The fine fields and both label maps live in one arena.  Consecutive
+    // fields start `stagger` bytes apart modulo 2 MiB, so the same node of the
+    // arrays a kernel streams together does not fall on the same HBM channel
+    // (RQ_FIELD_STAGGER overrides the offset for tuning).
+    size_t stagger = kFieldStagger;
+    if (const char* fs = std::getenv("RQ_FIELD_STAGGER")) stagger = (size_t)std::atoll(fs);
+    const size_t two_mb = size_t(2) << 20;
+    const size_t sizes[9] = {fb, fb, fb, fb, fb, fb, fb, lb, lb};
+    size_t offs[9], off = 0;
+    for (int f = 0; f < 9; ++f) {
+      offs[f] = off + (size_t)f * stagger % two_mb;
+      off += (sizes[f] + two_mb - 1) / two_mb * two_mb + two_mb;
+    }
+    ok = ok && alloc(&s->arena, off);
+    if (ok) {
+      char* base = static_cast<char*>(s->arena);
+      float** ff[7] = {&s->U, &s->U2, &s->K, &s->P, &s->P2, &s->bestU, &s->W};
+      for (int f = 0; f < 7; ++f) *ff[f] = reinterpret_cast<float*>(base + offs[f]);
+      s->lab = reinterpret_cast<uint16_t*>(base + offs[7]);
+      s->lab2 = reinterpret_cast<uint16_t*>(base + offs[8]);
+    }
+  }
   s->red_len = (size_t)n_atoms * 3 + 3;
   ok = ok && alloc((void**)&s->red, sizeof(double) * s->red_len);
   if (ok && levels > 0) {

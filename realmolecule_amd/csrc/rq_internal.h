@@ -16,6 +16,7 @@ constexpr int kMaxDomains = 4096;       // LDS bins of the per-domain reductions
 constexpr int kMaxCoarseInterior = 25;  // exact FP64 solve held in LDS (m^3 doubles)
 constexpr int kMaxCoarsePP = 19;        // table-driven exact solve: 2 m^3 doubles of LDS, matrix row in registers
 constexpr int kFuseMaxS = 65;           // one-pass prolongation+sweep up to 65^3 nodes
+constexpr size_t kFieldStagger = 4096;   // byte offset between consecutive fine fields (mod 2 MiB)
 constexpr int kLegsMaxS = 65;           // one down-leg + one up-leg launch per coarse level up to 65^3 nodes
 constexpr int kResResTiledMaxNc = 63;   // 3D-tiled residual+restriction up to 63 coarse interior nodes, march above
 constexpr int kBottomMaxS = 25;         // last two V-cycle levels in one LDS-resident workgroup up to 25^3 nodes
