@@ -62,6 +62,9 @@ struct rq_solver {
   float h = 0, h2 = 0, h3 = 0, half_dv = 0, dtv = 0, two_pi = 0;
   int n_atoms = 0;
   std::vector<Atom> atoms_h;
+  // uniform cell list of the atoms (Z > 0) for the score rule's argmax search
+  AtomCells cells;
+  int32_t *cell_start_d = nullptr, *cell_atoms_d = nullptr;
   std::vector<int32_t> lock_h;
   Atom* atoms_d = nullptr;
   float* Z_d = nullptr;
@@ -204,6 +207,28 @@ int upload_atoms(rq_solver* s) {
   std::vector<float> Z(s->n_atoms);
   for (int a = 0; a < s->n_atoms; ++a) Z[a] = s->atoms_h[a].Z;
   HIPCHK(s, hipMemcpyAsync(s->Z_d, Z.data(), sizeof(float) * s->n_atoms, hipMemcpyHostToDevice, s->st));
+  // cell list: atoms with Z > 0 binned by floor(position / cs), clamped
+  AtomCells& c = s->cells;
+  const int nc3 = c.n * c.n * c.n;
+  std::vector<int32_t> start(nc3 + 1, 0), order;
+  std::vector<int> cell_of(s->n_atoms, -1);
+  c.zmax = 0.0f;
+  auto bin = [&](float x) { return std::min(c.n - 1, std::max(0, (int)std::floor(x / (float)c.cs))); };
+  for (int a = 0; a < s->n_atoms; ++a) {
+    const Atom& at = s->atoms_h[a];
+    if (!(at.Z > 0.0f)) continue;
+    c.zmax = std::max(c.zmax, at.Z);
+    cell_of[a] = (bin(at.i) * c.n + bin(at.j)) * c.n + bin(at.k);
+    ++start[cell_of[a] + 1];
+  }
+  for (int q = 0; q < nc3; ++q) start[q + 1] += start[q];
+  order.assign(std::max(1, start[nc3]), 0);
+  std::vector<int32_t> fill(start.begin(), start.end() - 1);
+  for (int a = 0; a < s->n_atoms; ++a)
+    if (cell_of[a] >= 0) order[fill[cell_of[a]]++] = a;
+  HIPCHK(s, hipMemcpyAsync(s->cell_start_d, start.data(), sizeof(int32_t) * (nc3 + 1), hipMemcpyHostToDevice, s->st));
+  HIPCHK(s, hipMemcpyAsync(s->cell_atoms_d, order.data(), sizeof(int32_t) * order.size(), hipMemcpyHostToDevice,
+                           s->st));
   HIPCHK(s, hipStreamSynchronize(s->st));
   return RQ_OK;
 }
@@ -432,7 +457,7 @@ int boundary_step(rq_solver* s) {
     Timed t(s, F_BOUNDARY);
     if (s->p.boundary_mode == RQ_BOUNDARY_SCORE) {
       launch_boundary_score(s->st, s->tile, s->S, s->h, s->h2, s->atoms_d, s->n_atoms, s->lab, s->lab2,
-                            s->p.flip_ratio, s->p.flip_min_foreign);
+                            s->p.flip_ratio, s->p.flip_min_foreign, s->cells, s->cell_start_d, s->cell_atoms_d);
     } else {
       bool any_lock = false;
       for (int32_t g : s->lock_h) any_lock |= (g > 0);
@@ -469,6 +494,8 @@ void free_all(rq_solver* s) {
   fr(s->Z_d);
   fr(s->lock_d);
   fr(s->arena);
+  fr(s->cell_start_d);
+  fr(s->cell_atoms_d);
   fr(s->red);
   fr(s->dst_S);
   fr(s->dst_rden);
@@ -875,6 +902,15 @@ int rq_create(const rq_params* pp, const rq_atom* atoms, int32_t n_atoms, rq_sol
   const size_t fb = s->S3 * sizeof(float), lb = s->S3 * sizeof(uint16_t);
   ok = ok && alloc((void**)&s->atoms_d, sizeof(Atom) * n_atoms) && alloc((void**)&s->Z_d, sizeof(float) * n_atoms) &&
        alloc((void**)&s->lock_d, sizeof(int32_t) * n_atoms);
+  {
+    // cells of about 2 bohr (at least 4 nodes) for the score rule's search
+    AtomCells& c = s->cells;
+    c.cs = std::max(4, (int)std::lround(2.0 / (p.screen / p.NN)));
+    c.n = (s->S + c.cs - 1) / c.cs;
+    const size_t nc3 = (size_t)c.n * c.n * c.n;
+    ok = ok && alloc((void**)&s->cell_start_d, sizeof(int32_t) * (nc3 + 1)) &&
+         alloc((void**)&s->cell_atoms_d, sizeof(int32_t) * std::max(1, n_atoms));
+  }
   {
     // The fine fields and both label maps live in one arena.  Consecutive
     // fields start `stagger` bytes apart modulo 2 MiB, so the same node of the

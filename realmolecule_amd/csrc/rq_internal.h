@@ -17,6 +17,7 @@ constexpr int kMaxCoarseInterior = 25;  // exact FP64 solve held in LDS (m^3 dou
 constexpr int kMaxCoarsePP = 19;        // table-driven exact solve: 2 m^3 doubles of LDS, matrix row in registers
 constexpr int kFuseMaxS = 65;           // one-pass prolongation+sweep up to 65^3 nodes
 constexpr size_t kFieldStagger = 4096;   // byte offset between consecutive fine fields (mod 2 MiB)
+constexpr int kScoreScanMaxAtoms = 128;  // score rule: scan all atoms up to this count, cell search above
 constexpr int kLegsMaxS = 65;           // one down-leg + one up-leg launch per coarse level up to 65^3 nodes
 constexpr int kResResTiledMaxNc = 63;   // 3D-tiled residual+restriction up to 63 coarse interior nodes, march above
 constexpr int kBottomMaxS = 25;         // last two V-cycle levels in one LDS-resident workgroup up to 25^3 nodes
@@ -29,6 +30,13 @@ struct Tile {
 
 struct Atom {
   float i, j, k, Z, rc;
+};
+
+// Uniform cell list over the grid for nearest-atom searches.
+struct AtomCells {
+  int cs = 4;          // cell edge in nodes
+  int n = 1;           // cells per axis
+  float zmax = 0.0f;   // largest Z of the binned atoms
 };
 
 // Kernel families timed separately by rq_timing_*: fine-grid (level 0)
@@ -85,8 +93,14 @@ void launch_norm_sums(hipStream_t st, const Tile& t, int S, float h3, const floa
                       int n_dom, double* norms);
 void launch_normalize(hipStream_t st, const Tile& t, int S, float* U, const uint16_t* lab, const double* norms,
                       const float* Z, int n_dom);
+// Score rule.  Up to kScoreScanMaxAtoms atoms: a scan over all atoms.  Above:
+// the argmax over atoms is searched through a uniform cell list (cells of cs
+// nodes, n per axis; cell_start[n^3 + 1], cell_atoms sorted by cell; zmax =
+// largest Z), visiting shells of cells until no farther atom can reach the
+// best score found.  Both give the labels of the scan.
 void launch_boundary_score(hipStream_t st, const Tile& t, int S, float h, float h2, const Atom* atoms, int n_atoms,
-                           const uint16_t* lin, uint16_t* lout, float ratio, int min_foreign);
+                           const uint16_t* lin, uint16_t* lout, float ratio, int min_foreign, const AtomCells& cells,
+                           const int32_t* cell_start, const int32_t* cell_atoms);
 void launch_levelset(hipStream_t st, const Tile& t, int S, float h, const Atom* atoms, int n_atoms,
                      const int32_t* lock_group, const uint16_t* lin, uint16_t* lout, const float* U, float* W,
                      float dt_w, float curv_reg, int rc_flags);

@@ -1033,11 +1033,12 @@ __global__ __launch_bounds__(256) void k_normalize(int S, int ich, float* __rest
 // --------------------------------------------------------------------------
 // Nuclear-score flip (cloud/solver.py:92-147), reading labels from `lin`
 // and writing `lout` (double-buffered).
-__global__ __launch_bounds__(256) void k_boundary_score(int S, int ich, float h, float h2,
-                                                        const Atom* __restrict__ atoms, int n_atoms,
-                                                        const uint16_t* __restrict__ lin,
-                                                        uint16_t* __restrict__ lout, float ratio,
-                                                        int min_foreign) {
+// Score rule by a scan over all atoms (cloud/solver.py:92-147; the form for
+// small molecules, where a scan of a few dozen atoms beats any search).
+__global__ __launch_bounds__(256) void k_boundary_scan(int S, int ich, float h, float h2,
+                                                       const Atom* __restrict__ atoms, int n_atoms,
+                                                       const uint16_t* __restrict__ lin, uint16_t* __restrict__ lout,
+                                                       float ratio, int min_foreign) {
   const int n = S - 2;
   March m;
   if (!march_coords(n, ich, m)) return;
@@ -1071,6 +1072,78 @@ __global__ __launch_bounds__(256) void k_boundary_score(int S, int ich, float h,
           best = sc;
           bestN = a;
         }
+      }
+      if (bestN != cur && best > cur_score * ratio && other >= min_foreign) out = (uint16_t)bestN;
+    }
+    lout[id] = out;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_boundary_score(int S, int ich, float h, float h2,
+                                                        const Atom* __restrict__ atoms, int n_atoms,
+                                                        const uint16_t* __restrict__ lin,
+                                                        uint16_t* __restrict__ lout, float ratio, int min_foreign,
+                                                        int cs, int ncell, float zmax,
+                                                        const int32_t* __restrict__ cell_start,
+                                                        const int32_t* __restrict__ cell_atoms) {
+  const int n = S - 2;
+  March m;
+  if (!march_coords(n, ich, m)) return;
+  const int S2 = S * S;
+  for (int i = m.i0; i < m.i1; ++i) {
+    const int id = i * S2 + m.j * S + m.k;
+    const uint16_t raw = lin[id];
+    const int cur = raw & kLabelMask;
+    int other = 0;
+    other += ((lin[id + S2] & kLabelMask) != cur);
+    other += ((lin[id - S2] & kLabelMask) != cur);
+    other += ((lin[id + S] & kLabelMask) != cur);
+    other += ((lin[id - S] & kLabelMask) != cur);
+    other += ((lin[id + 1] & kLabelMask) != cur);
+    other += ((lin[id - 1] & kLabelMask) != cur);
+    uint16_t out = raw;
+    if (other > 0) {
+      const float xi = (float)i * h, yj = (float)m.j * h, zk = (float)m.k * h;
+      // score of atom a: Z / sqrt(|x - R_a|^2 + h^2) (cloud/solver.py:110-128)
+      auto score = [&](int a) -> float {
+        const float dx = xi - atoms[a].i * h;
+        const float dy = yj - atoms[a].j * h;
+        const float dz = zk - atoms[a].k * h;
+        const float r2 = dx * dx + dy * dy + dz * dz + h2;
+        return atoms[a].Z / sqrtf(r2);
+      };
+      // the reference scans all atoms in index order keeping the first
+      // strict maximum; here the candidates come from shells of cells, so a
+      // tie keeps the lower index -- the same winner.
+      const float cur_score = (cur < n_atoms && atoms[cur].Z > 0.0f) ? score(cur) : 0.0f;
+      float best = 0.0f;
+      int bestN = cur;
+      bool found = false;
+      const int hi = min(i / cs, ncell - 1), hj = min(m.j / cs, ncell - 1), hk = min(m.k / cs, ncell - 1);
+      const int rmax = max(max(max(hi, ncell - 1 - hi), max(hj, ncell - 1 - hj)), max(hk, ncell - 1 - hk));
+      for (int r = 0; r <= rmax; ++r) {
+        for (int ci = max(hi - r, 0); ci <= min(hi + r, ncell - 1); ++ci) {
+          for (int cj = max(hj - r, 0); cj <= min(hj + r, ncell - 1); ++cj) {
+            const bool edge_ij = abs(ci - hi) == r || abs(cj - hj) == r;
+            for (int ck = max(hk - r, 0); ck <= min(hk + r, ncell - 1); ++ck) {
+              if (!edge_ij && abs(ck - hk) != r) continue;
+              const int cell = (ci * ncell + cj) * ncell + ck;
+              for (int q = cell_start[cell]; q < cell_start[cell + 1]; ++q) {
+                const int a = cell_atoms[q];
+                const float sc = score(a);
+                if (sc > best || (found && sc == best && a < bestN)) {
+                  best = sc;
+                  bestN = a;
+                  found = true;
+                }
+              }
+            }
+          }
+        }
+        // every atom in shells > r lies at least r cells (r * cs nodes) away;
+        // 1e-4 relative slack covers the FP32 rounding of the scores
+        const float dmin = (float)(r * cs) * h * 0.9999f;
+        if (best > 0.0f && zmax / sqrtf(dmin * dmin + h2) < best) break;
       }
       if (bestN != cur && best > cur_score * ratio && other >= min_foreign) out = (uint16_t)bestN;
     }
@@ -1636,9 +1709,15 @@ void launch_normalize(hipStream_t st, const Tile& t, int S, float* U, const uint
 }
 
 void launch_boundary_score(hipStream_t st, const Tile& t, int S, float h, float h2, const Atom* atoms, int n_atoms,
-                           const uint16_t* lin, uint16_t* lout, float ratio, int min_foreign) {
+                           const uint16_t* lin, uint16_t* lout, float ratio, int min_foreign, const AtomCells& cells,
+                           const int32_t* cell_start, const int32_t* cell_atoms) {
+  if (n_atoms <= kScoreScanMaxAtoms) {
+    hipLaunchKernelGGL(k_boundary_scan, stencil_grid(S - 2, t), dim3(t.bx, t.by), 0, st, S, t.ich, h, h2, atoms,
+                       n_atoms, lin, lout, ratio, min_foreign);
+    return;
+  }
   hipLaunchKernelGGL(k_boundary_score, stencil_grid(S - 2, t), dim3(t.bx, t.by), 0, st, S, t.ich, h, h2, atoms,
-                     n_atoms, lin, lout, ratio, min_foreign);
+                     n_atoms, lin, lout, ratio, min_foreign, cells.cs, cells.n, cells.zmax, cell_start, cell_atoms);
 }
 
 void launch_levelset(hipStream_t st, const Tile& t, int S, float h, const Atom* atoms, int n_atoms,

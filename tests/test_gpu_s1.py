@@ -118,6 +118,33 @@ def test_boundary_score_exact():
     s.close()
 
 
+@pytest.mark.parametrize("n_atoms,NN,h", [(60, 40, 0.35), (150, 48, 0.6), (300, 48, 0.3)])
+def test_boundary_score_many_atoms_cell_search(n_atoms, NN, h):
+    """Many atoms (mixed Z, one with Z = 0; above 128 atoms the kernel searches a cell list): the argmax
+    search must pick the same atom as the oracle's scan over all atoms,
+    including boundary voxels far from every nucleus."""
+    rng = np.random.default_rng(11 + n_atoms)
+    rows = np.zeros((n_atoms, 5))
+    rows[:, :3] = rng.uniform(2.0, NN - 2.0, (n_atoms, 3))
+    rows[:, 3] = rng.choice([1.0, 2.0, 3.0, 4.0], n_atoms)
+    rows[3, 3] = 0.0
+    screen = NN * h
+    g, U, lab, K, P = _state(rows, NN, screen)
+    lab2 = lab.copy()
+    inner = lab2[1:-1, 1:-1, 1:-1]
+    m = rng.random(inner.shape) < 0.2
+    inner[m] = rng.integers(0, n_atoms, m.sum())
+    s = make_solver(rows, NN, screen)
+    s.U.set(U)
+    s.label.set(lab2)
+    s._boundary()
+    got = cube(s.label.get(), NN + 1)
+    ref = s1.boundary_score(U, lab2, rows, g)
+    assert (got != lab2).sum() > 0
+    assert np.array_equal(got, ref)
+    s.close()
+
+
 def test_energy_matches_oracle():
     rows = three_atoms(24)
     NN, screen = 24, 24 * 0.3
