@@ -77,6 +77,11 @@ class FrozenSolver:
     def _ekk(self, rows):
         return nuclear_energy(rows[:, :3].astype(np.float32), rows[:, 3], self.h)
 
+    def refreeze(self, pos_bohr):
+        """Re-freeze the labels at new nuclei: trial argmax at ``pos_bohr``."""
+        self._set_positions(pos_bohr)
+        self.s._chk(self.lib.rq_init_trial(self.hdl))
+
     def stored_position(self, x_bohr):
         """The position the solver actually uses: grid units held as FP32."""
         return float(np.float32(x_bohr / self.h)) * self.h

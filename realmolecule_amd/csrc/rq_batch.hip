@@ -14,6 +14,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -411,6 +412,7 @@ struct rq_batch {
   std::vector<int> maxS;               // largest node count per axis on each level
   int nnodes = 0, ndom = 0;
   int grid_stencil = 2048;
+  bool fuse_small = false;  // one-pass residual+restriction / prolongation+sweep on levels <= 65^3 (RQ_BATCH_FUSE_SMALL=1)
   BLev* lev_d = nullptr;  // L * nm
   BMol* mol_d = nullptr;
   Atom* atoms_d = nullptr;
@@ -492,7 +494,11 @@ void resolve(rq_batch* b) {
 
 const BLev* LD(rq_batch* b, int l) { return b->lev_d + (size_t)l * b->nm; }
 
-int grid_for(int items) { return std::max(8, std::min(((items + 7) / 8) * 8, 2048)); }
+// One workgroup per item up to 2^20 items (the grid-stride loops then run
+// once): the hardware keeps as many workgroups resident as occupancy allows,
+// which a capped persistent grid (2048, 8 per CU) did not -- it left the
+// fine passes at ~3 TB/s.  A multiple of 8 keeps item t on XCD t mod 8.
+int grid_for(int items) { return std::max(8, std::min(((items + 7) / 8) * 8, 1 << 20)); }
 
 int b_vcycle_coarse(rq_batch* b, int l) {
   const float a = b->p.mg_a, c = b->p.mg_c;
@@ -521,7 +527,10 @@ int b_vcycle_coarse(rq_batch* b, int l) {
     }
     std::swap(b->e0[l], b->e1[l]);
   }
-  const bool small = b->maxS[l] <= kFuseMaxS;
+  // separate residual and restriction passes and separate correction and
+  // sweeps: the one-pass variants recompute (27 residuals per coarse node, 7
+  // prolongations per sweep node) and measured slower on gfx950
+  const bool small = b->fuse_small && b->maxS[l] <= kFuseMaxS;
   {
     BTimed t(b, F_RESTRICT_C);
     const int ni = b->ncnodes[l];
@@ -577,7 +586,7 @@ int b_vcycle(rq_batch* b, bool fuse_itp, bool* did_itp) {
     }
     std::swap(b->P, b->P2);
   }
-  const bool small = b->maxS[0] <= kFuseMaxS;
+  const bool small = b->fuse_small && b->maxS[0] <= kFuseMaxS;
   {
     BTimed t(b, F_RESTRICT);
     const int ni = b->ncnodes[0];
@@ -785,6 +794,7 @@ int rq_batch_create(const rq_params* pp, const rq_molecule* mols, int32_t n_mols
   if (L < 2 || L > 10) return bfail(nullptr, RQ_EINVAL, "mg_levels %d outside [2, 10]", L);
   rq_batch* b = new rq_batch();
   b->p = p;
+  if (const char* fs = std::getenv("RQ_BATCH_FUSE_SMALL")) b->fuse_small = std::atoi(fs) != 0;
   b->nm = n_mols;
   b->L = L;
   std::vector<Atom> atoms;
