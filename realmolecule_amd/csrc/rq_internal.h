@@ -19,6 +19,7 @@ constexpr int kFuseMaxS = 65;           // one-pass prolongation+sweep up to 65^
 constexpr size_t kFieldStagger = 4096;   // byte offset between consecutive fine fields (mod 2 MiB)
 constexpr int kScoreScanMaxAtoms = 128;  // score rule: scan all atoms up to this count, cell search above
 constexpr int kLegsMaxS = 65;           // one down-leg + one up-leg launch per coarse level up to 65^3 nodes
+constexpr int kResResMarchCJ = 2;       // coarse j-columns per residual+restriction march workgroup (128 CJ threads)
 constexpr int kResResTiledMaxNc = 63;   // 3D-tiled residual+restriction up to 63 coarse interior nodes, march above
 constexpr int kBottomMaxS = 25;         // last two V-cycle levels in one LDS-resident workgroup up to 25^3 nodes
 

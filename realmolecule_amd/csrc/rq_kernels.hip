@@ -1607,7 +1607,7 @@ void launch_residual_restrict(hipStream_t st, bool fine, int Sf, const float* x,
   const int nc = (Sf - 1) / 2 - 1;
   if (nc > kResResTiledMaxNc) {
     // march: 32 x 2 coarse columns per workgroup, CI planes for ~2048 workgroups
-    constexpr int CJ = 2;
+    constexpr int CJ = kResResMarchCJ;
     const int gx = (nc + 31) / 32, gy = (nc + CJ - 1) / CJ;
     const int gz = std::max(1, std::min(nc, 2048 / (gx * gy)));
     const int CI = std::max(4, (nc + gz - 1) / gz);
