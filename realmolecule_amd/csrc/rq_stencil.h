@@ -55,6 +55,7 @@ __device__ __forceinline__ void smooth_col(int S, int k, int j, int i0, int i1, 
   const int S2 = S * S;
   int id = i0 * S2 + j * S + k;
   float pm = P[id - S2], pc = P[id];
+#pragma unroll 2
   for (int i = i0; i < i1; ++i, id += S2) {
     const float pp = P[id + S2];
     const float s6 = pp + pm + P[id + S] + P[id - S] + P[id + 1] + P[id - 1];
@@ -184,6 +185,7 @@ __device__ __forceinline__ void smooth_itp_col(int S, int k, int j, int i0, int 
   float pm = P[id - S2], pc = P[id];
   float um = U[id - S2], uc = U[id];
   uint16_t lm = lab[id - S2], lc = lab[id];
+#pragma unroll 2
   for (int i = i0; i < i1; ++i, id += S2) {
     const float pp = P[id + S2];
     const float s6 = pp + pm + P[id + S] + P[id - S] + P[id + 1] + P[id - 1];
