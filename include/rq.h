@@ -123,8 +123,19 @@ typedef struct rq_params {
   float   ls_curv_reg;    /* level set: curvature weight (0.15, molecule.js:34) */
   int32_t energy_mode;    /* RQ_ENERGY_* */
   int32_t rc_dirichlet;   /* 1: molecule.js rc rule in the ITP update (U'=0 inside own rc) */
-  int32_t reserved[8];    /* must be zero */
+  int32_t pother_mode;    /* RQ_POTHER_*: potential the ITP, energy and forces see */
+  int32_t reserved[7];    /* must be zero */
 } rq_params;
+
+/* pother_mode.  TOTAL: P solves the Poisson equation of all electrons (the
+ * cloud/solver.py potential).  DIRECT: molecule.js "direct Pother"
+ * (molecule.js:597-621, 664-688, 2834-2858): for every domain m with Z > 0 a
+ * persistent P_m solves nabla^2 P_m = -2 pi rho_other_m (the density of all
+ * OTHER domains) with one multigrid V-cycle per Poisson step, and P holds
+ * P_label(x)(x) -- each electron sees the others' repulsion, not its own.
+ * Needs RQ_POISSON_MULTIGRID. */
+#define RQ_POTHER_TOTAL 0
+#define RQ_POTHER_DIRECT 1
 
 /* Scheduling intervals of the run() loop (cloud/solver.py:550-586). */
 typedef struct rq_schedule {

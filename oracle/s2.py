@@ -14,6 +14,9 @@ Parity pinning (tests/test_oracle_s2.py):
                flips only to a neighbouring label at voxels with a foreign neighbour)
   energy_s2    explicit-loop restatement; dipole of a symmetric density
   gradp_forces linear P -> exact gradient (closed form); ball cell count
+  pother_direct linearity: the per-domain potentials of a two-domain density
+               sum to the total potential (same V-cycle, rhs split by label);
+               the gathered P equals the other domain's potential
 """
 from __future__ import annotations
 
@@ -255,3 +258,36 @@ class S2State:
         for _ in range(self.ls_steps if self.ls_steps is not None else max(1, n)):
             self.label, self.W = levelset(self.U, self.label, self.W, g, 0.5, self.curv_reg)
         return tot, per, dip
+
+
+def rho_other(U, label, m, alpha, beta, dtype):
+    """molecule.js:597-621 computeRhoOther in the V-cycle scaling: on the
+    interior (alpha U^2) beta where label != m, 0 where label == m."""
+    from .s1 import rho_rhs
+    I = _interior(U.shape[0])
+    r = rho_rhs(U, alpha, beta, dtype)
+    return np.where((label[I] & 0x7FFF) != m, r, dtype(0)).astype(dtype)
+
+
+def pother_direct(U, label, Pdom, Z, g: Grid, levels, nu1, nu2, a, c):
+    """One direct-Pother Poisson step (molecule.js:2834-2858, with one
+    multigrid V-cycle per domain in place of JACOBI_DIRECT sweeps): for every
+    domain m with Z[m] > 0, P_m <- vcycle(P_m, rho_other_m); then
+    P(x) = P_label(x)(x) on the interior (copyPotherForLabel, :664-688).
+    Returns (P, Pdom)."""
+    from . import mg
+    dt = g.dtype
+    S = g.S
+    Pdom = [p.copy() for p in Pdom]
+    for m in range(len(Pdom)):
+        if not Z[m] > 0:
+            continue
+        rhs = rho_other(U, label, m, g.two_pi, g.h2, dt)
+        Pdom[m] = mg.vcycle(Pdom[m], rhs, levels, nu1, nu2, a, c, dt)
+    I = _interior(S)
+    lab = (label[I] & 0x7FFF).astype(np.int64)
+    P = np.zeros((S, S, S), dt)
+    stack = np.stack([p[I] for p in Pdom])
+    valid = lab < len(Pdom)
+    P[I] = np.where(valid, np.take_along_axis(stack, np.minimum(lab, len(Pdom) - 1)[None], 0)[0], dt(0))
+    return P, Pdom

@@ -32,6 +32,8 @@ RQ_INIT_S1 = 0
 RQ_INIT_S2 = 1
 RQ_ENERGY_S1 = 0
 RQ_ENERGY_S2 = 1
+RQ_POTHER_TOTAL = 0
+RQ_POTHER_DIRECT = 1
 
 FIELDS = {"U": 0, "U2": 1, "K": 2, "P": 3, "P2": 4, "rho": 5, "label": 6, "bestU": 7, "W": 8}
 FAMILIES = ["itp", "smooth", "smooth_c", "restrict", "restrict_c", "prolong", "prolong_c", "coarse_solve",
@@ -50,7 +52,8 @@ class RqParams(C.Structure):
         ("mg_a", C.c_float), ("mg_c", C.c_float), ("twogrid_interval", C.c_int32),
         ("twogrid_coarse_sweeps", C.c_int32), ("boundary_mode", C.c_int32), ("flip_ratio", C.c_float),
         ("flip_min_foreign", C.c_int32), ("ls_dt_w", C.c_float), ("ls_curv_reg", C.c_float),
-        ("energy_mode", C.c_int32), ("rc_dirichlet", C.c_int32), ("reserved", C.c_int32 * 8),
+        ("energy_mode", C.c_int32), ("rc_dirichlet", C.c_int32), ("pother_mode", C.c_int32),
+        ("reserved", C.c_int32 * 7),
     ]
 
 

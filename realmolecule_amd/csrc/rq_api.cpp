@@ -71,6 +71,11 @@ struct rq_solver {
   int32_t* lock_d = nullptr;
   float *U = nullptr, *U2 = nullptr, *K = nullptr, *P = nullptr, *P2 = nullptr, *bestU = nullptr, *W = nullptr;
   void* arena = nullptr;  // one allocation holding the fine fields and labels
+  // direct per-domain potential (pother_mode DIRECT): persistent P_m per
+  // atom, its device pointer table and the masked right-hand side
+  std::vector<float*> Pdom;
+  float** Pdom_d = nullptr;
+  float* rhs_dom = nullptr;
   uint16_t *lab = nullptr, *lab2 = nullptr;
   // multigrid levels 1..L-1: S, two iterate buffers and the rhs
   int levels = 0;
@@ -418,8 +423,65 @@ int twogrid_correction(rq_solver* s) {
   return RQ_OK;
 }
 
+int copy_field(rq_solver* s, float* dst, const float* src);
+
+// One V-cycle on the fine grid for x (in place, x2 scratch) with the
+// precomputed right-hand side rhs: the vcycle_fine sequence with rhs = src[id]
+// in place of (2 pi U^2) h^2.
+int vcycle_rhs(rq_solver* s, float* x, float* x2, const float* rhs) {
+  const float a = s->p.mg_a, c = s->p.mg_c;
+  float* cur = x;
+  float* oth = x2;
+  for (int it = 0; it < s->p.mg_nu1; ++it) {
+    {
+      Timed t(s, F_SMOOTH);
+      launch_smooth_coarse(s->st, s->tile, s->S, cur, oth, rhs, a, c);
+    }
+    std::swap(cur, oth);
+  }
+  {
+    Timed t(s, F_RESTRICT);
+    if (s->fused) {
+      launch_residual_restrict(s->st, false, s->S, cur, rhs, 1.0f, 1.0f, s->g[1], 1.0f);
+    } else {
+      launch_residual_coarse(s->st, s->tile, s->S, cur, rhs, oth);
+      launch_restrict4(s->st, s->S, oth, s->g[1], 1.0f);
+    }
+  }
+  LAUNCHCHK(s);
+  if (int rc = vcycle_coarse(s, 1)) return rc;
+  {
+    Timed t(s, F_PROLONG);
+    launch_prolong_correct(s->st, s->tile, s->S, cur, s->e0[1], 1.0f);
+  }
+  for (int it = 0; it < s->p.mg_nu2; ++it) {
+    {
+      Timed t(s, F_SMOOTH);
+      launch_smooth_coarse(s->st, s->tile, s->S, cur, oth, rhs, a, c);
+    }
+    std::swap(cur, oth);
+  }
+  LAUNCHCHK(s);
+  if (cur != x) return copy_field(s, x, cur);
+  return RQ_OK;
+}
+
+// molecule.js direct Pother (molecule.js:2834-2858): per domain with Z > 0,
+// rho_other -> one V-cycle on its persistent potential; then the gather.
+int pother_direct_step(rq_solver* s) {
+  for (int m = 0; m < s->n_atoms; ++m) {
+    if (!(s->atoms_h[m].Z > 0.0f)) continue;
+    launch_rho_other(s->st, s->tile, s->S, s->U, s->lab, m, s->two_pi, s->h2, s->rhs_dom);
+    if (int rc = vcycle_rhs(s, s->Pdom[m], s->P2, s->rhs_dom)) return rc;
+  }
+  launch_pother_gather(s->st, s->tile, s->S, s->lab, s->Pdom_d, s->n_atoms, s->P);
+  LAUNCHCHK(s);
+  return RQ_OK;
+}
+
 int poisson_step(rq_solver* s, bool twogrid_due, bool fuse_itp = false, bool* did_itp = nullptr) {
   if (did_itp) *did_itp = false;
+  if (s->p.pother_mode == RQ_POTHER_DIRECT) return pother_direct_step(s);
   if (s->p.poisson_mode == RQ_POISSON_MULTIGRID) return vcycle_fine(s, fuse_itp, did_itp);
   for (int it = 0; it < s->p.jacobi_sweeps; ++it) {
     int rc = smooth_fine(s, s->p.smooth_a, s->p.smooth_c);
@@ -494,6 +556,9 @@ void free_all(rq_solver* s) {
   fr(s->Z_d);
   fr(s->lock_d);
   fr(s->arena);
+  for (float* q : s->Pdom) fr(q);
+  fr(s->Pdom_d);
+  fr(s->rhs_dom);
   fr(s->cell_start_d);
   fr(s->cell_atoms_d);
   fr(s->red);
@@ -661,6 +726,8 @@ int normalize_adj(rq_solver* s, const float* Upre) {
 }
 
 int frozen_run(rq_solver* s, const rq_adjoint_spec* sp, bool backward, double* E3, double* dEdR) {
+  if (s->p.pother_mode != RQ_POTHER_TOTAL)
+    return fail(s, RQ_EINVAL, "the adjoint differentiates the total-potential map (pother_mode TOTAL)");
   if (!sp) return fail(s, RQ_EINVAL, "null spec");
   for (int r : sp->reserved)
     if (r) return fail(s, RQ_EINVAL, "rq_adjoint_spec.reserved must be zero");
@@ -853,6 +920,9 @@ int rq_create(const rq_params* pp, const rq_atom* atoms, int32_t n_atoms, rq_sol
   if (p.boundary_mode < 0 || p.boundary_mode > 2) return fail(nullptr, RQ_EINVAL, "bad boundary_mode");
   if (p.init_mode < 0 || p.init_mode > 1) return fail(nullptr, RQ_EINVAL, "bad init_mode");
   if (p.energy_mode < 0 || p.energy_mode > 1) return fail(nullptr, RQ_EINVAL, "bad energy_mode");
+  if (p.pother_mode < 0 || p.pother_mode > 1) return fail(nullptr, RQ_EINVAL, "bad pother_mode");
+  if (p.pother_mode == RQ_POTHER_DIRECT && p.poisson_mode != RQ_POISSON_MULTIGRID)
+    return fail(nullptr, RQ_EINVAL, "pother_mode DIRECT needs RQ_POISSON_MULTIGRID");
   if (!(p.smooth_c != 0.f) || !(p.mg_c != 0.f)) return fail(nullptr, RQ_EINVAL, "smoother c must be non-zero");
   int levels = 0;
   if (p.poisson_mode == RQ_POISSON_MULTIGRID) {
@@ -969,6 +1039,22 @@ int rq_create(const rq_params* pp, const rq_atom* atoms, int32_t n_atoms, rq_sol
     ok = alloc((void**)&s->dst_S, sizeof(double) * Sm.size()) && alloc((void**)&s->dst_rden, sizeof(double) * rd.size());
     ok = ok && hipMemcpy(s->dst_S, Sm.data(), sizeof(double) * Sm.size(), hipMemcpyHostToDevice) == hipSuccess &&
          hipMemcpy(s->dst_rden, rd.data(), sizeof(double) * rd.size(), hipMemcpyHostToDevice) == hipSuccess;
+  }
+  if (ok && p.pother_mode == RQ_POTHER_DIRECT) {
+    const double bytes = (double)(n_atoms + 1) * (double)s->S3 * sizeof(float);
+    if (bytes > 64.0 * (1ull << 30)) {
+      free_all(s);
+      delete s;
+      return fail(nullptr, RQ_ENOMEM, "pother_mode DIRECT needs %.1f GiB for %d per-domain potentials", bytes / (1ull << 30),
+                  n_atoms);
+    }
+    s->Pdom.assign(n_atoms, nullptr);
+    for (int a = 0; a < n_atoms && ok; ++a) {
+      ok = alloc((void**)&s->Pdom[a], fb) && hipMemsetAsync(s->Pdom[a], 0, fb, s->st) == hipSuccess;
+    }
+    ok = ok && alloc((void**)&s->rhs_dom, fb) && hipMemsetAsync(s->rhs_dom, 0, fb, s->st) == hipSuccess &&
+         alloc((void**)&s->Pdom_d, sizeof(float*) * n_atoms) &&
+         hipMemcpy(s->Pdom_d, s->Pdom.data(), sizeof(float*) * n_atoms, hipMemcpyHostToDevice) == hipSuccess;
   }
   if (!ok) {
     (void)hipGetLastError();

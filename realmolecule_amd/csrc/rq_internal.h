@@ -76,6 +76,11 @@ bool bottom_fits(int S);
 // up leg (prolongation + correction and nu post-sweeps, nu in 1..3: returns
 // false for other nu without launching).
 void launch_down2(hipStream_t st, int S, const float* g, float* e_out, float* gc, float a, float c);
+// direct per-domain potential: masked right-hand side and the label gather
+void launch_rho_other(hipStream_t st, const Tile& t, int S, const float* U, const uint16_t* lab, int m, float alpha,
+                      float beta, float* rhs);
+void launch_pother_gather(hipStream_t st, const Tile& t, int S, const uint16_t* lab, const float* const* Pdom,
+                          int n_dom, float* P);
 bool launch_up(hipStream_t st, int S, int nu, const float* x, const float* e, float* x_out, const float* g, float a,
                float c);
 void launch_bottom(hipStream_t st, int S, int nu1, int nu2, float a, float c, const float* g, float* e,

@@ -959,6 +959,36 @@ __global__ __launch_bounds__(1024) void k_coarse_exact_rows(int Sc, const float*
   }
 }
 
+// ---- direct per-domain potential (molecule.js "direct Pother") ----
+// rhs_m = (alpha U^2) beta on interior nodes labelled != m, 0 on nodes of m
+// (molecule.js:597-621 computeRhoOther, in the V-cycle's 2 pi h^2 scaling)
+__global__ __launch_bounds__(256) void k_rho_other(int S, int ich, const float* __restrict__ U,
+                                                   const uint16_t* __restrict__ lab, int m, float alpha, float beta,
+                                                   float* __restrict__ rhs) {
+  March q;
+  if (!march_coords(S - 2, ich, q)) return;
+  const int S2 = S * S;
+  for (int i = q.i0; i < q.i1; ++i) {
+    const int id = i * S2 + q.j * S + q.k;
+    const float u = U[id];
+    rhs[id] = ((lab[id] & kLabelMask) != m) ? (alpha * (u * u)) * beta : 0.0f;
+  }
+}
+
+// P(x) = P_label(x)(x) on the interior (molecule.js:664-688 copyPotherForLabel)
+__global__ __launch_bounds__(256) void k_pother_gather(int S, int ich, const uint16_t* __restrict__ lab,
+                                                       const float* const* __restrict__ Pdom, int n_dom,
+                                                       float* __restrict__ P) {
+  March q;
+  if (!march_coords(S - 2, ich, q)) return;
+  const int S2 = S * S;
+  for (int i = q.i0; i < q.i1; ++i) {
+    const int id = i * S2 + q.j * S + q.k;
+    const int L = lab[id] & kLabelMask;
+    P[id] = (L < n_dom) ? Pdom[L][id] : 0.0f;
+  }
+}
+
 __global__ void k_fill(float* __restrict__ p, size_t n, float v) {
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t < n) p[t] = v;
@@ -1637,6 +1667,17 @@ static void down2_tiled(hipStream_t st, int S, const float* g, float* e_out, flo
 
 // 256 threads per workgroup on every level; smaller coarse tiles on smaller
 // levels keep a few nodes per thread (these launches are latency-bound)
+void launch_rho_other(hipStream_t st, const Tile& t, int S, const float* U, const uint16_t* lab, int m, float alpha,
+                      float beta, float* rhs) {
+  hipLaunchKernelGGL(k_rho_other, stencil_grid(S - 2, t), dim3(t.bx, t.by), 0, st, S, t.ich, U, lab, m, alpha, beta,
+                     rhs);
+}
+
+void launch_pother_gather(hipStream_t st, const Tile& t, int S, const uint16_t* lab, const float* const* Pdom,
+                          int n_dom, float* P) {
+  hipLaunchKernelGGL(k_pother_gather, stencil_grid(S - 2, t), dim3(t.bx, t.by), 0, st, S, t.ich, lab, Pdom, n_dom, P);
+}
+
 void launch_down2(hipStream_t st, int S, const float* g, float* e_out, float* gc, float a, float c) {
   const int nc = (S - 1) / 2 - 1;
   if (nc >= 32)

@@ -112,6 +112,9 @@ def make_params(NN, screen, config=None) -> L.RqParams:
     p.init_mode = _INIT[config.get("init", "s1")]
     p.energy_mode = _ENERGY[config.get("energy", "s1")]
     p.rc_dirichlet = int(bool(config.get("rc_dirichlet", False)))
+    # 'total': cloud/solver.py's potential of all electrons; 'direct': the
+    # molecule.js direct Pother (each domain sees the other domains' charge)
+    p.pother_mode = {"total": L.RQ_POTHER_TOTAL, "direct": L.RQ_POTHER_DIRECT}[config.get("pother", "total")]
     if "flip_ratio" in config:
         p.flip_ratio = float(config["flip_ratio"])
     if "ls_dt_w" in config:

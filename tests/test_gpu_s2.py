@@ -135,3 +135,38 @@ def test_twogrid_poisson_step():
     got = cube(s.P.get(), S)
     assert ulp_diff(got, ref) == 0
     s.close()
+
+
+def test_pother_direct_matches_oracle():
+    """Direct per-domain potential (molecule.js direct Pother): two Poisson
+    steps (one V-cycle per domain each, persistent P_m) and the label gather,
+    against the FP32 oracle; then one ITP update with that potential."""
+    rows = three_atoms(32)
+    NN, screen = 32, 32 * 0.25
+    S = NN + 1
+    g = s1.Grid(NN, screen, 0.12, F32)
+    U, lab, K, _ = s1.init_trial(rows, g)
+    U, _ = s1.normalize(U, lab, rows[:, 3], g)
+    s = make_solver(rows, NN, screen, poisson="multigrid", mg_levels=3, pother="direct")
+    s.U.set(U)
+    s.label.set(lab)
+    s._poisson_step()
+    s._poisson_step()
+    got = cube(s.P.get(), S)
+    Pdom = [np.zeros((S, S, S), F32) for _ in range(len(rows))]
+    P, Pdom = s2.pother_direct(U, lab, Pdom, rows[:, 3], g, 3, 2, 2, 1 / 7, 7.0)
+    P, Pdom = s2.pother_direct(U, lab, Pdom, rows[:, 3], g, 3, 2, 2, 1 / 7, 7.0)
+    err = np.abs(got - P).max() / np.abs(P).max()
+    print(f"direct Pother max rel diff {err:.2e}, ulp {ulp_diff(got, P)}")
+    assert err < 1e-6
+    s.K_buf.set(K)
+    s._update()
+    ref = s1.itp_update(U, K, got, lab, g)
+    assert ulp_diff(cube(s.U.get(), S), ref) == 0
+    s.close()
+
+
+def test_pother_direct_rejected_without_multigrid():
+    rows = three_atoms(16)
+    with pytest.raises(Exception):
+        make_solver(rows, 16, 16 * 0.3, poisson="jacobi", pother="direct")

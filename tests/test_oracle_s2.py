@@ -182,3 +182,29 @@ def test_gradp_linear_potential_closed_form():
     F = s2.gradp_forces(P, rows, g, R=5)
     assert np.allclose(F[0], [2 * 2 * a, 2 * 2 * b, 2 * 2 * c], rtol=1e-12)
     assert not F[1].any()
+
+
+def test_pother_direct_splits_the_total_potential():
+    """Two domains: the V-cycle is linear in its right-hand side, so from zero
+    P_0 + P_1 (each solved with the other domain's charge) equals one
+    V-cycle on the total charge; the gathered P holds P_1 on domain 0 and P_0
+    on domain 1."""
+    from oracle import mg
+    NN, screen = 16, 16 * 0.4
+    g = s1.Grid(NN, screen, 0.12, np.float64)
+    S = NN + 1
+    rng = np.random.default_rng(4)
+    U = np.zeros((S, S, S))
+    U[1:-1, 1:-1, 1:-1] = rng.uniform(0.1, 1.0, (S - 2,) * 3)
+    label = np.zeros((S, S, S), np.int32)
+    label[S // 2:] = 1
+    Pdom = [np.zeros((S, S, S)), np.zeros((S, S, S))]
+    P, Pd = s2.pother_direct(U, label, Pdom, [1.0, 1.0], g, 3, 2, 2, 1 / 7, 7.0)
+    tot = mg.vcycle(np.zeros((S, S, S)), s1.rho_rhs(U, g.two_pi, g.h2, np.float64), 3, 2, 2, 1 / 7, 7.0, np.float64)
+    assert np.allclose(Pd[0] + Pd[1], tot, rtol=1e-12, atol=1e-12)
+    I = (slice(1, -1),) * 3
+    li = label[I]
+    assert np.array_equal(P[I][li == 0], Pd[0][I][li == 0])
+    assert np.array_equal(P[I][li == 1], Pd[1][I][li == 1])
+    # domain 0's potential comes from domain 1's charge: larger there on average
+    assert Pd[0][I][li == 1].mean() > Pd[0][I][li == 0].mean()
