@@ -218,6 +218,12 @@ int rq_write_field(rq_solver* h, int32_t field, const void* host, size_t bytes);
 int rq_swap_u(rq_solver* h);                     /* exchange U and U2 (cloud/benchmark.py:104) */
 int rq_field_device_ptr(rq_solver* h, int32_t field, void** dptr);
 int rq_stream(rq_solver* h, void** hip_stream);
+/* Display slice at k = slice_k into host (3 S^2 + 2 S floats, S = NN+1):
+ * density U^2, label + 1000 Z, boundary marks (1 active, 2 broken) as S x S
+ * (i, j) planes, then the K line K(i, k, k) and the cutoff-weight line (all
+ * 1: no W cutoff).  Replaces molecule.js's extract pass + slice readback
+ * (molecule.js:1108-1192, 1818-1835). */
+int rq_display_slice(rq_solver* h, int32_t slice_k, float* host, size_t n_floats);
 
 /* --- differentiation: frozen-label fixed-step trajectory --- */
 /* The differentiated map: K(R) and the trial U0 = Z_L^2/sqrt(pi) exp(-Z_L r~)

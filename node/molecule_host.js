@@ -181,6 +181,12 @@ class MoleculeHost {
 
   readField(name) { return addon.readField(this.handle, name); }
 
+  // molecule.js extract + slice readback (molecule.js:1108-1192, 1818): the
+  // 3 S^2 + 2 S float buffer the page's renderer draws
+  extractSlice(sliceK) {
+    return addon.displaySlice(this.handle, sliceK === undefined ? this.N2 : sliceK);
+  }
+
   destroy() {
     if (this.handle) addon.destroy(this.handle);
     this.handle = null;

@@ -371,6 +371,26 @@ static napi_value js_read_field(napi_env env, napi_callback_info info) {
   return arr;
 }
 
+/* displaySlice(h, k) -> Float32Array of 3 S^2 + 2 S floats: the molecule.js
+ * extract buffer (density, label + 1000 Z, boundary planes; K line; W line) */
+static napi_value js_display_slice(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!get_args(env, info, 2, argv)) return NULL;
+  rq_solver* h = get_handle(env, argv[0]);
+  if (!h) return NULL;
+  int32_t k = 0;
+  NAPI_CALL(env, napi_get_value_int32(env, argv[1], &k));
+  int32_t S = 0;
+  rq_get_info(h, NULL, &S, NULL, NULL, NULL);
+  size_t n = 3 * (size_t)S * S + 2 * (size_t)S;
+  void* data = NULL;
+  napi_value ab, arr;
+  NAPI_CALL(env, napi_create_arraybuffer(env, n * 4, &data, &ab));
+  RQ_CALL(env, h, rq_display_slice(h, k, (float*)data, n));
+  NAPI_CALL(env, napi_create_typedarray(env, napi_float32_array, n, ab, 0, &arr));
+  return arr;
+}
+
 static napi_value js_write_field(napi_env env, napi_callback_info info) {
   napi_value argv[3];
   if (!get_args(env, info, 3, argv)) return NULL;
@@ -484,6 +504,7 @@ static napi_value init(napi_env env, napi_value exports) {
       {"forcesHF", 0, js_forces_hf, 0, 0, 0, napi_enumerable, 0},
       {"forcesGradP", 0, js_forces_gradp, 0, 0, 0, napi_enumerable, 0},
       {"readField", 0, js_read_field, 0, 0, 0, napi_enumerable, 0},
+      {"displaySlice", 0, js_display_slice, 0, 0, 0, napi_enumerable, 0},
       {"writeField", 0, js_write_field, 0, 0, 0, napi_enumerable, 0},
       {"setAtoms", 0, js_set_atoms, 0, 0, 0, napi_enumerable, 0},
       {"setLockGroups", 0, js_set_lock_groups, 0, 0, 0, napi_enumerable, 0},

@@ -291,3 +291,40 @@ def pother_direct(U, label, Pdom, Z, g: Grid, levels, nu1, nu2, a, c):
     valid = lab < len(Pdom)
     P[I] = np.where(valid, np.take_along_axis(stack, np.minimum(lab, len(Pdom) - 1)[None], 0)[0], dt(0))
     return P, Pdom
+
+
+def display_slice(U, label, K, atoms, sk, NN):
+    """molecule.js:1108-1192 extract at k = sk with W cutoff 0, as the flat
+    3 S^2 + 2 S float32 vector: density, label + 1000 Z, boundary marks,
+    K line, cutoff-weight line (all 1)."""
+    S = NN + 1
+    f = np.float32
+    atoms = np.asarray(atoms, np.float64).reshape(-1, 5)
+    out = np.zeros(3 * S * S + 2 * S, f)
+    u = U[:, :, sk].astype(f)
+    lab = (np.asarray(label)[:, :, sk] & 0x7FFF).astype(np.int64)
+    dens = u * u
+    Zl = np.where(lab < len(atoms), atoms[np.minimum(lab, len(atoms) - 1), 3], 0.0).astype(f)
+    inner = np.zeros((S, S), bool)
+    inner[1:NN, 1:NN] = True
+    out[:S * S] = np.where(inner, dens, f(0)).ravel()
+    out[S * S:2 * S * S] = np.where(inner, lab.astype(f) + Zl * f(1000.0), f(0)).ravel()
+    bnd = np.zeros((S, S), f)
+    ii, jj = np.meshgrid(np.arange(S), np.arange(S), indexing="ij")
+    # +i neighbour for 1 < i < NN-1
+    lab_ip = np.zeros_like(lab)
+    lab_ip[:-1] = lab[1:]
+    d_ip = np.zeros_like(dens)
+    d_ip[:-1] = dens[1:]
+    ci = inner & (ii > 1) & (ii < NN - 1) & (lab != lab_ip)
+    bnd = np.where(ci, np.where((dens > f(1e-6)) | (d_ip > f(1e-6)), f(1), f(2)), bnd)
+    lab_jp = np.zeros_like(lab)
+    lab_jp[:, :-1] = lab[:, 1:]
+    d_jp = np.zeros_like(dens)
+    d_jp[:, :-1] = dens[:, 1:]
+    cj = inner & (bnd < f(1.5)) & (jj > 1) & (jj < NN - 1) & (lab != lab_jp)
+    bnd = np.where(cj, np.where((dens > f(1e-6)) | (d_jp > f(1e-6)), np.maximum(bnd, f(1)), f(2)), bnd)
+    out[2 * S * S:3 * S * S] = np.where(inner, bnd, f(0)).ravel()
+    out[3 * S * S:3 * S * S + S] = K[:, sk, sk].astype(f)
+    out[3 * S * S + S:] = f(1)
+    return out

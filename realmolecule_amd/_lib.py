@@ -110,6 +110,7 @@ PROTOTYPES = {
     "rq_swap_u": (C.c_int, [_P]),
     "rq_field_device_ptr": (C.c_int, [_P, _I, C.POINTER(_P)]),
     "rq_stream": (C.c_int, [_P, C.POINTER(_P)]),
+    "rq_display_slice": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_float), C.c_size_t]),
     "rq_solve_frozen": (C.c_int, [_P, C.POINTER(RqAdjointSpec), _D]),
     "rq_adjoint_gradient": (C.c_int, [_P, C.POINTER(RqAdjointSpec), _D, _D]),
     "rq_batch_last_error": (C.c_char_p, [_P]),

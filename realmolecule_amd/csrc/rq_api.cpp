@@ -1452,6 +1452,21 @@ int rq_read_field(rq_solver* h, int32_t f, void* host, size_t bytes) {
   return RQ_OK;
 }
 
+int rq_display_slice(rq_solver* h, int32_t sk, float* host, size_t n) {
+  if (!h || !host) return fail(h, RQ_EINVAL, "null argument");
+  const size_t S = h->S, need = 3 * S * S + 2 * S;
+  if (n != need) return fail(h, RQ_EINVAL, "display slice needs %zu floats, got %zu", need, n);
+  if (sk < 0 || sk > h->NN) return fail(h, RQ_ERANGE, "slice_k %d outside [0, %d]", sk, h->NN);
+  float* tmp = nullptr;
+  HIPCHK(h, hipMallocAsync((void**)&tmp, need * sizeof(float), h->st));
+  launch_display_slice(h->st, h->S, sk, h->U, h->lab, h->K, h->atoms_d, h->n_atoms, tmp);
+  LAUNCHCHK(h);
+  HIPCHK(h, hipMemcpyAsync(host, tmp, need * sizeof(float), hipMemcpyDeviceToHost, h->st));
+  HIPCHK(h, hipFreeAsync(tmp, h->st));
+  HIPCHK(h, hipStreamSynchronize(h->st));
+  return RQ_OK;
+}
+
 int rq_write_field(rq_solver* h, int32_t f, const void* host, size_t bytes) {
   if (!h || !host) return fail(h, RQ_EINVAL, "null argument");
   const size_t need = h->S3 * 4;

@@ -76,6 +76,9 @@ bool bottom_fits(int S);
 // up leg (prolongation + correction and nu post-sweeps, nu in 1..3: returns
 // false for other nu without launching).
 void launch_down2(hipStream_t st, int S, const float* g, float* e_out, float* gc, float a, float c);
+// display slice (3 S^2 + 2 S floats) at k = sk
+void launch_display_slice(hipStream_t st, int S, int sk, const float* U, const uint16_t* lab, const float* K,
+                          const Atom* atoms, int n_atoms, float* out);
 // direct per-domain potential: masked right-hand side and the label gather
 void launch_rho_other(hipStream_t st, const Tile& t, int S, const float* U, const uint16_t* lab, int m, float alpha,
                       float beta, float* rhs);

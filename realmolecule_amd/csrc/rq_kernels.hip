@@ -989,6 +989,52 @@ __global__ __launch_bounds__(256) void k_pother_gather(int S, int ich, const uin
   }
 }
 
+// Display slice at k = sk (molecule.js:1108-1192 extract, W-cutoff 0):
+// out[0 .. S^2) density U^2, out[S^2 .. 2S^2) label + 1000 Z of the label's
+// atom, out[2S^2 .. 3S^2) boundary marks (1 = labels differ along +i or +j
+// with density on either side, 2 = they differ with none), then the K line
+// K(i, sk, sk) and the cutoff-weight line (1 everywhere).
+__global__ void k_display_slice(int S, int sk, const float* __restrict__ U, const uint16_t* __restrict__ lab,
+                                const float* __restrict__ K, const Atom* __restrict__ atoms, int n_atoms,
+                                float* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y * blockDim.y + threadIdx.y;
+  const int NN = S - 1, S2 = S * S, SS = S * S;
+  if (i > NN || j > NN) return;
+  if (j == 0) {
+    out[3 * SS + i] = K[i * S2 + sk * S + sk];
+    out[3 * SS + S + i] = 1.0f;
+  }
+  if (i < 1 || i >= NN || j < 1 || j >= NN) {
+    out[i * S + j] = 0.0f;
+    out[SS + i * S + j] = 0.0f;
+    out[2 * SS + i * S + j] = 0.0f;
+    return;
+  }
+  const int idx = i * S2 + j * S + sk;
+  const float u = U[idx];
+  const int lbl = lab[idx] & kLabelMask;
+  const float Zl = lbl < n_atoms ? atoms[lbl].Z : 0.0f;
+  out[i * S + j] = u * u;
+  out[SS + i * S + j] = (float)lbl + Zl * 1000.0f;
+  const float dens = u * u;
+  float bnd = 0.0f;
+  if (i > 1 && i < NN - 1) {
+    const int nb = idx + S2;
+    if (lbl != (lab[nb] & kLabelMask)) {
+      const float nd = U[nb] * U[nb];
+      bnd = (dens > 1e-6f || nd > 1e-6f) ? 1.0f : 2.0f;
+    }
+  }
+  if (bnd < 1.5f && j > 1 && j < NN - 1) {
+    const int nb = idx + S;
+    if (lbl != (lab[nb] & kLabelMask)) {
+      const float nd = U[nb] * U[nb];
+      bnd = (dens > 1e-6f || nd > 1e-6f) ? fmaxf(bnd, 1.0f) : 2.0f;
+    }
+  }
+  out[2 * SS + i * S + j] = bnd;
+}
+
 __global__ void k_fill(float* __restrict__ p, size_t n, float v) {
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t < n) p[t] = v;
@@ -1676,6 +1722,12 @@ void launch_rho_other(hipStream_t st, const Tile& t, int S, const float* U, cons
 void launch_pother_gather(hipStream_t st, const Tile& t, int S, const uint16_t* lab, const float* const* Pdom,
                           int n_dom, float* P) {
   hipLaunchKernelGGL(k_pother_gather, stencil_grid(S - 2, t), dim3(t.bx, t.by), 0, st, S, t.ich, lab, Pdom, n_dom, P);
+}
+
+void launch_display_slice(hipStream_t st, int S, int sk, const float* U, const uint16_t* lab, const float* K,
+                          const Atom* atoms, int n_atoms, float* out) {
+  hipLaunchKernelGGL(k_display_slice, dim3((S + 15) / 16, (S + 15) / 16), dim3(16, 16), 0, st, S, sk, U, lab, K, atoms,
+                     n_atoms, out);
 }
 
 void launch_down2(hipStream_t st, int S, const float* g, float* e_out, float* gc, float a, float c) {

@@ -422,6 +422,19 @@ class RealQMSolver:
         sched = L.RqSchedule(int(poisson_interval), int(norm_interval), int(bi))
         self._chk(self._lib.rq_step(self._h, int(first_step), int(n_steps), C.byref(sched)))
 
+    def display_slice(self, slice_k=None):
+        """molecule.js's visualisation slice at k = slice_k (default NN/2,
+        molecule.js:1827): dict of the (S, S) density, label + 1000 Z and
+        boundary planes and the K and cutoff-weight lines along i."""
+        S = self.S
+        k = self.NN // 2 if slice_k is None else int(slice_k)
+        out = np.empty(3 * S * S + 2 * S, np.float32)
+        self._chk(self._lib.rq_display_slice(self._h, k, out.ctypes.data_as(C.POINTER(C.c_float)), out.size))
+        SS = S * S
+        return {"density": out[:SS].reshape(S, S), "labelZ": out[SS:2 * SS].reshape(S, S),
+                "boundary": out[2 * SS:3 * SS].reshape(S, S), "K_line": out[3 * SS:3 * SS + S],
+                "w_line": out[3 * SS + S:], "raw": out}
+
     def domain_norms(self):
         out = (C.c_double * self.nAtoms)()
         self._chk(self._lib.rq_domain_norms(self._h, out))

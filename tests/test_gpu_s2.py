@@ -170,3 +170,23 @@ def test_pother_direct_rejected_without_multigrid():
     rows = three_atoms(16)
     with pytest.raises(Exception):
         make_solver(rows, 16, 16 * 0.3, poisson="jacobi", pother="direct")
+
+
+@pytest.mark.parametrize("sk", [16, 1, 31])
+def test_display_slice_bitexact(sk):
+    rows = three_atoms(32)
+    NN, screen = 32, 32 * 0.25
+    S = NN + 1
+    rng = np.random.default_rng(sk)
+    U = rng.uniform(0.0, 0.01, (S, S, S)).astype(F32)
+    U[rng.random((S, S, S)) < 0.3] = 0.0
+    lab = rng.integers(0, 3, (S, S, S)).astype(np.int32)
+    K = rng.uniform(0, 5, (S, S, S)).astype(F32)
+    s = make_solver(rows, NN, screen)
+    s.U.set(U)
+    s.label.set(lab)
+    s.K_buf.set(K)
+    got = s.display_slice(sk)["raw"]
+    ref = s2.display_slice(U, lab, K, rows, sk, NN)
+    assert np.array_equal(got, ref)
+    s.close()

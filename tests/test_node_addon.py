@@ -16,7 +16,7 @@ needs_node = pytest.mark.skipif(NODE is None or not os.path.exists(ADDON), reaso
 
 EXPORTS = ["abiVersion", "dvForElectrons", "create", "destroy", "info", "initTrial", "initialize", "poisson",
            "twogridCorrect", "update", "normalize", "boundary", "recomputeK", "step", "sync", "energy", "forcesHF",
-           "forcesGradP", "readField", "writeField", "setAtoms", "setLockGroups", "adjointGradient"]
+           "forcesGradP", "readField", "displaySlice", "writeField", "setAtoms", "setLockGroups", "adjointGradient"]
 
 
 def run_node(code, timeout=60):

@@ -208,3 +208,40 @@ def test_pother_direct_splits_the_total_potential():
     assert np.array_equal(P[I][li == 1], Pd[1][I][li == 1])
     # domain 0's potential comes from domain 1's charge: larger there on average
     assert Pd[0][I][li == 1].mean() > Pd[0][I][li == 0].mean()
+
+
+def test_display_slice_matches_scalar_loops():
+    """The vectorised slice against a per-pixel restatement of the WGSL body."""
+    NN = 10
+    S = NN + 1
+    rng = np.random.default_rng(8)
+    U = rng.uniform(0.0, 0.01, (S, S, S)).astype(np.float32)
+    U[:, :, 5][rng.random((S, S)) < 0.3] = 0.0
+    label = rng.integers(0, 3, (S, S, S)).astype(np.int32)
+    K = rng.uniform(0, 5, (S, S, S)).astype(np.float32)
+    atoms = np.array([[3, 3, 3, 1.0, 0], [6, 6, 6, 2.0, 0.4], [2, 8, 5, 4.0, 0.3]])
+    sk = 5
+    got = s2.display_slice(U, label, K, atoms, sk, NN)
+    f = np.float32
+    ref = np.zeros(3 * S * S + 2 * S, f)
+    for i in range(S):
+        ref[3 * S * S + i] = K[i, sk, sk]
+        ref[3 * S * S + S + i] = 1.0
+        for j in range(S):
+            if i < 1 or i >= NN or j < 1 or j >= NN:
+                continue
+            u = f(U[i, j, sk])
+            lbl = int(label[i, j, sk])
+            ref[i * S + j] = u * u
+            ref[S * S + i * S + j] = f(lbl) + f(atoms[lbl, 3]) * f(1000.0)
+            dens = u * u
+            bnd = 0.0
+            if 1 < i < NN - 1 and lbl != label[i + 1, j, sk]:
+                nd = f(U[i + 1, j, sk]) * f(U[i + 1, j, sk])
+                bnd = 1.0 if (dens > f(1e-6) or nd > f(1e-6)) else 2.0
+            if bnd < 1.5 and 1 < j < NN - 1 and lbl != label[i, j + 1, sk]:
+                nd = f(U[i, j + 1, sk]) * f(U[i, j + 1, sk])
+                bnd = max(bnd, 1.0) if (dens > f(1e-6) or nd > f(1e-6)) else 2.0
+            ref[2 * S * S + i * S + j] = bnd
+    assert np.array_equal(got, ref)
+    assert (got[2 * S * S:3 * S * S] == 2).any() and (got[2 * S * S:3 * S * S] == 1).any()
