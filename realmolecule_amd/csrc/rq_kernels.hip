@@ -312,17 +312,18 @@ __device__ __forceinline__ void lds_barrier() {
 // workgroup owns 32 (k) x CJ (j) coarse columns and CI coarse planes; per
 // fine plane it evaluates the residual on the (65 x 2CJ+1) fine region those
 // columns' stencils cover (each thread keeps the i-1 / i values of its nodes
-// in registers, so a fine plane is fetched once) into a 3-plane LDS ring, and
-// after every odd fine plane 2I+1 it restricts coarse plane I.  The loads of
-// the next two fine planes are issued before the restriction's barriers.
-// Same residual_at / restrict27 expressions as k_residual + k_restrict4.
+// in registers, so a fine plane is fetched once) into a 6-plane LDS ring.
+// Each round forms four fine planes and then restricts two coarse planes
+// (fine planes 2I-1 .. 2I+3), so two of every four waves restrict and a
+// barrier pair serves two coarse planes.  Same residual_at / restrict27
+// expressions as k_residual + k_restrict4.
 template <bool FINE, int CJ>
 __global__ __launch_bounds__(128 * CJ) void k_resres_march(int Sf, int CI, const float* __restrict__ x,
                                                             const float* __restrict__ src, float alpha, float beta,
                                                             float* __restrict__ gc, float sign) {
   constexpr int CK = 32, RK = 2 * CK + 1, RJ = 2 * CJ + 1, PL = RK * RJ;
-  constexpr int NT = 128 * CJ, PTS = (PL + NT - 1) / NT;
-  __shared__ float ring[3 * PL];
+  constexpr int NT = 128 * CJ, PTS = (PL + NT - 1) / NT, RING = 6;
+  __shared__ float ring[RING * PL];
   const int nc = (Sf - 1) / 2 - 1, n = Sf - 2;
   const int Sc = nc + 2, S2 = Sf * Sf;
   const int gx = (nc + CK - 1) / CK, gy = (nc + CJ - 1) / CJ;
@@ -344,69 +345,47 @@ __global__ __launch_bounds__(128 * CJ) void k_resres_march(int Sf, int CI, const
     xm[p] = x[(f0 - 1) * S2 + col[p]];
     xc[p] = x[f0 * S2 + col[p]];
   }
-  auto rhs_of = [&](int id) -> float {
-    if (FINE) {
-      const float u = src[id];
-      return (alpha * (u * u)) * beta;
-    }
-    return src[id];
-  };
-  // residual at one region node from its centre line and in-plane neighbours
-  auto store = [&](int f, int p, float xmv, float xcv, float xpv, const float* nb, float rhs) {
-    const float s6 = xpv + xmv + nb[0] + nb[1] + nb[2] + nb[3];
-    const float r = rhs - (6.0f * xcv - s6);
-    if (tid + p * NT < PL) ring[(f % 3) * PL + tid + p * NT] = in[p] ? r : 0.0f;
-  };
-  // plane f0 (the low halo plane of the first coarse plane)
-#pragma unroll
-  for (int p = 0; p < PTS; ++p) {
-    const int id = f0 * S2 + col[p];
-    const float nb[4] = {x[id + Sf], x[id - Sf], x[id + 1], x[id - 1]};
-    const float xp = x[id + S2];
-    store(f0, p, xm[p], xc[p], xp, nb, rhs_of(id));
-    xm[p] = xc[p];
-    xc[p] = xp;
-  }
-  // loads of fine planes fa = 2I and fa + 1
-  float pa[PTS], pb[PTS], na[PTS][4], nb2[PTS][4], ra[PTS], rb[PTS];
-  auto fetch = [&](int fa) {
+  // residual of fine plane f into ring slot f % RING (carried xm / xc advance)
+  auto plane = [&](int f) {
 #pragma unroll
     for (int p = 0; p < PTS; ++p) {
-      const int ida = fa * S2 + col[p], idb = ida + S2;
-      pa[p] = x[idb];
-      pb[p] = x[idb + S2];
-      na[p][0] = x[ida + Sf];
-      na[p][1] = x[ida - Sf];
-      na[p][2] = x[ida + 1];
-      na[p][3] = x[ida - 1];
-      nb2[p][0] = x[idb + Sf];
-      nb2[p][1] = x[idb - Sf];
-      nb2[p][2] = x[idb + 1];
-      nb2[p][3] = x[idb - 1];
-      ra[p] = rhs_of(ida);
-      rb[p] = rhs_of(idb);
+      const int id = f * S2 + col[p];
+      const float xp = x[id + S2];
+      const float s6 = xp + xm[p] + x[id + Sf] + x[id - Sf] + x[id + 1] + x[id - 1];
+      float rhs;
+      if (FINE) {
+        const float u = src[id];
+        rhs = (alpha * (u * u)) * beta;
+      } else {
+        rhs = src[id];
+      }
+      const float r = rhs - (6.0f * xc[p] - s6);
+      if (tid + p * NT < PL) ring[(f % RING) * PL + tid + p * NT] = in[p] ? r : 0.0f;
+      xm[p] = xc[p];
+      xc[p] = xp;
     }
   };
-  fetch(2 * I0);
-  const int tk = tid % CK, tj = tid / CK;
-  for (int I = I0; I < I1; ++I) {
-#pragma unroll
-    for (int p = 0; p < PTS; ++p) {
-      store(2 * I, p, xm[p], xc[p], pa[p], na[p], ra[p]);
-      store(2 * I + 1, p, xc[p], pa[p], pb[p], nb2[p], rb[p]);
-      xm[p] = pa[p];
-      xc[p] = pb[p];
+  plane(f0);
+  // thread -> (coarse plane of the round, coarse column)
+  const int half = tid / (CK * CJ), tk = tid % CK, tj = (tid / CK) % CJ;
+  for (int I = I0; I < I1; I += 2) {
+    const bool two = I + 1 < I1;
+    plane(2 * I);
+    plane(2 * I + 1);
+    if (two) {
+      plane(2 * I + 2);
+      plane(2 * I + 3);
     }
-    if (I + 1 < I1) fetch(2 * I + 2);
     lds_barrier();
-    if (tj < CJ) {
+    const int Ih = I + half;
+    if (half < 2 && Ih < I1) {
       const int K = K0 + tk, J = J0 + tj;
       if (K <= nc && J <= nc) {
         const int c = (2 * tj + 1) * RK + (2 * tk + 1);
-        float out = restrict27_planes(ring, ((2 * I - 1) % 3) * PL + c, ((2 * I) % 3) * PL + c,
-                                      ((2 * I + 1) % 3) * PL + c, RK, 1);
+        float out = restrict27_planes(ring, ((2 * Ih - 1) % RING) * PL + c, ((2 * Ih) % RING) * PL + c,
+                                      ((2 * Ih + 1) % RING) * PL + c, RK, 1);
         if (sign < 0.0f) out = -out;
-        gc[I * Sc * Sc + J * Sc + K] = out;
+        gc[Ih * Sc * Sc + J * Sc + K] = out;
       }
     }
     lds_barrier();
