@@ -98,6 +98,11 @@ struct rq_solver {
   bool fused = true;  // fused multigrid passes (bit-identical to the unfused sequence)
   bool bottom = false;  // last two levels in one LDS-resident workgroup (with fused)
   bool legs = true;     // one launch per coarse level and direction (with fused)
+  bool smooth2 = true;  // sweep pairs as one two-sweep march (with fused); RQ_SMOOTH2=0 turns it off
+  // smallest level (nodes per axis) whose sweep pairs run as one march: below
+  // it a level's three fields stay in the 256 MB MALL and two plain sweeps
+  // measured faster (RQ_SMOOTH2_MIN_S overrides)
+  int smooth2_min_s = kSmooth2MinS;
   // largest level (nodes per axis) that runs the one-pass prolongation+sweep;
   // 0: none -- on gfx950 the separate correction and sweep launches measured
   // faster on every level of the 192^3 workload.  RQ_FUSE_MAX_S overrides it.
@@ -249,6 +254,21 @@ int smooth_fine(rq_solver* s, float a, float c) {
   return RQ_OK;
 }
 
+// count fine sweeps; with fused, pairs on large grids run as one two-sweep march (bit-identical)
+int smooth_fine_n(rq_solver* s, float a, float c, int count) {
+  for (; s->fused && s->smooth2 && s->S >= s->smooth2_min_s && count >= 2; count -= 2) {
+    {
+      Timed t(s, F_SMOOTH);
+      launch_smooth2(s->st, true, s->S, s->P, s->P2, s->U, a, c, s->two_pi, s->h2);
+    }
+    LAUNCHCHK(s);
+    std::swap(s->P, s->P2);
+  }
+  for (; count > 0; --count)
+    if (int rc = smooth_fine(s, a, c)) return rc;
+  return RQ_OK;
+}
+
 // V-cycle on coarse level l (x = e0[l], rhs g[l]); zero initial guess.
 int vcycle_coarse(rq_solver* s, int l) {
   const int S = s->Sl[l];
@@ -332,6 +352,13 @@ int vcycle_coarse(rq_solver* s, int l) {
     Timed t(s, F_PROLONG_C);
     launch_prolong_correct(s->st, tl, S, s->e0[l], s->e0[l + 1], 1.0f);
   }
+  for (; s->fused && s->smooth2 && S >= s->smooth2_min_s && post >= 2; post -= 2) {
+    {
+      Timed t(s, F_SMOOTH_C);
+      launch_smooth2(s->st, false, S, s->e0[l], s->e1[l], s->g[l], a, c, 1.0f, 1.0f);
+    }
+    std::swap(s->e0[l], s->e1[l]);
+  }
   for (int it = 0; it < post; ++it) {
     {
       Timed t(s, F_SMOOTH_C);
@@ -348,10 +375,7 @@ int vcycle_coarse(rq_solver* s, int l) {
 int vcycle_fine(rq_solver* s, bool fuse_itp, bool* did_itp) {
   if (did_itp) *did_itp = false;
   const float a = s->p.mg_a, c = s->p.mg_c;
-  for (int it = 0; it < s->p.mg_nu1; ++it) {
-    int rc = smooth_fine(s, a, c);
-    if (rc) return rc;
-  }
+  if (int rc = smooth_fine_n(s, a, c, s->p.mg_nu1)) return rc;
   const bool small = s->fused && s->S <= s->fuse_max_s;
   {
     Timed t(s, F_RESTRICT);
@@ -378,6 +402,13 @@ int vcycle_fine(rq_solver* s, bool fuse_itp, bool* did_itp) {
     launch_prolong_correct(s->st, s->tile, s->S, s->P, s->e0[1], 1.0f);
   }
   LAUNCHCHK(s);
+  // plain sweeps before the last one (which may carry the ITP update)
+  if (s->fused && fuse_itp && post >= 3) {
+    if (int rc2 = smooth_fine_n(s, a, c, post - 1)) return rc2;
+    post = 1;
+  } else if (!(s->fused && fuse_itp)) {
+    return smooth_fine_n(s, a, c, post);
+  }
   for (int it = 0; it < post; ++it) {
     if (s->fused && fuse_itp && it == post - 1) {
       {
@@ -483,10 +514,7 @@ int poisson_step(rq_solver* s, bool twogrid_due, bool fuse_itp = false, bool* di
   if (did_itp) *did_itp = false;
   if (s->p.pother_mode == RQ_POTHER_DIRECT) return pother_direct_step(s);
   if (s->p.poisson_mode == RQ_POISSON_MULTIGRID) return vcycle_fine(s, fuse_itp, did_itp);
-  for (int it = 0; it < s->p.jacobi_sweeps; ++it) {
-    int rc = smooth_fine(s, s->p.smooth_a, s->p.smooth_c);
-    if (rc) return rc;
-  }
+  if (int rc = smooth_fine_n(s, s->p.smooth_a, s->p.smooth_c, s->p.jacobi_sweeps)) return rc;
   if (s->p.poisson_mode == RQ_POISSON_TWOGRID && twogrid_due) return twogrid_correction(s);
   return RQ_OK;
 }
@@ -945,6 +973,8 @@ int rq_create(const rq_params* pp, const rq_atom* atoms, int32_t n_atoms, rq_sol
   rq_solver* s = new rq_solver();
   s->p = p;
   if (const char* fm = std::getenv("RQ_FUSE_MAX_S")) s->fuse_max_s = std::atoi(fm);
+  if (const char* s2 = std::getenv("RQ_SMOOTH2")) s->smooth2 = std::atoi(s2) != 0;
+  if (const char* s2m = std::getenv("RQ_SMOOTH2_MIN_S")) s->smooth2_min_s = std::atoi(s2m);
   s->NN = p.NN;
   s->S = p.NN + 1;
   s->S3 = (size_t)s->S * s->S * s->S;

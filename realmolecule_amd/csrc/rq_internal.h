@@ -20,6 +20,9 @@ constexpr size_t kFieldStagger = 4096;   // byte offset between consecutive fine
 constexpr int kScoreScanMaxAtoms = 128;  // score rule: scan all atoms up to this count, cell search above
 constexpr int kLegsMaxS = 65;           // one down-leg + one up-leg launch per coarse level up to 65^3 nodes
 constexpr int kResResMarchCJ = 2;       // coarse j-columns per residual+restriction march workgroup (128 CJ threads)
+constexpr int kResResMarchCP = 2;
+constexpr int kSmooth2Blocks = 8192;    // target workgroup count of the two-sweep march
+constexpr int kSmooth2MinS = 320;       // smallest level (nodes per axis) that runs sweep pairs as that march       // coarse planes restricted per barrier round of that march
 constexpr int kResResTiledMaxNc = 63;   // 3D-tiled residual+restriction up to 63 coarse interior nodes, march above
 constexpr int kBottomMaxS = 25;         // last two V-cycle levels in one LDS-resident workgroup up to 25^3 nodes
 
@@ -90,6 +93,9 @@ void launch_bottom(hipStream_t st, int S, int nu1, int nu2, float a, float c, co
                    const double* Smat, const double* rden);
 // fused multigrid passes (bit-identical to the unfused sequences)
 void launch_smooth2_from_zero(hipStream_t st, const Tile& t, int S, const float* g, float* e_out, float a, float c);
+// two damped-Jacobi sweeps x -> xn in one march (equal to two launch_smooth_* calls)
+void launch_smooth2(hipStream_t st, bool fine, int S, const float* x, float* xn, const float* src, float a, float c,
+                    float alpha, float beta);
 void launch_residual_restrict(hipStream_t st, bool fine, int Sf, const float* x, const float* src, float alpha,
                               float beta, float* gc, float sign);
 void launch_prolong_smooth(hipStream_t st, const Tile& t, bool fine, int Sf, const float* x, const float* e,

@@ -317,12 +317,13 @@ __device__ __forceinline__ void lds_barrier() {
 // (fine planes 2I-1 .. 2I+3), so two of every four waves restrict and a
 // barrier pair serves two coarse planes.  Same residual_at / restrict27
 // expressions as k_residual + k_restrict4.
-template <bool FINE, int CJ>
+template <bool FINE, int CJ, int CP>
 __global__ __launch_bounds__(128 * CJ) void k_resres_march(int Sf, int CI, const float* __restrict__ x,
                                                             const float* __restrict__ src, float alpha, float beta,
                                                             float* __restrict__ gc, float sign) {
   constexpr int CK = 32, RK = 2 * CK + 1, RJ = 2 * CJ + 1, PL = RK * RJ;
-  constexpr int NT = 128 * CJ, PTS = (PL + NT - 1) / NT, RING = 6;
+  constexpr int NT = 128 * CJ, PTS = (PL + NT - 1) / NT, RING = 2 * CP + 2;
+  static_assert(CP * CK * CJ <= NT, "one thread per coarse column and plane");
   __shared__ float ring[RING * PL];
   const int nc = (Sf - 1) / 2 - 1, n = Sf - 2;
   const int Sc = nc + 2, S2 = Sf * Sf;
@@ -368,17 +369,17 @@ __global__ __launch_bounds__(128 * CJ) void k_resres_march(int Sf, int CI, const
   plane(f0);
   // thread -> (coarse plane of the round, coarse column)
   const int half = tid / (CK * CJ), tk = tid % CK, tj = (tid / CK) % CJ;
-  for (int I = I0; I < I1; I += 2) {
-    const bool two = I + 1 < I1;
-    plane(2 * I);
-    plane(2 * I + 1);
-    if (two) {
-      plane(2 * I + 2);
-      plane(2 * I + 3);
+  for (int I = I0; I < I1; I += CP) {
+#pragma unroll
+    for (int h = 0; h < CP; ++h) {
+      if (I + h < I1) {
+        plane(2 * (I + h));
+        plane(2 * (I + h) + 1);
+      }
     }
     lds_barrier();
     const int Ih = I + half;
-    if (half < 2 && Ih < I1) {
+    if (half < CP && Ih < I1) {
       const int K = K0 + tk, J = J0 + tj;
       if (K <= nc && J <= nc) {
         const int c = (2 * tj + 1) * RK + (2 * tk + 1);
@@ -387,6 +388,121 @@ __global__ __launch_bounds__(128 * CJ) void k_resres_march(int Sf, int CI, const
         if (sign < 0.0f) out = -out;
         gc[Ih * Sc * Sc + J * Sc + K] = out;
       }
+    }
+    lds_barrier();
+  }
+}
+
+// Two damped-Jacobi sweeps fused as a march along i (temporal blocking).  A
+// workgroup owns TK (k) x TJ (j) interior columns and planes [I0, I1).  The
+// first sweep is formed on the (TK+2) x (TJ+2) region the second sweep's
+// stencils read (each thread carries the i-1 / i values of its region nodes
+// in registers) into an LDS ring of CP+2 planes; face nodes keep x there (the
+// two ping-pong buffers hold the same faces).  Each round forms CP planes of
+// the first sweep and then CP planes of the second from the ring.  Same
+// smooth_col expressions in the same order, so the output equals two k_smooth
+// launches bit for bit, with one read of x and src and one write.
+template <bool FINE, int TK, int TJ, int CP>
+__global__ __launch_bounds__(256) void k_smooth2_march(int S, int CI, const float* __restrict__ x,
+                                                       float* __restrict__ xn, const float* __restrict__ src,
+                                                       float a, float c, float alpha, float beta) {
+  constexpr int NT = 256, RK = TK + 2, RJ = TJ + 2, PL = RK * RJ, PTS = (PL + NT - 1) / NT, RING = CP + 2;
+  constexpr int OUT = TK * TJ * CP, OPT = (OUT + NT - 1) / NT;
+  __shared__ float ring[RING * PL];
+  const int n = S - 2, S2 = S * S;
+  const int gx = (n + TK - 1) / TK, gy = (n + TJ - 1) / TJ;
+  const int t = xcd_block();
+  const int K0 = 1 + (t % gx) * TK, J0 = 1 + ((t / gx) % gy) * TJ, I0 = 1 + (t / (gx * gy)) * CI;
+  const int I1 = min(I0 + CI, n + 1);
+  const int tid = threadIdx.x;
+  // region nodes of this thread: (j, k) clamped into the grid for the loads;
+  // interior nodes read their four in-plane neighbours at offsets oj / ok,
+  // faces (and unused slots beyond S - 1) read themselves and keep x, so every
+  // load of a plane is issued without a branch
+  int col[PTS], oj[PTS], ok[PTS];
+  bool upd[PTS];
+  float xm[PTS], xc[PTS];
+#pragma unroll
+  for (int p = 0; p < PTS; ++p) {
+    const int q = min(tid + p * NT, PL - 1);
+    const int j = J0 - 1 + q / RK, k = K0 - 1 + q % RK;
+    upd[p] = tid + p * NT < PL && j >= 1 && j <= n && k >= 1 && k <= n;
+    oj[p] = upd[p] ? S : 0;
+    ok[p] = upd[p] ? 1 : 0;
+    col[p] = min(j, S - 1) * S + min(k, S - 1);
+    xc[p] = x[(I0 - 1) * S2 + col[p]];
+    xm[p] = (I0 >= 2) ? x[(I0 - 2) * S2 + col[p]] : 0.0f;
+  }
+  // first sweep at plane f (I0 - 1 <= f <= I1) into ring slot f % RING
+  auto sweep1 = [&](int f) {
+    float* dst = ring + (f % RING) * PL + tid;
+    if (f >= 1 && f <= n) {
+      float xp[PTS], nb[PTS][4], u[PTS];
+#pragma unroll
+      for (int p = 0; p < PTS; ++p) {
+        const int id = f * S2 + col[p];
+        xp[p] = x[id + S2];
+        nb[p][0] = x[id + oj[p]];
+        nb[p][1] = x[id - oj[p]];
+        nb[p][2] = x[id + ok[p]];
+        nb[p][3] = x[id - ok[p]];
+        u[p] = src[id];
+      }
+#pragma unroll
+      for (int p = 0; p < PTS; ++p) {
+        const float s6 = xp[p] + xm[p] + nb[p][0] + nb[p][1] + nb[p][2] + nb[p][3];
+        const float rhs = FINE ? (alpha * (u[p] * u[p])) * beta : u[p];
+        float v = (s6 + rhs) / c;
+        if (a != 0.0f) v = a * xc[p] + v;
+        if (tid + p * NT < PL) dst[p * NT] = upd[p] ? v : xc[p];
+        xm[p] = xc[p];
+        xc[p] = xp[p];
+      }
+    } else {
+      // face plane (f == 0 or f == S - 1): the first sweep keeps x
+#pragma unroll
+      for (int p = 0; p < PTS; ++p) {
+        if (tid + p * NT < PL) dst[p * NT] = xc[p];
+        const float xp = (f + 1 <= S - 1) ? x[(f + 1) * S2 + col[p]] : 0.0f;
+        xm[p] = xc[p];
+        xc[p] = xp;
+      }
+    }
+  };
+  // second-sweep outputs of this thread in a round: plane offset, LDS centre
+  int oh[OPT], occ[OPT], oid[OPT];
+#pragma unroll
+  for (int q = 0; q < OPT; ++q) {
+    const int o = min(tid + q * NT, OUT - 1);
+    const int tj = (o / TK) % TJ, tk = o % TK;
+    oh[q] = o / (TK * TJ);
+    occ[q] = (tj + 1) * RK + (tk + 1);
+    const bool inside = tid + q * NT < OUT && J0 + tj <= n && K0 + tk <= n;
+    oid[q] = inside ? (J0 + tj) * S + K0 + tk : -1;
+  }
+  sweep1(I0 - 1);
+  sweep1(I0);
+  for (int I = I0; I < I1; I += CP) {
+#pragma unroll
+    for (int h = 1; h <= CP; ++h)
+      if (I + h <= I1) sweep1(I + h);
+    float u[OPT];
+#pragma unroll
+    for (int q = 0; q < OPT; ++q) {
+      const int Ih = min(I + oh[q], I1 - 1);
+      u[q] = src[Ih * S2 + max(oid[q], 0)];
+    }
+    lds_barrier();
+#pragma unroll
+    for (int q = 0; q < OPT; ++q) {
+      const int Ih = I + oh[q];
+      const int cc = occ[q];
+      const int s0 = (Ih % RING) * PL + cc, sm = ((Ih + RING - 1) % RING) * PL + cc, sp = ((Ih + 1) % RING) * PL + cc;
+      const float s6 = ring[sp] + ring[sm] + ring[s0 + RK] + ring[s0 - RK] + ring[s0 + 1] + ring[s0 - 1];
+      const float rhs = FINE ? (alpha * (u[q] * u[q])) * beta : u[q];
+      float v = (s6 + rhs) / c;
+      if (a != 0.0f) v = a * ring[s0] + v;
+      if (oid[q] >= 0 && Ih < I1) xn[Ih * S2 + oid[q]] = v;
     }
     lds_barrier();
   }
@@ -1657,21 +1773,46 @@ void resres_tiled(hipStream_t st, bool fine, int Sf, const float* x, const float
                        gc, sign);
 }
 
+namespace {
+template <int TK, int TJ, int CP>
+void smooth2_march(hipStream_t st, bool fine, int S, const float* x, float* xn, const float* src, float a, float c,
+                   float alpha, float beta) {
+  const int n = S - 2;
+  const int gx = (n + TK - 1) / TK, gy = (n + TJ - 1) / TJ;
+  const int gz = std::max(1, std::min(n, kSmooth2Blocks / (gx * gy)));
+  const int CI = std::max(2 * CP, (n + gz - 1) / gz);
+  const dim3 grid(gx, gy, (n + CI - 1) / CI);
+  if (fine)
+    hipLaunchKernelGGL((k_smooth2_march<true, TK, TJ, CP>), grid, dim3(256), 0, st, S, CI, x, xn, src, a, c, alpha,
+                       beta);
+  else
+    hipLaunchKernelGGL((k_smooth2_march<false, TK, TJ, CP>), grid, dim3(256), 0, st, S, CI, x, xn, src, a, c, 1.0f,
+                       1.0f);
+}
+}  // namespace
+
+void launch_smooth2(hipStream_t st, bool fine, int S, const float* x, float* xn, const float* src, float a, float c,
+                    float alpha, float beta) {
+  // 64 x 8 columns, one plane per round: the fastest of the 32/64 x 4/8,
+  // 1/2-plane tiles on the 513^3 grid
+  smooth2_march<64, 8, 1>(st, fine, S, x, xn, src, a, c, alpha, beta);
+}
+
 void launch_residual_restrict(hipStream_t st, bool fine, int Sf, const float* x, const float* src, float alpha,
                               float beta, float* gc, float sign) {
   const int nc = (Sf - 1) / 2 - 1;
   if (nc > kResResTiledMaxNc) {
     // march: 32 x 2 coarse columns per workgroup, CI planes for ~2048 workgroups
-    constexpr int CJ = kResResMarchCJ;
+    constexpr int CJ = kResResMarchCJ, CP = kResResMarchCP;
     const int gx = (nc + 31) / 32, gy = (nc + CJ - 1) / CJ;
     const int gz = std::max(1, std::min(nc, 2048 / (gx * gy)));
     const int CI = std::max(4, (nc + gz - 1) / gz);
     const dim3 grid(gx, gy, (nc + CI - 1) / CI);
     if (fine)
-      hipLaunchKernelGGL((k_resres_march<true, CJ>), grid, dim3(128 * CJ), 0, st, Sf, CI, x, src, alpha, beta, gc,
+      hipLaunchKernelGGL((k_resres_march<true, CJ, CP>), grid, dim3(128 * CJ), 0, st, Sf, CI, x, src, alpha, beta, gc,
                          sign);
     else
-      hipLaunchKernelGGL((k_resres_march<false, CJ>), grid, dim3(128 * CJ), 0, st, Sf, CI, x, src, 1.0f, 1.0f, gc,
+      hipLaunchKernelGGL((k_resres_march<false, CJ, CP>), grid, dim3(128 * CJ), 0, st, Sf, CI, x, src, 1.0f, 1.0f, gc,
                          sign);
     return;
   }

@@ -82,3 +82,46 @@ def test_level_legs_bitexact(NN, levels, nu2):
     for u, p in out[1:]:
         assert ulp_diff(out[0][0], u) == 0
         assert ulp_diff(out[0][1], p) == 0
+
+
+@pytest.mark.parametrize("NN,sweeps,min_s", [(48, 2, 0), (100, 5, 0), (130, 4, 0), (320, 2, None)])
+def test_sweep_pairs_jacobi_bitexact(NN, sweeps, min_s, monkeypatch):
+    """Jacobi sweep pairs run as one two-sweep march (fused) and equal the
+    one-sweep launches bit for bit, on grids that are no multiple of the
+    march's 64 x 8 column tile, with an odd count leaving one single sweep.
+    Small grids lower the size gate of the march (RQ_SMOOTH2_MIN_S=0);
+    the 321^3 grid takes it by default."""
+    if min_s is not None:
+        monkeypatch.setenv("RQ_SMOOTH2_MIN_S", str(min_s))
+    rows = three_atoms(NN)
+    out = []
+    for mode in (0, 1):
+        s = make_solver(rows, NN, NN * 0.2, poisson="jacobi", jacobi_sweeps=sweeps)
+        s._chk(s._lib.rq_set_fused(s._h, mode))
+        s.initialize()
+        s.step(0, 30, poisson_interval=1)
+        out.append((s.U.get(), s.P.get()))
+        s.close()
+    assert ulp_diff(out[0][0], out[1][0]) == 0
+    assert ulp_diff(out[0][1], out[1][1]) == 0
+
+
+@pytest.mark.parametrize("NN,levels,nu1,nu2", [(136, 4, 3, 4), (192, 4, 2, 2)])
+def test_sweep_pairs_multigrid_bitexact(NN, levels, nu1, nu2, monkeypatch):
+    """Fine and coarse sweep pairs of the V-cycle as two-sweep marches
+    (pre: nu1 sweeps, post: nu2 - 1 plain sweeps before the one fused with
+    the ITP update; coarse post pairs on levels above the per-level legs)
+    against the unfused sequence, with the march's size gate lowered."""
+    monkeypatch.setenv("RQ_SMOOTH2_MIN_S", "0")
+    rows = three_atoms(NN)
+    out = []
+    for mode in (0, 1):
+        s = make_solver(rows, NN, NN * 0.12, poisson="multigrid", mg_levels=levels, mg_nu1=nu1, mg_nu2=nu2,
+                        bootstrap_poisson=2)
+        s._chk(s._lib.rq_set_fused(s._h, mode))
+        s.initialize()
+        s.step(0, 8, poisson_interval=1)
+        out.append((s.U.get(), s.P.get()))
+        s.close()
+    assert ulp_diff(out[0][0], out[1][0]) == 0
+    assert ulp_diff(out[0][1], out[1][1]) == 0
