@@ -198,7 +198,7 @@ def energy_and_gradient(rows, label, NN, screen, n_steps, **kw):
     E, *_ = trajectory_energy(R, rows, label, NN, screen, n_steps, **kw)
     E.backward()
     h = screen / NN
-    return float(E), R.grad.numpy() / h
+    return float(E.detach()), R.grad.numpy() / h
 
 
 def frozen_labels(rows, NN, screen):
