@@ -70,9 +70,30 @@ class RqAdjointSpec(C.Structure):
                 ("schedule", RqSchedule), ("reserved", C.c_int32 * 4)]
 
 
+RQ_MF_MAX_ELEC = 16
+RQ_MF_SPLIT = {"sphere": 0, "hemi": 2, "third": 3, "tetra": 4, "hemi_third": 5}
+RQ_MF_FIELDS = {"U": 0, "W": 1, "P": 2, "K": 3}
+
+
+class RqMfNucleus(C.Structure):
+    _fields_ = [("i", C.c_double), ("j", C.c_double), ("k", C.c_double), ("Z", C.c_double), ("rc", C.c_double),
+                ("r_out", C.c_double), ("tilt_x", C.c_double), ("tilt_y", C.c_double), ("hs", C.c_double),
+                ("z_init", C.c_double), ("target", C.c_double), ("axis", C.c_double * 3), ("split", C.c_int32),
+                ("split_idx", C.c_int32), ("split_fix", C.c_int32), ("reserved", C.c_int32)]
+
+
+class RqMfParams(C.Structure):
+    _fields_ = [("NN", C.c_int32), ("shell_mode", C.c_int32), ("full_self", C.c_int32), ("vcycle", C.c_int32),
+                ("vcycle_interval", C.c_int32), ("coarse_iters", C.c_int32), ("norm_interval", C.c_int32),
+                ("reserved0", C.c_int32), ("screen", C.c_double), ("kinetic_coeff", C.c_double),
+                ("kc_multi", C.c_double), ("beta_pauli", C.c_double), ("alpha_tf", C.c_double),
+                ("field_x", C.c_double), ("reserved", C.c_int32 * 8)]
+
+
 _P = C.c_void_p
 _I = C.c_int32
 _D = C.POINTER(C.c_double)
+_F = C.POINTER(C.c_float)
 
 # name -> (restype, argtypes)
 PROTOTYPES = {
@@ -128,6 +149,23 @@ PROTOTYPES = {
     "rq_batch_timing_read": (C.c_int, [_P, _D, C.POINTER(C.c_int64), _I]),
     "rq_timing_enable": (C.c_int, [_P, _I]),
     "rq_timing_read": (C.c_int, [_P, _D, C.POINTER(C.c_int64), _I]),
+    # include/rq_molfast.h
+    "rq_mf_default_params": (None, [C.POINTER(RqMfParams)]),
+    "rq_mf_last_error": (C.c_char_p, [_P]),
+    "rq_mf_create": (C.c_int, [C.POINTER(RqMfParams), C.POINTER(RqMfNucleus), _I, C.POINTER(_P)]),
+    "rq_mf_destroy": (C.c_int, [_P]),
+    "rq_mf_sync": (C.c_int, [_P]),
+    "rq_mf_init": (C.c_int, [_P]),
+    "rq_mf_do_steps": (C.c_int, [_P, _I, _D]),
+    "rq_mf_set_r_out": (C.c_int, [_P, _I]),
+    "rq_mf_forces": (C.c_int, [_P, _F]),
+    "rq_mf_set_positions": (C.c_int, [_P, _D]),
+    "rq_mf_recompute_k": (C.c_int, [_P]),
+    "rq_mf_extract": (C.c_int, [_P, _F, C.c_size_t]),
+    "rq_mf_read_field": (C.c_int, [_P, _I, _I, _F, C.c_size_t]),
+    "rq_mf_write_field": (C.c_int, [_P, _I, _I, _F, C.c_size_t]),
+    "rq_mf_set_timing": (C.c_int, [_P, _I]),
+    "rq_mf_update_time": (C.c_int, [_P, _D, C.POINTER(_I)]),
 }
 
 _lib = None
