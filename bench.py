@@ -64,6 +64,10 @@ def workload(name):
     elif name == "c5":
         w = B.c5_waters()
         w["desc"] = "1000 H2O (seed 4, O-O>=2.5A) in 58.58 bohr cube, 512^3 FP32, 7-level V-cycle/step"
+    elif name == "mf":
+        w = {"mf": True, "cfg": B.mf_h2o(),
+             "desc": "mol_fast.js H2O (O Z=3 rc=0.5 occupation 3, two H), 200^3 FP32, 3 electrons, "
+                     "fused u/w/P update per step, normalise every 20, 2-grid per electron every 50"}
     else:
         raise SystemExit(f"unknown workload {name}")
     return w
@@ -227,6 +231,78 @@ def bench_c3(steps, warmup):
     }
 
 
+def bench_mf(args):
+    """mol_fast.js water: frames of 500 fused steps (the page's doSteps),
+    inputs resident on the device; roofline of the fused update from HIP
+    events around each launch."""
+    from realmolecule_amd.molfast import MolFast
+    w = workload("mf")
+    host = MolFast(w["cfg"])
+    NN, NE = host.NN, host.NE
+    n3 = (NN - 1) ** 3
+    log(f"[bench] mf: {w['desc']}; warmup {args.warmup} steps")
+    host.do_steps(args.warmup)
+    log(f"[bench] mf: timed {args.steps} steps in frames of 500")
+    left = args.steps
+    t0 = time.perf_counter()
+    while left > 0:
+        n = min(500, left)
+        host.do_steps(n)  # returns after the frame's reduction is copied back
+        left -= n
+    dt = time.perf_counter() - t0
+    ms_per_step = dt / args.steps * 1e3
+    value = n3 * NE * args.steps / dt / 1e9
+    log(f"[bench] mf: {ms_per_step:.4f} ms/step, {value:.2f} Gvoxel-electron updates/s, E={host.E:.6f}")
+    host.set_timing(True)
+    host.do_steps(args.timing_steps)
+    ms, cnt = host.update_time()
+    host.set_timing(False)
+    avg_us = ms / cnt * 1e3
+    bpn = 24.0 * NE + 4.0  # u, w, P in and out per electron, K in
+    achieved = bpn * n3 / (avg_us * 1e-6) / 1e9
+    roof = {"kernel": "mf_update", "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "traffic_unit": "bytes per launch", "alg_bytes_per_node": bpn, "alg_bytes_per_launch": bpn * n3,
+            "avg_us": round(avg_us, 3), "launches": cnt}
+    forces = host.compute_forces()
+    E = host.E
+    K = host.field("K")
+    fields = [np.stack([host.field(f, m) for m in range(NE)]) for f in ("U", "W", "P")]
+    host.close()
+    if not args.no_pmc:
+        log("[bench] mf: HBM traffic of the fused update: rocprofv3 FETCH_SIZE / WRITE_SIZE passes")
+        traffic, info = pmc_traffic("mf", "k_mf_update")
+        if traffic is not None:
+            roof["traffic"] = round(traffic)
+            roof["traffic_over_alg"] = round(traffic / (bpn * n3), 4)
+        roof["traffic_pmc"] = info
+    cpu = None
+    if not args.no_cpu:
+        from oracle import molfast as O
+        nuc = [O.Nucleus(n["i"], n["j"], n["k"], n["Z"], rc=n["rc"]) for n in w["cfg"]["USER_NUCLEI"]]
+        mf = O.MolFast(nuc, NN=NN, screen=w["cfg"]["USER_SCREEN"], targets=w["cfg"]["USER_NORM_TARGETS"])
+        U, W, P = fields
+        steps = 2
+        log(f"[bench] mf: cpu baseline, {steps} fused updates of the NumPy FP32 oracle")
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            U, W, P = O.update(mf, K, U, W, P)
+        cdt = time.perf_counter() - t0
+        cpu = {"value": round(n3 * NE * steps / cdt / 1e9, 6), "unit": "Gvoxel-electron updates/s", "cores": 1,
+               "kind": "port", "sample": f"{steps} fused updates of the same {NN}^3, 3-electron fields (read back "
+                                         f"from the device), NumPy FP32 restatement, {cdt:.1f} s"}
+    return {
+        "metric": "mol_fast fused per-electron steps: Gvoxel-electron updates/s & %HBM roofline",
+        "value": round(value, 3), "unit": "Gvoxel-electron updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms_per_step, 5), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+        "config": {"workload": w["desc"], "NN": NN, "electrons": NE, "interior_voxels": n3,
+                   "steps_per_frame": 500, "parallelism": "single-gpu"},
+        "roofline": roof, "cpu_baseline": cpu, "energy_Ha": E,
+        "forces": [[round(x, 6) for x in f] for f in forces],
+    }
+
+
 def pmc_child(kind, w, steps):
     """Body of one counter pass: either a few eager solver steps or a plain
     device copy of COPY_MIB MiB, three times (the calibration kernel)."""
@@ -238,6 +314,11 @@ def pmc_child(kind, w, steps):
         for _ in range(3):
             b.copy_(a)
         torch.cuda.synchronize()
+        return
+    if w.get("mf"):
+        from realmolecule_amd.molfast import MolFast
+        with MolFast(w["cfg"]) as host:
+            host.do_steps(steps)
         return
     s = make_solver(w, graph=False)
     s.initialize()
@@ -318,7 +399,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=50)
-    ap.add_argument("--workload", default="c2", choices=("c1", "c2", "c3", "c4", "c5"))
+    ap.add_argument("--workload", default="c2", choices=("c1", "c2", "c3", "c4", "c5", "mf"))
     ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--timing-steps", type=int, default=100)
@@ -338,6 +419,9 @@ def main():
         return
     if args.workload == "c3":
         print(json.dumps(bench_c3(args.steps, args.warmup)))
+        return
+    if args.workload == "mf":
+        print(json.dumps(bench_mf(args)))
         return
 
     import ctypes as C

@@ -282,3 +282,20 @@ def gaussian_init(rows, NN, screen, label, seed=0, sigma_H=0.9, sigma_O=0.6, noi
     U[:, 0, :] = U[:, -1, :] = 0
     U[:, :, 0] = U[:, :, -1] = 0
     return U.astype(np.float32).ravel()
+
+
+def mf_h2o(N=200, bond_au=None, screen=12.0):
+    """The mol_fast.js water scenario (mol_fast_h2o.html:18-46): O (Z = 3,
+    rc = 0.5, triply occupied) and two H at 104.5 degrees, all fixed, on an
+    N^3 grid of ``screen`` au; returns the page's USER_* configuration."""
+    A = 1.8897
+    b_oh = bond_au if bond_au is not None else 0.96 * A
+    h = screen / N
+    N2 = round(N / 2)
+    ang = 104.5 * math.pi / 180
+    hx, hy = b_oh * math.sin(ang / 2), b_oh * math.cos(ang / 2)
+    nuclei = [dict(i=N2, j=N2, k=N2, Z=3, rc=0.5, tilt_y=0.0, fixed=True),
+              dict(i=N2 + round(hx / h), j=N2 + round(hy / h), k=N2, Z=1, rc=0.0, fixed=True),
+              dict(i=N2 - round(hx / h), j=N2 + round(hy / h), k=N2, Z=1, rc=0.0, fixed=True)]
+    return {"USER_NN": N, "USER_SCREEN": screen, "USER_STEPS": 100000, "USER_NUCLEI": nuclei,
+            "USER_NORM_TARGETS": [3, 1, 1]}

@@ -283,12 +283,12 @@ __global__ __launch_bounds__(256) void k_mf_update(Kc c, const float* __restrict
   if (k > n || j > n) return;
   const int S = c.S, S2 = c.S2;
   const size_t S3 = (size_t)c.S3;
-  const size_t id = (size_t)i * S2 + (size_t)j * S + k;
+  const int id = i * S2 + j * S + k;  // per-lane offsets 32-bit, electron bases uniform
   float un[NE], pn[NE];
 #pragma unroll
   for (int e = 0; e < NE; ++e) {
-    un[e] = Ui[e * S3 + id];
-    pn[e] = Pi[e * S3 + id];
+    un[e] = (Ui + e * S3)[id];
+    pn[e] = (Pi + e * S3)[id];
   }
   const float Kv = K[id];
   const float fi = (float)i, fj = (float)j, fk = (float)k;
@@ -298,7 +298,10 @@ __global__ __launch_bounds__(256) void k_mf_update(Kc c, const float* __restrict
   for (int e = 0; e < NE; ++e) P_total = P_total + pn[e];
 #pragma unroll
   for (int m = 0; m < NE; ++m) {
-    const size_t o = m * S3 + id;
+    const float* __restrict__ Um = Ui + m * S3;
+    const float* __restrict__ Wm = Wi + m * S3;
+    const float* __restrict__ Pm = Pi + m * S3;
+    const int o = id;
     const Orb& ob = c.orb[m];
     const float uc = un[m];
     float umax = 0.0f;
@@ -307,8 +310,8 @@ __global__ __launch_bounds__(256) void k_mf_update(Kc c, const float* __restrict
       if (e != m && un[e] > umax) umax = un[e];
     const float cm = uc - umax;
     // w: free-boundary advection + diffusion, then cusps / confinement / split
-    const float wc = Wi[o], wip = Wi[o + S2], wim = Wi[o - S2], wjp = Wi[o + S], wjm = Wi[o - S];
-    const float wkp = Wi[o + 1], wkm = Wi[o - 1];
+    const float wc = Wm[o], wip = Wm[o + S2], wim = Wm[o - S2], wjp = Wm[o + S], wjm = Wm[o - S];
+    const float wkp = Wm[o + 1], wkm = Wm[o - 1];
     const float lw = (((((wip + wim) + wjp) + wjm) + wkp) + wkm - 6.0f * wc) * c.inv_h2;
     const float gx = (wip - wim) * c.inv_h, gy = (wjp - wjm) * c.inv_h, gz = (wkp - wkm) * c.inv_h;
     float nw = (wc + ((0.05f * c.dt) * fabsf(cm)) * lw) + ((1.0f * c.dt) * cm) * sqrtf((gx * gx + gy * gy) + gz * gz);
@@ -329,7 +332,7 @@ __global__ __launch_bounds__(256) void k_mf_update(Kc c, const float* __restrict
         if (rca > 0.0f && r < rca) nw = fminf(nw, smooth_rc(r, rca, c.h));
       }
     }
-    Wo[o] = nw;
+    (Wo + m * S3)[o] = nw;
     // u: w-weighted Laplacian + potential terms
     float P_group = 0.0f;
 #pragma unroll
@@ -348,18 +351,18 @@ __global__ __launch_bounds__(256) void k_mf_update(Kc c, const float* __restrict
     for (int e = 0; e < NE; ++e)
       if (e != m) rho_other = rho_other + un[e] * un[e];
     const float V_pauli = c.beta * rho_other;
-    const float uip = Ui[o + S2], uim = Ui[o - S2], ujp = Ui[o + S], ujm = Ui[o - S], ukp = Ui[o + 1],
-                ukm = Ui[o - 1];
+    const float uip = Um[o + S2], uim = Um[o - S2], ujp = Um[o + S], ujm = Um[o - S], ukp = Um[o + 1],
+                ukm = Um[o - 1];
     const float hd = c.half_d * ((selfN > 1.0f) ? c.kc_multi : 1.0f);
     const float A = hd * (((uip - uc) * (wip + nw)) * 0.5f - ((uc - uim) * (nw + wim)) * 0.5f);
     const float B = hd * (((ujp - uc) * (wjp + nw)) * 0.5f - ((uc - ujm) * (nw + wjm)) * 0.5f);
     const float C = hd * (((ukp - uc) * (wkp + nw)) * 0.5f - ((uc - ukm) * (nw + wkm)) * 0.5f);
     const float D = ((c.dt * (((((Kv - Vother) - V_TF) - V_tilt) - V_pauli) - c.field_x * x_au)) * uc) * wc;
-    Uo[o] = (((uc + A) + B) + C) + D;
+    (Uo + m * S3)[o] = (((uc + A) + B) + C) + D;
     // P_m: pseudo-time Jacobi step sourced by electron m's own density
     const float Pc = pn[m];
-    const float s6 = ((((Pi[o + S2] + Pi[o - S2]) + Pi[o + S]) + Pi[o - S]) + Pi[o + 1]) + Pi[o - 1];
-    Po[o] = (Pc + (c.dt * (s6 - 6.0f * Pc)) * c.inv_h2) + ((c.two_pi * c.dt) * uc) * uc;
+    const float s6 = ((((Pm[o + S2] + Pm[o - S2]) + Pm[o + S]) + Pm[o - S]) + Pm[o + 1]) + Pm[o - 1];
+    (Po + m * S3)[o] = (Pc + (c.dt * (s6 - 6.0f * Pc)) * c.inv_h2) + ((c.two_pi * c.dt) * uc) * uc;
   }
 }
 
