@@ -87,6 +87,7 @@ extern "C" {
 #define RQ_FIELD_LABEL  6  /* int32 S^3 on the host (stored as uint16 on the device) */
 #define RQ_FIELD_BESTU  7  /* float32 S^3, trial maximum from initialisation */
 #define RQ_FIELD_W      8  /* float32 S^3, level-set value (RQ_BOUNDARY_LEVELSET) */
+#define RQ_FIELD_POTHER 9  /* float32 S^3, frame potential (RQ_POTHER_FRAME) */
 
 typedef struct rq_solver rq_solver;
 
@@ -124,7 +125,8 @@ typedef struct rq_params {
   int32_t energy_mode;    /* RQ_ENERGY_* */
   int32_t rc_dirichlet;   /* 1: molecule.js rc rule in the ITP update (U'=0 inside own rc) */
   int32_t pother_mode;    /* RQ_POTHER_*: potential the ITP, energy and forces see */
-  int32_t reserved[7];    /* must be zero */
+  int32_t sic_interval;   /* RQ_POTHER_FRAME: self-interaction correction every n frames (0: never) */
+  int32_t reserved[6];    /* must be zero */
 } rq_params;
 
 /* pother_mode.  TOTAL: P solves the Poisson equation of all electrons (the
@@ -136,6 +138,13 @@ typedef struct rq_params {
  * Needs RQ_POISSON_MULTIGRID. */
 #define RQ_POTHER_TOTAL 0
 #define RQ_POTHER_DIRECT 1
+/* FRAME: molecule.js's frame-stale PotherBuf (molecule.js:2436-2470,
+ * 3009-3076).  The ITP update and the energy read Pother, which
+ * rq_pother_frame rebuilds at each frame's end as a copy of P, minus, on
+ * frames that are a multiple of sic_interval, each domain's self-potential
+ * on its own nodes (scaled by 1/Z for Z > 1).  Needs a coarse level
+ * (RQ_POISSON_TWOGRID or RQ_POISSON_MULTIGRID) when sic_interval > 0. */
+#define RQ_POTHER_FRAME 2
 
 /* Scheduling intervals of the run() loop (cloud/solver.py:550-586). */
 typedef struct rq_schedule {
@@ -224,6 +233,13 @@ int rq_stream(rq_solver* h, void** hip_stream);
  * 1: no W cutoff).  Replaces molecule.js's extract pass + slice readback
  * (molecule.js:1108-1192, 1818-1835). */
 int rq_display_slice(rq_solver* h, int32_t slice_k, float* host, size_t n_floats);
+/* End-of-frame potential of RQ_POTHER_FRAME: Pother = P, then for frame > 0
+ * and frame % sic_interval == 0, for every domain m with Z != 0, the
+ * self-potential of its density (4 sweeps from zero, one two-grid
+ * correction, 4 sweeps, in the solver's S2 smoother) is subtracted on the
+ * domain's nodes.  Replaces the Pother block of the molecule.js frame
+ * (molecule.js:3009-3076). */
+int rq_pother_frame(rq_solver* h, int64_t frame);
 
 /* --- differentiation: frozen-label fixed-step trajectory --- */
 /* The differentiated map: K(R) and the trial U0 = Z_L^2/sqrt(pi) exp(-Z_L r~)

@@ -17,6 +17,10 @@ Parity pinning (tests/test_oracle_s2.py):
   pother_direct linearity: the per-domain potentials of a two-domain density
                sum to the total potential (same V-cycle, rhs split by label);
                the gathered P equals the other domain's potential
+  self_potential / pother_frame  linearity (scaling U by sqrt 2 doubles the
+               self-potential), additivity over domains (the self-potentials
+               of all domains sum to the solve of the total density), Pother
+               = P off SIC frames and changed only on the domain's own nodes
 """
 from __future__ import annotations
 
@@ -291,6 +295,51 @@ def pother_direct(U, label, Pdom, Z, g: Grid, levels, nu1, nu2, a, c):
     valid = lab < len(Pdom)
     P[I] = np.where(valid, np.take_along_axis(stack, np.minimum(lab, len(Pdom) - 1)[None], 0)[0], dt(0))
     return P, Pdom
+
+
+def rho_self(U, label, m, alpha, beta, dtype):
+    """molecule.js:572-593 computeRhoSelf in the V-cycle scaling: on the
+    interior (alpha U^2) beta where label == m, else 0."""
+    from .s1 import rho_rhs
+    I = _interior(U.shape[0])
+    r = rho_rhs(U, alpha, beta, dtype)
+    return np.where((label[I] & 0x7FFF) == m, r, dtype(0)).astype(dtype)
+
+
+def self_potential(U, label, m, g: Grid, a, c):
+    """Domain m's self-potential solve of the SIC frame step
+    (molecule.js:3018-3069): from zero, 4 damped sweeps, the sign-inverted
+    two-grid correction (10 coarse sweeps, half the prolongation), 4 sweeps."""
+    from . import mg
+    dt = g.dtype
+    rhs = rho_self(U, label, m, g.two_pi, g.h2, dt)
+    x = np.zeros((g.S,) * 3, dt)
+    for _ in range(4):
+        x = mg.smooth(x, rhs, a, c, dt)
+    x = mg.twogrid_s2(x, rhs, dt, a, c, 10, 0.5)
+    for _ in range(4):
+        x = mg.smooth(x, rhs, a, c, dt)
+    return x
+
+
+def pother_frame(U, label, P, Z, g: Grid, frame, sic_interval, a, c):
+    """End-of-frame PotherBuf (molecule.js:3009-3076): a copy of P, minus on
+    SIC frames (frame > 0, frame % sic_interval == 0) each domain's
+    self-potential on its own nodes, scaled by 1/Z for Z > 1
+    (subtractPself, :694-721)."""
+    dt = g.dtype
+    out = P.copy()
+    if sic_interval > 0 and frame > 0 and frame % sic_interval == 0:
+        I = _interior(g.S)
+        lab = label[I] & 0x7FFF
+        for m, z in enumerate(Z):
+            z = dt(z)
+            if z == 0:
+                continue
+            x = self_potential(U, label, m, g, a, c)
+            frac = dt(dt(1.0) / z) if z > 1 else dt(1.0)
+            out[I] = np.where(lab == m, out[I] - x[I] * frac, out[I])
+    return out
 
 
 def display_slice(U, label, K, atoms, sk, NN):

@@ -34,8 +34,9 @@ RQ_ENERGY_S1 = 0
 RQ_ENERGY_S2 = 1
 RQ_POTHER_TOTAL = 0
 RQ_POTHER_DIRECT = 1
+RQ_POTHER_FRAME = 2
 
-FIELDS = {"U": 0, "U2": 1, "K": 2, "P": 3, "P2": 4, "rho": 5, "label": 6, "bestU": 7, "W": 8}
+FIELDS = {"U": 0, "U2": 1, "K": 2, "P": 3, "P2": 4, "rho": 5, "label": 6, "bestU": 7, "W": 8, "Pother": 9}
 FAMILIES = ["itp", "smooth", "smooth_c", "restrict", "restrict_c", "prolong", "prolong_c", "coarse_solve",
             "normalize", "boundary", "energy", "force", "smooth_itp", "bottom"]
 
@@ -53,7 +54,7 @@ class RqParams(C.Structure):
         ("twogrid_coarse_sweeps", C.c_int32), ("boundary_mode", C.c_int32), ("flip_ratio", C.c_float),
         ("flip_min_foreign", C.c_int32), ("ls_dt_w", C.c_float), ("ls_curv_reg", C.c_float),
         ("energy_mode", C.c_int32), ("rc_dirichlet", C.c_int32), ("pother_mode", C.c_int32),
-        ("reserved", C.c_int32 * 7),
+        ("sic_interval", C.c_int32), ("reserved", C.c_int32 * 6),
     ]
 
 
@@ -132,6 +133,7 @@ PROTOTYPES = {
     "rq_field_device_ptr": (C.c_int, [_P, _I, C.POINTER(_P)]),
     "rq_stream": (C.c_int, [_P, C.POINTER(_P)]),
     "rq_display_slice": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_float), C.c_size_t]),
+    "rq_pother_frame": (C.c_int, [_P, C.c_int64]),
     "rq_solve_frozen": (C.c_int, [_P, C.POINTER(RqAdjointSpec), _D]),
     "rq_adjoint_gradient": (C.c_int, [_P, C.POINTER(RqAdjointSpec), _D, _D]),
     "rq_batch_last_error": (C.c_char_p, [_P]),

@@ -76,6 +76,12 @@ struct rq_solver {
   std::vector<float*> Pdom;
   float** Pdom_d = nullptr;
   float* rhs_dom = nullptr;
+  // frame potential (pother_mode FRAME): Pother, and the self-potential solve's
+  // iterate pair and rhs (sic_interval > 0)
+  float* Pother = nullptr;
+  float* sic_x = nullptr;
+  float* sic_x2 = nullptr;
+  float* sic_g = nullptr;
   uint16_t *lab = nullptr, *lab2 = nullptr;
   // multigrid levels 1..L-1: S, two iterate buffers and the rhs
   int levels = 0;
@@ -522,7 +528,8 @@ int poisson_step(rq_solver* s, bool twogrid_due, bool fuse_itp = false, bool* di
 int update_step(rq_solver* s) {
   {
     Timed t(s, F_ITP);
-    launch_itp(s->st, s->tile, s->S, s->U, s->U2, s->K, s->P, s->lab, s->half_dv, s->dtv);
+    const float* Pv = (s->p.pother_mode == RQ_POTHER_FRAME) ? s->Pother : s->P;
+    launch_itp(s->st, s->tile, s->S, s->U, s->U2, s->K, Pv, s->lab, s->half_dv, s->dtv);
   }
   LAUNCHCHK(s);
   std::swap(s->U, s->U2);
@@ -572,6 +579,7 @@ float* field_ptr(rq_solver* s, int f) {
     case RQ_FIELD_P2: return s->P2;
     case RQ_FIELD_BESTU: return s->bestU;
     case RQ_FIELD_W: return s->W;
+    case RQ_FIELD_POTHER: return s->Pother;
     default: return nullptr;
   }
 }
@@ -587,6 +595,10 @@ void free_all(rq_solver* s) {
   for (float* q : s->Pdom) fr(q);
   fr(s->Pdom_d);
   fr(s->rhs_dom);
+  fr(s->Pother);
+  fr(s->sic_x);
+  fr(s->sic_x2);
+  fr(s->sic_g);
   fr(s->cell_start_d);
   fr(s->cell_atoms_d);
   fr(s->red);
@@ -948,7 +960,12 @@ int rq_create(const rq_params* pp, const rq_atom* atoms, int32_t n_atoms, rq_sol
   if (p.boundary_mode < 0 || p.boundary_mode > 2) return fail(nullptr, RQ_EINVAL, "bad boundary_mode");
   if (p.init_mode < 0 || p.init_mode > 1) return fail(nullptr, RQ_EINVAL, "bad init_mode");
   if (p.energy_mode < 0 || p.energy_mode > 1) return fail(nullptr, RQ_EINVAL, "bad energy_mode");
-  if (p.pother_mode < 0 || p.pother_mode > 1) return fail(nullptr, RQ_EINVAL, "bad pother_mode");
+  if (p.pother_mode < 0 || p.pother_mode > 2) return fail(nullptr, RQ_EINVAL, "bad pother_mode");
+  if (p.sic_interval < 0) return fail(nullptr, RQ_EINVAL, "negative sic_interval");
+  if (p.sic_interval > 0 && p.pother_mode != RQ_POTHER_FRAME)
+    return fail(nullptr, RQ_EINVAL, "sic_interval needs pother_mode FRAME");
+  if (p.pother_mode == RQ_POTHER_FRAME && p.sic_interval > 0 && p.poisson_mode == RQ_POISSON_JACOBI)
+    return fail(nullptr, RQ_EINVAL, "the self-interaction solve needs a coarse level (TWOGRID or MULTIGRID)");
   if (p.pother_mode == RQ_POTHER_DIRECT && p.poisson_mode != RQ_POISSON_MULTIGRID)
     return fail(nullptr, RQ_EINVAL, "pother_mode DIRECT needs RQ_POISSON_MULTIGRID");
   if (!(p.smooth_c != 0.f) || !(p.mg_c != 0.f)) return fail(nullptr, RQ_EINVAL, "smoother c must be non-zero");
@@ -1086,6 +1103,12 @@ int rq_create(const rq_params* pp, const rq_atom* atoms, int32_t n_atoms, rq_sol
          alloc((void**)&s->Pdom_d, sizeof(float*) * n_atoms) &&
          hipMemcpy(s->Pdom_d, s->Pdom.data(), sizeof(float*) * n_atoms, hipMemcpyHostToDevice) == hipSuccess;
   }
+  if (ok && p.pother_mode == RQ_POTHER_FRAME) {
+    ok = alloc((void**)&s->Pother, fb) && hipMemsetAsync(s->Pother, 0, fb, s->st) == hipSuccess;
+    if (p.sic_interval > 0)
+      for (float** q : {&s->sic_x, &s->sic_x2, &s->sic_g})
+        ok = ok && alloc((void**)q, fb) && hipMemsetAsync(*q, 0, fb, s->st) == hipSuccess;
+  }
   if (!ok) {
     (void)hipGetLastError();
     free_all(s);
@@ -1177,6 +1200,9 @@ int rq_initialize(rq_solver* h, int32_t bootstrap) {
     rc = poisson_step(h, false);
     if (rc) return rc;
   }
+  // the frame potential starts as P (molecule.js:2091)
+  if (h->p.pother_mode == RQ_POTHER_FRAME)
+    HIPCHK(h, hipMemcpyAsync(h->Pother, h->P, h->S3 * sizeof(float), hipMemcpyDeviceToDevice, h->st));
   return RQ_OK;
 }
 
@@ -1227,7 +1253,8 @@ static int eager_step(rq_solver* h, int64_t step, const rq_schedule* sc, bool wi
   bool did_itp = false;
   if (step % sc->poisson_interval == 0) {
     const bool tg = h->p.poisson_mode == RQ_POISSON_TWOGRID && step % h->p.twogrid_interval == 0;
-    if ((rc = poisson_step(h, tg, true, &did_itp))) return rc;
+    // the fused last sweep + ITP reads the fresh P; the frame mode's ITP reads Pother
+    if ((rc = poisson_step(h, tg, h->p.pother_mode != RQ_POTHER_FRAME, &did_itp))) return rc;
   }
   if (!did_itp && (rc = update_step(h))) return rc;
   if (step % sc->norm_interval == 0 && (rc = normalize_step(h))) return rc;
@@ -1392,7 +1419,8 @@ int rq_energy(rq_solver* h, double* out6, double* per_domain) {
   zero_doubles(h, h->red, h->red_len);
   {
     Timed t(h, F_ENERGY);
-    launch_energy(h->st, h->tile, h->p.energy_mode, h->S, h->h, h->h3, h->U, h->K, h->P, h->lab, h->atoms_d, nd, per,
+    const float* Pv = (h->p.pother_mode == RQ_POTHER_FRAME) ? h->Pother : h->P;
+    launch_energy(h->st, h->tile, h->p.energy_mode, h->S, h->h, h->h3, h->U, h->K, Pv, h->lab, h->atoms_d, nd, per,
                   dip);
   }
   LAUNCHCHK(h);
@@ -1477,6 +1505,54 @@ int rq_read_field(rq_solver* h, int32_t f, void* host, size_t bytes) {
     float* p = field_ptr(h, f);
     if (!p) return fail(h, RQ_EINVAL, "unknown field %d", f);
     HIPCHK(h, hipMemcpyAsync(host, p, need, hipMemcpyDeviceToHost, h->st));
+  }
+  HIPCHK(h, hipStreamSynchronize(h->st));
+  return RQ_OK;
+}
+
+// Self-potential of domain m (molecule.js:3018-3069): from zero, 4 sweeps of
+// the S2 smoother, one two-grid correction (restricted residual with the
+// inverted sign, 10 coarse sweeps from zero, half the prolongation added),
+// 4 sweeps; the result is in s->sic_x.
+static int sic_solve(rq_solver* s, int m) {
+  const float a = s->p.smooth_a, c = s->p.smooth_c;
+  launch_rho_self(s->st, s->tile, s->S, s->U, s->lab, m, s->two_pi, s->h2, s->sic_g);
+  launch_zero(s->st, s->sic_x, s->S3);
+  for (int it = 0; it < 4; ++it) {
+    launch_smooth_coarse(s->st, s->tile, s->S, s->sic_x, s->sic_x2, s->sic_g, a, c);
+    std::swap(s->sic_x, s->sic_x2);
+  }
+  const int S1 = s->Sl[1];
+  launch_residual_coarse(s->st, s->tile, s->S, s->sic_x, s->sic_g, s->sic_x2);
+  launch_restrict4(s->st, s->S, s->sic_x2, s->g[1], -1.0f);
+  launch_zero(s->st, s->e0[1], (size_t)S1 * S1 * S1);
+  for (int it = 0; it < 10; ++it) {
+    launch_smooth_coarse(s->st, s->ctile[1], S1, s->e0[1], s->e1[1], s->g[1], a, c);
+    std::swap(s->e0[1], s->e1[1]);
+  }
+  launch_prolong_correct(s->st, s->tile, s->S, s->sic_x, s->e0[1], 0.5f);
+  for (int it = 0; it < 4; ++it) {
+    launch_smooth_coarse(s->st, s->tile, s->S, s->sic_x, s->sic_x2, s->sic_g, a, c);
+    std::swap(s->sic_x, s->sic_x2);
+  }
+  LAUNCHCHK(s);
+  return RQ_OK;
+}
+
+int rq_pother_frame(rq_solver* h, int64_t frame) {
+  if (!h) return fail(nullptr, RQ_EINVAL, "null handle");
+  if (h->p.pother_mode != RQ_POTHER_FRAME) return fail(h, RQ_ESTATE, "rq_pother_frame needs pother_mode FRAME");
+  if (!h->initialized) return fail(h, RQ_ESTATE, "rq_pother_frame before initialisation");
+  HIPCHK(h, hipMemcpyAsync(h->Pother, h->P, h->S3 * sizeof(float), hipMemcpyDeviceToDevice, h->st));
+  if (h->p.sic_interval > 0 && frame > 0 && frame % h->p.sic_interval == 0) {
+    for (int m = 0; m < h->n_atoms; ++m) {
+      const float Z = h->atoms_h[m].Z;
+      if (Z == 0.0f) continue;
+      if (int rc = sic_solve(h, m)) return rc;
+      const float frac = (Z > 1.0f) ? 1.0f / Z : 1.0f;
+      launch_sub_pself(h->st, h->tile, h->S, h->sic_x, h->lab, m, frac, h->Pother);
+      LAUNCHCHK(h);
+    }
   }
   HIPCHK(h, hipStreamSynchronize(h->st));
   return RQ_OK;

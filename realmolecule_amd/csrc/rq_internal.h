@@ -85,6 +85,10 @@ void launch_display_slice(hipStream_t st, int S, int sk, const float* U, const u
 // direct per-domain potential: masked right-hand side and the label gather
 void launch_rho_other(hipStream_t st, const Tile& t, int S, const float* U, const uint16_t* lab, int m, float alpha,
                       float beta, float* rhs);
+void launch_rho_self(hipStream_t st, const Tile& t, int S, const float* U, const uint16_t* lab, int m, float alpha,
+                     float beta, float* rhs);
+void launch_sub_pself(hipStream_t st, const Tile& t, int S, const float* Pself, const uint16_t* lab, int m, float frac,
+                      float* Pother);
 void launch_pother_gather(hipStream_t st, const Tile& t, int S, const uint16_t* lab, const float* const* Pdom,
                           int n_dom, float* P);
 bool launch_up(hipStream_t st, int S, int nu, const float* x, const float* e, float* x_out, const float* g, float a,

@@ -1070,6 +1070,34 @@ __global__ __launch_bounds__(256) void k_rho_other(int S, int ich, const float* 
   }
 }
 
+// rhs of domain m's self-potential: (alpha u^2) beta where label == m, else 0
+// (molecule.js:572-593 computeRhoSelf, in the build's rhs form)
+__global__ __launch_bounds__(256) void k_rho_self(int S, int ich, const float* __restrict__ U,
+                                                  const uint16_t* __restrict__ lab, int m, float alpha, float beta,
+                                                  float* __restrict__ rhs) {
+  March q;
+  if (!march_coords(S - 2, ich, q)) return;
+  const int S2 = S * S;
+  for (int i = q.i0; i < q.i1; ++i) {
+    const int id = i * S2 + q.j * S + q.k;
+    const float u = U[id];
+    rhs[id] = ((lab[id] & kLabelMask) == m) ? (alpha * (u * u)) * beta : 0.0f;
+  }
+}
+
+// Pother -= frac * P_self on domain m's nodes (molecule.js:694-721 subtractPself)
+__global__ __launch_bounds__(256) void k_sub_pself(int S, int ich, const float* __restrict__ Pself,
+                                                   const uint16_t* __restrict__ lab, int m, float frac,
+                                                   float* __restrict__ Pother) {
+  March q;
+  if (!march_coords(S - 2, ich, q)) return;
+  const int S2 = S * S;
+  for (int i = q.i0; i < q.i1; ++i) {
+    const int id = i * S2 + q.j * S + q.k;
+    if ((lab[id] & kLabelMask) == m) Pother[id] = Pother[id] - Pself[id] * frac;
+  }
+}
+
 // P(x) = P_label(x)(x) on the interior (molecule.js:664-688 copyPotherForLabel)
 __global__ __launch_bounds__(256) void k_pother_gather(int S, int ich, const uint16_t* __restrict__ lab,
                                                        const float* const* __restrict__ Pdom, int n_dom,
@@ -1837,6 +1865,18 @@ void launch_rho_other(hipStream_t st, const Tile& t, int S, const float* U, cons
                       float beta, float* rhs) {
   hipLaunchKernelGGL(k_rho_other, stencil_grid(S - 2, t), dim3(t.bx, t.by), 0, st, S, t.ich, U, lab, m, alpha, beta,
                      rhs);
+}
+
+void launch_rho_self(hipStream_t st, const Tile& t, int S, const float* U, const uint16_t* lab, int m, float alpha,
+                     float beta, float* rhs) {
+  hipLaunchKernelGGL(k_rho_self, stencil_grid(S - 2, t), dim3(t.bx, t.by), 0, st, S, t.ich, U, lab, m, alpha, beta,
+                     rhs);
+}
+
+void launch_sub_pself(hipStream_t st, const Tile& t, int S, const float* Pself, const uint16_t* lab, int m, float frac,
+                      float* Pother) {
+  hipLaunchKernelGGL(k_sub_pself, stencil_grid(S - 2, t), dim3(t.bx, t.by), 0, st, S, t.ich, Pself, lab, m, frac,
+                     Pother);
 }
 
 void launch_pother_gather(hipStream_t st, const Tile& t, int S, const uint16_t* lab, const float* const* Pdom,

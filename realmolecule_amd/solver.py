@@ -20,7 +20,10 @@ Extra constructor keys (all optional; defaults reproduce cloud/solver.py):
 ``mg_nu2``, ``mg_omega``, ``jacobi_sweeps``, ``smooth_a``, ``smooth_c``,
 ``boundary`` ('score' | 'levelset' | 'off'), ``init`` ('s1' | 's2'),
 ``energy`` ('s1' | 's2'), ``rc_dirichlet`` (bool), ``dv``, ``graph`` (bool: replay
-step chunks from HIP graphs), ``bootstrap_poisson`` (Poisson steps in initialize, 100).
+step chunks from HIP graphs), ``bootstrap_poisson`` (Poisson steps in initialize, 100),
+``pother`` ('total' | 'direct' | 'frame': the potential the ITP update and the
+energy see) and ``sic_interval`` (self-interaction correction every n frames
+with 'frame').
 """
 from __future__ import annotations
 
@@ -114,7 +117,9 @@ def make_params(NN, screen, config=None) -> L.RqParams:
     p.rc_dirichlet = int(bool(config.get("rc_dirichlet", False)))
     # 'total': cloud/solver.py's potential of all electrons; 'direct': the
     # molecule.js direct Pother (each domain sees the other domains' charge)
-    p.pother_mode = {"total": L.RQ_POTHER_TOTAL, "direct": L.RQ_POTHER_DIRECT}[config.get("pother", "total")]
+    p.pother_mode = {"total": L.RQ_POTHER_TOTAL, "direct": L.RQ_POTHER_DIRECT,
+                     "frame": L.RQ_POTHER_FRAME}[config.get("pother", "total")]
+    p.sic_interval = int(config.get("sic_interval", 0))
     if "flip_ratio" in config:
         p.flip_ratio = float(config["flip_ratio"])
     if "ls_dt_w" in config:
@@ -232,6 +237,11 @@ class RealQMSolver:
     @P2.setter
     def P2(self, v):
         self._assign("P2", v)
+
+    @property
+    def Pother(self):
+        """Frame potential of pother 'frame' (molecule.js PotherBuf)."""
+        return self._fields["Pother"]
 
     @property
     def K_buf(self):
@@ -421,6 +431,13 @@ class RealQMSolver:
         bi = boundary_interval if boundary_interval is not None else 5 * norm_interval
         sched = L.RqSchedule(int(poisson_interval), int(norm_interval), int(bi))
         self._chk(self._lib.rq_step(self._h, int(first_step), int(n_steps), C.byref(sched)))
+
+    def pother_frame(self, frame):
+        """End of a molecule.js frame with pother "frame": rebuild the
+        frame potential (P minus the self-interaction terms on SIC frames,
+        molecule.js:3009-3076); the next frame's ITP steps and the energy
+        read it."""
+        self._chk(self._lib.rq_pother_frame(self._h, int(frame)))
 
     def display_slice(self, slice_k=None):
         """molecule.js's visualisation slice at k = slice_k (default NN/2,

@@ -190,3 +190,43 @@ def test_display_slice_bitexact(sk):
     ref = s2.display_slice(U, lab, K, rows, sk, NN)
     assert np.array_equal(got, ref)
     s.close()
+
+
+def test_pother_frame_sic_matches_oracle():
+    """molecule.js frame potential (PotherBuf, :3009-3076): Pother = P after
+    initialisation; after a SIC frame it is P minus each domain's
+    self-potential on its own nodes, bit-exact against the oracle on the
+    device's own fields; the next step's ITP and the energy read it."""
+    rows = three_atoms(32)
+    NN, screen = 32, 32 * 0.25
+    S = NN + 1
+    g = s1.Grid(NN, screen, 0.12, F32)
+    s = make_solver(rows, NN, screen, poisson="twogrid", twogrid_interval=2, smooth_a=0.3333, smooth_c=9.0,
+                    pother="frame", sic_interval=2, bootstrap_poisson=4)
+    s.initialize()
+    assert ulp_diff(cube(s.Pother.get(), S), cube(s.P.get(), S)) == 0
+    s.step(0, 6, poisson_interval=1)
+    U, P, K = cube(s.U.get(), S), cube(s.P.get(), S), cube(s.K_buf.get(), S)
+    lab = cube(s.label.get(), S)
+    s.pother_frame(1)  # not a SIC frame: a copy of P
+    assert ulp_diff(cube(s.Pother.get(), S), P) == 0
+    s.pother_frame(2)
+    got = cube(s.Pother.get(), S)
+    ref = s2.pother_frame(U, lab, P, rows[:, 3], g, 2, 2, 0.3333, 9.0)
+    assert ulp_diff(got, ref) == 0
+    assert np.abs(got - P).max() > 0
+    # the ITP update reads Pother (one step without a Poisson step or normalisation)
+    s.step(6, 1, poisson_interval=10 ** 6, norm_interval=10 ** 6, boundary_interval=10 ** 6)
+    assert ulp_diff(cube(s.U.get(), S), s1.itp_update(U, K, ref, lab, g)) == 0
+    s._compute_energy()
+    tot, _ = s1.energy(cube(s.U.get(), S), K, ref, lab, g, 3)
+    assert np.allclose([s.E_T, s.E_eK, s.E_ee], tot, rtol=1e-12)
+    s.close()
+
+
+def test_pother_frame_sic_needs_a_coarse_level():
+    rows = three_atoms(16)
+    with pytest.raises(Exception):
+        make_solver(rows, 16, 16 * 0.3, poisson="jacobi", pother="frame", sic_interval=1)
+    with pytest.raises(Exception):
+        make_solver(rows, 16, 16 * 0.3, poisson="twogrid", sic_interval=1)  # SIC without the frame mode

@@ -245,3 +245,45 @@ def test_display_slice_matches_scalar_loops():
             ref[2 * S * S + i * S + j] = bnd
     assert np.array_equal(got, ref)
     assert (got[2 * S * S:3 * S * S] == 2).any() and (got[2 * S * S:3 * S * S] == 1).any()
+
+
+def _sic_case(NN=16):
+    from oracle import s1
+    rows = np.array([[5.3, 8.2, 8.1, 1.0, 0.0], [10.6, 8.4, 7.9, 2.0, 0.0]])
+    g = s1.Grid(NN, NN * 0.3, 0.12, np.float32)
+    U, lab, K, _ = s1.init_trial(rows, g)
+    U, _ = s1.normalize(U, lab, rows[:, 3], g)
+    P = np.random.default_rng(3).uniform(0, 1, U.shape).astype(np.float32)
+    return rows, g, U, lab, P
+
+
+def test_self_potential_linear_and_additive():
+    from oracle import s2
+    rows, g, U, lab, P = _sic_case()
+    a, c = 0.3333, 9.0
+    x0 = s2.self_potential(U, lab, 0, g, a, c)
+    x1 = s2.self_potential(U, lab, 1, g, a, c)
+    # linear in the density: U -> sqrt(2) U doubles the self-potential
+    x0b = s2.self_potential((U * np.float32(np.sqrt(2.0))).astype(np.float32), lab, 0, g, a, c)
+    assert np.allclose(x0b, 2 * x0, rtol=1e-5, atol=1e-7)
+    # additive over domains: the solve of the total density (every node one domain)
+    allone = np.where(lab >= 0, 0, 0).astype(lab.dtype)
+    xt = s2.self_potential(U, allone, 0, g, a, c)
+    assert np.allclose(x0 + x1, xt, rtol=1e-5, atol=1e-6)
+    assert x0.max() > 0 and x1.max() > 0
+
+
+def test_pother_frame_schedule_and_mask():
+    from oracle import s2
+    rows, g, U, lab, P = _sic_case()
+    Z = rows[:, 3]
+    assert np.array_equal(s2.pother_frame(U, lab, P, Z, g, 0, 1, 0.3333, 9.0), P)  # frame 0: no SIC
+    assert np.array_equal(s2.pother_frame(U, lab, P, Z, g, 3, 2, 0.3333, 9.0), P)  # not a SIC frame
+    out = s2.pother_frame(U, lab, P, Z, g, 4, 2, 0.3333, 9.0)
+    I = (slice(1, -1),) * 3
+    x1 = s2.self_potential(U, lab, 1, g, 0.3333, 9.0)
+    on1 = (lab[I] & 0x7FFF) == 1
+    # domain 1 (Z = 2) loses half its self-potential on its own nodes only
+    d = (P - out)[I]
+    assert np.allclose(d[on1], 0.5 * x1[I][on1], rtol=1e-6, atol=1e-7)
+    assert np.array_equal(out[0], P[0])  # faces untouched
