@@ -49,6 +49,10 @@ class MoleculeHost {
     this.dynamicsEnabled = !!cfg.USER_DYNAMICS;
     this.curvReg = cfg.USER_CURV_REG !== undefined ? cfg.USER_CURV_REG : 0.15;
     this.threshold = cfg.CONVERGENCE_THRESHOLD || 0;
+    // frame potential (PotherBuf) with self-interaction correction, molecule.js:112, 3009-3076
+    this.POTHER_FRAME = cfg.USER_POTHER_FRAME !== undefined ? !!cfg.USER_POTHER_FRAME : true;
+    this.SIC_INTERVAL = cfg.USER_SIC_INTERVAL !== undefined ? cfg.USER_SIC_INTERVAL :
+      (this.NELEC <= 15 ? 1 : this.NELEC <= 30 ? 5 : 0);
     this.nucVel = this.nucPos.map(() => [0, 0, 0]);
     this.nucForce = this.nucPos.map(() => [0, 0, 0]);
     this.tStep = 0;
@@ -72,12 +76,17 @@ class MoleculeHost {
       twogridInterval: this.POISSON_INTERVAL, twogridCoarseSweeps: 10,
       boundary: this.cfg.FREEZE_BOUNDARY ? 'off' : 'levelset', curvReg: this.curvReg, dtW: 0.5,
       init: 's2', energy: 's2', rcDirichlet: 1,
+      pother: this.POTHER_FRAME ? 'frame' : 'total', sicInterval: this.POTHER_FRAME ? this.SIC_INTERVAL : 0,
     }, this._atomRows());
     addon.initTrial(this.handle);
+    if (this._framePotential()) addon.potherFrame(this.handle, 0);  // PotherBuf = P (molecule.js:2091)
     return this.handle;
   }
 
   isGpuReady() { return this.handle !== null; }
+
+  // PotherBuf stays zero with one electron or without V_ee (molecule.js:3010-3011)
+  _framePotential() { return this.POTHER_FRAME && !this.cfg.NO_VEE && this.NELEC > 1; }
 
   // One frame of molecule.js doSteps (total-Poisson path).
   doSteps(n) {
@@ -93,6 +102,8 @@ class MoleculeHost {
       if ((s + 1) % this.NORM_INTERVAL === 0 || s === n - 1) addon.normalize(h);
       if ((this.tStep + s + 1) % this.N_MOVE === 0) forcesDue = true;
     }
+    // end-of-frame potential, counted from 1 like the page's draw loop
+    if (this._framePotential()) addon.potherFrame(h, this.frameCount + 1);
     const e = addon.energy(h);
     if (!this.cfg.FREEZE_BOUNDARY) {
       for (let w = 0; w < Math.max(1, this.STEPS_PER_FRAME); w++) addon.boundary(h);

@@ -165,6 +165,8 @@ static napi_value js_create(napi_env env, napi_callback_info info) {
   if (prop_str_eq(env, o, "boundary", "off")) p.boundary_mode = RQ_BOUNDARY_OFF;
   if (prop_str_eq(env, o, "init", "s2")) p.init_mode = RQ_INIT_S2;
   if (prop_str_eq(env, o, "energy", "s2")) p.energy_mode = RQ_ENERGY_S2;
+  if (prop_str_eq(env, o, "pother", "frame")) p.pother_mode = RQ_POTHER_FRAME;
+  p.sic_interval = (int32_t)prop_num(env, o, "sicInterval", 0);
   p.rc_dirichlet = (int32_t)prop_num(env, o, "rcDirichlet", 0);
   p.ls_dt_w = (float)prop_num(env, o, "dtW", p.ls_dt_w);
   p.ls_curv_reg = (float)prop_num(env, o, "curvReg", p.ls_curv_reg);
@@ -342,8 +344,8 @@ static int field_id(napi_env env, napi_value v) {
   char buf[32];
   size_t n = 0;
   if (napi_get_value_string_utf8(env, v, buf, sizeof buf, &n) != napi_ok) return -1;
-  static const char* names[] = {"U", "U2", "K", "P", "P2", "rho", "label", "bestU", "W"};
-  for (int f = 0; f < 9; ++f)
+  static const char* names[] = {"U", "U2", "K", "P", "P2", "rho", "label", "bestU", "W", "Pother"};
+  for (int f = 0; f < 10; ++f)
     if (strcmp(buf, names[f]) == 0) return f;
   return -1;
 }
@@ -389,6 +391,18 @@ static napi_value js_display_slice(napi_env env, napi_callback_info info) {
   RQ_CALL(env, h, rq_display_slice(h, k, (float*)data, n));
   NAPI_CALL(env, napi_create_typedarray(env, napi_float32_array, n, ab, 0, &arr));
   return arr;
+}
+
+/* potherFrame(h, frame): end-of-frame potential (molecule.js:3009-3076) */
+static napi_value js_pother_frame(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!get_args(env, info, 2, argv)) return NULL;
+  rq_solver* h = get_handle(env, argv[0]);
+  if (!h) return NULL;
+  int64_t frame = 0;
+  NAPI_CALL(env, napi_get_value_int64(env, argv[1], &frame));
+  RQ_CALL(env, h, rq_pother_frame(h, frame));
+  return NULL;
 }
 
 static napi_value js_write_field(napi_env env, napi_callback_info info) {
@@ -505,6 +519,7 @@ static napi_value init(napi_env env, napi_value exports) {
       {"forcesGradP", 0, js_forces_gradp, 0, 0, 0, napi_enumerable, 0},
       {"readField", 0, js_read_field, 0, 0, 0, napi_enumerable, 0},
       {"displaySlice", 0, js_display_slice, 0, 0, 0, napi_enumerable, 0},
+      {"potherFrame", 0, js_pother_frame, 0, 0, 0, napi_enumerable, 0},
       {"writeField", 0, js_write_field, 0, 0, 0, napi_enumerable, 0},
       {"setAtoms", 0, js_set_atoms, 0, 0, 0, napi_enumerable, 0},
       {"setLockGroups", 0, js_set_lock_groups, 0, 0, 0, napi_enumerable, 0},

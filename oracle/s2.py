@@ -228,10 +228,14 @@ class S2State:
     per step rho + ``jacobi`` damped sweeps (0.3333, 9), the two-grid correction
     when s > 0 and s % poisson_interval == 0, the rc-rule ITP update, normalise at
     (s+1) % 20 == 0 or the frame's last step; per frame the energy and
-    ``ls_steps`` level-set steps.  FP32-faithful when dtype is float32."""
+    ``ls_steps`` level-set steps.  FP32-faithful when dtype is float32.
+    With ``pother_frame`` the ITP update and the energy read the frame
+    potential (PotherBuf), rebuilt after each frame's steps by
+    ``pother_frame`` with SIC every ``sic_interval`` frames (frames counted
+    from 1, as the page's draw loop counts them)."""
 
     def __init__(self, rows, NN, screen, dtype=np.float32, dv=0.12, jacobi=2, poisson_interval=50,
-                 curv_reg=0.15, ls_steps=None):
+                 curv_reg=0.15, ls_steps=None, pother_frame=False, sic_interval=0):
         from . import s1
         self.rows = np.asarray(rows, np.float64)
         self.g = s1.Grid(NN, screen, dv, dtype)
@@ -242,6 +246,10 @@ class S2State:
         self.poisson_interval = poisson_interval
         self.curv_reg = curv_reg
         self.ls_steps = ls_steps
+        self.use_frame = pother_frame
+        self.sic_interval = sic_interval
+        self.frame_no = 0
+        self.Pother = self.P.copy()
 
     def frame(self, n):
         from . import s1, mg
@@ -255,10 +263,16 @@ class S2State:
             if s > 0 and s % self.poisson_interval == 0:
                 self.P = mg.twogrid_s2(self.P, rhs, dt)
             mask = rc_mask(self.label, self.rows, g)
-            self.U = s1.itp_update(self.U, self.K, self.P, self.label, g, rc_mask=mask)
+            Pv = self.Pother if self.use_frame else self.P
+            self.U = s1.itp_update(self.U, self.K, Pv, self.label, g, rc_mask=mask)
             if (s + 1) % 20 == 0 or s == n - 1:
                 self.U, _ = s1.normalize(self.U, self.label, Z, g)
-        tot, per, dip = energy_s2(self.U, self.K, self.P, self.label, self.rows, g)
+        if self.use_frame:
+            self.frame_no += 1
+            self.Pother = pother_frame(self.U, self.label, self.P, Z, g, self.frame_no, self.sic_interval,
+                                       0.3333, 9.0)
+        Pv = self.Pother if self.use_frame else self.P
+        tot, per, dip = energy_s2(self.U, self.K, Pv, self.label, self.rows, g)
         for _ in range(self.ls_steps if self.ls_steps is not None else max(1, n)):
             self.label, self.W = levelset(self.U, self.label, self.W, g, 0.5, self.curv_reg)
         return tot, per, dip

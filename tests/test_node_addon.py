@@ -16,7 +16,7 @@ needs_node = pytest.mark.skipif(NODE is None or not os.path.exists(ADDON), reaso
 
 EXPORTS = ["abiVersion", "dvForElectrons", "create", "destroy", "info", "initTrial", "initialize", "poisson",
            "twogridCorrect", "update", "normalize", "boundary", "recomputeK", "step", "sync", "energy", "forcesHF",
-           "forcesGradP", "readField", "displaySlice", "writeField", "setAtoms", "setLockGroups", "adjointGradient"]
+           "forcesGradP", "readField", "displaySlice", "potherFrame", "writeField", "setAtoms", "setLockGroups", "adjointGradient"]
 
 
 def run_node(code, timeout=60):
@@ -45,7 +45,8 @@ console.log(JSON.stringify(out));
 @pytest.mark.gpu
 def test_js_host_frames_match_s2_oracle(tmp_path):
     """Three 20-step frames of the JS host (two-grid correction every 8 steps,
-    rc rule, level set) against the S2 oracle schedule in FP32."""
+    rc rule, level set, frame potential with SIC) against the S2 oracle
+    schedule in FP32."""
     from oracle import s2
     from tests.helpers import three_atoms, rel_l2
     NN, screen = 24, 24 * 0.3
@@ -64,7 +65,8 @@ def test_js_host_frames_match_s2_oracle(tmp_path):
     S = NN + 1
     U = np.fromfile(tmp_path / "node_U.f32", np.float32).reshape(S, S, S)
     L = np.fromfile(tmp_path / "node_label.i32", np.int32).reshape(S, S, S)
-    st = s2.S2State(rows, NN, screen, np.float32, dv=0.12, poisson_interval=8)
+    # the host's default frame potential: PotherBuf with SIC every frame (3 electrons)
+    st = s2.S2State(rows, NN, screen, np.float32, dv=0.12, poisson_interval=8, pother_frame=True, sic_interval=1)
     for f in range(3):
         tot, per, dip = st.frame(20)
         E_el = tot.sum()
