@@ -76,6 +76,17 @@ struct rq_solver {
   std::vector<float*> Pdom;
   float** Pdom_d = nullptr;
   float* rhs_dom = nullptr;
+  // concurrent per-domain V-cycles (pother_mode DIRECT): domain m runs on
+  // context m % dom.size(), each with its own stream, scratch and coarse levels
+  struct DomCtx {
+    hipStream_t st = nullptr;
+    hipEvent_t done = nullptr;
+    float* x2 = nullptr;
+    float* rhs = nullptr;
+    std::vector<float*> e0, e1, g;
+  };
+  std::vector<DomCtx> dom;
+  hipEvent_t dom_fork = nullptr;
   // frame potential (pother_mode FRAME): Pother, and the self-potential solve's
   // iterate pair and rhs (sic_interval > 0)
   float* Pother = nullptr;
@@ -506,10 +517,43 @@ int vcycle_rhs(rq_solver* s, float* x, float* x2, const float* rhs) {
 // molecule.js direct Pother (molecule.js:2834-2858): per domain with Z > 0,
 // rho_other -> one V-cycle on its persistent potential; then the gather.
 int pother_direct_step(rq_solver* s) {
-  for (int m = 0; m < s->n_atoms; ++m) {
-    if (!(s->atoms_h[m].Z > 0.0f)) continue;
-    launch_rho_other(s->st, s->tile, s->S, s->U, s->lab, m, s->two_pi, s->h2, s->rhs_dom);
-    if (int rc = vcycle_rhs(s, s->Pdom[m], s->P2, s->rhs_dom)) return rc;
+  if (s->dom.empty()) {
+    for (int m = 0; m < s->n_atoms; ++m) {
+      if (!(s->atoms_h[m].Z > 0.0f)) continue;
+      launch_rho_other(s->st, s->tile, s->S, s->U, s->lab, m, s->two_pi, s->h2, s->rhs_dom);
+      if (int rc = vcycle_rhs(s, s->Pdom[m], s->P2, s->rhs_dom)) return rc;
+    }
+  } else {
+    // fork: every context stream waits for the main stream (U current), runs
+    // its domains' V-cycles with its own scratch, and the main stream joins
+    // them before the gather.  The launch code reads the stream and the
+    // coarse buffers from the handle, so each context's are swapped in while
+    // its launches are enqueued (host-side; the streams then run concurrently).
+    HIPCHK(s, hipEventRecord(s->dom_fork, s->st));
+    for (auto& d : s->dom) HIPCHK(s, hipStreamWaitEvent(d.st, s->dom_fork, 0));
+    hipStream_t main_st = s->st;
+    float* main_x2 = s->P2;
+    int rc = RQ_OK;
+    for (int m = 0; m < s->n_atoms && rc == RQ_OK; ++m) {
+      if (!(s->atoms_h[m].Z > 0.0f)) continue;
+      auto& d = s->dom[m % s->dom.size()];
+      s->st = d.st;
+      std::swap(s->e0, d.e0);
+      std::swap(s->e1, d.e1);
+      std::swap(s->g, d.g);
+      launch_rho_other(s->st, s->tile, s->S, s->U, s->lab, m, s->two_pi, s->h2, d.rhs);
+      rc = vcycle_rhs(s, s->Pdom[m], d.x2, d.rhs);
+      std::swap(s->e0, d.e0);
+      std::swap(s->e1, d.e1);
+      std::swap(s->g, d.g);
+      s->st = main_st;
+    }
+    s->P2 = main_x2;
+    if (rc) return rc;
+    for (auto& d : s->dom) {
+      HIPCHK(s, hipEventRecord(d.done, d.st));
+      HIPCHK(s, hipStreamWaitEvent(s->st, d.done, 0));
+    }
   }
   launch_pother_gather(s->st, s->tile, s->S, s->lab, s->Pdom_d, s->n_atoms, s->P);
   LAUNCHCHK(s);
@@ -595,6 +639,16 @@ void free_all(rq_solver* s) {
   for (float* q : s->Pdom) fr(q);
   fr(s->Pdom_d);
   fr(s->rhs_dom);
+  for (auto& d : s->dom) {
+    fr(d.x2);
+    fr(d.rhs);
+    for (float* q : d.e0) fr(q);
+    for (float* q : d.e1) fr(q);
+    for (float* q : d.g) fr(q);
+    if (d.done) (void)hipEventDestroy(d.done);
+    if (d.st) (void)hipStreamDestroy(d.st);
+  }
+  if (s->dom_fork) (void)hipEventDestroy(s->dom_fork);
   fr(s->Pother);
   fr(s->sic_x);
   fr(s->sic_x2);
@@ -1102,6 +1156,29 @@ int rq_create(const rq_params* pp, const rq_atom* atoms, int32_t n_atoms, rq_sol
     ok = ok && alloc((void**)&s->rhs_dom, fb) && hipMemsetAsync(s->rhs_dom, 0, fb, s->st) == hipSuccess &&
          alloc((void**)&s->Pdom_d, sizeof(float*) * n_atoms) &&
          hipMemcpy(s->Pdom_d, s->Pdom.data(), sizeof(float*) * n_atoms, hipMemcpyHostToDevice) == hipSuccess;
+    // concurrent contexts: one per charged domain up to kDomStreams (RQ_DOM_STREAMS overrides; <= 1: sequential)
+    int n_ctx = 0;
+    for (int a = 0; a < n_atoms; ++a) n_ctx += atoms[a].Z > 0.0f;
+    n_ctx = std::min(n_ctx, kDomStreams);
+    if (const char* ds = std::getenv("RQ_DOM_STREAMS")) n_ctx = std::min(n_ctx, std::max(0, std::atoi(ds)));
+    if (ok && n_ctx > 1) {
+      ok = hipEventCreateWithFlags(&s->dom_fork, hipEventDisableTiming) == hipSuccess;
+      s->dom.resize(n_ctx);
+      for (auto& d : s->dom) {
+        ok = ok && hipStreamCreateWithFlags(&d.st, hipStreamNonBlocking) == hipSuccess &&
+             hipEventCreateWithFlags(&d.done, hipEventDisableTiming) == hipSuccess &&
+             alloc((void**)&d.x2, fb) && hipMemsetAsync(d.x2, 0, fb, s->st) == hipSuccess &&
+             alloc((void**)&d.rhs, fb) && hipMemsetAsync(d.rhs, 0, fb, s->st) == hipSuccess;
+        d.e0.assign(levels, nullptr);
+        d.e1.assign(levels, nullptr);
+        d.g.assign(levels, nullptr);
+        for (int l = 1; l < levels && ok; ++l) {
+          const size_t b = (size_t)s->Sl[l] * s->Sl[l] * s->Sl[l] * sizeof(float);
+          for (float** q : {&d.e0[l], &d.e1[l], &d.g[l]})
+            ok = ok && alloc((void**)q, b) && hipMemsetAsync(*q, 0, b, s->st) == hipSuccess;
+        }
+      }
+    }
   }
   if (ok && p.pother_mode == RQ_POTHER_FRAME) {
     ok = alloc((void**)&s->Pother, fb) && hipMemsetAsync(s->Pother, 0, fb, s->st) == hipSuccess;

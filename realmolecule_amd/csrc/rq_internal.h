@@ -21,7 +21,8 @@ constexpr int kScoreScanMaxAtoms = 128;  // score rule: scan all atoms up to thi
 constexpr int kLegsMaxS = 65;           // one down-leg + one up-leg launch per coarse level up to 65^3 nodes
 constexpr int kResResMarchCJ = 2;       // coarse j-columns per residual+restriction march workgroup (128 CJ threads)
 constexpr int kResResMarchCP = 2;
-constexpr int kSmooth2Blocks = 8192;    // target workgroup count of the two-sweep march
+constexpr int kSmooth2Blocks = 8192;
+constexpr int kDomStreams = 4;         // concurrent per-domain V-cycles of the direct potential (one stream each)    // target workgroup count of the two-sweep march
 constexpr int kSmooth2MinS = 320;       // smallest level (nodes per axis) that runs sweep pairs as that march       // coarse planes restricted per barrier round of that march
 constexpr int kResResTiledMaxNc = 63;   // 3D-tiled residual+restriction up to 63 coarse interior nodes, march above
 constexpr int kBottomMaxS = 25;         // last two V-cycle levels in one LDS-resident workgroup up to 25^3 nodes
