@@ -32,6 +32,13 @@ namespace {
 
 constexpr int kMaxE = RQ_MF_MAX_ELEC;
 constexpr int kFinalThreads = 256;
+// LDS plane ring of the update: 2 x 3 NELEC planes of 66 x 6 floats (28.5 KB
+// at 3 electrons, 57 KB at 6); above 6 electrons the per-node kernel runs
+constexpr int kMfLdsMaxE = 6;
+#ifndef MF_LDS_WAVES
+#define MF_LDS_WAVES 3  // waves per SIMD the LDS march is compiled for (4 spills to scratch)
+#endif
+constexpr int kMfLdsBlocks = 4096;  // target workgroups of the LDS march
 
 // FP32 constants of the WGSL passes (writeParams and the templated arrays,
 // mol_fast.js:96-131, 838-876)
@@ -40,6 +47,11 @@ struct Orb {
   float ax, ay, az, rot;               // split axis (normalised, toFixed(6))
   float e1x, e1y, e1z, e2x, e2y, e2z;  // in-plane basis of the sector split
   int split, sidx, fix, group;
+  // per-electron constants of the update, formed on the host in FP32 with the
+  // shader's expressions: the self-interaction fraction f_m and hd = half_d k_mult
+  float fm, hd;
+  // squared grid distance beyond which sqrt(d2) h >= rc surely (skips the cusp test)
+  float rc_skip2;
 };
 
 struct Kc {
@@ -272,7 +284,96 @@ __device__ bool in_sector_f(const Orb& o, float rx, float ry, float rz, float r_
   }
 }
 
-// The fused update (mol_fast.js:155-350) at one interior node for every electron.
+// Per-node quantities shared by every electron's update at that node.
+struct NodeIn {
+  float fi, fj, fk, x_au, y_au, Kv, P_total, tmin;
+};
+
+// frozen-core cusps: w_m = min(w_m, t_a) over every nucleus a with r_a < rc_a;
+// the t_a do not depend on m and min is exact, so their minimum is formed once
+template <int NE>
+__device__ __forceinline__ float cusp_min(const Kc& c, float fi, float fj, float fk) {
+  float tmin = INFINITY;
+  if (!c.shell) {
+#pragma unroll
+    for (int a = 0; a < NE; ++a) {
+      const float rca = c.orb[a].rc;
+      if (rca > 0.0f) {
+        const float dx = fi - c.px[a], dy = fj - c.py[a], dz = fk - c.pz[a];
+        const float d2 = (dx * dx + dy * dy) + dz * dz;
+        if (d2 <= c.orb[a].rc_skip2) {
+          const float r = sqrtf(d2) * c.h;
+          if (r < rca) tmin = fminf(tmin, smooth_rc(r, rca, c.h));
+        }
+      }
+    }
+  }
+  return tmin;
+}
+
+// Electron m's fused update (mol_fast.js:155-350) at one interior node from
+// its 7-point values of w, u and P (centre, +i, -i, +j, -j, +k, -k) and the
+// centre values un / pn of every electron.
+template <int NE>
+__device__ __forceinline__ void mf_electron(const Kc& c, int m, const NodeIn& nd, const float* un, const float* pn,
+                                            const float* w7, const float* u7, const float* p7, float& w_out,
+                                            float& u_out, float& p_out) {
+  const Orb& ob = c.orb[m];
+  const float uc = un[m];
+  float umax = 0.0f;
+#pragma unroll
+  for (int e = 0; e < NE; ++e)
+    if (e != m && un[e] > umax) umax = un[e];
+  const float cm = uc - umax;
+  // w: free-boundary advection + diffusion, then cusps / confinement / split
+  const float wc = w7[0], wip = w7[1], wim = w7[2], wjp = w7[3], wjm = w7[4], wkp = w7[5], wkm = w7[6];
+  const float lw = (((((wip + wim) + wjp) + wjm) + wkp) + wkm - 6.0f * wc) * c.inv_h2;
+  const float gx = (wip - wim) * c.inv_h, gy = (wjp - wjm) * c.inv_h, gz = (wkp - wkm) * c.inv_h;
+  float nw = (wc + ((0.05f * c.dt) * fabsf(cm)) * lw) + ((1.0f * c.dt) * cm) * sqrtf((gx * gx + gy * gy) + gz * gz);
+  nw = fminf(fmaxf(nw, 0.0f), 1.0f);
+  if (c.shell) {
+    const float rx = nd.fi - c.px[m], ry = nd.fj - c.py[m], rz = nd.fk - c.pz[m];
+    const float r_m = sqrtf((rx * rx + ry * ry) + rz * rz) * c.h;
+    const float ro_m = ob.r_out * c.R_out;
+    if (ob.rc > 0.0f && r_m < ob.rc) nw = fminf(nw, smooth_rc(r_m, ob.rc, c.h));
+    if (ro_m > 0.0f && r_m > ro_m) nw = (r_m <= ro_m + c.h) ? nw : 0.0f;
+    if (ob.split != 0 && ob.fix == 1 && !in_sector_f(ob, rx, ry, rz, r_m)) nw = 0.0f;
+  } else {
+    nw = fminf(nw, nd.tmin);
+  }
+  w_out = nw;
+  // u: w-weighted Laplacian + potential terms
+  float P_group = 0.0f;
+#pragma unroll
+  for (int e = 0; e < NE; ++e)
+    if (c.orb[e].group == ob.group) P_group = P_group + pn[e];
+  const float selfN = ob.target;
+  const float Vother = 2.0f * (nd.P_total - ob.fm * P_group);
+  const float rho_m = uc * uc;
+  const float V_TF =
+      (selfN > 1.0f && c.alpha_tf > 0.0f) ? c.alpha_tf * powf(fmaxf(rho_m, 1e-20f), 0.6666667f) : 0.0f;
+  const float V_tilt = (-ob.tilt_x) * nd.x_au - ob.tilt_y * nd.y_au;
+  float rho_other = 0.0f;
+#pragma unroll
+  for (int e = 0; e < NE; ++e)
+    if (e != m) rho_other = rho_other + un[e] * un[e];
+  const float V_pauli = c.beta * rho_other;
+  const float uip = u7[1], uim = u7[2], ujp = u7[3], ujm = u7[4], ukp = u7[5], ukm = u7[6];
+  const float hd = ob.hd;
+  const float A = hd * (((uip - uc) * (wip + nw)) * 0.5f - ((uc - uim) * (nw + wim)) * 0.5f);
+  const float B = hd * (((ujp - uc) * (wjp + nw)) * 0.5f - ((uc - ujm) * (nw + wjm)) * 0.5f);
+  const float C = hd * (((ukp - uc) * (wkp + nw)) * 0.5f - ((uc - ukm) * (nw + wkm)) * 0.5f);
+  const float D = ((c.dt * (((((nd.Kv - Vother) - V_TF) - V_tilt) - V_pauli) - c.field_x * nd.x_au)) * uc) * wc;
+  u_out = (((uc + A) + B) + C) + D;
+  // P_m: pseudo-time Jacobi step sourced by electron m's own density
+  const float Pc = pn[m];
+  const float s6 = ((((p7[1] + p7[2]) + p7[3]) + p7[4]) + p7[5]) + p7[6];
+  p_out = (Pc + (c.dt * (s6 - 6.0f * Pc)) * c.inv_h2) + ((c.two_pi * c.dt) * uc) * uc;
+}
+
+// The fused update at one interior node per thread, every electron; the
+// 7-point values come straight from global memory (used above kMfLdsMaxE
+// electrons, where the LDS ring below would not fit).
 template <int NE>
 __global__ __launch_bounds__(256) void k_mf_update(Kc c, const float* __restrict__ K, const float* __restrict__ Ui,
                                                     const float* __restrict__ Wi, const float* __restrict__ Pi,
@@ -290,79 +391,158 @@ __global__ __launch_bounds__(256) void k_mf_update(Kc c, const float* __restrict
     un[e] = (Ui + e * S3)[id];
     pn[e] = (Pi + e * S3)[id];
   }
-  const float Kv = K[id];
-  const float fi = (float)i, fj = (float)j, fk = (float)k;
-  const float x_au = (fi - (float)c.N2) * c.h, y_au = (fj - (float)c.N2) * c.h;
-  float P_total = 0.0f;
+  NodeIn nd;
+  nd.fi = (float)i;
+  nd.fj = (float)j;
+  nd.fk = (float)k;
+  nd.x_au = (nd.fi - (float)c.N2) * c.h;
+  nd.y_au = (nd.fj - (float)c.N2) * c.h;
+  nd.Kv = K[id];
+  nd.P_total = 0.0f;
 #pragma unroll
-  for (int e = 0; e < NE; ++e) P_total = P_total + pn[e];
+  for (int e = 0; e < NE; ++e) nd.P_total = nd.P_total + pn[e];
+  nd.tmin = cusp_min<NE>(c, nd.fi, nd.fj, nd.fk);
 #pragma unroll
   for (int m = 0; m < NE; ++m) {
     const float* __restrict__ Um = Ui + m * S3;
     const float* __restrict__ Wm = Wi + m * S3;
     const float* __restrict__ Pm = Pi + m * S3;
-    const int o = id;
-    const Orb& ob = c.orb[m];
-    const float uc = un[m];
-    float umax = 0.0f;
-#pragma unroll
-    for (int e = 0; e < NE; ++e)
-      if (e != m && un[e] > umax) umax = un[e];
-    const float cm = uc - umax;
-    // w: free-boundary advection + diffusion, then cusps / confinement / split
-    const float wc = Wm[o], wip = Wm[o + S2], wim = Wm[o - S2], wjp = Wm[o + S], wjm = Wm[o - S];
-    const float wkp = Wm[o + 1], wkm = Wm[o - 1];
-    const float lw = (((((wip + wim) + wjp) + wjm) + wkp) + wkm - 6.0f * wc) * c.inv_h2;
-    const float gx = (wip - wim) * c.inv_h, gy = (wjp - wjm) * c.inv_h, gz = (wkp - wkm) * c.inv_h;
-    float nw = (wc + ((0.05f * c.dt) * fabsf(cm)) * lw) + ((1.0f * c.dt) * cm) * sqrtf((gx * gx + gy * gy) + gz * gz);
-    nw = fminf(fmaxf(nw, 0.0f), 1.0f);
-    if (c.shell) {
-      const float rx = fi - c.px[m], ry = fj - c.py[m], rz = fk - c.pz[m];
-      const float r_m = sqrtf((rx * rx + ry * ry) + rz * rz) * c.h;
-      const float ro_m = ob.r_out * c.R_out;
-      if (ob.rc > 0.0f && r_m < ob.rc) nw = fminf(nw, smooth_rc(r_m, ob.rc, c.h));
-      if (ro_m > 0.0f && r_m > ro_m) nw = (r_m <= ro_m + c.h) ? nw : 0.0f;
-      if (ob.split != 0 && ob.fix == 1 && !in_sector_f(ob, rx, ry, rz, r_m)) nw = 0.0f;
+    const float w7[7] = {Wm[id], Wm[id + S2], Wm[id - S2], Wm[id + S], Wm[id - S], Wm[id + 1], Wm[id - 1]};
+    const float u7[7] = {un[m], Um[id + S2], Um[id - S2], Um[id + S], Um[id - S], Um[id + 1], Um[id - 1]};
+    const float p7[7] = {pn[m], Pm[id + S2], Pm[id - S2], Pm[id + S], Pm[id - S], Pm[id + 1], Pm[id - 1]};
+    float wn, unw, pnw;
+    mf_electron<NE>(c, m, nd, un, pn, w7, u7, p7, wn, unw, pnw);
+    (Wo + m * S3)[id] = wn;
+    (Uo + m * S3)[id] = unw;
+    (Po + m * S3)[id] = pnw;
+  }
+}
+
+// Linear block id remapped so that each of the 8 XCDs (workgroups are dealt
+// round-robin over them) sweeps one contiguous range of tiles, whose halo
+// rows then stay in that XCD's L2.
+__device__ __forceinline__ int xcd_tile() {
+  const int gx = gridDim.x, gy = gridDim.y;
+  const int nblk = gx * gy * gridDim.z;
+  const int b = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+  const int per = nblk >> 3, rem = nblk & 7;
+  const int xcd = b & 7, local = b >> 3;
+  return (xcd < rem) ? xcd * (per + 1) + local : rem * (per + 1) + (xcd - rem) * per + local;
+}
+
+// The fused update as a march along i with an LDS plane ring: a workgroup
+// owns 64 (k) x 4 (j) columns and ci planes.  Per plane every field of every
+// electron (u, w, P) is staged once into LDS with a one-node halo, the i-1 /
+// i+1 values stay in the owning thread's registers, and each node's 7-point
+// values are read from LDS and registers: about 3 NELEC + 1 global loads per
+// node instead of 21 NELEC + 1.  Double-buffered LDS, one barrier per plane.
+template <int NE>
+__global__ __launch_bounds__(256, MF_LDS_WAVES) void k_mf_update_lds(Kc c, int ci, const float* __restrict__ K,
+                                                        const float* __restrict__ Ui, const float* __restrict__ Wi,
+                                                        const float* __restrict__ Pi, float* __restrict__ Uo,
+                                                        float* __restrict__ Wo, float* __restrict__ Po) {
+  constexpr int TK = 64, TJ = 4, RK = TK + 2, RJ = TJ + 2, PL = RK * RJ, NF = 3 * NE, NH = 2 * RK + 2 * TJ;
+  __shared__ float lds[2][NF][PL];
+  const int n = c.NN - 1, S = c.S, S2 = c.S2;
+  const size_t S3 = (size_t)c.S3;
+  const int t = xcd_tile();
+  const int gx = gridDim.x, gy = gridDim.y;
+  const int K0 = 1 + (t % gx) * TK, J0 = 1 + ((t / gx) % gy) * TJ, I0 = 1 + (t / (gx * gy)) * ci;
+  const int I1 = min(I0 + ci, n + 1);
+  const int tk = threadIdx.x, tj = threadIdx.y, tid = tj * TK + tk;
+  const int k = K0 + tk, j = J0 + tj;
+  const bool active = k <= n && j <= n;
+  const int col = min(j, S - 1) * S + min(k, S - 1);
+  const int oslot = (tj + 1) * RK + (tk + 1);
+  // this thread's halo node of the (RJ x RK) region, if any
+  const bool has_halo = tid < NH;
+  int hcol = 0, hslot = 0;
+  if (has_halo) {
+    int hj, hk;
+    if (tid < RK) {
+      hj = 0;
+      hk = tid;
+    } else if (tid < 2 * RK) {
+      hj = RJ - 1;
+      hk = tid - RK;
     } else {
+      const int q = tid - 2 * RK;
+      hj = 1 + (q >> 1);
+      hk = (q & 1) ? RK - 1 : 0;
+    }
+    hcol = min(J0 - 1 + hj, S - 1) * S + min(K0 - 1 + hk, S - 1);
+    hslot = hj * RK + hk;
+  }
+  // field f: u of electron f (f < NE), w of f - NE, P of f - 2 NE
+  auto fld = [&](int f) -> const float* {
+    return f < NE ? Ui + f * S3 : (f < 2 * NE ? Wi + (f - NE) * S3 : Pi + (f - 2 * NE) * S3);
+  };
+  float prv[NF], cur[NF];
 #pragma unroll
-      for (int a = 0; a < NE; ++a) {
-        const float dx = fi - c.px[a], dy = fj - c.py[a], dz = fk - c.pz[a];
-        const float r = sqrtf((dx * dx + dy * dy) + dz * dz) * c.h;
-        const float rca = c.orb[a].rc;
-        if (rca > 0.0f && r < rca) nw = fminf(nw, smooth_rc(r, rca, c.h));
+  for (int f = 0; f < NF; ++f) {
+    prv[f] = fld(f)[(I0 - 1) * S2 + col];
+    cur[f] = fld(f)[I0 * S2 + col];
+  }
+  const float fj = (float)j, fk = (float)k;
+  int b = 0;
+  for (int i = I0; i < I1; ++i) {
+    const int id = i * S2 + col;
+    float nxt[NF];
+#pragma unroll
+    for (int f = 0; f < NF; ++f) nxt[f] = fld(f)[id + S2];
+    float halo[NF];
+    if (has_halo) {
+#pragma unroll
+      for (int f = 0; f < NF; ++f) halo[f] = fld(f)[i * S2 + hcol];
+    }
+    const float Kv = K[id];
+#pragma unroll
+    for (int f = 0; f < NF; ++f) lds[b][f][oslot] = cur[f];
+    if (has_halo) {
+#pragma unroll
+      for (int f = 0; f < NF; ++f) lds[b][f][hslot] = halo[f];
+    }
+    __syncthreads();
+    if (active) {
+      float un[NE], pn[NE];
+#pragma unroll
+      for (int e = 0; e < NE; ++e) {
+        un[e] = cur[e];
+        pn[e] = cur[2 * NE + e];
+      }
+      NodeIn nd;
+      nd.fi = (float)i;
+      nd.fj = fj;
+      nd.fk = fk;
+      nd.x_au = (nd.fi - (float)c.N2) * c.h;
+      nd.y_au = (fj - (float)c.N2) * c.h;
+      nd.Kv = Kv;
+      nd.P_total = 0.0f;
+#pragma unroll
+      for (int e = 0; e < NE; ++e) nd.P_total = nd.P_total + pn[e];
+      nd.tmin = cusp_min<NE>(c, nd.fi, fj, fk);
+#pragma unroll
+      for (int m = 0; m < NE; ++m) {
+        const int fu = m, fw = NE + m, fp = 2 * NE + m;
+        const float w7[7] = {cur[fw], nxt[fw], prv[fw], lds[b][fw][oslot + RK], lds[b][fw][oslot - RK],
+                             lds[b][fw][oslot + 1], lds[b][fw][oslot - 1]};
+        const float u7[7] = {cur[fu], nxt[fu], prv[fu], lds[b][fu][oslot + RK], lds[b][fu][oslot - RK],
+                             lds[b][fu][oslot + 1], lds[b][fu][oslot - 1]};
+        const float p7[7] = {cur[fp], nxt[fp], prv[fp], lds[b][fp][oslot + RK], lds[b][fp][oslot - RK],
+                             lds[b][fp][oslot + 1], lds[b][fp][oslot - 1]};
+        float wn, unw, pnw;
+        mf_electron<NE>(c, m, nd, un, pn, w7, u7, p7, wn, unw, pnw);
+        (Wo + m * S3)[id] = wn;
+        (Uo + m * S3)[id] = unw;
+        (Po + m * S3)[id] = pnw;
       }
     }
-    (Wo + m * S3)[o] = nw;
-    // u: w-weighted Laplacian + potential terms
-    float P_group = 0.0f;
 #pragma unroll
-    for (int e = 0; e < NE; ++e)
-      if (c.orb[e].group == ob.group) P_group = P_group + pn[e];
-    const float selfN = ob.target;
-    const float fm_multi = (c.full_self > 0.5f) ? 0.0f : 1.0f / selfN;
-    const float fm = (selfN > 1.0f) ? fm_multi : 1.0f;
-    const float Vother = 2.0f * (P_total - fm * P_group);
-    const float rho_m = uc * uc;
-    const float V_TF =
-        (selfN > 1.0f && c.alpha_tf > 0.0f) ? c.alpha_tf * powf(fmaxf(rho_m, 1e-20f), 0.6666667f) : 0.0f;
-    const float V_tilt = (-ob.tilt_x) * x_au - ob.tilt_y * y_au;
-    float rho_other = 0.0f;
-#pragma unroll
-    for (int e = 0; e < NE; ++e)
-      if (e != m) rho_other = rho_other + un[e] * un[e];
-    const float V_pauli = c.beta * rho_other;
-    const float uip = Um[o + S2], uim = Um[o - S2], ujp = Um[o + S], ujm = Um[o - S], ukp = Um[o + 1],
-                ukm = Um[o - 1];
-    const float hd = c.half_d * ((selfN > 1.0f) ? c.kc_multi : 1.0f);
-    const float A = hd * (((uip - uc) * (wip + nw)) * 0.5f - ((uc - uim) * (nw + wim)) * 0.5f);
-    const float B = hd * (((ujp - uc) * (wjp + nw)) * 0.5f - ((uc - ujm) * (nw + wjm)) * 0.5f);
-    const float C = hd * (((ukp - uc) * (wkp + nw)) * 0.5f - ((uc - ukm) * (nw + wkm)) * 0.5f);
-    const float D = ((c.dt * (((((Kv - Vother) - V_TF) - V_tilt) - V_pauli) - c.field_x * x_au)) * uc) * wc;
-    (Uo + m * S3)[o] = (((uc + A) + B) + C) + D;
-    // P_m: pseudo-time Jacobi step sourced by electron m's own density
-    const float Pc = pn[m];
-    const float s6 = ((((Pm[o + S2] + Pm[o - S2]) + Pm[o + S]) + Pm[o - S]) + Pm[o + 1]) + Pm[o - 1];
-    (Po + m * S3)[o] = (Pc + (c.dt * (s6 - 6.0f * Pc)) * c.inv_h2) + ((c.two_pi * c.dt) * uc) * uc;
+    for (int f = 0; f < NF; ++f) {
+      prv[f] = cur[f];
+      cur[f] = nxt[f];
+    }
+    b ^= 1;
   }
 }
 
@@ -664,6 +844,16 @@ __global__ void k_mf_extract(Kc c, int NE, const float* __restrict__ U, const fl
     default: { constexpr int E = 16; CALL; } break; \
   }
 
+#define MF_DISPATCH_LDS(NE_, CALL)                 \
+  switch (NE_) {                                  \
+    case 1: { constexpr int E = 1; CALL; } break; \
+    case 2: { constexpr int E = 2; CALL; } break; \
+    case 3: { constexpr int E = 3; CALL; } break; \
+    case 4: { constexpr int E = 4; CALL; } break; \
+    case 5: { constexpr int E = 5; CALL; } break; \
+    default: { constexpr int E = 6; CALL; } break; \
+  }
+
 float fixed6(double x) {
   char b[64];
   snprintf(b, sizeof b, "%.6f", x);
@@ -700,6 +890,9 @@ struct rq_mf {
   int cur = 0;
   int red_ich = 8;
   int nparts = 0;
+  bool lds_march = true;  // RQ_MF_LDS=0: per-node update kernel
+  int lds_ci = 1;
+  dim3 lds_grid;
   bool timing = false;
   std::vector<hipEvent_t> ev;
   int ev_used = 0;
@@ -816,6 +1009,12 @@ void build_kc(rq_mf* h) {
       }
     }
     o.group = g;
+    const float selfN = o.target;
+    const float fm_multi = (c.full_self > 0.5f) ? 0.0f : 1.0f / selfN;
+    o.fm = (selfN > 1.0f) ? fm_multi : 1.0f;
+    o.hd = c.half_d * ((selfN > 1.0f) ? c.kc_multi : 1.0f);
+    const double rg = (double)o.rc / (double)c.h * (1.0 + 1e-4);
+    o.rc_skip2 = (float)(rg * rg);
   }
 }
 
@@ -1006,6 +1205,13 @@ int rq_mf_create(const rq_mf_params* p, const rq_mf_nucleus* nuclei, int32_t n, 
   if ((e = hipMemcpyAsync(h->tgt_d, tg.data(), n * 8, hipMemcpyHostToDevice, h->st)) != hipSuccess)
     return bad(e, "hipMemcpy");
   if ((e = hipStreamSynchronize(h->st)) != hipSuccess) return bad(e, "hipStreamSynchronize");
+  {
+    const int gx = (nn + 63) / 64, gy = (nn + 3) / 4;
+    const int gz = std::max(1, std::min(nn, kMfLdsBlocks / (gx * gy)));
+    h->lds_ci = (nn + gz - 1) / gz;
+    h->lds_grid = dim3(gx, gy, (nn + h->lds_ci - 1) / h->lds_ci);
+    if (const char* lv = std::getenv("RQ_MF_LDS")) h->lds_march = std::atoi(lv) != 0;
+  }
   *out = h;
   return RQ_OK;
 }
@@ -1068,8 +1274,15 @@ int rq_mf_do_steps(rq_mf* h, int32_t n, double* sums) {
   for (int s = 0; s < n; ++s) {
     const int cur = h->cur, nxt = 1 - cur;
     if (h->timing) MFCHK(h, hipEventRecord(h->ev[2 * s], h->st));
-    MF_DISPATCH(h->NE, hipLaunchKernelGGL((k_mf_update<E>), grid, blk, 0, h->st, c, h->K, h->U[cur], h->W[cur],
-                                          h->P[cur], h->U[nxt], h->W[nxt], h->P[nxt]));
+    if (h->NE <= kMfLdsMaxE && h->lds_march) {
+      const int ci = h->lds_ci;
+      MF_DISPATCH_LDS(h->NE, hipLaunchKernelGGL((k_mf_update_lds<E>), h->lds_grid, dim3(64, 4), 0, h->st, c, ci,
+                                                h->K, h->U[cur], h->W[cur], h->P[cur], h->U[nxt], h->W[nxt],
+                                                h->P[nxt]));
+    } else {
+      MF_DISPATCH(h->NE, hipLaunchKernelGGL((k_mf_update<E>), grid, blk, 0, h->st, c, h->K, h->U[cur], h->W[cur],
+                                            h->P[cur], h->U[nxt], h->W[nxt], h->P[nxt]));
+    }
     MFLAUNCH(h);
     if (h->timing) {
       MFCHK(h, hipEventRecord(h->ev[2 * s + 1], h->st));
