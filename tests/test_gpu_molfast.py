@@ -165,3 +165,34 @@ def test_frame_with_dynamics_moves_free_nuclei():
         assert host.nuclei[0]["i"] != before[0] or host.nuclei[0]["j"] != 12
         assert host.E_KK == pytest.approx(O.e_kk(O.MolFast([O.Nucleus(**{k: n[k] for k in ("i", "j", "k", "Z")})
                                                             for n in host.nuclei], NN=24, screen=8.0)))
+
+
+def test_update_kernels_agree(monkeypatch):
+    """The LDS plane-ring march and the per-node kernel (RQ_MF_LDS=0) give
+    bit-identical fields over 30 steps (two normalisations)."""
+    out = []
+    for lds in ("1", "0"):
+        monkeypatch.setenv("RQ_MF_LDS", lds)
+        mf, host = _pair(24, WATER, targets=[3, 1, 1], USER_BETA_PAULI=0.2)
+        with host:
+            host.do_steps(30)
+            out.append(_fields(host, 3))
+    for a, b in zip(out[0], out[1]):
+        assert ulp_diff(a, b) == 0
+
+
+def test_seven_electrons_per_node_kernel_matches_oracle():
+    """Above six electrons the per-node kernel runs: one step against the oracle."""
+    nuc = [dict(i=8 + 2 * (a % 4), j=9 + 3 * (a // 4), k=10 + (a % 2), Z=1 + (a == 0), rc=0.4 * (a == 0))
+           for a in range(7)]
+    mf, host = _pair(20, nuc, targets=[2, 1, 1, 1, 1, 1, 1], screen=7.0)
+    with host:
+        Ko, Uo, Wo, Po = O.init_fields(mf)
+        _load(host, Ko, Uo, Wo, Po)
+        host.do_steps(1)
+        _, U, W, P = _fields(host, 7)
+        U1, W1, P1 = O.update(mf, Ko, Uo, Wo, Po)
+        U1 = O.normalize(mf, U1, O.reduce_sums(mf, Ko, U1, W1, P1))
+        assert ulp_diff(W, W1) == 0
+        assert ulp_diff(P, P1) == 0
+        assert ulp_diff(U, U1) <= 1
