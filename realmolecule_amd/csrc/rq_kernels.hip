@@ -299,98 +299,22 @@ __global__ __launch_bounds__(CK * CJ * CI) void k_resres_tiled(int Sf, const flo
   gc[I * Sc * Sc + J * Sc + K] = out;
 }
 
-// Workgroup barrier that orders LDS only: global loads issued before it stay
-// in flight (a full __syncthreads would also wait for this wave's global
-// stores, which on gfx9 share one counter with its loads).
-__device__ __forceinline__ void lds_barrier() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-  __builtin_amdgcn_s_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-}
 
-// Residual + 4 FW restriction as a march along i for the large levels.  A
-// workgroup owns 32 (k) x CJ (j) coarse columns and CI coarse planes; per
-// fine plane it evaluates the residual on the (65 x 2CJ+1) fine region those
-// columns' stencils cover (each thread keeps the i-1 / i values of its nodes
-// in registers, so a fine plane is fetched once) into a 6-plane LDS ring.
-// Each round forms four fine planes and then restricts two coarse planes
-// (fine planes 2I-1 .. 2I+3), so two of every four waves restrict and a
-// barrier pair serves two coarse planes.  Same residual_at / restrict27
-// expressions as k_residual + k_restrict4.
+// Residual + 4 FW restriction as a march along i for the large levels: one
+// workgroup per 32 x CJ coarse-column tile and CI coarse planes
+// (resres_march_tile; with CP = 2, two of every four waves restrict and a
+// barrier pair serves two coarse planes).
 template <bool FINE, int CJ, int CP>
 __global__ __launch_bounds__(128 * CJ) void k_resres_march(int Sf, int CI, const float* __restrict__ x,
                                                             const float* __restrict__ src, float alpha, float beta,
                                                             float* __restrict__ gc, float sign) {
-  constexpr int CK = 32, RK = 2 * CK + 1, RJ = 2 * CJ + 1, PL = RK * RJ;
-  constexpr int NT = 128 * CJ, PTS = (PL + NT - 1) / NT, RING = 2 * CP + 2;
-  static_assert(CP * CK * CJ <= NT, "one thread per coarse column and plane");
+  constexpr int PL = 65 * (2 * CJ + 1), RING = 2 * CP + 2;
   __shared__ float ring[RING * PL];
-  const int nc = (Sf - 1) / 2 - 1, n = Sf - 2;
-  const int Sc = nc + 2, S2 = Sf * Sf;
-  const int gx = (nc + CK - 1) / CK, gy = (nc + CJ - 1) / CJ;
+  const int nc = (Sf - 1) / 2 - 1;
+  const int gx = (nc + 31) / 32, gy = (nc + CJ - 1) / CJ;
   const int t = xcd_block();
-  const int K0 = 1 + (t % gx) * CK, J0 = 1 + ((t / gx) % gy) * CJ, I0 = 1 + (t / (gx * gy)) * CI;
-  const int I1 = min(I0 + CI, nc + 1);
-  const int tid = threadIdx.x;
-  // this thread's region nodes (clamped into the interior for the loads)
-  int col[PTS];
-  bool in[PTS];
-  float xm[PTS], xc[PTS];
-  const int f0 = 2 * I0 - 1;
-#pragma unroll
-  for (int p = 0; p < PTS; ++p) {
-    const int q = min(tid + p * NT, PL - 1);
-    const int j = 2 * J0 - 1 + q / RK, k = 2 * K0 - 1 + q % RK;
-    in[p] = tid + p * NT < PL && j <= n && k <= n;
-    col[p] = min(j, n) * Sf + min(k, n);
-    xm[p] = x[(f0 - 1) * S2 + col[p]];
-    xc[p] = x[f0 * S2 + col[p]];
-  }
-  // residual of fine plane f into ring slot f % RING (carried xm / xc advance)
-  auto plane = [&](int f) {
-#pragma unroll
-    for (int p = 0; p < PTS; ++p) {
-      const int id = f * S2 + col[p];
-      const float xp = x[id + S2];
-      const float s6 = xp + xm[p] + x[id + Sf] + x[id - Sf] + x[id + 1] + x[id - 1];
-      float rhs;
-      if (FINE) {
-        const float u = src[id];
-        rhs = (alpha * (u * u)) * beta;
-      } else {
-        rhs = src[id];
-      }
-      const float r = rhs - (6.0f * xc[p] - s6);
-      if (tid + p * NT < PL) ring[(f % RING) * PL + tid + p * NT] = in[p] ? r : 0.0f;
-      xm[p] = xc[p];
-      xc[p] = xp;
-    }
-  };
-  plane(f0);
-  // thread -> (coarse plane of the round, coarse column)
-  const int half = tid / (CK * CJ), tk = tid % CK, tj = (tid / CK) % CJ;
-  for (int I = I0; I < I1; I += CP) {
-#pragma unroll
-    for (int h = 0; h < CP; ++h) {
-      if (I + h < I1) {
-        plane(2 * (I + h));
-        plane(2 * (I + h) + 1);
-      }
-    }
-    lds_barrier();
-    const int Ih = I + half;
-    if (half < CP && Ih < I1) {
-      const int K = K0 + tk, J = J0 + tj;
-      if (K <= nc && J <= nc) {
-        const int c = (2 * tj + 1) * RK + (2 * tk + 1);
-        float out = restrict27_planes(ring, ((2 * Ih - 1) % RING) * PL + c, ((2 * Ih) % RING) * PL + c,
-                                      ((2 * Ih + 1) % RING) * PL + c, RK, 1);
-        if (sign < 0.0f) out = -out;
-        gc[Ih * Sc * Sc + J * Sc + K] = out;
-      }
-    }
-    lds_barrier();
-  }
+  resres_march_tile<FINE, CJ, CP>(Sf, CI, 1 + (t % gx) * 32, 1 + ((t / gx) % gy) * CJ, 1 + (t / (gx * gy)) * CI, x,
+                                  src, alpha, beta, gc, sign, ring);
 }
 
 // Two damped-Jacobi sweeps fused as a march along i (temporal blocking).  A
