@@ -8,7 +8,8 @@
 //   USER_CURV_REG, USER_N_MOVE, USER_DT_NUC, USER_DAMPING, USER_FORCE_SCALE,
 //   USER_DYNAMICS, FREEZE_BOUNDARY, NO_VEE, CONVERGENCE_THRESHOLD.
 // Host API mirrored: initGPU, doSteps(n) (molecule.js:2822-3261),
-// restartWithPositions (:2127), isGpuReady (:2196), onSweepDone (:4653).
+// restartWithPositions (:2127), activateAtoms (:2151), isGpuReady (:2196),
+// onSweepDone (:4653).
 //
 // Not reproduced (outside the grid-solver path): the per-electron direct
 // Poisson / SIC (molecule.js:572-720), angular sector splits, bare nuclei,
@@ -32,8 +33,11 @@ class MoleculeHost {
     const rcs = cfg.USER_RC || atoms.map(a => a.rc || 0);
     this.nAtoms = atoms.length;
     this.Z = atoms.map((a, n) => Zs[n] || 0);
+    this.Zorig = [...this.Z];
+    this.activeCount = this.nAtoms;
     this.rc = atoms.map((a, n) => rcs[n] || 0);
     this.nucPos = atoms.map(a => [a.i, a.j, a.k !== undefined ? a.k : this.N2]);
+    this.molPos = this.nucPos.map(p => [...p]);  // the molecule's positions (restart / activateAtoms)
     this.NELEC = this.Z.filter(z => z > 0).length || 3;
     this.dv = cfg.USER_DV || (this.NELEC > 500 ? 0.01 : this.NELEC > 100 ? 0.03 : 0.12);
     this.STEPS_PER_FRAME = cfg.USER_STEPS_PER_FRAME ||
@@ -167,11 +171,32 @@ class MoleculeHost {
   }
 
   restartWithPositions(pos) {
-    this.nucPos = pos.map(p => [p[0], p[1], p[2]]);
+    for (let n = 0; n < pos.length && n < this.nAtoms; n++) {
+      this.nucPos[n] = [pos[n][0], pos[n][1], pos[n][2]];
+      this.molPos[n] = [...this.nucPos[n]];
+    }
     this.nucVel = this.nucPos.map(() => [0, 0, 0]);
+    this._restart();
+  }
+
+  // molecule.js:2150-2195: keep the first `count` nuclei charged (the rest
+  // Z = 0), put the nuclei back at the molecule's positions and restart
+  activateAtoms(count) {
+    count = Math.max(1, Math.min(count, this.nAtoms));
+    this.Z = this.Zorig.map((z, n) => (n < count ? z : 0));
+    this.nucPos = this.molPos.map(p => [...p]);
+    this.nucVel = this.nucPos.map(() => [0, 0, 0]);
+    this.nucForce = this.nucPos.map(() => [0, 0, 0]);
+    this.activeCount = count;
+    this._restart();
+  }
+
+  _restart() {
     addon.setAtoms(this.handle, this._atomRows());
     addon.initTrial(this.handle);
+    if (this._framePotential()) addon.potherFrame(this.handle, 0);
     this.tStep = 0;
+    this.frameCount = 0;
     this._stable = 0;
     this._lastE = Infinity;
   }

@@ -9,6 +9,7 @@ const out = process.argv[2];
 const spec = JSON.parse(process.argv[3]);
 const host = new MoleculeHost(spec.cfg);
 host.initGPU();
+if (spec.activate) host.activateAtoms(spec.activate);
 const frames = [];
 for (let f = 0; f < spec.frames; f++) {
   host.doSteps(spec.steps);

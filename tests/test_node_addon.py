@@ -43,10 +43,12 @@ console.log(JSON.stringify(out));
 
 @needs_node
 @pytest.mark.gpu
-def test_js_host_frames_match_s2_oracle(tmp_path):
+@pytest.mark.parametrize("activate", [None, 2])
+def test_js_host_frames_match_s2_oracle(tmp_path, activate):
     """Three 20-step frames of the JS host (two-grid correction every 8 steps,
     rc rule, level set, frame potential with SIC) against the S2 oracle
-    schedule in FP32."""
+    schedule in FP32; with activate = 2, after activateAtoms(2) (the third
+    nucleus uncharged)."""
     from oracle import s2
     from tests.helpers import three_atoms, rel_l2
     NN, screen = 24, 24 * 0.3
@@ -56,7 +58,7 @@ def test_js_host_frames_match_s2_oracle(tmp_path):
            "USER_ATOMS": [{"i": float(r[0]), "j": float(r[1]), "k": float(r[2]), "Z": float(r[3])} for r in rows],
            "USER_Z": [float(z) for z in rows[:, 3]], "USER_RC": [float(x) for x in rows[:, 4]],
            "USER_STEPS_PER_FRAME": 20, "USER_POISSON_INTERVAL": 8}
-    spec = {"cfg": cfg, "frames": 3, "steps": 20}
+    spec = {"cfg": cfg, "frames": 3, "steps": 20, "activate": activate}
     print("running node host", flush=True)
     r = subprocess.run([NODE, os.path.join(ROOT, "node", "test_host_gpu.js"), str(tmp_path), json.dumps(spec)],
                        capture_output=True, text=True, timeout=120)
@@ -66,7 +68,10 @@ def test_js_host_frames_match_s2_oracle(tmp_path):
     U = np.fromfile(tmp_path / "node_U.f32", np.float32).reshape(S, S, S)
     L = np.fromfile(tmp_path / "node_label.i32", np.int32).reshape(S, S, S)
     # the host's default frame potential: PotherBuf with SIC every frame (3 electrons)
-    st = s2.S2State(rows, NN, screen, np.float32, dv=0.12, poisson_interval=8, pother_frame=True, sic_interval=1)
+    orows = rows.copy()
+    if activate:
+        orows[activate:, 3] = 0.0
+    st = s2.S2State(orows, NN, screen, np.float32, dv=0.12, poisson_interval=8, pother_frame=True, sic_interval=1)
     for f in range(3):
         tot, per, dip = st.frame(20)
         E_el = tot.sum()
