@@ -103,6 +103,7 @@ struct rq_solver {
   size_t red_len = 0;
   hipStream_t st = nullptr;
   Tile tile;                 // fine-level stencil launch shape
+  Tile ptile;                // fine-level prolongation launch shape (two fine planes per trip want longer chunks)
   std::vector<Tile> ctile;   // coarse levels (index = level)
   bool initialized = false;
   bool timing = false;
@@ -204,11 +205,13 @@ struct Timed {
 };
 
 // Built-in launch shape per level, from sweeps on gfx950 (192^3 workload):
-// large levels 64x4 tiles marching 2..8 planes for about 4096 workgroups;
-// mid levels 32x8; levels of <= 64 interior nodes per axis one plane per
-// thread, since there a launch is latency-bound and a march only lengthens
-// each thread's dependent chain.
-Tile tile_for(int S) {
+// large levels 64x4 tiles marching 2..8 planes for about ``want`` workgroups
+// (8192: 4 planes at 193^3 for the sweeps; the prolongation, which advances
+// two fine planes per trip, is launched with 4096: 7 planes); mid levels
+// 32x8; levels of <= 64 interior nodes per axis one plane per thread, since
+// there a launch is latency-bound and a march only lengthens each thread's
+// dependent chain.
+Tile tile_for(int S, int want = 8192) {
   Tile t;
   const int n = S - 2;
   t.bx = n >= 128 ? 64 : (n >= 24 ? 32 : 16);
@@ -218,7 +221,6 @@ Tile tile_for(int S) {
     return t;
   }
   const int gx = (n + t.bx - 1) / t.bx, gy = (n + t.by - 1) / t.by;
-  const int want = 4096;
   int nz = std::max(1, std::min(n, want / std::max(1, gx * gy)));
   t.ich = std::min(8, std::max(2, (n + nz - 1) / nz));
   return t;
@@ -416,7 +418,7 @@ int vcycle_fine(rq_solver* s, bool fuse_itp, bool* did_itp) {
     --post;
   } else {
     Timed t(s, F_PROLONG);
-    launch_prolong_correct(s->st, s->tile, s->S, s->P, s->e0[1], 1.0f);
+    launch_prolong_correct(s->st, s->ptile, s->S, s->P, s->e0[1], 1.0f);
   }
   LAUNCHCHK(s);
   // plain sweeps before the last one (which may carry the ITP update)
@@ -465,7 +467,7 @@ int twogrid_correction(rq_solver* s) {
   }
   {
     Timed t(s, F_PROLONG);
-    launch_prolong_correct(s->st, s->tile, s->S, s->P, s->e0[1], 0.5f);
+    launch_prolong_correct(s->st, s->ptile, s->S, s->P, s->e0[1], 0.5f);
   }
   LAUNCHCHK(s);
   return RQ_OK;
@@ -500,7 +502,7 @@ int vcycle_rhs(rq_solver* s, float* x, float* x2, const float* rhs) {
   if (int rc = vcycle_coarse(s, 1)) return rc;
   {
     Timed t(s, F_PROLONG);
-    launch_prolong_correct(s->st, s->tile, s->S, cur, s->e0[1], 1.0f);
+    launch_prolong_correct(s->st, s->ptile, s->S, cur, s->e0[1], 1.0f);
   }
   for (int it = 0; it < s->p.mg_nu2; ++it) {
     {
@@ -1066,6 +1068,7 @@ int rq_create(const rq_params* pp, const rq_atom* atoms, int32_t n_atoms, rq_sol
   s->dtv = (float)(s->dv * (s->h64 * s->h64));
   s->two_pi = (float)(2.0 * M_PI);
   s->tile = tile_for(s->S);
+  s->ptile = tile_for(s->S, 4096);
   s->levels = levels;
 
   auto alloc = [&](void** ptr, size_t bytes) -> bool { return hipMalloc(ptr, bytes) == hipSuccess; };
@@ -1465,13 +1468,19 @@ int rq_set_tile(rq_solver* h, int32_t level, int32_t bx, int32_t by, int32_t ich
     if (level != -1 && level != l) continue;
     const int S = (l == 0) ? h->S : h->Sl[l];
     Tile t = tile_for(S);
+    Tile pt = tile_for(S, 4096);
     if (!dflt) {
       t.bx = bx;
       t.by = by;
       t.ich = ich;
+      pt = t;
     }
-    if (l == 0) h->tile = t;
-    else h->ctile[l] = t;
+    if (l == 0) {
+      h->tile = t;
+      h->ptile = pt;
+    } else {
+      h->ctile[l] = t;
+    }
   }
   drop_graphs(h);
   return RQ_OK;
@@ -1607,7 +1616,7 @@ static int sic_solve(rq_solver* s, int m) {
     launch_smooth_coarse(s->st, s->ctile[1], S1, s->e0[1], s->e1[1], s->g[1], a, c);
     std::swap(s->e0[1], s->e1[1]);
   }
-  launch_prolong_correct(s->st, s->tile, s->S, s->sic_x, s->e0[1], 0.5f);
+  launch_prolong_correct(s->st, s->ptile, s->S, s->sic_x, s->e0[1], 0.5f);
   for (int it = 0; it < 4; ++it) {
     launch_smooth_coarse(s->st, s->tile, s->S, s->sic_x, s->sic_x2, s->sic_g, a, c);
     std::swap(s->sic_x, s->sic_x2);
