@@ -35,6 +35,12 @@ constexpr int kFinalThreads = 256;
 // LDS plane ring of the update: 2 x 3 NELEC planes of 66 x 6 floats (28.5 KB
 // at 3 electrons, 57 KB at 6); above 6 electrons the per-node kernel runs
 constexpr int kMfLdsMaxE = 6;
+// three-plane ring of the update with K: 3 x (3 NELEC + 1) planes of
+// kMfRingPlane floats (47.5 KB at 3 electrons); at 4 electrons it would hold
+// 2 waves per SIMD (208 VGPRs, 61.8 KB) against the two-plane march's 3
+constexpr int kMfRingMaxE = 3;
+constexpr int kMfRingPlane = 396;  // (TK + 2)(TJ + 2) bound of a ring tile (66 x 6)
+constexpr int kMfRingPlanes = 8;   // target planes per ring workgroup
 #ifndef MF_LDS_WAVES
 #define MF_LDS_WAVES 3  // waves per SIMD the LDS march is compiled for (4 spills to scratch)
 #endif
@@ -546,6 +552,187 @@ __global__ __launch_bounds__(256, MF_LDS_WAVES) void k_mf_update_lds(Kc c, int c
   }
 }
 
+// Store sink of the lanes of k_mf_update_ring that lie past the last
+// interior node (tile overhang): their stores go here, so every store
+// instruction is unconditional.
+__device__ float g_mf_sink[256];
+
+// The fused update as an i-march with a three-plane LDS ring (NELEC <= 3).
+// While plane i is computed the ring holds planes i - 1 (own columns), i
+// (with its one-node halo) and i + 1 of every field of every electron and K,
+// and the registers hold plane i + 2, loaded during the previous plane.
+// After plane i's stores, plane i + 2 goes into the slot of plane i - 1 and
+// the loads of plane i + 3 are issued, so a load has a whole plane's compute
+// to arrive and no plane waits on a load issued in its own iteration.  Every
+// register set is written to LDS before it is reloaded (no register rotation
+// that would force a wait), and the overhang lanes store to a sink instead of
+// branching around their stores: every path reaching the LDS write then has
+// the same stores after the loads it needs, so the wait for those loads does
+// not wait for the stores.  Every node reads its 7-point values from LDS.
+//
+// Grid (k-tiles, j-tiles, gz): each tile's planes are split into gz ranges
+// of balanced length (plane counts differ by at most one), so the host can
+// pick gz to make the workgroup count a whole number of resident rounds.
+// Workgroups go to the XCDs in slab order (all tiles of one plane range
+// before the next), so neighbouring tiles march the same planes at the same
+// time and their halo reads meet in L2.
+template <int NE>
+__global__ __launch_bounds__(256, MF_LDS_WAVES) void k_mf_update_ring(Kc c, int TK, int TJ,
+                                                         const float* __restrict__ K, const float* __restrict__ Ui,
+                                                         const float* __restrict__ Wi, const float* __restrict__ Pi,
+                                                         float* __restrict__ Uo, float* __restrict__ Wo,
+                                                         float* __restrict__ Po) {
+  // a (TK x TJ)-node tile per 256-thread workgroup, TK * TJ <= 256 and
+  // (TK + 2)(TJ + 2) <= PL; the LDS field planes keep the fixed stride PL
+  constexpr int PL = kMfRingPlane, NF = 3 * NE, NL = NF + 1;
+  __shared__ float ring[3][NL][PL];
+  const int RK = TK + 2, RJ = TJ + 2, NH = 2 * RK + 2 * TJ;
+  const int n = c.NN - 1, S = c.S, S2 = c.S2;
+  const size_t S3 = (size_t)c.S3;
+  const int gxt = gridDim.x, gyt = gridDim.y, gz = gridDim.z;
+  const int t = xcd_tile();
+  const int tile = t % (gxt * gyt), z = t / (gxt * gyt);
+  const int I0 = 1 + (z * n) / gz, I1 = 1 + ((z + 1) * n) / gz;
+  const int tid = threadIdx.x;
+  // threads past TK * TJ repeat a node of the last row: they load and write
+  // the same values to the same LDS slots as its owner and store to the sink
+  const int tk = tid % TK, tj = min(tid / TK, TJ - 1);
+  const bool spare = tid >= TK * TJ;
+  const int oslot = (tj + 1) * RK + (tk + 1);
+  // this thread's halo node of the (RJ x RK) region, if any
+  const bool has_halo = tid < NH;
+  int hj = 0, hk = 0;
+  if (has_halo) {
+    if (tid < RK) {
+      hk = tid;
+    } else if (tid < 2 * RK) {
+      hj = RJ - 1;
+      hk = tid - RK;
+    } else {
+      const int q = tid - 2 * RK;
+      hj = 1 + (q >> 1);
+      hk = (q & 1) ? RK - 1 : 0;
+    }
+  }
+  const int hslot = hj * RK + hk;
+  // field f: u of electron f (f < NE), w of f - NE, P of f - 2 NE, K (f = NF)
+  auto fld = [&](int f) -> const float* {
+    return f < NE ? Ui + f * S3 : (f < 2 * NE ? Wi + (f - NE) * S3 : (f < NF ? Pi + (f - 2 * NE) * S3 : K));
+  };
+  {
+    const int K0 = 1 + (tile % gxt) * TK, J0 = 1 + (tile / gxt) * TJ;
+    const int k = K0 + tk, j = J0 + tj;
+    const bool active = k <= n && j <= n && !spare;
+    const int col = min(j, S - 1) * S + min(k, S - 1);
+    // threads without a halo node load their own column again (a cache
+    // hit) so the halo load is unconditional
+    const int hcol = has_halo ? min(J0 - 1 + hj, S - 1) * S + min(K0 - 1 + hk, S - 1) : col;
+    // prologue: planes I0 - 1 (own columns, slot 2), I0 (slot 0) and I0 + 1
+    // (slot 1) with their halos, loads issued together; plane I0 + 2 into
+    // registers.  Planes past I1 (the i + 1 neighbour of the range's last
+    // plane) are clamped to I1 <= S - 1: their values are never read, and
+    // the repeated loads of plane I1 hit in cache instead of fetching planes
+    // no plane of the range needs.
+    {
+      const int i1 = min(I0 + 1, S - 1);
+      float A[NL], B[NL], HB[NF], C[NL], HC[NF];
+#pragma unroll
+      for (int f = 0; f < NL; ++f) {
+        A[f] = fld(f)[(I0 - 1) * S2 + col];
+        B[f] = fld(f)[I0 * S2 + col];
+        C[f] = fld(f)[i1 * S2 + col];
+      }
+#pragma unroll
+      for (int f = 0; f < NF; ++f) {
+        HB[f] = fld(f)[I0 * S2 + hcol];
+        HC[f] = fld(f)[i1 * S2 + hcol];
+      }
+#pragma unroll
+      for (int f = 0; f < NL; ++f) {
+        ring[2][f][oslot] = A[f];
+        ring[0][f][oslot] = B[f];
+        ring[1][f][oslot] = C[f];
+      }
+      if (has_halo) {
+#pragma unroll
+        for (int f = 0; f < NF; ++f) {
+          ring[0][f][hslot] = HB[f];
+          ring[1][f][hslot] = HC[f];
+        }
+      }
+    }
+    float R[NL], H[NF];
+    {
+      const int i2 = min(I0 + 2, I1);
+#pragma unroll
+      for (int f = 0; f < NL; ++f) R[f] = fld(f)[i2 * S2 + col];
+#pragma unroll
+      for (int f = 0; f < NF; ++f) H[f] = fld(f)[i2 * S2 + hcol];
+    }
+    const float fj = (float)j, fk = (float)k;
+    int q = 0;
+    for (int i = I0; i < I1; ++i) {
+      const int qn = q == 2 ? 0 : q + 1, qp = q == 0 ? 2 : q - 1;
+      __syncthreads();
+      const int id = i * S2 + col;
+      float un[NE], pn[NE];
+#pragma unroll
+      for (int e = 0; e < NE; ++e) {
+        un[e] = ring[q][e][oslot];
+        pn[e] = ring[q][2 * NE + e][oslot];
+      }
+      NodeIn nd;
+      nd.fi = (float)i;
+      nd.fj = fj;
+      nd.fk = fk;
+      nd.x_au = (nd.fi - (float)c.N2) * c.h;
+      nd.y_au = (fj - (float)c.N2) * c.h;
+      nd.Kv = ring[q][NF][oslot];
+      nd.P_total = 0.0f;
+#pragma unroll
+      for (int e = 0; e < NE; ++e) nd.P_total = nd.P_total + pn[e];
+      nd.tmin = cusp_min<NE>(c, nd.fi, fj, fk);
+      float wo[NE], uo[NE], po[NE];
+#pragma unroll
+      for (int m = 0; m < NE; ++m) {
+        const int fu = m, fw = NE + m, fp = 2 * NE + m;
+        const float w7[7] = {ring[q][fw][oslot],      ring[qn][fw][oslot],     ring[qp][fw][oslot],
+                             ring[q][fw][oslot + RK], ring[q][fw][oslot - RK], ring[q][fw][oslot + 1],
+                             ring[q][fw][oslot - 1]};
+        const float u7[7] = {un[m],                   ring[qn][fu][oslot],     ring[qp][fu][oslot],
+                             ring[q][fu][oslot + RK], ring[q][fu][oslot - RK], ring[q][fu][oslot + 1],
+                             ring[q][fu][oslot - 1]};
+        const float p7[7] = {pn[m],                   ring[qn][fp][oslot],     ring[qp][fp][oslot],
+                             ring[q][fp][oslot + RK], ring[q][fp][oslot - RK], ring[q][fp][oslot + 1],
+                             ring[q][fp][oslot - 1]};
+        mf_electron<NE>(c, m, nd, un, pn, w7, u7, p7, wo[m], uo[m], po[m]);
+      }
+      // all stores after the last branch of the compute
+#pragma unroll
+      for (int m = 0; m < NE; ++m) {
+        *(active ? Wo + m * S3 + id : g_mf_sink + tid) = wo[m];
+        *(active ? Uo + m * S3 + id : g_mf_sink + tid) = uo[m];
+        *(active ? Po + m * S3 + id : g_mf_sink + tid) = po[m];
+      }
+      // plane i + 2 into the slot of plane i - 1: its halo and neighbour
+      // reads ended before this plane's barrier, its own-column reads are
+      // this thread's and done; then the loads of plane i + 3 (clamped)
+#pragma unroll
+      for (int f = 0; f < NL; ++f) ring[qp][f][oslot] = R[f];
+      if (has_halo) {
+#pragma unroll
+        for (int f = 0; f < NF; ++f) ring[qp][f][hslot] = H[f];
+      }
+      const int i3 = min(i + 3, I1);
+#pragma unroll
+      for (int f = 0; f < NL; ++f) R[f] = fld(f)[i3 * S2 + col];
+#pragma unroll
+      for (int f = 0; f < NF; ++f) H[f] = fld(f)[i3 * S2 + hcol];
+      q = qn;
+    }
+  }
+}
+
 // reduce (mol_fast.js:363-436): per-node FP32 terms, FP64 per-block partials
 template <int NE>
 __global__ __launch_bounds__(256) void k_mf_reduce(Kc c, int ich, const float* __restrict__ K,
@@ -844,6 +1031,13 @@ __global__ void k_mf_extract(Kc c, int NE, const float* __restrict__ U, const fl
     default: { constexpr int E = 16; CALL; } break; \
   }
 
+#define MF_DISPATCH_RING(NE_, CALL)                \
+  switch (NE_) {                                  \
+    case 1: { constexpr int E = 1; CALL; } break; \
+    case 2: { constexpr int E = 2; CALL; } break; \
+    default: { constexpr int E = 3; CALL; } break; \
+  }
+
 #define MF_DISPATCH_LDS(NE_, CALL)                 \
   switch (NE_) {                                  \
     case 1: { constexpr int E = 1; CALL; } break; \
@@ -890,9 +1084,13 @@ struct rq_mf {
   int cur = 0;
   int red_ich = 8;
   int nparts = 0;
-  bool lds_march = true;  // RQ_MF_LDS=0: per-node update kernel
+  // update kernel: 2 = three-plane LDS ring (NELEC <= kMfRingMaxE, else the
+  // two-plane march), 1 = two-plane LDS march, 0 = per-node (RQ_MF_LDS)
+  int lds_march = 2;
   int lds_ci = 1;
   dim3 lds_grid;
+  int ring_tk = 64, ring_tj = 4;  // tile and grid of the three-plane ring
+  dim3 ring_grid;
   bool timing = false;
   std::vector<hipEvent_t> ev;
   int ev_used = 0;
@@ -1207,10 +1405,37 @@ int rq_mf_create(const rq_mf_params* p, const rq_mf_nucleus* nuclei, int32_t n, 
   if ((e = hipStreamSynchronize(h->st)) != hipSuccess) return bad(e, "hipStreamSynchronize");
   {
     const int gx = (nn + 63) / 64, gy = (nn + 3) / 4;
-    const int gz = std::max(1, std::min(nn, kMfLdsBlocks / (gx * gy)));
+    int blocks = kMfLdsBlocks;  // RQ_MF_BLOCKS: target workgroups (launch-shape sweeps)
+    if (const char* bv = std::getenv("RQ_MF_BLOCKS")) blocks = std::max(1, std::atoi(bv));
+    const int gz = std::max(1, std::min(nn, blocks / (gx * gy)));
     h->lds_ci = (nn + gz - 1) / gz;
     h->lds_grid = dim3(gx, gy, (nn + h->lds_ci - 1) / h->lds_ci);
-    if (const char* lv = std::getenv("RQ_MF_LDS")) h->lds_march = std::atoi(lv) != 0;
+    // ring tiles: the fewest k-tiles of at most 64 nodes, split evenly, so
+    // little of a wave falls past the last interior node (199 nodes: 4 x 50
+    // instead of 4 x 64), then as many j-rows as 256 threads and the LDS
+    // plane allow
+    const int ntk = (nn + 63) / 64;
+    const int tkr = (nn + ntk - 1) / ntk;
+    const int tjr = std::max(1, std::min({256 / tkr, kMfRingPlane / (tkr + 2) - 2, 16}));
+    h->ring_tk = tkr;
+    h->ring_tj = tjr;
+    // plane ranges per tile: about kMfRingPlanes planes each, rounded so
+    // the workgroups fill whole resident rounds (workgroups per CU the
+    // kernel holds x CUs); RQ_MF_RING_GZ sets the count directly
+    const int tiles = ((nn + tkr - 1) / tkr) * ((nn + tjr - 1) / tjr);
+    int per_cu = 0, cus = 0, dev = 0;
+    if ((e = hipGetDevice(&dev)) != hipSuccess) return bad(e, "hipGetDevice");
+    if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess)
+      return bad(e, "hipDeviceGetAttribute");
+    MF_DISPATCH_RING(h->NE, e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_mf_update_ring<E>, 256, 0));
+    if (e != hipSuccess) return bad(e, "hipOccupancyMaxActiveBlocksPerMultiprocessor");
+    const long slots = (long)std::max(1, per_cu) * std::max(1, cus);
+    const long rounds = std::max(1L, std::lround((double)tiles * nn / kMfRingPlanes / slots));
+    int rgz = (int)std::max(1L, std::lround((double)rounds * slots / tiles));
+    if (const char* gv = std::getenv("RQ_MF_RING_GZ")) rgz = std::atoi(gv);
+    rgz = std::max(1, std::min(nn, rgz));
+    h->ring_grid = dim3((nn + tkr - 1) / tkr, (nn + tjr - 1) / tjr, rgz);
+    if (const char* lv = std::getenv("RQ_MF_LDS")) h->lds_march = std::min(2, std::max(0, std::atoi(lv)));
   }
   *out = h;
   return RQ_OK;
@@ -1274,7 +1499,12 @@ int rq_mf_do_steps(rq_mf* h, int32_t n, double* sums) {
   for (int s = 0; s < n; ++s) {
     const int cur = h->cur, nxt = 1 - cur;
     if (h->timing) MFCHK(h, hipEventRecord(h->ev[2 * s], h->st));
-    if (h->NE <= kMfLdsMaxE && h->lds_march) {
+    if (h->NE <= kMfRingMaxE && h->lds_march == 2) {
+      MF_DISPATCH_RING(h->NE, hipLaunchKernelGGL((k_mf_update_ring<E>), h->ring_grid, dim3(256), 0, h->st,
+                                                 c, h->ring_tk, h->ring_tj,
+                                                 h->K, h->U[cur], h->W[cur], h->P[cur], h->U[nxt], h->W[nxt],
+                                                 h->P[nxt]));
+    } else if (h->NE <= kMfLdsMaxE && h->lds_march) {
       const int ci = h->lds_ci;
       MF_DISPATCH_LDS(h->NE, hipLaunchKernelGGL((k_mf_update_lds<E>), h->lds_grid, dim3(64, 4), 0, h->st, c, ci,
                                                 h->K, h->U[cur], h->W[cur], h->P[cur], h->U[nxt], h->W[nxt],

@@ -167,18 +167,32 @@ def test_frame_with_dynamics_moves_free_nuclei():
                                                             for n in host.nuclei], NN=24, screen=8.0)))
 
 
-def test_update_kernels_agree(monkeypatch):
-    """The LDS plane-ring march and the per-node kernel (RQ_MF_LDS=0) give
-    bit-identical fields over 30 steps (two normalisations)."""
+@pytest.mark.parametrize("NN,blocks", [(24, None), (70, None), (70, "180")])
+def test_update_kernels_agree(monkeypatch, NN, blocks):
+    """The three-plane LDS ring (default at three electrons, RQ_MF_LDS=2),
+    the two-plane LDS march (1) and the per-node kernel (0) give bit-identical
+    fields over 30 steps (two normalisations).  At NN = 70 the 69 interior
+    nodes per axis leave ragged tiles: 5 of 64 k-columns and 1 of 4 j-rows
+    in the two-plane march, 1 of 35 k-columns in the ring's 35 x 7 tiles
+    (11 spare threads).  By default the ring splits each tile's planes into
+    ranges of one or two planes; with RQ_MF_RING_GZ=5 and 180 target march
+    workgroups the ring ranges hold 13 or 14 planes and the march blocks 14
+    (the last 13)."""
+    if blocks is not None:
+        monkeypatch.setenv("RQ_MF_BLOCKS", blocks)
+        monkeypatch.setenv("RQ_MF_RING_GZ", "5")
     out = []
-    for lds in ("1", "0"):
+    c = NN // 2
+    nuc = [dict(i=c, j=c, k=c, Z=3, rc=0.6), dict(i=c + 3, j=c + 2, k=c, Z=1), dict(i=c - 3, j=c + 2, k=c, Z=1)]
+    for lds in ("2", "1", "0"):
         monkeypatch.setenv("RQ_MF_LDS", lds)
-        mf, host = _pair(24, WATER, targets=[3, 1, 1], USER_BETA_PAULI=0.2)
+        mf, host = _pair(NN, nuc, targets=[3, 1, 1], USER_BETA_PAULI=0.2)
         with host:
             host.do_steps(30)
             out.append(_fields(host, 3))
-    for a, b in zip(out[0], out[1]):
-        assert ulp_diff(a, b) == 0
+    for other in out[1:]:
+        for a, b in zip(out[0], other):
+            assert ulp_diff(a, b) == 0
 
 
 def test_seven_electrons_per_node_kernel_matches_oracle():
