@@ -20,10 +20,11 @@ constexpr size_t kFieldStagger = 4096;   // byte offset between consecutive fine
 constexpr int kScoreScanMaxAtoms = 128;  // score rule: scan all atoms up to this count, cell search above
 constexpr int kLegsMaxS = 65;           // one down-leg + one up-leg launch per coarse level up to 65^3 nodes
 constexpr int kResResMarchCJ = 2;       // coarse j-columns per residual+restriction march workgroup (128 CJ threads)
-constexpr int kResResMarchCP = 2;
-constexpr int kSmooth2Blocks = 8192;
-constexpr int kDomStreams = 4;         // concurrent per-domain V-cycles of the direct potential (one stream each)    // target workgroup count of the two-sweep march
-constexpr int kSmooth2MinS = 320;       // smallest level (nodes per axis) that runs sweep pairs as that march       // coarse planes restricted per barrier round of that march
+constexpr int kResResMarchCP = 2;       // coarse planes restricted per barrier round of that march
+constexpr int kSmooth2Blocks = 8192;    // target workgroup count of the two-sweep march
+constexpr int kSmooth2MinS = 320;       // smallest level (nodes per axis) that runs sweep pairs as that march
+constexpr int kDomStreams = 4;          // concurrent per-domain V-cycles of the direct potential (one stream each)
+constexpr int kReduceIch = 24;          // planes per thread of the per-domain reductions (fewer global atomics)
 constexpr int kResResTiledMaxNc = 63;   // 3D-tiled residual+restriction up to 63 coarse interior nodes, march above
 constexpr int kBottomMaxS = 25;         // last two V-cycle levels in one LDS-resident workgroup up to 25^3 nodes
 

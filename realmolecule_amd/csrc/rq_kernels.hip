@@ -1178,7 +1178,9 @@ __global__ __launch_bounds__(256) void k_boundary_scan(int S, int ich, float h, 
     other += ((lin[id + 1] & kLabelMask) != cur);
     other += ((lin[id - 1] & kLabelMask) != cur);
     uint16_t out = raw;
-    if (other > 0) {
+    // a flip needs other >= min_foreign (cloud/solver.py:144), so nodes with
+    // fewer foreign neighbours keep their label without the score search
+    if (other > 0 && other >= min_foreign) {
       const float xi = (float)i * h, yj = (float)m.j * h, zk = (float)m.k * h;
       float cur_score = 0.0f, best = 0.0f;
       int bestN = cur;
@@ -1225,7 +1227,9 @@ __global__ __launch_bounds__(256) void k_boundary_score(int S, int ich, float h,
     other += ((lin[id + 1] & kLabelMask) != cur);
     other += ((lin[id - 1] & kLabelMask) != cur);
     uint16_t out = raw;
-    if (other > 0) {
+    // a flip needs other >= min_foreign (cloud/solver.py:144), so nodes with
+    // fewer foreign neighbours keep their label without the score search
+    if (other > 0 && other >= min_foreign) {
       const float xi = (float)i * h, yj = (float)m.j * h, zk = (float)m.k * h;
       // score of atom a: Z / sqrt(|x - R_a|^2 + h^2) (cloud/solver.py:110-128)
       auto score = [&](int a) -> float {
@@ -1876,7 +1880,11 @@ void launch_fill(hipStream_t st, float* p, size_t n, float v) {
 
 void launch_norm_sums(hipStream_t st, const Tile& t, int S, float h3, const float* U, const uint16_t* lab, int n_dom,
                       double* norms) {
-  hipLaunchKernelGGL(k_norm_sums, stencil_grid(S - 2, t), dim3(t.bx, t.by), sizeof(double) * n_dom, st, S, t.ich, h3,
+  // longer columns than the sweeps' tile: a few thousand workgroups, so a
+  // few thousand FP64 atomics into the per-domain sums instead of ~10^4
+  Tile r = t;
+  r.ich = std::max(t.ich, kReduceIch);
+  hipLaunchKernelGGL(k_norm_sums, stencil_grid(S - 2, r), dim3(r.bx, r.by), sizeof(double) * n_dom, st, S, r.ich, h3,
                      U, lab, n_dom, norms);
 }
 

@@ -392,21 +392,35 @@ __device__ __forceinline__ void prolong_correct_col(int Sf, int k, int j, int i0
 __device__ __forceinline__ void norm_sums_col(int S, int k, int j, int i0, int i1, float h3,
                                               const float* __restrict__ U, const uint16_t* __restrict__ lab,
                                               int n_dom, double* bins) {
-  const int S2 = S * S;
-  int id = i0 * S2 + j * S + k;
+  // planes in batches of B: the batch's loads are issued together (indices
+  // clamped to the column's last plane, whose repeats are skipped), then the
+  // run-length accumulation walks them in order, so the sums are those of a
+  // plane-by-plane walk
+  constexpr int B = 8;
+  const int S2 = S * S, col = j * S + k;
   int cur = -1;
   double acc = 0.0;
-  for (int i = i0; i < i1; ++i, id += S2) {
-    int L = lab[id] & kLabelMask;
-    L = L < n_dom ? L : n_dom - 1;
-    const float v = U[id];
-    const float term = (v * v) * h3;
-    if (L != cur) {
-      if (cur >= 0) atomicAdd(&bins[cur], acc);
-      cur = L;
-      acc = 0.0;
+  for (int ib = i0; ib < i1; ib += B) {
+    int Lb[B];
+    float vb[B];
+#pragma unroll
+    for (int q = 0; q < B; ++q) {
+      const int id = min(ib + q, i1 - 1) * S2 + col;
+      Lb[q] = lab[id] & kLabelMask;
+      vb[q] = U[id];
     }
-    acc += (double)term;
+#pragma unroll
+    for (int q = 0; q < B; ++q) {
+      if (ib + q >= i1) break;
+      const int L = Lb[q] < n_dom ? Lb[q] : n_dom - 1;
+      const float term = (vb[q] * vb[q]) * h3;
+      if (L != cur) {
+        if (cur >= 0) atomicAdd(&bins[cur], acc);
+        cur = L;
+        acc = 0.0;
+      }
+      acc += (double)term;
+    }
   }
   if (cur >= 0) atomicAdd(&bins[cur], acc);
 }
