@@ -65,6 +65,10 @@ struct rq_solver {
   // uniform cell list of the atoms (Z > 0) for the score rule's argmax search
   AtomCells cells;
   int32_t *cell_start_d = nullptr, *cell_atoms_d = nullptr;
+  // score rule: each node's best atom, filled on first need after the atoms
+  // were last set (0xFFFF = not yet searched); allocated at the first use
+  uint16_t* best_atom_d = nullptr;
+  bool best_atom_stale = true;
   std::vector<int32_t> lock_h;
   Atom* atoms_d = nullptr;
   float* Z_d = nullptr;
@@ -233,6 +237,7 @@ void zero_doubles(rq_solver* s, double* p, size_t n) {
 
 int upload_atoms(rq_solver* s) {
   HIPCHK(s, hipMemcpyAsync(s->atoms_d, s->atoms_h.data(), sizeof(Atom) * s->n_atoms, hipMemcpyHostToDevice, s->st));
+  s->best_atom_stale = true;
   std::vector<float> Z(s->n_atoms);
   for (int a = 0; a < s->n_atoms; ++a) Z[a] = s->atoms_h[a].Z;
   HIPCHK(s, hipMemcpyAsync(s->Z_d, Z.data(), sizeof(float) * s->n_atoms, hipMemcpyHostToDevice, s->st));
@@ -599,8 +604,20 @@ int boundary_step(rq_solver* s) {
   {
     Timed t(s, F_BOUNDARY);
     if (s->p.boundary_mode == RQ_BOUNDARY_SCORE) {
+      // S^3 uint16 entries, rounded up to whole floats for the fill kernel
+      const size_t nf = ((size_t)s->S3 + 1) / 2;
+      if (!s->best_atom_d) HIPCHK(s, hipMalloc(&s->best_atom_d, nf * sizeof(float)));
+      if (s->best_atom_stale) {
+        // every entry 0xFFFF (not yet searched): a fill kernel, not a memset
+        const uint32_t ones = 0xFFFFFFFFu;
+        float v;
+        std::memcpy(&v, &ones, sizeof v);
+        launch_fill(s->st, reinterpret_cast<float*>(s->best_atom_d), nf, v);
+        s->best_atom_stale = false;
+      }
       launch_boundary_score(s->st, s->tile, s->S, s->h, s->h2, s->atoms_d, s->n_atoms, s->lab, s->lab2,
-                            s->p.flip_ratio, s->p.flip_min_foreign, s->cells, s->cell_start_d, s->cell_atoms_d);
+                            s->p.flip_ratio, s->p.flip_min_foreign, s->cells, s->cell_start_d, s->cell_atoms_d,
+                            s->best_atom_d);
     } else {
       bool any_lock = false;
       for (int32_t g : s->lock_h) any_lock |= (g > 0);
@@ -635,6 +652,7 @@ void free_all(rq_solver* s) {
     if (p) (void)hipFree(p);
   };
   fr(s->atoms_d);
+  fr(s->best_atom_d);
   fr(s->Z_d);
   fr(s->lock_d);
   fr(s->arena);

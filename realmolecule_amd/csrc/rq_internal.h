@@ -121,7 +121,8 @@ void launch_normalize(hipStream_t st, const Tile& t, int S, float* U, const uint
 // best score found.  Both give the labels of the scan.
 void launch_boundary_score(hipStream_t st, const Tile& t, int S, float h, float h2, const Atom* atoms, int n_atoms,
                            const uint16_t* lin, uint16_t* lout, float ratio, int min_foreign, const AtomCells& cells,
-                           const int32_t* cell_start, const int32_t* cell_atoms);
+                           const int32_t* cell_start, const int32_t* cell_atoms,
+                           uint16_t* best_atom);
 void launch_levelset(hipStream_t st, const Tile& t, int S, float h, const Atom* atoms, int n_atoms,
                      const int32_t* lock_group, const uint16_t* lin, uint16_t* lout, const float* U, float* W,
                      float dt_w, float curv_reg, int rc_flags);

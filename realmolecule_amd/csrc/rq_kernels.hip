@@ -1154,6 +1154,24 @@ __global__ __launch_bounds__(256) void k_normalize(int S, int ich, float* __rest
 // --------------------------------------------------------------------------
 // Boundary rules.
 // --------------------------------------------------------------------------
+// The score rule's decision at a node with enough foreign neighbours, from
+// the node's best atom bN (the first strict maximum of Z_a / sqrt(r_a^2 +
+// h^2) over atoms with Z > 0, or kBestNone): flip to bN if it is not the
+// node's own domain and its score beats the own atom's score by `ratio`
+// (cloud/solver.py:128-145).  The best atom depends only on the geometry, so
+// the kernels keep it per node (best_atom, kBestUnknown until first needed
+// after the atoms were last set) and each later application of the rule
+// evaluates two scores instead of searching the atoms again.
+constexpr int kBestUnknown = 0xFFFF, kBestNone = 0xFFFE;
+template <class Score>
+__device__ __forceinline__ uint16_t flip_label(uint16_t raw, int cur, int bN, int n_atoms,
+                                               const Atom* __restrict__ atoms, float ratio, Score score) {
+  if (bN == kBestNone || bN == cur) return raw;
+  const float best = score(bN);
+  const float cur_score = (cur < n_atoms && atoms[cur].Z > 0.0f) ? score(cur) : 0.0f;
+  return best > cur_score * ratio ? (uint16_t)bN : raw;
+}
+
 // Nuclear-score flip (cloud/solver.py:92-147), reading labels from `lin`
 // and writing `lout` (double-buffered).
 // Score rule by a scan over all atoms (cloud/solver.py:92-147; the form for
@@ -1161,7 +1179,7 @@ __global__ __launch_bounds__(256) void k_normalize(int S, int ich, float* __rest
 __global__ __launch_bounds__(256) void k_boundary_scan(int S, int ich, float h, float h2,
                                                        const Atom* __restrict__ atoms, int n_atoms,
                                                        const uint16_t* __restrict__ lin, uint16_t* __restrict__ lout,
-                                                       float ratio, int min_foreign) {
+                                                       float ratio, int min_foreign, uint16_t* __restrict__ best_atom) {
   const int n = S - 2;
   March m;
   if (!march_coords(n, ich, m)) return;
@@ -1182,23 +1200,29 @@ __global__ __launch_bounds__(256) void k_boundary_scan(int S, int ich, float h, 
     // fewer foreign neighbours keep their label without the score search
     if (other > 0 && other >= min_foreign) {
       const float xi = (float)i * h, yj = (float)m.j * h, zk = (float)m.k * h;
-      float cur_score = 0.0f, best = 0.0f;
-      int bestN = cur;
-      for (int a = 0; a < n_atoms; ++a) {
-        const float Za = atoms[a].Z;
-        if (!(Za > 0.0f)) continue;
+      auto score = [&](int a) -> float {
         const float dx = xi - atoms[a].i * h;
         const float dy = yj - atoms[a].j * h;
         const float dz = zk - atoms[a].k * h;
         const float r2 = dx * dx + dy * dy + dz * dz + h2;
-        const float sc = Za / sqrtf(r2);
-        if (a == cur) cur_score = sc;
-        if (sc > best) {
-          best = sc;
-          bestN = a;
+        return atoms[a].Z / sqrtf(r2);
+      };
+      int bN = best_atom[id];
+      if (bN == kBestUnknown) {
+        // the reference's scan: the first strict maximum among atoms with Z > 0
+        float best = 0.0f;
+        bN = kBestNone;
+        for (int a = 0; a < n_atoms; ++a) {
+          if (!(atoms[a].Z > 0.0f)) continue;
+          const float sc = score(a);
+          if (sc > best) {
+            best = sc;
+            bN = a;
+          }
         }
+        best_atom[id] = (uint16_t)bN;
       }
-      if (bestN != cur && best > cur_score * ratio && other >= min_foreign) out = (uint16_t)bestN;
+      out = flip_label(raw, cur, bN, n_atoms, atoms, ratio, score);
     }
     lout[id] = out;
   }
@@ -1210,7 +1234,8 @@ __global__ __launch_bounds__(256) void k_boundary_score(int S, int ich, float h,
                                                         uint16_t* __restrict__ lout, float ratio, int min_foreign,
                                                         int cs, int ncell, float zmax,
                                                         const int32_t* __restrict__ cell_start,
-                                                        const int32_t* __restrict__ cell_atoms) {
+                                                        const int32_t* __restrict__ cell_atoms,
+                                                        uint16_t* __restrict__ best_atom) {
   const int n = S - 2;
   March m;
   if (!march_coords(n, ich, m)) return;
@@ -1242,37 +1267,41 @@ __global__ __launch_bounds__(256) void k_boundary_score(int S, int ich, float h,
       // the reference scans all atoms in index order keeping the first
       // strict maximum; here the candidates come from shells of cells, so a
       // tie keeps the lower index -- the same winner.
-      const float cur_score = (cur < n_atoms && atoms[cur].Z > 0.0f) ? score(cur) : 0.0f;
-      float best = 0.0f;
-      int bestN = cur;
-      bool found = false;
-      const int hi = min(i / cs, ncell - 1), hj = min(m.j / cs, ncell - 1), hk = min(m.k / cs, ncell - 1);
-      const int rmax = max(max(max(hi, ncell - 1 - hi), max(hj, ncell - 1 - hj)), max(hk, ncell - 1 - hk));
-      for (int r = 0; r <= rmax; ++r) {
-        for (int ci = max(hi - r, 0); ci <= min(hi + r, ncell - 1); ++ci) {
-          for (int cj = max(hj - r, 0); cj <= min(hj + r, ncell - 1); ++cj) {
-            const bool edge_ij = abs(ci - hi) == r || abs(cj - hj) == r;
-            for (int ck = max(hk - r, 0); ck <= min(hk + r, ncell - 1); ++ck) {
-              if (!edge_ij && abs(ck - hk) != r) continue;
-              const int cell = (ci * ncell + cj) * ncell + ck;
-              for (int q = cell_start[cell]; q < cell_start[cell + 1]; ++q) {
-                const int a = cell_atoms[q];
-                const float sc = score(a);
-                if (sc > best || (found && sc == best && a < bestN)) {
-                  best = sc;
-                  bestN = a;
-                  found = true;
+      int bN = best_atom[id];
+      if (bN == kBestUnknown) {
+        float best = 0.0f;
+        int bestN = kBestNone;
+        bool found = false;
+        const int hi = min(i / cs, ncell - 1), hj = min(m.j / cs, ncell - 1), hk = min(m.k / cs, ncell - 1);
+        const int rmax = max(max(max(hi, ncell - 1 - hi), max(hj, ncell - 1 - hj)), max(hk, ncell - 1 - hk));
+        for (int r = 0; r <= rmax; ++r) {
+          for (int ci = max(hi - r, 0); ci <= min(hi + r, ncell - 1); ++ci) {
+            for (int cj = max(hj - r, 0); cj <= min(hj + r, ncell - 1); ++cj) {
+              const bool edge_ij = abs(ci - hi) == r || abs(cj - hj) == r;
+              for (int ck = max(hk - r, 0); ck <= min(hk + r, ncell - 1); ++ck) {
+                if (!edge_ij && abs(ck - hk) != r) continue;
+                const int cell = (ci * ncell + cj) * ncell + ck;
+                for (int q = cell_start[cell]; q < cell_start[cell + 1]; ++q) {
+                  const int a = cell_atoms[q];
+                  const float sc = score(a);
+                  if (sc > best || (found && sc == best && a < bestN)) {
+                    best = sc;
+                    bestN = a;
+                    found = true;
+                  }
                 }
               }
             }
           }
+          // every atom in shells > r lies at least r cells (r * cs nodes) away;
+          // 1e-4 relative slack covers the FP32 rounding of the scores
+          const float dmin = (float)(r * cs) * h * 0.9999f;
+          if (best > 0.0f && zmax / sqrtf(dmin * dmin + h2) < best) break;
         }
-        // every atom in shells > r lies at least r cells (r * cs nodes) away;
-        // 1e-4 relative slack covers the FP32 rounding of the scores
-        const float dmin = (float)(r * cs) * h * 0.9999f;
-        if (best > 0.0f && zmax / sqrtf(dmin * dmin + h2) < best) break;
+        bN = bestN;
+        best_atom[id] = (uint16_t)bN;
       }
-      if (bestN != cur && best > cur_score * ratio && other >= min_foreign) out = (uint16_t)bestN;
+      out = flip_label(raw, cur, bN, n_atoms, atoms, ratio, score);
     }
     lout[id] = out;
   }
@@ -1895,14 +1924,15 @@ void launch_normalize(hipStream_t st, const Tile& t, int S, float* U, const uint
 
 void launch_boundary_score(hipStream_t st, const Tile& t, int S, float h, float h2, const Atom* atoms, int n_atoms,
                            const uint16_t* lin, uint16_t* lout, float ratio, int min_foreign, const AtomCells& cells,
-                           const int32_t* cell_start, const int32_t* cell_atoms) {
+                           const int32_t* cell_start, const int32_t* cell_atoms, uint16_t* best_atom) {
   if (n_atoms <= kScoreScanMaxAtoms) {
     hipLaunchKernelGGL(k_boundary_scan, stencil_grid(S - 2, t), dim3(t.bx, t.by), 0, st, S, t.ich, h, h2, atoms,
-                       n_atoms, lin, lout, ratio, min_foreign);
+                       n_atoms, lin, lout, ratio, min_foreign, best_atom);
     return;
   }
   hipLaunchKernelGGL(k_boundary_score, stencil_grid(S - 2, t), dim3(t.bx, t.by), 0, st, S, t.ich, h, h2, atoms,
-                     n_atoms, lin, lout, ratio, min_foreign, cells.cs, cells.n, cells.zmax, cell_start, cell_atoms);
+                     n_atoms, lin, lout, ratio, min_foreign, cells.cs, cells.n, cells.zmax, cell_start, cell_atoms,
+                     best_atom);
 }
 
 void launch_levelset(hipStream_t st, const Tile& t, int S, float h, const Atom* atoms, int n_atoms,

@@ -145,6 +145,51 @@ def test_boundary_score_many_atoms_cell_search(n_atoms, NN, h):
     s.close()
 
 
+@pytest.mark.parametrize("n_atoms,NN,h", [(3, 24, 0.3), (300, 48, 0.3)])
+def test_boundary_score_best_atom_cache(n_atoms, NN, h):
+    """The per-node best atom kept between applications of the score rule:
+    two applications in a row (the second reads the atoms found by the
+    first), then an atom moved (rq_set_atoms must drop every kept entry) and
+    a third application, each against the oracle's scan over all atoms on
+    the same labels; 300 atoms take the cell-search kernel."""
+    rng = np.random.default_rng(23 + n_atoms)
+    if n_atoms == 3:
+        rows = three_atoms(NN)
+    else:
+        rows = np.zeros((n_atoms, 5))
+        rows[:, :3] = rng.uniform(2.0, NN - 2.0, (n_atoms, 3))
+        rows[:, 3] = rng.choice([1.0, 2.0, 3.0], n_atoms)
+    screen = NN * h
+    g, U, lab, K, P = _state(rows, NN, screen)
+    lab1 = lab.copy()
+    inner = lab1[1:-1, 1:-1, 1:-1]
+    m = rng.random(inner.shape) < 0.2
+    inner[m] = rng.integers(0, n_atoms, m.sum())
+    s = make_solver(rows, NN, screen)
+    s.U.set(U)
+    s.label.set(lab1)
+    s._boundary()
+    got1 = cube(s.label.get(), NN + 1)
+    ref1 = s1.boundary_score(U, lab1, rows, g)
+    assert np.array_equal(got1, ref1)
+    s._boundary()
+    got2 = cube(s.label.get(), NN + 1)
+    assert np.array_equal(got2, s1.boundary_score(U, got1, rows, g))
+    # move atom 1 across the grid: its best-atom regions change
+    rows2 = rows.copy()
+    rows2[1, :3] = rows[0, :3] + np.array([2.0, -3.0, 1.5])
+    s.atom_pos[1][:] = rows2[1, :3]
+    s._chk(s._lib.rq_set_atoms(s._h, s._atoms_c(), s.nAtoms))
+    lab3 = lab1.copy()
+    s.label.set(lab3)
+    s._boundary()
+    got3 = cube(s.label.get(), NN + 1)
+    ref3 = s1.boundary_score(U, lab3, rows2, g)
+    assert (ref3 != s1.boundary_score(U, lab3, rows, g)).sum() > 0
+    assert np.array_equal(got3, ref3)
+    s.close()
+
+
 def test_energy_matches_oracle():
     rows = three_atoms(24)
     NN, screen = 24, 24 * 0.3
