@@ -167,7 +167,7 @@ def test_frame_with_dynamics_moves_free_nuclei():
                                                             for n in host.nuclei], NN=24, screen=8.0)))
 
 
-@pytest.mark.parametrize("NN,blocks", [(24, None), (70, None), (70, "180")])
+@pytest.mark.parametrize("NN,blocks", [(24, None), (70, None), (70, "180"), (130, None)])
 def test_update_kernels_agree(monkeypatch, NN, blocks):
     """The three-plane LDS ring (default at three electrons, RQ_MF_LDS=2),
     the two-plane LDS march (1) and the per-node kernel (0) give bit-identical
@@ -177,7 +177,8 @@ def test_update_kernels_agree(monkeypatch, NN, blocks):
     (11 spare threads).  By default the ring splits each tile's planes into
     ranges of one or two planes; with RQ_MF_RING_GZ=5 and 180 target march
     workgroups the ring ranges hold 13 or 14 planes and the march blocks 14
-    (the last 13)."""
+    (the last 13).  At NN = 130 the ring tiles are 43 x 5 (three k-tiles
+    over 129 nodes, 41 spare threads)."""
     if blocks is not None:
         monkeypatch.setenv("RQ_MF_BLOCKS", blocks)
         monkeypatch.setenv("RQ_MF_RING_GZ", "5")
