@@ -113,12 +113,15 @@ def forces_bench(w, steps, bootstrap, n_batch, fd_delta, log_fn):
     import threading
     from realmolecule_amd.adjoint import FrozenSolver
     NN, screen, rows = w["NN"], w["screen"], w["rows"]
-    kw = dict(bootstrap=bootstrap, checkpoint=64, poisson_interval=1, norm_interval=20, poisson="multigrid",
+    kw = dict(bootstrap=bootstrap, checkpoint=0, poisson_interval=1, norm_interval=20, poisson="multigrid",
               mg_levels=w["mg_levels"])
     fz = FrozenSolver(rows, NN, screen, steps, **kw)
     pos = rows[:, :3] * fz.h
     n = len(rows)
-    fz.energy(pos)  # warm
+    # warm both paths on this handle: one energy and one gradient (the
+    # first gradient allocates the recorded trajectory, once per handle)
+    fz.energy(pos)
+    fz.energy_and_gradient(pos)
     fz.s.sync()
     t0 = time.perf_counter()
     E_adj, g = fz.energy_and_gradient(pos)
@@ -138,7 +141,7 @@ def forces_bench(w, steps, bootstrap, n_batch, fd_delta, log_fn):
     solvers = [FrozenSolver(rows, NN, screen, steps, **kw) for _ in range(n_batch)]
     geos = [pos + rng.normal(0, 0.05, pos.shape) for _ in range(n_batch)]
     for s_, p_ in zip(solvers, geos):
-        s_.energy(p_)
+        s_.energy_and_gradient(p_)
     out = [None] * n_batch
 
     def run(i):
@@ -155,7 +158,8 @@ def forces_bench(w, steps, bootstrap, n_batch, fd_delta, log_fn):
         s_.close()
     log_fn(f"[forces] batched adjoint: {n_batch} force vectors in {t_batch:.3f} s")
     return {
-        "workload": f"{w['desc']}; frozen-label solve of {bootstrap} bootstrap V-cycles + {steps} steps",
+        "workload": f"{w['desc']}; frozen-label solve of {bootstrap} bootstrap V-cycles + {steps} steps; "
+                    "each handle warmed with one energy and one gradient",
         "n_atoms": n,
         "energy_Ha": E0,
         "energy_rel_diff_adjoint_vs_plain": abs(E_adj - E0) / abs(E0),

@@ -252,7 +252,9 @@ int rq_pother_frame(rq_solver* h, int64_t frame);
 typedef struct rq_adjoint_spec {
   int64_t     n_steps;
   int32_t     bootstrap;           /* Poisson steps after the initial normalise */
-  int32_t     checkpoint_interval; /* steps per stored (U, P) checkpoint; <= 0: 64 */
+  int32_t     checkpoint_interval; /* steps per stored (U, P) checkpoint; <= 0: the whole
+                                      trajectory when its 2 n_steps + 1 fields fit in a quarter
+                                      of the free device memory (no recomputation), else 64 */
   rq_schedule schedule;            /* boundary_interval is ignored (frozen labels) */
   int32_t     reserved[4];         /* must be zero */
 } rq_adjoint_spec;

@@ -22,7 +22,9 @@ from . import _lib as L
 from .solver import RealQMSolver, atoms_array, nuclear_energy, nuclear_forces
 
 
-def make_spec(n_steps, bootstrap=100, checkpoint=64, poisson_interval=2, norm_interval=20):
+def make_spec(n_steps, bootstrap=100, checkpoint=0, poisson_interval=2, norm_interval=20):
+    """rq_adjoint_spec; ``checkpoint`` <= 0 lets the library keep the whole
+    trajectory when it fits (no recomputation in the reverse sweep)."""
     sp = L.RqAdjointSpec()
     sp.n_steps = int(n_steps)
     sp.bootstrap = int(bootstrap)
@@ -37,7 +39,7 @@ class FrozenSolver:
     ``rows`` are (i, j, k, Z, rc) in grid units; the labels come from the
     trial argmax at that geometry (cloud/solver.py:36-53)."""
 
-    def __init__(self, rows, NN, screen, n_steps, bootstrap=100, checkpoint=64, poisson_interval=2,
+    def __init__(self, rows, NN, screen, n_steps, bootstrap=100, checkpoint=0, poisson_interval=2,
                  norm_interval=20, **cfg):
         from .builders import to_config_atoms
         self.rows = np.asarray(rows, np.float64).copy()

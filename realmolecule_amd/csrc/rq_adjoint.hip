@@ -25,10 +25,21 @@ struct M3 {
   int k, j, i0, i1;
 };
 
+// Thread coordinates of a 2.5D launch, with the linear workgroup id remapped
+// so each XCD (workgroups are dealt to the 8 XCDs round-robin) takes one
+// contiguous range of tiles: neighbouring tiles, whose halos overlap, then
+// share that XCD's L2.
 __device__ __forceinline__ bool coords(int n, int ich, M3& m) {
-  m.k = 1 + blockIdx.x * blockDim.x + threadIdx.x;
-  m.j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
-  m.i0 = 1 + blockIdx.z * ich;
+  const int gx = gridDim.x, gy = gridDim.y;
+  const int nblk = gx * gy * gridDim.z;
+  const int b = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+  const int per = nblk >> 3, rem = nblk & 7;
+  const int xcd = b & 7, local = b >> 3;
+  const int t = (xcd < rem) ? xcd * (per + 1) + local : rem * (per + 1) + (xcd - rem) * per + local;
+  const int tx = t % gx, ty = (t / gx) % gy, tz = t / (gx * gy);
+  m.k = 1 + tx * blockDim.x + threadIdx.x;
+  m.j = 1 + ty * blockDim.y + threadIdx.y;
+  m.i0 = 1 + tz * ich;
   m.i1 = min(m.i0 + ich, n + 1);
   return m.k <= n && m.j <= n;
 }
@@ -110,12 +121,15 @@ __global__ __launch_bounds__(256) void k_itp_adj(int S, int ich, const float* __
   M3 m;
   if (!coords(n, ich, m)) return;
   const int S2 = S * S;
-  for (int i = m.i0; i < m.i1; ++i) {
-    const int id = i * S2 + m.j * S + m.k;
-    const uint16_t lc = lab[id];
-    const float lc_v = lam[id];
-    const float t0 = (lab[id + S2] == lc) ? lam[id + S2] : lc_v;
-    const float t1 = (lab[id - S2] == lc) ? lam[id - S2] : lc_v;
+  // an i-march: labels and lam of planes i - 1 and i stay in registers
+  int id = m.i0 * S2 + m.j * S + m.k;
+  uint16_t lbm = lab[id - S2], lc = lab[id];
+  float lvm = lam[id - S2], lc_v = lam[id];
+  for (int i = m.i0; i < m.i1; ++i, id += S2) {
+    const uint16_t lbp = lab[id + S2];
+    const float lvp = lam[id + S2];
+    const float t0 = (lbp == lc) ? lvp : lc_v;
+    const float t1 = (lbm == lc) ? lvm : lc_v;
     const float t2 = (lab[id + S] == lc) ? lam[id + S] : lc_v;
     const float t3 = (lab[id - S] == lc) ? lam[id - S] : lc_v;
     const float t4 = (lab[id + 1] == lc) ? lam[id + 1] : lc_v;
@@ -126,29 +140,51 @@ __global__ __launch_bounds__(256) void k_itp_adj(int S, int ich, const float* __
     const float ul = U[id] * lc_v;
     gK[id] += dtv * ul;
     lamP[id] += -2.0f * dtv * ul;
+    lbm = lc;
+    lc = lbp;
+    lvm = lc_v;
+    lc_v = lvp;
   }
 }
 
-// Smoother adjoint: lam_out = a lam + sum6 lam / c, and
-//   acc += coef * src * lam   (USE_SRC: Jacobi mode, src = U, coef = 2 alpha beta / c)
-//   acc += coef * lam         (!USE_SRC: V-cycle rhs adjoint, coef = 1 / c)
-template <bool USE_SRC>
+// Smoother adjoint: lam_out = a lam + sum6 lam / c, and by MODE
+//   kAdjAcc:   acc += coef * lam                   (V-cycle rhs adjoint, coef = 1 / c)
+//   kAdjSrc:   acc += coef * src * lam             (Jacobi mode: src = U, acc = lamU, coef = 2 alpha beta / c)
+//   kAdjFirst: acc  = 0 + coef * lam               (first sweep of a V-cycle adjoint: no zero fill before it)
+//   kAdjLast:  lamU += coef2 * src * (acc + coef * lam)
+//              (last sweep: the accumulated rhs adjoint goes straight into lamU, src = U)
+// The same FP32 operations as accumulating into a zeroed acc and adding
+// coef2 U acc to lamU in a pass of its own.  An i-march: lam of planes i - 1
+// and i stay in registers.
+template <int MODE>
 __global__ __launch_bounds__(256) void k_smooth_adj(int S, int ich, const float* __restrict__ lam,
                                                     float* __restrict__ lam_out, const float* __restrict__ src,
-                                                    float* __restrict__ acc, float a, float c, float coef) {
+                                                    float* __restrict__ acc, float* __restrict__ lamU, float a,
+                                                    float c, float coef, float coef2) {
   const int n = S - 2;
   M3 m;
   if (!coords(n, ich, m)) return;
   const int S2 = S * S;
-  for (int i = m.i0; i < m.i1; ++i) {
-    const int id = i * S2 + m.j * S + m.k;
-    const float l = lam[id];
-    const float s6 = lam[id + S2] + lam[id - S2] + lam[id + S] + lam[id - S] + lam[id + 1] + lam[id - 1];
+  int id = m.i0 * S2 + m.j * S + m.k;
+  float lm = lam[id - S2], lc = lam[id];
+  for (int i = m.i0; i < m.i1; ++i, id += S2) {
+    const float lp = lam[id + S2];
+    const float s6 = lp + lm + lam[id + S] + lam[id - S] + lam[id + 1] + lam[id - 1];
     float v = s6 / c;
-    if (a != 0.0f) v = a * l + v;
+    if (a != 0.0f) v = a * lc + v;
     lam_out[id] = v;
-    if (USE_SRC) acc[id] += coef * src[id] * l;
-    else acc[id] += coef * l;
+    if (MODE == kAdjAcc) {
+      acc[id] += coef * lc;
+    } else if (MODE == kAdjSrc) {
+      acc[id] += coef * src[id] * lc;
+    } else if (MODE == kAdjFirst) {
+      acc[id] = 0.0f + coef * lc;
+    } else {
+      const float af = acc[id] + coef * lc;
+      lamU[id] += coef2 * src[id] * af;
+    }
+    lm = lc;
+    lc = lp;
   }
 }
 
@@ -160,12 +196,16 @@ __global__ __launch_bounds__(256) void k_correct_adj(int S, int ich, const float
   M3 m;
   if (!coords(n, ich, m)) return;
   const int S2 = S * S;
-  for (int i = m.i0; i < m.i1; ++i) {
-    const int id = i * S2 + m.j * S + m.k;
-    const float zc = z[id];
-    const float s6 = z[id + S2] + z[id - S2] + z[id + S] + z[id - S] + z[id + 1] + z[id - 1];
+  // an i-march: z of planes i - 1 and i stay in registers
+  int id = m.i0 * S2 + m.j * S + m.k;
+  float zm = z[id - S2], zc = z[id];
+  for (int i = m.i0; i < m.i1; ++i, id += S2) {
+    const float zp = z[id + S2];
+    const float s6 = zp + zm + z[id + S] + z[id - S] + z[id + 1] + z[id - 1];
     lam_out[id] = lam[id] - (6.0f * zc - s6);
     acc[id] += zc;
+    zm = zc;
+    zc = zp;
   }
 }
 
@@ -357,14 +397,25 @@ void launch_itp_adj(hipStream_t st, const Tile& t, int S, const float* lam, floa
                      half_dv, dtv, gK, lamP);
 }
 
-void launch_smooth_adj(hipStream_t st, const Tile& t, int S, const float* lam, float* lam_out, const float* src,
-                       float* acc, float a, float c, float coef) {
-  if (src)
-    hipLaunchKernelGGL(k_smooth_adj<true>, sgrid(S - 2, t), dim3(t.bx, t.by), 0, st, S, t.ich, lam, lam_out, src,
-                       acc, a, c, coef);
-  else
-    hipLaunchKernelGGL(k_smooth_adj<false>, sgrid(S - 2, t), dim3(t.bx, t.by), 0, st, S, t.ich, lam, lam_out, src,
-                       acc, a, c, coef);
+void launch_smooth_adj(hipStream_t st, const Tile& t, int S, int mode, const float* lam, float* lam_out,
+                       const float* src, float* acc, float* lamU, float a, float c, float coef, float coef2) {
+  const dim3 g = sgrid(S - 2, t), b(t.bx, t.by);
+  switch (mode) {
+    case kAdjAcc:
+      hipLaunchKernelGGL(k_smooth_adj<kAdjAcc>, g, b, 0, st, S, t.ich, lam, lam_out, src, acc, lamU, a, c, coef, coef2);
+      break;
+    case kAdjSrc:
+      hipLaunchKernelGGL(k_smooth_adj<kAdjSrc>, g, b, 0, st, S, t.ich, lam, lam_out, src, acc, lamU, a, c, coef, coef2);
+      break;
+    case kAdjFirst:
+      hipLaunchKernelGGL(k_smooth_adj<kAdjFirst>, g, b, 0, st, S, t.ich, lam, lam_out, src, acc, lamU, a, c, coef,
+                         coef2);
+      break;
+    default:
+      hipLaunchKernelGGL(k_smooth_adj<kAdjLast>, g, b, 0, st, S, t.ich, lam, lam_out, src, acc, lamU, a, c, coef,
+                         coef2);
+      break;
+  }
 }
 
 void launch_correct_adj(hipStream_t st, const Tile& t, int S, const float* lam, const float* z, float* lam_out,

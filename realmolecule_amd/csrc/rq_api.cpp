@@ -735,9 +735,14 @@ int copy_field(rq_solver* s, float* dst, const float* src) {
 float* other_of(float* busy, float* t1, float* t2) { return busy == t1 ? t2 : t1; }
 
 // One Poisson step P_out = Poisson(P_in, U) with explicit buffers; t1, t2 are
-// scratch distinct from P_in, P_out and U.
-int poisson_into(rq_solver* s, const float* P_in, float* P_out, const float* U, float* t1, float* t2) {
+// scratch distinct from P_in, P_out, U and U_out.  With U_out given, a fused
+// multigrid step runs its last post-smoothing sweep together with the ITP
+// update U -> U_out on the new P (k_smooth_itp, the values of the sweep then
+// k_itp) and sets *did_itp; otherwise the caller applies the ITP itself.
+int poisson_into(rq_solver* s, const float* P_in, float* P_out, const float* U, float* t1, float* t2,
+                 float* U_out = nullptr, bool* did_itp = nullptr) {
   const Tile& T = s->tile;
+  if (did_itp) *did_itp = false;
   if (s->p.poisson_mode == RQ_POISSON_MULTIGRID) {
     float* x;
     const float a = s->p.mg_a, c = s->p.mg_c;
@@ -754,16 +759,29 @@ int poisson_into(rq_solver* s, const float* P_in, float* P_out, const float* U, 
       }
       x = const_cast<float*>(src);
     }
-    launch_residual_fine(s->st, T, s->S, x, U, s->two_pi, s->h2, P_out);
-    launch_restrict4(s->st, s->S, P_out, s->g[1], 1.0f);
+    // residual + restriction in one pass (the values of k_residual then
+    // k_restrict4, without the fine residual array)
+    if (s->fused) {
+      launch_residual_restrict(s->st, true, s->S, x, U, s->two_pi, s->h2, s->g[1], 1.0f);
+    } else {
+      launch_residual_fine(s->st, T, s->S, x, U, s->two_pi, s->h2, P_out);
+      launch_restrict4(s->st, s->S, P_out, s->g[1], 1.0f);
+    }
     LAUNCHCHK(s);
     if (int rc = vcycle_coarse(s, 1)) return rc;
-    launch_prolong_correct(s->st, T, s->S, x, s->e0[1], 1.0f);
+    launch_prolong_correct(s->st, s->ptile, s->S, x, s->e0[1], 1.0f);
     if (s->p.mg_nu2 == 0) return copy_field(s, P_out, x);
     float* src = x;
     for (int it = 0; it < s->p.mg_nu2; ++it) {
-      float* dst = (it == s->p.mg_nu2 - 1) ? P_out : other_of(src, t1, t2);
-      launch_smooth_fine(s->st, T, s->S, src, dst, U, a, c, s->two_pi, s->h2);
+      const bool last = it == s->p.mg_nu2 - 1;
+      float* dst = last ? P_out : other_of(src, t1, t2);
+      if (last && U_out && s->fused) {
+        launch_smooth_itp(s->st, T, s->S, src, dst, U, U_out, s->K, s->lab, a, c, s->two_pi, s->h2, s->half_dv,
+                          s->dtv);
+        if (did_itp) *did_itp = true;
+      } else {
+        launch_smooth_fine(s->st, T, s->S, src, dst, U, a, c, s->two_pi, s->h2);
+      }
       src = dst;
     }
     LAUNCHCHK(s);
@@ -797,29 +815,34 @@ int poisson_adj(rq_solver* s, const float* U) {
   const float ab2 = 2.0f * s->two_pi * s->h2;
   if (s->p.poisson_mode == RQ_POISSON_MULTIGRID) {
     const float a = s->p.mg_a, c = s->p.mg_c;
-    launch_fill(s->st, A.acc, s->S3, 0.0f);
+    // the first sweep's adjoint writes acc (no zero fill before it); the
+    // last one adds the rhs adjoint into lamU (no separate pass)
+    if (s->p.mg_nu2 == 0) launch_fill(s->st, A.acc, s->S3, 0.0f);
     for (int it = 0; it < s->p.mg_nu2; ++it) {
-      launch_smooth_adj(s->st, T, s->S, A.lamP, A.lamP2, nullptr, A.acc, a, c, 1.0f / c);
+      launch_smooth_adj(s->st, T, s->S, it == 0 ? kAdjFirst : kAdjAcc, A.lamP, A.lamP2, nullptr, A.acc, nullptr, a,
+                        c, 1.0f / c, 0.0f);
       std::swap(A.lamP, A.lamP2);
     }
     launch_restrict4(s->st, s->S, A.lamP, s->g[1], 1.0f);
     LAUNCHCHK(s);
     if (int rc = vcycle_coarse(s, 1)) return rc;
-    launch_fill(s->st, A.t3, s->S3, 0.0f);
-    launch_prolong_correct(s->st, T, s->S, A.t3, s->e0[1], 1.0f);
+    // t3's faces are zero since its allocation and never written
+    launch_prolong_set(s->st, T, s->S, A.t3, s->e0[1]);
     launch_correct_adj(s->st, T, s->S, A.lamP, A.t3, A.lamP2, A.acc);
     std::swap(A.lamP, A.lamP2);
     for (int it = 0; it < s->p.mg_nu1; ++it) {
-      launch_smooth_adj(s->st, T, s->S, A.lamP, A.lamP2, nullptr, A.acc, a, c, 1.0f / c);
+      const bool last = it == s->p.mg_nu1 - 1;
+      launch_smooth_adj(s->st, T, s->S, last ? kAdjLast : kAdjAcc, A.lamP, A.lamP2, last ? U : nullptr, A.acc,
+                        last ? A.lamU : nullptr, a, c, 1.0f / c, ab2);
       std::swap(A.lamP, A.lamP2);
     }
-    launch_axpy_src(s->st, T, s->S, U, A.acc, A.lamU, ab2);
+    if (s->p.mg_nu1 == 0) launch_axpy_src(s->st, T, s->S, U, A.acc, A.lamU, ab2);
     LAUNCHCHK(s);
     return RQ_OK;
   }
   const float a = s->p.smooth_a, c = s->p.smooth_c;
   for (int it = 0; it < s->p.jacobi_sweeps; ++it) {
-    launch_smooth_adj(s->st, T, s->S, A.lamP, A.lamP2, U, A.lamU, a, c, ab2 / c);
+    launch_smooth_adj(s->st, T, s->S, kAdjSrc, A.lamP, A.lamP2, U, A.lamU, nullptr, a, c, ab2 / c, 0.0f);
     std::swap(A.lamP, A.lamP2);
   }
   LAUNCHCHK(s);
@@ -856,7 +879,21 @@ int frozen_run(rq_solver* s, const rq_adjoint_spec* sp, bool backward, double* E
     return fail(s, RQ_EINVAL, "adjoint implements the cloud/solver.py semantics (init/energy S1, no rc rule)");
   AdjState& A = s->adj;
   const int64_t Tn = sp->n_steps;
-  const int C = sp->checkpoint_interval > 0 ? sp->checkpoint_interval : 64;
+  // checkpoint interval; <= 0: the whole trajectory as one segment when its
+  // 2 T + 1 fields (beyond those already held) fit in a quarter of the free
+  // device memory -- the forward pass then records every step's inputs and
+  // the reverse sweep recomputes nothing -- else 64
+  int C = sp->checkpoint_interval;
+  if (C <= 0) {
+    C = 64;
+    const int64_t T1 = std::max<int64_t>(Tn, 1);
+    size_t free_b = 0, total_b = 0;
+    if (backward && T1 <= (int64_t(1) << 20) && hipMemGetInfo(&free_b, &total_b) == hipSuccess) {
+      const size_t want = 2 * (size_t)T1 + 1, have = A.segU.size() + A.segP.size();
+      const size_t extra = want > have ? want - have : 0;
+      if (extra * s->S3 * sizeof(float) <= free_b / 4) C = (int)T1;
+    }
+  }
   const int64_t nck = backward ? (Tn + C - 1) / C + 1 : 0;
   int rc;
   for (float** p : {&A.t1, &A.t2, &A.t3})
@@ -885,15 +922,25 @@ int frozen_run(rq_solver* s, const rq_adjoint_spec* sp, bool backward, double* E
   if (backward) {
     if ((rc = copy_field(s, A.ckU[0], s->U)) || (rc = copy_field(s, A.ckP[0], s->P))) return rc;
   }
+  // the last segment's inputs (U before each step, P of each Poisson step)
+  // are recorded here, so the reverse sweep does not recompute that segment
+  const int64_t nseg = backward ? (Tn + C - 1) / C : 0;
+  const int64_t t0_last = nseg > 0 ? (nseg - 1) * C : Tn;
   for (int64_t t = 1; t <= Tn; ++t) {
+    const bool rec = backward && t > t0_last;
+    const int idx = rec ? (int)(t - t0_last - 1) : 0;
+    if (rec && idx == 0 && (rc = copy_field(s, A.segU[0], s->U))) return rc;
+    bool did_itp = false;
     if (t % sc.poisson_interval == 0) {
-      if ((rc = poisson_into(s, s->P, s->P2, s->U, A.t1, A.t2))) return rc;
+      if ((rc = poisson_into(s, s->P, s->P2, s->U, A.t1, A.t2, s->U2, &did_itp))) return rc;
       std::swap(s->P, s->P2);
+      if (rec && (rc = copy_field(s, A.segP[idx], s->P))) return rc;
     }
-    launch_itp(s->st, T, s->S, s->U, s->U2, s->K, s->P, s->lab, s->half_dv, s->dtv);
+    if (!did_itp) launch_itp(s->st, T, s->S, s->U, s->U2, s->K, s->P, s->lab, s->half_dv, s->dtv);
     LAUNCHCHK(s);
     std::swap(s->U, s->U2);
     if (t % sc.norm_interval == 0 && (rc = normalize_on(s, s->U))) return rc;
+    if (rec && t < Tn && (rc = copy_field(s, A.segU[idx + 1], s->U))) return rc;
     if (backward && t % C == 0 && t < Tn) {
       if ((rc = copy_field(s, A.ckU[t / C], s->U)) || (rc = copy_field(s, A.ckP[t / C], s->P))) return rc;
     }
@@ -921,22 +968,27 @@ int frozen_run(rq_solver* s, const rq_adjoint_spec* sp, bool backward, double* E
   launch_energy_grad(s->st, T, s->S, s->h, s->h3, s->U, s->K, s->P, s->lab, A.lamU, A.gK, A.lamP);
   LAUNCHCHK(s);
   std::vector<const float*> Pused(C);
-  const int64_t nseg = (Tn + C - 1) / C;
   for (int64_t sg = nseg - 1; sg >= 0; --sg) {
     const int64_t t0 = sg * C, t1 = std::min<int64_t>(Tn, t0 + C);
-    if ((rc = copy_field(s, A.segU[0], A.ckU[sg]))) return rc;
+    // the last segment was recorded by the forward pass
+    const bool recompute = sg != nseg - 1;
+    if (recompute && (rc = copy_field(s, A.segU[0], A.ckU[sg]))) return rc;
     const float* Pprev = A.ckP[sg];
     for (int64_t t = t0 + 1; t <= t1; ++t) {
       const int idx = (int)(t - t0 - 1);
+      bool did_itp = false;
       if (t % sc.poisson_interval == 0) {
-        if ((rc = poisson_into(s, Pprev, A.segP[idx], A.segU[idx], A.t1, A.t2))) return rc;
+        if (recompute && (rc = poisson_into(s, Pprev, A.segP[idx], A.segU[idx], A.t1, A.t2,
+                                            t < t1 ? A.segU[idx + 1] : nullptr, &did_itp)))
+          return rc;
         Pused[idx] = A.segP[idx];
       } else {
         Pused[idx] = Pprev;
       }
       Pprev = Pused[idx];
-      if (t < t1) {
-        launch_itp(s->st, T, s->S, A.segU[idx], A.segU[idx + 1], s->K, Pused[idx], s->lab, s->half_dv, s->dtv);
+      if (recompute && t < t1) {
+        if (!did_itp)
+          launch_itp(s->st, T, s->S, A.segU[idx], A.segU[idx + 1], s->K, Pused[idx], s->lab, s->half_dv, s->dtv);
         LAUNCHCHK(s);
         if (t % sc.norm_interval == 0 && (rc = normalize_on(s, A.segU[idx + 1]))) return rc;
       }

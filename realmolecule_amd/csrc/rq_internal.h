@@ -69,6 +69,9 @@ void launch_residual_fine(hipStream_t st, const Tile& t, int S, const float* P, 
 void launch_residual_coarse(hipStream_t st, const Tile& t, int S, const float* E, const float* g, float* r);
 void launch_restrict4(hipStream_t st, int Sf, const float* r, float* gc, float sign);
 void launch_prolong_correct(hipStream_t st, const Tile& t, int Sf, float* x, const float* ec, float scale);
+// x = 0 + prolongation of ec on the interior (x's faces untouched): the
+// correction into a zero-filled x without the fill or the read of x
+void launch_prolong_set(hipStream_t st, const Tile& t, int Sf, float* x, const float* ec);
 // Smat (m x m DST matrix) and rden (m^3 reciprocal eigenvalue sums) may be
 // null, which selects the line-transform variant that computes them itself.
 void launch_coarse_exact(hipStream_t st, int Sc, const float* g, float* e, const double* Smat, const double* rden);
@@ -146,8 +149,10 @@ void launch_energy_grad(hipStream_t st, const Tile& t, int S, float h, float h3,
 void launch_itp_adj(hipStream_t st, const Tile& t, int S, const float* lam, float* lam_out, const float* U,
                     const float* K, const float* P, const uint16_t* lab, float half_dv, float dtv, float* gK,
                     float* lamP);
-void launch_smooth_adj(hipStream_t st, const Tile& t, int S, const float* lam, float* lam_out, const float* src,
-                       float* acc, float a, float c, float coef);
+// smoother-adjoint accumulation modes (k_smooth_adj)
+enum { kAdjAcc = 0, kAdjSrc = 1, kAdjFirst = 2, kAdjLast = 3 };
+void launch_smooth_adj(hipStream_t st, const Tile& t, int S, int mode, const float* lam, float* lam_out,
+                       const float* src, float* acc, float* lamU, float a, float c, float coef, float coef2);
 void launch_correct_adj(hipStream_t st, const Tile& t, int S, const float* lam, const float* z, float* lam_out,
                         float* acc);
 void launch_axpy_src(hipStream_t st, const Tile& t, int S, const float* U, const float* acc, float* lamU, float coef);

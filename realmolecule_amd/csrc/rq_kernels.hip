@@ -617,11 +617,12 @@ __global__ __launch_bounds__(256) void k_smooth_itp(int S, int ich, const float*
   smooth_itp_col(S, m.k, m.j, m.i0, m.i1, P, Pn, U, Un, K, lab, a, c, alpha, beta, half_dv, dtv);
 }
 
+template <bool SET>
 __global__ __launch_bounds__(256) void k_prolong_correct(int Sf, int ich, float* __restrict__ x,
                                                          const float* __restrict__ e, float scale) {
   March m;
   if (!march_coords(Sf - 2, ich, m)) return;
-  prolong_correct_col(Sf, m.k, m.j, m.i0, m.i1, x, e, scale);
+  prolong_correct_col<SET>(Sf, m.k, m.j, m.i0, m.i1, x, e, scale);
 }
 
 // Exact coarsest-level solve of 6e - sum6 e = g (zero Dirichlet faces) in
@@ -1678,7 +1679,13 @@ void launch_restrict4(hipStream_t st, int Sf, const float* r, float* gc, float s
 }
 
 void launch_prolong_correct(hipStream_t st, const Tile& t, int Sf, float* x, const float* ec, float scale) {
-  hipLaunchKernelGGL(k_prolong_correct, stencil_grid(Sf - 2, t), dim3(t.bx, t.by), 0, st, Sf, t.ich, x, ec, scale);
+  hipLaunchKernelGGL(k_prolong_correct<false>, stencil_grid(Sf - 2, t), dim3(t.bx, t.by), 0, st, Sf, t.ich, x, ec,
+                     scale);
+}
+
+void launch_prolong_set(hipStream_t st, const Tile& t, int Sf, float* x, const float* ec) {
+  hipLaunchKernelGGL(k_prolong_correct<true>, stencil_grid(Sf - 2, t), dim3(t.bx, t.by), 0, st, Sf, t.ich, x, ec,
+                     1.0f);
 }
 
 template <int M>

@@ -357,18 +357,21 @@ __device__ __forceinline__ void restrict4_pt(int Sf, int I, int J, int K2, const
 // coarse plane m: P e = 0.5 (T(m) + T(m)) at 2m and 0.5 (T(m) + T(m+1)) at
 // 2m+1 (T = prolong_plane, the prolong_at terms), T(m+1) carried to the next
 // trip, so every load of a trip is issued without a branch.
+// SET: x = 0 + correction, without reading x (the sum into a zero-filled x)
+template <bool SET = false>
 __device__ __forceinline__ void prolong_correct_col(int Sf, int k, int j, int i0, int i1, float* __restrict__ x,
                                                     const float* __restrict__ e, float scale) {
   const int Sc = (Sf - 1) / 2 + 1;
   const int S2 = Sf * Sf;
   const int col = j * Sf + k;
+  auto xat = [&](int id) -> float { return SET ? 0.0f : x[id]; };
   auto apply = [&](int id, float xv, float corr) { x[id] = (scale == 1.0f) ? xv + corr : xv + scale * corr; };
   int i = i0, m = i0 >> 1;
   float ta = prolong_plane(e, Sc, m, j, k);
   if ((i & 1) && i < i1) {
     const float tb = prolong_plane(e, Sc, m + 1, j, k);
     const int id = i * S2 + col;
-    apply(id, x[id], 0.5f * (ta + tb));
+    apply(id, xat(id), 0.5f * (ta + tb));
     ta = tb;
     ++i;
     ++m;
@@ -377,14 +380,14 @@ __device__ __forceinline__ void prolong_correct_col(int Sf, int k, int j, int i0
   for (; i + 1 < i1; i += 2, ++m) {
     const float tb = prolong_plane(e, Sc, m + 1, j, k);
     const int id = i * S2 + col;
-    const float x0 = x[id], x1 = x[id + S2];
+    const float x0 = xat(id), x1 = xat(id + S2);
     apply(id, x0, 0.5f * (ta + ta));
     apply(id + S2, x1, 0.5f * (ta + tb));
     ta = tb;
   }
   if (i < i1) {
     const int id = i * S2 + col;
-    apply(id, x[id], 0.5f * (ta + ta));
+    apply(id, xat(id), 0.5f * (ta + ta));
   }
 }
 

@@ -81,12 +81,16 @@ def test_checkpoint_interval_does_not_change_gradient():
     NN, screen = 24, 24 * 0.3
     rows = three_atoms(NN)
     out = []
-    for ck in (7, 64):
+    for ck in (7, 64, 0):
         fz = FrozenSolver(rows, NN, screen, 50, bootstrap=4, checkpoint=ck)
         out.append(fz.energy_and_gradient(rows[:, :3] * fz.h))
         fz.close()
     assert out[0][0] == pytest.approx(out[1][0], rel=1e-12)
     assert np.allclose(out[0][1], out[1][1], rtol=1e-5, atol=1e-7)
+    # one segment recorded by the forward pass (64 >= 50 steps, and the
+    # automatic interval) against seven, six of them recomputed
+    assert out[2][0] == pytest.approx(out[1][0], rel=1e-12)
+    assert np.allclose(out[2][1], out[1][1], rtol=1e-5, atol=1e-7)
 
 
 def test_autograd_function_backward():
