@@ -522,8 +522,9 @@ int vcycle_rhs(rq_solver* s, float* x, float* x2, const float* rhs) {
 }
 
 // molecule.js direct Pother (molecule.js:2834-2858): per domain with Z > 0,
-// rho_other -> one V-cycle on its persistent potential; then the gather.
-int pother_direct_step(rq_solver* s) {
+// rho_other -> one V-cycle on its persistent potential; then the gather,
+// or with fuse_itp the gather fused into the step's ITP update (*did_itp).
+int pother_direct_step(rq_solver* s, bool fuse_itp, bool* did_itp) {
   if (s->dom.empty()) {
     for (int m = 0; m < s->n_atoms; ++m) {
       if (!(s->atoms_h[m].Z > 0.0f)) continue;
@@ -562,6 +563,17 @@ int pother_direct_step(rq_solver* s) {
       HIPCHK(s, hipStreamWaitEvent(s->st, d.done, 0));
     }
   }
+  if (fuse_itp) {
+    {
+      Timed t(s, F_ITP);
+      launch_itp_gather(s->st, s->tile, s->S, s->U, s->U2, s->K, s->lab, s->Pdom_d, s->n_atoms, s->P, s->half_dv,
+                        s->dtv);
+    }
+    LAUNCHCHK(s);
+    std::swap(s->U, s->U2);
+    if (did_itp) *did_itp = true;
+    return RQ_OK;
+  }
   launch_pother_gather(s->st, s->tile, s->S, s->lab, s->Pdom_d, s->n_atoms, s->P);
   LAUNCHCHK(s);
   return RQ_OK;
@@ -569,7 +581,7 @@ int pother_direct_step(rq_solver* s) {
 
 int poisson_step(rq_solver* s, bool twogrid_due, bool fuse_itp = false, bool* did_itp = nullptr) {
   if (did_itp) *did_itp = false;
-  if (s->p.pother_mode == RQ_POTHER_DIRECT) return pother_direct_step(s);
+  if (s->p.pother_mode == RQ_POTHER_DIRECT) return pother_direct_step(s, fuse_itp && s->fused, did_itp);
   if (s->p.poisson_mode == RQ_POISSON_MULTIGRID) return vcycle_fine(s, fuse_itp, did_itp);
   if (int rc = smooth_fine_n(s, s->p.smooth_a, s->p.smooth_c, s->p.jacobi_sweeps)) return rc;
   if (s->p.poisson_mode == RQ_POISSON_TWOGRID && twogrid_due) return twogrid_correction(s);

@@ -96,6 +96,9 @@ void launch_sub_pself(hipStream_t st, const Tile& t, int S, const float* Pself, 
                       float* Pother);
 void launch_pother_gather(hipStream_t st, const Tile& t, int S, const uint16_t* lab, const float* const* Pdom,
                           int n_dom, float* P);
+// k_pother_gather then the ITP update on the gathered P, in one pass
+void launch_itp_gather(hipStream_t st, const Tile& t, int S, const float* U, float* Un, const float* K,
+                       const uint16_t* lab, const float* const* Pdom, int n_dom, float* P, float half_dv, float dtv);
 bool launch_up(hipStream_t st, int S, int nu, const float* x, const float* e, float* x_out, const float* g, float a,
                float c);
 void launch_bottom(hipStream_t st, int S, int nu1, int nu2, float a, float c, const float* g, float* e,

@@ -1024,6 +1024,50 @@ __global__ __launch_bounds__(256) void k_sub_pself(int S, int ich, const float* 
 }
 
 // P(x) = P_label(x)(x) on the interior (molecule.js:664-688 copyPotherForLabel)
+// The ITP update (itp_col) with the direct potential gathered on the fly:
+// the centre P is the node's own domain potential Pdom[label] (0 past the
+// domains), also stored into P -- the values of k_pother_gather then k_itp
+// in one pass, without re-reading the labels and P.
+__global__ __launch_bounds__(256) void k_itp_gather(int S, int ich, const float* __restrict__ U,
+                                                    float* __restrict__ Un, const float* __restrict__ K,
+                                                    const uint16_t* __restrict__ lab,
+                                                    const float* const* __restrict__ Pdom, int n_dom,
+                                                    float* __restrict__ P, float half_dv, float dtv) {
+  March q;
+  if (!march_coords(S - 2, ich, q)) return;
+  const int S2 = S * S;
+  int id = q.i0 * S2 + q.j * S + q.k;
+  float um = U[id - S2], uc = U[id];
+  uint16_t lm = lab[id - S2], lc = lab[id];
+  for (int i = q.i0; i < q.i1; ++i, id += S2) {
+    const float up = U[id + S2];
+    const uint16_t lp = lab[id + S2];
+    const int L = lc & kLabelMask;
+    const float pv = (L < n_dom) ? Pdom[L][id] : 0.0f;
+    P[id] = pv;
+    float out;
+    if (lc & kRcFlag) {
+      out = 0.0f;
+    } else {
+      const float ujp = U[id + S], ujm = U[id - S], ukp = U[id + 1], ukm = U[id - 1];
+      const uint16_t ljp = lab[id + S], ljm = lab[id - S], lkp = lab[id + 1], lkm = lab[id - 1];
+      const float t0 = (lp == lc) ? up : uc;
+      const float t1 = (lm == lc) ? um : uc;
+      const float t2 = (ljp == lc) ? ujp : uc;
+      const float t3 = (ljm == lc) ? ujm : uc;
+      const float t4 = (lkp == lc) ? ukp : uc;
+      const float t5 = (lkm == lc) ? ukm : uc;
+      const float lap = t0 + t1 + t2 + t3 + t4 + t5 - 6.0f * uc;
+      out = uc + half_dv * lap + dtv * (K[id] - 2.0f * pv) * uc;
+    }
+    Un[id] = out;
+    um = uc;
+    uc = up;
+    lm = lc;
+    lc = lp;
+  }
+}
+
 __global__ __launch_bounds__(256) void k_pother_gather(int S, int ich, const uint16_t* __restrict__ lab,
                                                        const float* const* __restrict__ Pdom, int n_dom,
                                                        float* __restrict__ P) {
@@ -1846,6 +1890,12 @@ void launch_sub_pself(hipStream_t st, const Tile& t, int S, const float* Pself, 
 void launch_pother_gather(hipStream_t st, const Tile& t, int S, const uint16_t* lab, const float* const* Pdom,
                           int n_dom, float* P) {
   hipLaunchKernelGGL(k_pother_gather, stencil_grid(S - 2, t), dim3(t.bx, t.by), 0, st, S, t.ich, lab, Pdom, n_dom, P);
+}
+
+void launch_itp_gather(hipStream_t st, const Tile& t, int S, const float* U, float* Un, const float* K,
+                       const uint16_t* lab, const float* const* Pdom, int n_dom, float* P, float half_dv, float dtv) {
+  hipLaunchKernelGGL(k_itp_gather, stencil_grid(S - 2, t), dim3(t.bx, t.by), 0, st, S, t.ich, U, Un, K, lab, Pdom,
+                     n_dom, P, half_dv, dtv);
 }
 
 void launch_display_slice(hipStream_t st, int S, int sk, const float* U, const uint16_t* lab, const float* K,
