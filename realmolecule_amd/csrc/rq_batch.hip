@@ -31,8 +31,12 @@ struct BLev {
   int S, ich, gx, gy, gz;
   int tile0;     // first tile of this molecule in the level's tile space
   int cnode0;    // first coarse interior node (restriction items from this level)
+  int ctile0;    // first coarse tile (kBatchResCK x CJ x CI coarse nodes) of the tiled restriction
   long long off; // element offset of this molecule in the level's arrays
 };
+
+// coarse tile of the batch's residual + restriction (the single-grid tile)
+constexpr int kBatchResCK = 8, kBatchResCJ = 8, kBatchResCI = 4;
 
 struct BMol {
   int atom0, n_atoms, dom0, node0;
@@ -55,6 +59,16 @@ __device__ __forceinline__ int find_cnode(const BLev* __restrict__ L, int nm, in
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
     if (L[mid].cnode0 <= t) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ int find_ctile(const BLev* __restrict__ L, int nm, int t) {
+  int lo = 0, hi = nm - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (L[mid].ctile0 <= t) lo = mid;
     else hi = mid - 1;
   }
   return lo;
@@ -164,6 +178,31 @@ __global__ __launch_bounds__(256) void kb_itp(const BLev* __restrict__ L, const 
 }
 
 // items = coarse interior nodes of level l+1 over all molecules
+// Residual + 4 FW restriction over the concatenated coarse tiles of every
+// molecule (resres_tile: residuals of a tile's fine region once into LDS,
+// then 27-point sums) -- the values of kb_residual then kb_restrict4 without
+// the fine residual array.
+template <bool FINE>
+__global__ __launch_bounds__(kBatchResCK * kBatchResCJ * kBatchResCI) void kb_resres_tiled(
+    const BLev* __restrict__ Lf, const BLev* __restrict__ Lc, const BMol* __restrict__ M, int nm, int nct,
+    const float* __restrict__ x, const float* __restrict__ src, float* __restrict__ gc) {
+  constexpr int CK = kBatchResCK, CJ = kBatchResCJ, CI = kBatchResCI;
+  __shared__ float r[(2 * CK + 1) * (2 * CJ + 1) * (2 * CI + 1)];
+  for (int t = blockIdx.x; t < nct; t += gridDim.x) {
+    const int u = xcd_item(t, nct);
+    const int m = find_ctile(Lf, nm, u);
+    const BLev d = Lf[m];
+    const int nc = (d.S - 1) / 2 - 1;
+    const int gx = (nc + CK - 1) / CK, gy = (nc + CJ - 1) / CJ;
+    const int local = u - d.ctile0;
+    const float al = FINE ? M[m].two_pi : 1.0f, be = FINE ? M[m].h2 : 1.0f;
+    resres_tile<FINE, CK, CJ, CI>(d.S, 1 + (local % gx) * CK, 1 + ((local / gx) % gy) * CJ,
+                                  1 + (local / (gx * gy)) * CI, x + d.off, src + d.off, al, be, gc + Lc[m].off,
+                                  1.0f, r);
+    __syncthreads();
+  }
+}
+
 template <bool FINE>
 __global__ __launch_bounds__(256) void kb_residual_restrict(const BLev* __restrict__ Lf, const BLev* __restrict__ Lc,
                                                             const BMol* __restrict__ M, int nm, int nitems,
@@ -179,32 +218,6 @@ __global__ __launch_bounds__(256) void kb_residual_restrict(const BLev* __restri
     const float al = FINE ? M[m].two_pi : 1.0f, be = FINE ? M[m].h2 : 1.0f;
     residual_restrict_pt<FINE>(Sf, local / (nc * nc) + 1, (local / nc) % nc + 1, local % nc + 1, x + Lf[m].off,
                                src + Lf[m].off, al, be, gc + Lc[m].off, 1.0f);
-  }
-}
-
-template <bool FINE>
-__global__ __launch_bounds__(256) void kb_residual(const BLev* __restrict__ L, const BMol* __restrict__ M, int nm,
-                                                   int ntiles, const float* __restrict__ x,
-                                                   const float* __restrict__ src, float* __restrict__ r) {
-  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
-    Col q;
-    if (!item_col(L, nm, ntiles, t, q)) continue;
-    const long long o = L[q.m].off;
-    const float al = FINE ? M[q.m].two_pi : 1.0f, be = FINE ? M[q.m].h2 : 1.0f;
-    residual_col<FINE>(L[q.m].S, q.k, q.j, q.i0, q.i1, x + o, src + o, al, be, r + o);
-  }
-}
-
-__global__ __launch_bounds__(256) void kb_restrict4(const BLev* __restrict__ Lf, const BLev* __restrict__ Lc, int nm,
-                                                    int nitems, const float* __restrict__ r, float* __restrict__ gc) {
-  const int stride = gridDim.x * blockDim.x;
-  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nitems; t += stride) {
-    const int m = find_cnode(Lf, nm, t);
-    const int Sf = Lf[m].S;
-    const int nc = (Sf - 1) / 2 - 1;
-    const int local = t - Lf[m].cnode0;
-    restrict4_pt(Sf, local / (nc * nc) + 1, (local / nc) % nc + 1, local % nc + 1, r + Lf[m].off, gc + Lc[m].off,
-                 1.0f);
   }
 }
 
@@ -407,7 +420,7 @@ struct rq_batch {
   std::vector<double> hh, dv;
   std::vector<std::vector<BLev>> lev;  // [level][mol]
   std::vector<BMol> mol;
-  std::vector<int> ntiles, ncnodes;    // per level
+  std::vector<int> ntiles, ncnodes, nctiles;  // per level
   std::vector<size_t> len;             // elements per level array
   std::vector<int> maxS;               // largest node count per axis on each level
   int nnodes = 0, ndom = 0;
@@ -538,10 +551,9 @@ int b_vcycle_coarse(rq_batch* b, int l) {
       hipLaunchKernelGGL(kb_residual_restrict<false>, dim3(std::min((ni + 255) / 256, 4096)), dim3(256), 0, b->st,
                          LD(b, l), LD(b, l + 1), b->mol_d, b->nm, ni, b->e0[l], b->g[l], b->g[l + 1]);
     } else {
-      hipLaunchKernelGGL(kb_residual<false>, dim3(grid_for(nt)), blk, 0, b->st, LD(b, l), b->mol_d, b->nm, nt,
-                         b->e0[l], b->g[l], b->e1[l]);
-      hipLaunchKernelGGL(kb_restrict4, dim3(std::min((ni + 255) / 256, 4096)), dim3(256), 0, b->st, LD(b, l),
-                         LD(b, l + 1), b->nm, ni, b->e1[l], b->g[l + 1]);
+      const int nct = b->nctiles[l];
+      hipLaunchKernelGGL(kb_resres_tiled<false>, dim3(grid_for(nct)), dim3(kBatchResCK * kBatchResCJ * kBatchResCI),
+                         0, b->st, LD(b, l), LD(b, l + 1), b->mol_d, b->nm, nct, b->e0[l], b->g[l], b->g[l + 1]);
     }
   }
   BLAUNCH(b);
@@ -594,9 +606,9 @@ int b_vcycle(rq_batch* b, bool fuse_itp, bool* did_itp) {
       hipLaunchKernelGGL(kb_residual_restrict<true>, dim3(std::min((ni + 255) / 256, 4096)), dim3(256), 0, b->st,
                          LD(b, 0), LD(b, 1), b->mol_d, b->nm, ni, b->P, b->U, b->g[1]);
     } else {
-      hipLaunchKernelGGL(kb_residual<true>, grd, blk, 0, b->st, LD(b, 0), b->mol_d, b->nm, nt, b->P, b->U, b->P2);
-      hipLaunchKernelGGL(kb_restrict4, dim3(std::min((ni + 255) / 256, 4096)), dim3(256), 0, b->st, LD(b, 0),
-                         LD(b, 1), b->nm, ni, b->P2, b->g[1]);
+      const int nct = b->nctiles[0];
+      hipLaunchKernelGGL(kb_resres_tiled<true>, dim3(grid_for(nct)), dim3(kBatchResCK * kBatchResCJ * kBatchResCI),
+                         0, b->st, LD(b, 0), LD(b, 1), b->mol_d, b->nm, nct, b->P, b->U, b->g[1]);
     }
   }
   BLAUNCH(b);
@@ -757,8 +769,9 @@ int upload_descriptors(rq_batch* b) {
 void layout(rq_batch* b, const std::vector<int>* ich_fine) {
   b->ntiles.assign(b->L, 0);
   b->ncnodes.assign(b->L, 0);
+  b->nctiles.assign(b->L, 0);
   for (int l = 0; l < b->L; ++l) {
-    int tile = 0, cnode = 0;
+    int tile = 0, cnode = 0, ctile = 0;
     for (int m = 0; m < b->nm; ++m) {
       BLev& d = b->lev[l][m];
       const int n = d.S - 2;
@@ -770,11 +783,17 @@ void layout(rq_batch* b, const std::vector<int>* ich_fine) {
       d.tile0 = tile;
       tile += d.gx * d.gy * d.gz;
       d.cnode0 = cnode;
+      d.ctile0 = ctile;
       const int nc = (d.S - 1) / 2 - 1;
-      if (l + 1 < b->L) cnode += nc * nc * nc;
+      if (l + 1 < b->L) {
+        cnode += nc * nc * nc;
+        ctile += ((nc + kBatchResCK - 1) / kBatchResCK) * ((nc + kBatchResCJ - 1) / kBatchResCJ) *
+                 ((nc + kBatchResCI - 1) / kBatchResCI);
+      }
     }
     b->ntiles[l] = tile;
     b->ncnodes[l] = cnode;
+    b->nctiles[l] = ctile;
   }
 }
 

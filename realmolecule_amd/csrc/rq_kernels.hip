@@ -267,36 +267,12 @@ template <bool FINE, int CK, int CJ, int CI>
 __global__ __launch_bounds__(CK * CJ * CI) void k_resres_tiled(int Sf, const float* __restrict__ x,
                                                                 const float* __restrict__ src, float alpha,
                                                                 float beta, float* __restrict__ gc, float sign) {
-  constexpr int FK = 2 * CK + 1, FJ = 2 * CJ + 1, FI = 2 * CI + 1;
-  constexpr int NT = CK * CJ * CI, NF = FK * FJ * FI;
-  __shared__ float r[NF];
-  const int nc = (Sf - 1) / 2 - 1, n = Sf - 2;
-  const int Sc = nc + 2, S2 = Sf * Sf;
+  __shared__ float r[(2 * CK + 1) * (2 * CJ + 1) * (2 * CI + 1)];
+  const int nc = (Sf - 1) / 2 - 1;
   const int gx = (nc + CK - 1) / CK, gy = (nc + CJ - 1) / CJ;
   const int t = xcd_block();
-  const int K0 = 1 + (t % gx) * CK, J0 = 1 + ((t / gx) % gy) * CJ, I0 = 1 + (t / (gx * gy)) * CI;
-  const int fk0 = 2 * K0 - 1, fj0 = 2 * J0 - 1, fi0 = 2 * I0 - 1;
-  const int tid = threadIdx.x;
-  // all residual loads of a thread are independent and (indices clamped to
-  // the interior) unconditional, so the unrolled loop has them in flight
-  // together; nodes past the interior store 0 (no valid coarse node reads them)
-#pragma unroll
-  for (int it = 0; it < (NF + NT - 1) / NT; ++it) {
-    const int q = tid + it * NT;
-    const int qq = q < NF ? q : NF - 1;
-    const int li = qq / (FJ * FK), lj = (qq / FK) % FJ, lk = qq % FK;
-    const int fi = fi0 + li, fj = fj0 + lj, fk = fk0 + lk;
-    const bool in = fi <= n && fj <= n && fk <= n;
-    const float v = residual_at<FINE>(x, src, min(fi, n) * S2 + min(fj, n) * Sf + min(fk, n), S2, Sf, alpha, beta);
-    if (q < NF) r[q] = in ? v : 0.0f;
-  }
-  __syncthreads();
-  const int tk = tid % CK, tj = (tid / CK) % CJ, ti = tid / (CK * CJ);
-  const int K = K0 + tk, J = J0 + tj, I = I0 + ti;
-  if (K > nc || J > nc || I > nc) return;
-  float out = restrict27(r, (2 * ti + 1) * (FJ * FK) + (2 * tj + 1) * FK + (2 * tk + 1), FJ * FK, FK, 1);
-  if (sign < 0.0f) out = -out;
-  gc[I * Sc * Sc + J * Sc + K] = out;
+  resres_tile<FINE, CK, CJ, CI>(Sf, 1 + (t % gx) * CK, 1 + ((t / gx) % gy) * CJ, 1 + (t / (gx * gy)) * CI, x, src,
+                                alpha, beta, gc, sign, r);
 }
 
 
