@@ -129,6 +129,7 @@ struct rq_solver {
   // 0: none -- on gfx950 the separate correction and sweep launches measured
   // faster on every level of the 192^3 workload.  RQ_FUSE_MAX_S overrides it.
   int fuse_max_s = 0;
+  int itp_fuse_max_s = kItpFuseMaxS;  // largest grid whose last sweep carries the ITP update (RQ_ITP_FUSE_MAX_S)
   std::vector<GraphEntry> graphs;
   std::string err;
 };
@@ -398,6 +399,7 @@ int vcycle_coarse(rq_solver* s, int l) {
 // and the ITP update run as one kernel and *did_itp is set.
 int vcycle_fine(rq_solver* s, bool fuse_itp, bool* did_itp) {
   if (did_itp) *did_itp = false;
+  fuse_itp = fuse_itp && s->S <= s->itp_fuse_max_s;
   const float a = s->p.mg_a, c = s->p.mg_c;
   if (int rc = smooth_fine_n(s, a, c, s->p.mg_nu1)) return rc;
   const bool small = s->fused && s->S <= s->fuse_max_s;
@@ -787,7 +789,7 @@ int poisson_into(rq_solver* s, const float* P_in, float* P_out, const float* U, 
     for (int it = 0; it < s->p.mg_nu2; ++it) {
       const bool last = it == s->p.mg_nu2 - 1;
       float* dst = last ? P_out : other_of(src, t1, t2);
-      if (last && U_out && s->fused) {
+      if (last && U_out && s->fused && s->S <= s->itp_fuse_max_s) {
         launch_smooth_itp(s->st, T, s->S, src, dst, U, U_out, s->K, s->lab, a, c, s->two_pi, s->h2, s->half_dv,
                           s->dtv);
         if (did_itp) *did_itp = true;
@@ -1128,6 +1130,7 @@ int rq_create(const rq_params* pp, const rq_atom* atoms, int32_t n_atoms, rq_sol
   rq_solver* s = new rq_solver();
   s->p = p;
   if (const char* fm = std::getenv("RQ_FUSE_MAX_S")) s->fuse_max_s = std::atoi(fm);
+  if (const char* fi = std::getenv("RQ_ITP_FUSE_MAX_S")) s->itp_fuse_max_s = std::atoi(fi);
   if (const char* s2 = std::getenv("RQ_SMOOTH2")) s->smooth2 = std::atoi(s2) != 0;
   if (const char* s2m = std::getenv("RQ_SMOOTH2_MIN_S")) s->smooth2_min_s = std::atoi(s2m);
   s->NN = p.NN;

@@ -25,6 +25,11 @@ constexpr int kSmooth2Blocks = 8192;    // target workgroup count of the two-swe
 constexpr int kSmooth2MinS = 320;       // smallest level (nodes per axis) that runs sweep pairs as that march
 constexpr int kDomStreams = 4;          // concurrent per-domain V-cycles of the direct potential (one stream each)
 constexpr int kReduceIch = 24;          // planes per thread of the per-domain reductions (fewer global atomics)
+// largest fine grid (nodes per axis) whose last post-sweep carries the ITP
+// update: fused while the fields fit the 256 MB MALL (193^3: 28.5 us against
+// 14.4 + 22.8 separate); from HBM the two streaming passes win (257^3:
+// 0.379 against 0.385 ms per step, 513^3: 2.96 against 3.08)
+constexpr int kItpFuseMaxS = 200;
 constexpr int kResResTiledMaxNc = 63;   // 3D-tiled residual+restriction up to 63 coarse interior nodes, march above
 constexpr int kBottomMaxS = 25;         // last two V-cycle levels in one LDS-resident workgroup up to 25^3 nodes
 
