@@ -426,6 +426,7 @@ struct rq_batch {
   int nnodes = 0, ndom = 0;
   int grid_stencil = 2048;
   bool fuse_small = false;  // one-pass residual+restriction / prolongation+sweep on levels <= 65^3 (RQ_BATCH_FUSE_SMALL=1)
+  bool fuse_itp = true;     // last post-sweep carries the ITP update (RQ_BATCH_FUSE_ITP=0: two passes)
   BLev* lev_d = nullptr;  // L * nm
   BMol* mol_d = nullptr;
   Atom* atoms_d = nullptr;
@@ -664,7 +665,7 @@ int b_normalize(rq_batch* b) {
 int b_step(rq_batch* b, int64_t step, const rq_schedule* sc) {
   bool did = false;
   if (step % sc->poisson_interval == 0)
-    if (int rc = b_vcycle(b, true, &did)) return rc;
+    if (int rc = b_vcycle(b, b->fuse_itp, &did)) return rc;
   if (!did) {
     BTimed t(b, F_ITP);
     const int nt = b->ntiles[0];
@@ -814,6 +815,7 @@ int rq_batch_create(const rq_params* pp, const rq_molecule* mols, int32_t n_mols
   rq_batch* b = new rq_batch();
   b->p = p;
   if (const char* fs = std::getenv("RQ_BATCH_FUSE_SMALL")) b->fuse_small = std::atoi(fs) != 0;
+  if (const char* fi = std::getenv("RQ_BATCH_FUSE_ITP")) b->fuse_itp = std::atoi(fi) != 0;
   b->nm = n_mols;
   b->L = L;
   std::vector<Atom> atoms;
