@@ -221,6 +221,15 @@ Tile tile_for(int S, int want = 8192) {
   const int n = S - 2;
   t.bx = n >= 128 ? 64 : (n >= 24 ? 32 : 16);
   t.by = 256 / t.bx;
+  if (n >= 255) {
+    // fields beyond the MALL stream from HBM: 128 x 2 tiles; the sweeps'
+    // target (8192) becomes 16384 workgroups, 4 planes at 257^3 (sweep 33.3
+    // us against 36.0 for 64 x 4 x 8, the ITP pass 77 against 80), 8 at
+    // 513^3; the prolongation's (4096) stays, 8 planes
+    t.bx = 128;
+    t.by = 2;
+    if (want >= 8192) want = 16384;
+  }
   if (n <= 64) {
     t.ich = 1;
     return t;
