@@ -93,6 +93,23 @@ def test_checkpoint_interval_does_not_change_gradient():
     assert np.allclose(out[2][1], out[1][1], rtol=1e-5, atol=1e-7)
 
 
+@pytest.mark.parametrize("tile", [(128, 2, 4), (32, 8, 16), (256, 1, 3)])
+def test_tile_shape_does_not_change_gradient(tile):
+    """The forward and reverse sweeps take the fine level's tile (128 x 2 on
+    grids of 255 or more nodes per axis): every shape gives the same energy
+    and gradient as the default, up to the order of the FP64 block sums."""
+    NN, screen = 24, 24 * 0.3
+    rows = three_atoms(NN)
+    fz = FrozenSolver(rows, NN, screen, 30, bootstrap=4, poisson="multigrid", mg_levels=3)
+    pos = rows[:, :3] * fz.h
+    E0, g0 = fz.energy_and_gradient(pos)
+    fz.s._chk(fz.s._lib.rq_set_tile(fz.s._h, 0, *tile))
+    E1, g1 = fz.energy_and_gradient(pos)
+    fz.close()
+    assert E1 == pytest.approx(E0, rel=1e-12)
+    assert np.allclose(g1, g0, rtol=1e-5, atol=1e-7)
+
+
 def test_autograd_function_backward():
     NN, screen = 16, 16 * 0.3
     rows = three_atoms(NN)
