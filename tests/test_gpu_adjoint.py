@@ -1,7 +1,7 @@
 """GPU adjoint forces: hand-written reverse sweep (librq.so) against autograd of
 the FP64 torch oracle of the same frozen-label trajectory, against central
-finite differences of the GPU solve, checkpoint-interval invariance, and the
-torch.autograd.Function wrapper."""
+finite differences of the GPU solve, checkpoint-interval and tile-shape
+invariance, and the torch.autograd.Function wrapper."""
 import numpy as np
 import pytest
 import torch
