@@ -118,6 +118,12 @@ int rq_mf_write_field(rq_mf* h, int32_t field, int32_t m, const float* host, siz
  * device time (ms) and count for the last rq_mf_do_steps call */
 int rq_mf_set_timing(rq_mf* h, int32_t on);
 int rq_mf_update_time(rq_mf* h, double* ms, int32_t* launches);
+/* Host-only query (no device call): the (tk, tj) node tile of the
+ * three-plane LDS-ring update for a grid of NN cells, and the LDS slot its
+ * lanes past tk * tj write instead of a node's slot.  A tile's LDS region
+ * is (tk + 2) x (tj + 2) slots, all below sink_slot.  No counterpart in the
+ * reference (the WGSL update has no LDS tiles); exported for tests. */
+void rq_mf_ring_tile(int32_t NN, int32_t* tk, int32_t* tj, int32_t* sink_slot);
 
 #ifdef __cplusplus
 }

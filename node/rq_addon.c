@@ -405,21 +405,35 @@ static napi_value js_pother_frame(napi_env env, napi_callback_info info) {
   return NULL;
 }
 
+/* writeField(h, name, array): a Float32Array (an Int32Array for "label") of
+ * S^3 elements.  The array type is checked before the handle, so a wrong
+ * type is a TypeError whatever the handle. */
 static napi_value js_write_field(napi_env env, napi_callback_info info) {
   napi_value argv[3];
   if (!get_args(env, info, 3, argv)) return NULL;
-  rq_solver* h = get_handle(env, argv[0]);
-  if (!h) return NULL;
   int f = field_id(env, argv[1]);
   if (f < 0) {
     napi_throw_type_error(env, NULL, "unknown field name");
     return NULL;
   }
-  napi_typedarray_type t;
+  napi_typedarray_type t = napi_int8_array;
   size_t len = 0, off = 0;
   void* data = NULL;
   napi_value ab;
-  NAPI_CALL(env, napi_get_typedarray_info(env, argv[2], &t, &len, &data, &ab, &off));
+  bool is_ta = false;
+  NAPI_CALL(env, napi_is_typedarray(env, argv[2], &is_ta));
+  if (is_ta) NAPI_CALL(env, napi_get_typedarray_info(env, argv[2], &t, &len, &data, &ab, &off));
+  /* the label map is int32 on the host, every other field float32: the
+   * byte count passed on is that of the array's own elements */
+  const napi_typedarray_type want = f == RQ_FIELD_LABEL ? napi_int32_array : napi_float32_array;
+  if (!is_ta || t != want) {
+    napi_throw_type_error(env, NULL,
+                          f == RQ_FIELD_LABEL ? "writeField: label needs an Int32Array"
+                                              : "writeField: field needs a Float32Array");
+    return NULL;
+  }
+  rq_solver* h = get_handle(env, argv[0]);
+  if (!h) return NULL;
   RQ_CALL(env, h, rq_write_field(h, f, data, len * 4));
   return NULL;
 }
@@ -438,16 +452,23 @@ static napi_value js_set_atoms(napi_env env, napi_callback_info info) {
   return NULL;
 }
 
+/* setLockGroups(h, Int32Array(nAtoms)) (molecule.js:487-500) */
 static napi_value js_set_lock_groups(napi_env env, napi_callback_info info) {
   napi_value argv[2];
   if (!get_args(env, info, 2, argv)) return NULL;
-  rq_solver* h = get_handle(env, argv[0]);
-  if (!h) return NULL;
-  napi_typedarray_type t;
+  napi_typedarray_type t = napi_int8_array;
   size_t len = 0, off = 0;
   void* data = NULL;
   napi_value ab;
-  NAPI_CALL(env, napi_get_typedarray_info(env, argv[1], &t, &len, &data, &ab, &off));
+  bool is_ta = false;
+  NAPI_CALL(env, napi_is_typedarray(env, argv[1], &is_ta));
+  if (is_ta) NAPI_CALL(env, napi_get_typedarray_info(env, argv[1], &t, &len, &data, &ab, &off));
+  if (!is_ta || t != napi_int32_array) {
+    napi_throw_type_error(env, NULL, "setLockGroups: expected an Int32Array");
+    return NULL;
+  }
+  rq_solver* h = get_handle(env, argv[0]);
+  if (!h) return NULL;
   RQ_CALL(env, h, rq_set_lock_groups(h, (const int32_t*)data, (int32_t)len));
   return NULL;
 }

@@ -168,6 +168,7 @@ PROTOTYPES = {
     "rq_mf_write_field": (C.c_int, [_P, _I, _I, _F, C.c_size_t]),
     "rq_mf_set_timing": (C.c_int, [_P, _I]),
     "rq_mf_update_time": (C.c_int, [_P, _D, C.POINTER(_I)]),
+    "rq_mf_ring_tile": (None, [_I, C.POINTER(_I), C.POINTER(_I), C.POINTER(_I)]),
 }
 
 _lib = None

@@ -671,6 +671,14 @@ float* field_ptr(rq_solver* s, int f) {
 }
 
 void free_all(rq_solver* s) {
+  // drain the stream, then destroy the step graphs before the buffers their
+  // kernel nodes reference are freed
+  if (s->st) (void)hipStreamSynchronize(s->st);
+  for (auto& ge : s->graphs) {
+    (void)hipGraphExecDestroy(ge.exec);
+    (void)hipGraphDestroy(ge.graph);
+  }
+  s->graphs.clear();
   auto fr = [](void* p) {
     if (p) (void)hipFree(p);
   };
@@ -711,11 +719,6 @@ void free_all(rq_solver* s) {
     (void)hipEventDestroy(ep.b);
   }
   for (auto e : s->ev_free) (void)hipEventDestroy(e);
-  for (auto& ge : s->graphs) {
-    (void)hipGraphExecDestroy(ge.exec);
-    (void)hipGraphDestroy(ge.graph);
-  }
-  s->graphs.clear();
   AdjState& A = s->adj;
   for (float* p : {A.lamU, A.lamU2, A.lamP, A.lamP2, A.gK, A.acc, A.t1, A.t2, A.t3}) fr(p);
   for (float* p : A.ckU) fr(p);
@@ -1450,6 +1453,20 @@ static void set_ptr_state(rq_solver* h, const PtrState& p) {
   h->e1 = p.e1;
 }
 
+// Destroys every cached step graph.  A launched executable may still be
+// running on the handle's stream (or be read by a tracing tool's completion
+// callback), and HIP does not document that hipGraphExecDestroy waits for
+// it, so the stream is drained first.
+static void drop_graphs(rq_solver* h) {
+  if (h->graphs.empty()) return;
+  (void)hipStreamSynchronize(h->st);
+  for (auto& ge : h->graphs) {
+    (void)hipGraphExecDestroy(ge.exec);
+    (void)hipGraphDestroy(ge.graph);
+  }
+  h->graphs.clear();
+}
+
 // One aligned chunk of norm_interval steps (no boundary rule inside) from a
 // cached HIP graph, capturing it on first use for the current buffer state.
 static int graph_chunk(rq_solver* h, int64_t c0, const rq_schedule* sc) {
@@ -1461,13 +1478,7 @@ static int graph_chunk(rq_solver* h, int64_t c0, const rq_schedule* sc) {
       return RQ_OK;
     }
   }
-  if (h->graphs.size() >= 8) {
-    for (auto& ge : h->graphs) {
-      (void)hipGraphExecDestroy(ge.exec);
-      (void)hipGraphDestroy(ge.graph);
-    }
-    h->graphs.clear();
-  }
+  if (h->graphs.size() >= 8) drop_graphs(h);
   GraphEntry ge;
   ge.before = cur;
   ge.pi = sc->poisson_interval;
@@ -1529,11 +1540,7 @@ int rq_set_fused(rq_solver* h, int32_t on) {
   h->fused = on != 0;
   h->bottom = on == 3;
   h->legs = on != 2;
-  for (auto& ge : h->graphs) {
-    (void)hipGraphExecDestroy(ge.exec);
-    (void)hipGraphDestroy(ge.graph);
-  }
-  h->graphs.clear();
+  drop_graphs(h);
   return RQ_OK;
 }
 
@@ -1541,14 +1548,6 @@ int rq_set_graph(rq_solver* h, int32_t on) {
   if (!h) return fail(nullptr, RQ_EINVAL, "null handle");
   h->use_graph = on != 0;
   return RQ_OK;
-}
-
-static void drop_graphs(rq_solver* h) {
-  for (auto& ge : h->graphs) {
-    (void)hipGraphExecDestroy(ge.exec);
-    (void)hipGraphDestroy(ge.graph);
-  }
-  h->graphs.clear();
 }
 
 int rq_set_tile(rq_solver* h, int32_t level, int32_t bx, int32_t by, int32_t ich) {

@@ -690,7 +690,12 @@ void set_bptrs(rq_batch* b, const BatchPtrs& p) {
   b->e1 = p.e1;
 }
 
+// Drains the batch stream before destroying its cached step graphs: a
+// launched executable may still be running (HIP does not document that
+// hipGraphExecDestroy waits for it).
 void drop_bgraphs(rq_batch* b) {
+  if (b->graphs.empty()) return;
+  if (b->st) (void)hipStreamSynchronize(b->st);
   for (auto& g : b->graphs) {
     (void)hipGraphExecDestroy(g.exec);
     (void)hipGraphDestroy(g.graph);

@@ -36,7 +36,8 @@ constexpr int kFinalThreads = 256;
 // at 3 electrons, 57 KB at 6); above 6 electrons the per-node kernel runs
 constexpr int kMfLdsMaxE = 6;
 // three-plane ring of the update with K: 3 x (3 NELEC + 1) planes of
-// kMfRingPlane floats (47.5 KB at 3 electrons); at 4 electrons it would hold
+// kMfRingPlane + 1 floats (46.5 KiB at 3 electrons; the last slot of a plane
+// is the write sink of the lanes past the tile); at 4 electrons it would hold
 // 2 waves per SIMD (208 VGPRs, 61.8 KB) against the two-plane march's 3
 constexpr int kMfRingMaxE = 3;
 constexpr int kMfRingPlane = 396;  // (TK + 2)(TJ + 2) bound of a ring tile (66 x 6)
@@ -584,7 +585,7 @@ __global__ __launch_bounds__(256, MF_LDS_WAVES) void k_mf_update_ring(Kc c, int 
                                                          float* __restrict__ Po) {
   // a (TK x TJ)-node tile per 256-thread workgroup, TK * TJ <= 256 and
   // (TK + 2)(TJ + 2) <= PL; the LDS field planes keep the fixed stride PL
-  constexpr int PL = kMfRingPlane, NF = 3 * NE, NL = NF + 1;
+  constexpr int PL = kMfRingPlane + 1, NF = 3 * NE, NL = NF + 1;
   __shared__ float ring[3][NL][PL];
   const int RK = TK + 2, RJ = TJ + 2, NH = 2 * RK + 2 * TJ;
   const int n = c.NN - 1, S = c.S, S2 = c.S2;
@@ -594,11 +595,15 @@ __global__ __launch_bounds__(256, MF_LDS_WAVES) void k_mf_update_ring(Kc c, int 
   const int tile = t % (gxt * gyt), z = t / (gxt * gyt);
   const int I0 = 1 + (z * n) / gz, I1 = 1 + ((z + 1) * n) / gz;
   const int tid = threadIdx.x;
-  // threads past TK * TJ repeat a node of the last row: they load and write
-  // the same values to the same LDS slots as its owner and store to the sink
+  // threads past TK * TJ repeat a node of the last row: they read its LDS
+  // slots, but write their own-column values to the sink slot kMfRingPlane
+  // (never read) and their results to the global sink.  Writing the owner's
+  // slot instead would race: a spare in another wave could store plane i + 2
+  // into the slot of plane i - 1 while the owner still reads plane i - 1.
   const int tk = tid % TK, tj = min(tid / TK, TJ - 1);
   const bool spare = tid >= TK * TJ;
   const int oslot = (tj + 1) * RK + (tk + 1);
+  const int wslot = spare ? kMfRingPlane : oslot;
   // this thread's halo node of the (RJ x RK) region, if any
   const bool has_halo = tid < NH;
   int hj = 0, hk = 0;
@@ -649,9 +654,9 @@ __global__ __launch_bounds__(256, MF_LDS_WAVES) void k_mf_update_ring(Kc c, int 
       }
 #pragma unroll
       for (int f = 0; f < NL; ++f) {
-        ring[2][f][oslot] = A[f];
-        ring[0][f][oslot] = B[f];
-        ring[1][f][oslot] = C[f];
+        ring[2][f][wslot] = A[f];
+        ring[0][f][wslot] = B[f];
+        ring[1][f][wslot] = C[f];
       }
       if (has_halo) {
 #pragma unroll
@@ -718,7 +723,7 @@ __global__ __launch_bounds__(256, MF_LDS_WAVES) void k_mf_update_ring(Kc c, int 
       // reads ended before this plane's barrier, its own-column reads are
       // this thread's and done; then the loads of plane i + 3 (clamped)
 #pragma unroll
-      for (int f = 0; f < NL; ++f) ring[qp][f][oslot] = R[f];
+      for (int f = 0; f < NL; ++f) ring[qp][f][wslot] = R[f];
       if (has_halo) {
 #pragma unroll
         for (int f = 0; f < NF; ++f) ring[qp][f][hslot] = H[f];
@@ -1333,6 +1338,19 @@ void rq_mf_default_params(rq_mf_params* p) {
 
 const char* rq_mf_last_error(const rq_mf* h) { return h ? h->err.c_str() : g_err.c_str(); }
 
+void rq_mf_ring_tile(int32_t NN, int32_t* tk, int32_t* tj, int32_t* sink_slot) {
+  // the fewest k-tiles of at most 64 nodes, split evenly, so little of a
+  // wave falls past the last interior node (199 nodes: 4 x 50 instead of
+  // 4 x 64), then as many j-rows as 256 threads and the LDS plane allow
+  const int nn = std::max(1, NN - 1);
+  const int ntk = (nn + 63) / 64;
+  const int tkr = (nn + ntk - 1) / ntk;
+  const int tjr = std::max(1, std::min({256 / tkr, kMfRingPlane / (tkr + 2) - 2, 16}));
+  if (tk) *tk = tkr;
+  if (tj) *tj = tjr;
+  if (sink_slot) *sink_slot = kMfRingPlane;
+}
+
 int rq_mf_create(const rq_mf_params* p, const rq_mf_nucleus* nuclei, int32_t n, rq_mf** out) {
   if (!p || !nuclei || !out) return fail(nullptr, RQ_EINVAL, "null argument");
   *out = nullptr;
@@ -1410,13 +1428,9 @@ int rq_mf_create(const rq_mf_params* p, const rq_mf_nucleus* nuclei, int32_t n, 
     const int gz = std::max(1, std::min(nn, blocks / (gx * gy)));
     h->lds_ci = (nn + gz - 1) / gz;
     h->lds_grid = dim3(gx, gy, (nn + h->lds_ci - 1) / h->lds_ci);
-    // ring tiles: the fewest k-tiles of at most 64 nodes, split evenly, so
-    // little of a wave falls past the last interior node (199 nodes: 4 x 50
-    // instead of 4 x 64), then as many j-rows as 256 threads and the LDS
-    // plane allow
-    const int ntk = (nn + 63) / 64;
-    const int tkr = (nn + ntk - 1) / ntk;
-    const int tjr = std::max(1, std::min({256 / tkr, kMfRingPlane / (tkr + 2) - 2, 16}));
+    // ring tiles (rq_mf_ring_tile)
+    int tkr = 0, tjr = 0;
+    rq_mf_ring_tile(h->NN, &tkr, &tjr, nullptr);
     h->ring_tk = tkr;
     h->ring_tj = tjr;
     // plane ranges per tile: about kMfRingPlanes planes each, rounded so

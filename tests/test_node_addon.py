@@ -30,6 +30,11 @@ const a = require('./node/rq_addon.node');
 const out = {keys: Object.keys(a), abi: a.abiVersion(), dv: [a.dvForElectrons(3), a.dvForElectrons(150)]};
 try { a.create({NN: 9, screen: 3}, [[4, 4, 4, 1, 0]]); out.err = 'none'; } catch (e) { out.err = e.message; }
 try { a.create({NN: 32, screen: 3}, 'x'); } catch (e) { out.err2 = e.message; }
+// typed-array types are checked before the handle: no device call is made
+const te = (f) => { try { f(); return 'none'; } catch (e) { return e.constructor.name + ': ' + e.message; } };
+out.wf = [te(() => a.writeField({}, 'U', new Uint8Array(64))), te(() => a.writeField({}, 'U', new Float64Array(8))),
+          te(() => a.writeField({}, 'label', new Float32Array(8))), te(() => a.writeField({}, 'U', [1, 2])),
+          te(() => a.setLockGroups({}, new Uint32Array(3))), te(() => a.writeField({}, 'U', new Float32Array(8)))];
 console.log(JSON.stringify(out));
 """
     r = run_node(code)
@@ -39,6 +44,13 @@ console.log(JSON.stringify(out));
     assert out["abi"] == 1 and out["dv"] == [0.12, 0.02]
     assert "NN=9" in out["err"]
     assert "atoms must be an array" in out["err2"]
+    wf = out["wf"]
+    assert wf[0] == wf[1] == "TypeError: writeField: field needs a Float32Array"
+    assert wf[2] == "TypeError: writeField: label needs an Int32Array"
+    assert wf[3] == "TypeError: writeField: field needs a Float32Array"
+    assert wf[4] == "TypeError: setLockGroups: expected an Int32Array"
+    # a well-typed array reaches the handle check
+    assert wf[5] == "TypeError: expected a solver handle"
 
 
 @needs_node
