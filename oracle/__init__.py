@@ -17,10 +17,12 @@ Contents
 ``s2``      restatement of the ``molecule.js`` WGSL kernels on the hot path:
             level-set boundary update, damped smoother and two-grid correction,
             exclusion/dipole energy, sphere-averaged grad-P force, trial init.
-``adjoint`` reverse-mode derivative of the S1 fixed-step trajectory energy
-            with respect to atom positions (frozen labels), FP64.
-
-
+``adjoint`` the S1 fixed-step trajectory energy (frozen labels) in torch
+            FP64, whose autograd gives dE/dR with respect to atom positions.
+``molfast`` restatement of the ``mol_fast.js`` fused per-electron engine:
+            set-up in float64, the WGSL update, reduction, normalisation,
+            per-electron two-grid correction, forces, K refresh and extract
+            in FP32 operand order, the host loop's nuclear terms in float64.
 
 Every function takes an explicit dtype: ``np.float64`` is the FP64 oracle and
 ``np.float32`` is the FP32-faithful mode, which performs the same IEEE

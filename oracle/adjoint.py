@@ -7,8 +7,14 @@ steps from P = 0, then ``n_steps`` steps of the cloud/solver.py loop body
 (Poisson every ``poisson_interval`` steps, ITP, normalise every
 ``norm_interval``) with the labels held fixed (no boundary rule).  Written
 in torch FP64 on the CPU so autograd gives an independent dE/dR for the
-hand-written HIP adjoint; pinned by central differences of the same
-function (tests/test_oracle_adjoint.py).
+hand-written HIP adjoint.
+
+Pins (tests/test_oracle_adjoint.py): trajectory_energy against the NumPy
+oracle's passes on the same trajectory (oracle/s1.py init_trial, normalize,
+jacobi, itp_update, energy and oracle/mg.py vcycle), Jacobi and multigrid
+mode, to 1e-12 relative -- so _restrict4, _prolong, _vcycle and
+_exact_solve are checked through it; its autograd gradient against central
+differences of itself (delta = 1e-4 bohr) to 1e-7 Ha/bohr.
 
 Positions R are fractional grid units; gradients returned per bohr.
 The Poisson step is either the cloud/solver.py Jacobi step (``mode='jacobi'``:
