@@ -302,9 +302,20 @@ class RealQMSolver:
         return L.check(self._lib, self._h, code)
 
     # ---- reference methods ----
-    def initialize(self):
-        """initTrial, normalise, bootstrap Poisson (cloud/solver.py:402-419)."""
-        self._chk(self._lib.rq_initialize(self._h, self._bootstrap))
+    def initialize(self, U0=None):
+        """initTrial, normalise, bootstrap Poisson (cloud/solver.py:402-419).
+
+        ``U0`` (optional, S^3 floats): start amplitudes written over the trial
+        U after the trial pass has set K and the labels (e.g. the seeded
+        Gaussians of ``builders.gaussian_init``); they are normalised per
+        domain before the bootstrap Poisson steps, as the trial U is."""
+        if U0 is None:
+            self._chk(self._lib.rq_initialize(self._h, self._bootstrap))
+        else:
+            self._chk(self._lib.rq_init_trial(self._h))
+            self.U.set(U0)
+            self._chk(self._lib.rq_normalize(self._h))
+            self._chk(self._lib.rq_poisson(self._h, self._bootstrap))
         self._chk(self._lib.rq_sync(self._h))
         self._initialized = True
 
@@ -405,25 +416,32 @@ class RealQMSolver:
                     'nucPos': [[float(x) for x in row] for row in self.atom_pos],
                 }
                 if getattr(self, '_fold_atoms', None):
-                    fa = self._fold_atoms
-                    a = self.atom_pos[fa[0]] * self.h
-                    b = self.atom_pos[fa[1]] * self.h
-                    c = self.atom_pos[fa[2]] * self.h
-                    ba, bc = a - b, c - b
-                    mag = np.linalg.norm(ba) * np.linalg.norm(bc)
-                    angle = np.degrees(np.arccos(np.clip(np.dot(ba, bc) / mag, -1, 1))) if mag > 0 else 0
-                    snapshot['fold_angle'] = float(angle)
+                    snapshot['fold_angle'] = self._fold_angle()
                 results.append(snapshot)
-                fmax = float(np.max(np.linalg.norm(self.nuc_force[:self._nProtein], axis=1)))
-                print(f"  Step {step}/{total_steps}: E={self.E:.6f} "
-                      f"E_bind={self.E - self.E_atoms_sum:.6f} "
-                      f"F_max={fmax:.4f} ({ms_per_step:.1f} ms/step)")
+                self._log_report(step, total_steps, ms_per_step)
         self._chk(self._lib.rq_sync(self._h))
         self._step += total_steps
         return results
 
     def set_fold_atoms(self, indices):
         self._fold_atoms = indices
+
+    def _fold_angle(self):
+        """Angle (degrees) at the middle fold atom between the other two,
+        from atan2(|u x v|, u . v) of the bond vectors (0 if one is null)."""
+        pa, pb, pc = (np.asarray(self.atom_pos[i], np.float64) * self.h for i in self._fold_atoms[:3])
+        u, v = pa - pb, pc - pb
+        if not (np.any(u) and np.any(v)):
+            return 0.0
+        return float(np.degrees(np.arctan2(np.linalg.norm(np.cross(u, v)), float(u @ v))))
+
+    def _log_report(self, step, total, ms_per_step):
+        """One progress line per report: energies in Ha, the largest force
+        on a moving nucleus in Ha/bohr, the wall time per step so far."""
+        moving = self.nuc_force[:self._nProtein]
+        fmax = float(np.sqrt((moving.astype(np.float64) ** 2).sum(1)).max()) if len(moving) else 0.0
+        print(f"[realqm] {step:>7d}/{total} steps  E {self.E:+.6f} Ha  "
+              f"E-E_atoms {self.E - self.E_atoms_sum:+.6f} Ha  |F|max {fmax:.4f}  {ms_per_step:.3f} ms/step")
 
     # ---- extras ----
     def step(self, first_step, n_steps, poisson_interval=2, norm_interval=20, boundary_interval=None):

@@ -293,3 +293,64 @@ def test_trajectory_s1_matches_fp32_oracle():
     assert np.allclose(s.E_domains, per32, rtol=1e-6, atol=1e-7)
     assert np.array_equal(cube(s.label.get(), S), st32.label)
     s.close()
+
+
+def test_recompute_k_bitexact_after_move():
+    """rq_set_atoms + rq_recompute_k (cloud/solver.py:538-548, 149-179)
+    against the FP32 oracle's K refresh at the moved positions: bit-exact."""
+    rows = three_atoms(32)
+    NN, screen = 32, 32 * 0.25
+    s = make_solver(rows, NN, screen)
+    s.initialize()
+    moved = rows.copy()
+    moved[:, :3] += np.array([[0.37, -0.21, 0.05], [-0.13, 0.29, -0.41], [0.08, 0.11, -0.33]])
+    from realmolecule_amd.solver import atoms_array
+    s._chk(s._lib.rq_set_atoms(s._h, atoms_array(moved), len(moved)))
+    s._chk(s._lib.rq_recompute_k(s._h))
+    got = cube(s.K_buf.get(), NN + 1)
+    ref = s1.recompute_k(moved, s1.Grid(NN, screen, 0.12, F32))
+    print(f"recompute_k: max ulp {ulp_diff(got, ref)}")
+    assert ulp_diff(got, ref) == 0
+    s.close()
+
+
+def test_run_with_dynamics_matches_oracle_schedule():
+    """RealQMSolver.run() with dynamics on (cloud/solver.py:550-634): 100
+    steps, forces + damped nuclear move + K refresh every 50 steps, a
+    report every 50 steps.  The snapshots carry the reference's keys; their
+    energies and nucleus positions follow the FP32 oracle schedule
+    (oracle/s1.py State, hf_forces + nuclear_forces, move_nuclei,
+    recompute_k)."""
+    rows, els, screen = small_h2o(32, 0.25)
+    NN = 32
+    s = make_solver(rows, NN, screen, dynamics=True, forceScale=500.0, force_interval=50, bootstrap_poisson=20)
+    snaps = s.run(100, report_interval=50)
+    keys = {"step", "E", "E_T", "E_eK", "E_ee", "E_KK", "E_bind", "ms_per_step", "elapsed_sec", "nucPos"}
+    assert [sn["step"] for sn in snaps] == [50, 100]
+    assert all(set(sn) == keys for sn in snaps)
+
+    st = s1.State(rows, NN, screen, F32)
+    st.initialize(bootstrap=20)
+    pos = rows[:, :3].astype(np.float64).copy()
+    vel = np.zeros_like(pos)
+    Z = rows[:, 3]
+    h = screen / NN
+    n_el = int((Z > 0).sum())
+    for step in range(1, 101):
+        st.step(step)
+        if step % 50 == 0:
+            F = s1.hf_forces(st.U, st.atoms, st.g) + s1.nuclear_forces(pos, Z, h)
+            pos, vel = s1.move_nuclei(pos, vel, F, Z, h, NN, 0.8, 500.0, 0.98, 0.3 if n_el <= 5 else 0.1)
+            st.atoms[:, :3] = pos
+            st.K = s1.recompute_k(st.atoms, st.g)
+            tot, _, ekk = st.energy()
+            sn = snaps[step // 50 - 1]
+            E_ref = tot.sum() + ekk
+            dpos = np.abs(np.array(sn["nucPos"]) - pos).max()
+            print(f"step {step}: E gpu {sn['E']:.9f} oracle {E_ref:.9f}; max |dpos| {dpos:.2e} grid units; "
+                  f"moved {np.abs(pos - rows[:, :3]).max():.3f}")
+            assert dpos < 1e-4
+            assert abs(sn["E"] - E_ref) <= 1e-6 * abs(E_ref)
+            assert abs(sn["E_KK"] - ekk) <= 1e-6 * abs(ekk)
+    assert np.abs(pos - rows[:, :3]).max() > 1e-3  # the nuclei did move
+    s.close()
