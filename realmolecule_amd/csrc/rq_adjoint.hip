@@ -223,14 +223,15 @@ __global__ __launch_bounds__(256) void k_axpy_src(int S, int ich, const float* _
   }
 }
 
-// Per-domain sum of (A * B) * w, FP64 (labels clipped to n_dom - 1).
+// Per-domain sum of (A * B) * w, FP64 runs into fixed-point sums (labels
+// clipped to n_dom - 1).
 __global__ __launch_bounds__(256) void k_domain_dot(int S, int ich, float w, const float* __restrict__ A,
                                                     const float* __restrict__ B, const uint16_t* __restrict__ lab,
-                                                    int n_dom, double* __restrict__ out) {
-  extern __shared__ double bins[];
+                                                    int n_dom, Fx* __restrict__ out) {
+  extern __shared__ Fx fbins[];
   const int nthr = blockDim.x * blockDim.y;
   const int tid = threadIdx.y * blockDim.x + threadIdx.x;
-  for (int t = tid; t < n_dom; t += nthr) bins[t] = 0.0;
+  for (int t = tid; t < kFxWords * n_dom; t += nthr) fbins[t] = 0;
   __syncthreads();
   const int n = S - 2;
   M3 m;
@@ -244,38 +245,35 @@ __global__ __launch_bounds__(256) void k_domain_dot(int S, int ich, float w, con
       L = L < n_dom ? L : n_dom - 1;
       const float term = (A[id] * B[id]) * w;
       if (L != cur) {
-        if (cur >= 0) atomicAdd(&bins[cur], accv);
+        if (cur >= 0) fx_add(fbins + kFxWords * cur, accv);
         cur = L;
         accv = 0.0;
       }
       accv += (double)term;
     }
-    if (cur >= 0) atomicAdd(&bins[cur], accv);
+    if (cur >= 0) fx_add(fbins + kFxWords * cur, accv);
   }
   __syncthreads();
-  for (int t = tid; t < n_dom; t += nthr) {
-    const double b = bins[t];
-    if (b != 0.0) atomicAdd(&out[t], b);
-  }
+  for (int t = tid; t < n_dom; t += nthr) fx_add_words(out + kFxWords * t, fbins + kFxWords * t);
 }
 
 // Normalise adjoint: for domains with Z > 0 and n > 1e-20,
 //   lam <- s (lam - (h^3 / n) U_pre m),  s = sqrt(Z / n),  m = sum_d U_pre lam.
 __global__ __launch_bounds__(256) void k_normalize_adj(int S, int ich, float h3, const float* __restrict__ Upre,
                                                        float* __restrict__ lam, const uint16_t* __restrict__ lab,
-                                                       const double* __restrict__ norms,
-                                                       const double* __restrict__ dots, const float* __restrict__ Z,
+                                                       const Fx* __restrict__ norms,
+                                                       const Fx* __restrict__ dots, const float* __restrict__ Z,
                                                        int n_dom) {
   __shared__ float sc[kMaxDomains];
   __shared__ float be[kMaxDomains];
   const int nthr = blockDim.x * blockDim.y;
   const int tid = threadIdx.y * blockDim.x + threadIdx.x;
   for (int t = tid; t < n_dom; t += nthr) {
-    const double z = (double)Z[t], nv = norms[t];
+    const double z = (double)Z[t], nv = fx_value(norms + kFxWords * t);
     if (z > 0.0 && nv > 1e-20) {
       const double s = sqrt(z / nv);
       sc[t] = (float)s;
-      be[t] = (float)(s * (double)h3 * dots[t] / nv);
+      be[t] = (float)(s * (double)h3 * fx_value(dots + kFxWords * t) / nv);
     } else {
       sc[t] = 1.0f;
       be[t] = 0.0f;
@@ -302,7 +300,7 @@ constexpr int kChunk = 4;
 __global__ __launch_bounds__(256) void k_pos_grad(int S, float h, float h2, const float* __restrict__ gK,
                                                   const float* __restrict__ lamU0, const uint16_t* __restrict__ lab,
                                                   const Atom* __restrict__ atoms, int n_atoms,
-                                                  double* __restrict__ out) {
+                                                  Fx* __restrict__ out) {
   __shared__ double red[4][kChunk * 3];
   const int n = S - 2;
   const int tid = threadIdx.y * blockDim.x + threadIdx.x;
@@ -368,7 +366,7 @@ __global__ __launch_bounds__(256) void k_pos_grad(int S, float h, float h2, cons
     __syncthreads();
     if (tid < kChunk * 3) {
       const int a = a0 + tid / 3;
-      if (a < n_atoms) atomicAdd(&out[a * 3 + tid % 3], red[0][tid] + red[1][tid] + red[2][tid] + red[3][tid]);
+      if (a < n_atoms) fx_add(out + kFxWords * (a * 3 + tid % 3), red[0][tid] + red[1][tid] + red[2][tid] + red[3][tid]);
     }
     __syncthreads();
   }
@@ -429,19 +427,24 @@ void launch_axpy_src(hipStream_t st, const Tile& t, int S, const float* U, const
 }
 
 void launch_domain_dot(hipStream_t st, const Tile& t, int S, float w, const float* A, const float* B,
-                       const uint16_t* lab, int n_dom, double* out) {
-  hipLaunchKernelGGL(k_domain_dot, sgrid(S - 2, t), dim3(t.bx, t.by), sizeof(double) * n_dom, st, S, t.ich, w, A, B,
-                     lab, n_dom, out);
+                       const uint16_t* lab, int n_dom, Fx* out) {
+  static bool attr_set = false;
+  if (!attr_set) {
+    (void)hipFuncSetAttribute((const void*)k_domain_dot, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    attr_set = true;
+  }
+  hipLaunchKernelGGL(k_domain_dot, sgrid(S - 2, t), dim3(t.bx, t.by), sizeof(Fx) * kFxWords * n_dom, st, S, t.ich, w,
+                     A, B, lab, n_dom, out);
 }
 
 void launch_normalize_adj(hipStream_t st, const Tile& t, int S, float h3, const float* Upre, float* lam,
-                          const uint16_t* lab, const double* norms, const double* dots, const float* Z, int n_dom) {
+                          const uint16_t* lab, const Fx* norms, const Fx* dots, const float* Z, int n_dom) {
   hipLaunchKernelGGL(k_normalize_adj, sgrid(S - 2, t), dim3(t.bx, t.by), 0, st, S, t.ich, h3, Upre, lam, lab, norms,
                      dots, Z, n_dom);
 }
 
 void launch_pos_grad(hipStream_t st, int S, float h, float h2, const float* gK, const float* lamU0,
-                     const uint16_t* lab, const Atom* atoms, int n_atoms, double* out) {
+                     const uint16_t* lab, const Atom* atoms, int n_atoms, Fx* out) {
   const int n = S - 2;
   dim3 grid((n + 63) / 64, (n + 3) / 4, (n + kCol - 1) / kCol);
   hipLaunchKernelGGL(k_pos_grad, grid, dim3(64, 4), 0, st, S, h, h2, gK, lamU0, lab, atoms, n_atoms, out);

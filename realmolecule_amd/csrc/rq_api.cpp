@@ -50,7 +50,7 @@ struct AdjState {
   float *t1 = nullptr, *t2 = nullptr, *t3 = nullptr;
   std::vector<float*> ckU, ckP;    // checkpoints (U after the step, P in use)
   std::vector<float*> segU, segP;  // one segment of recomputed inputs
-  double* red = nullptr;           // 2 * n_dom (norms, dots) + 3 * n_atoms (dE/dR) + 3 * n_dom + 3 (energy)
+  Fx* red = nullptr;  // fixed-point sums: 2 n_dom (norms, dots) + 3 n_atoms (dE/dR) + 3 n_dom + 3 (energy)
 };
 }  // namespace
 
@@ -102,7 +102,7 @@ struct rq_solver {
   int levels = 0;
   std::vector<int> Sl;
   std::vector<float*> e0, e1, g;
-  double* red = nullptr;  // n_atoms*3 + 3 + n_atoms*3 doubles
+  Fx* red = nullptr;  // red_len fixed-point sums (per-domain energies + dipole, or norms)
   double *dst_S = nullptr, *dst_rden = nullptr;  // coarsest-level DST tables
   size_t red_len = 0;
   hipStream_t st = nullptr;
@@ -240,9 +240,19 @@ Tile tile_for(int S, int want = 8192) {
   return t;
 }
 
-void zero_doubles(rq_solver* s, double* p, size_t n) {
-  // zero with a fill kernel (not hipMemsetAsync) so the pass is graph-capturable
-  launch_fill(s->st, reinterpret_cast<float*>(p), n * 2, 0.0f);
+// Zeroes n fixed-point sums with a fill kernel (not hipMemsetAsync) so the
+// pass is graph-capturable.
+void zero_fx(rq_solver* s, Fx* p, size_t n) {
+  launch_fill(s->st, reinterpret_cast<float*>(p), n * kFxWords * (sizeof(Fx) / sizeof(float)), 0.0f);
+}
+
+// Values of n fixed-point sums copied from the device (synchronises the stream).
+int read_fx(rq_solver* s, const Fx* d, size_t n, double* out) {
+  std::vector<Fx> w(n * kFxWords);
+  HIPCHK(s, hipMemcpyAsync(w.data(), d, sizeof(Fx) * w.size(), hipMemcpyDeviceToHost, s->st));
+  HIPCHK(s, hipStreamSynchronize(s->st));
+  for (size_t i = 0; i < n; ++i) out[i] = fx_value(&w[i * kFxWords]);
+  return RQ_OK;
 }
 
 int upload_atoms(rq_solver* s) {
@@ -611,8 +621,8 @@ int update_step(rq_solver* s) {
 }
 
 int normalize_step(rq_solver* s) {
-  double* norms = s->red;
-  zero_doubles(s, norms, s->n_atoms);
+  Fx* norms = s->red;
+  zero_fx(s, norms, s->n_atoms);
   {
     Timed t(s, F_NORM);
     launch_norm_sums(s->st, s->tile, s->S, s->h3, s->U, s->lab, s->n_atoms, norms);
@@ -826,7 +836,7 @@ int poisson_into(rq_solver* s, const float* P_in, float* P_out, const float* U, 
 }
 
 int normalize_on(rq_solver* s, float* U) {
-  zero_doubles(s, s->red, s->n_atoms);
+  zero_fx(s, s->red, s->n_atoms);
   launch_norm_sums(s->st, s->tile, s->S, s->h3, U, s->lab, s->n_atoms, s->red);
   launch_normalize(s->st, s->tile, s->S, U, s->lab, s->red, s->Z_d, s->n_atoms);
   LAUNCHCHK(s);
@@ -878,9 +888,9 @@ int poisson_adj(rq_solver* s, const float* U) {
 // Adjoint of normalise (on lamU in place) given the pre-normalise field Upre.
 int normalize_adj(rq_solver* s, const float* Upre) {
   AdjState& A = s->adj;
-  double* norms = A.red;
-  double* dots = A.red + s->n_atoms;
-  zero_doubles(s, A.red, 2 * (size_t)s->n_atoms);
+  Fx* norms = A.red;
+  Fx* dots = A.red + kFxWords * (size_t)s->n_atoms;
+  zero_fx(s, A.red, 2 * (size_t)s->n_atoms);
   launch_norm_sums(s->st, s->tile, s->S, s->h3, Upre, s->lab, s->n_atoms, norms);
   launch_domain_dot(s->st, s->tile, s->S, 1.0f, Upre, A.lamU, s->lab, s->n_atoms, dots);
   launch_normalize_adj(s->st, s->tile, s->S, s->h3, Upre, A.lamU, s->lab, norms, dots, s->Z_d, s->n_atoms);
@@ -926,7 +936,7 @@ int frozen_run(rq_solver* s, const rq_adjoint_spec* sp, bool backward, double* E
     if ((rc = adj_alloc(s, p))) return rc;
   if (!A.red) {
     const size_t nr = 2 * (size_t)s->n_atoms + 3 * (size_t)s->n_atoms + 3 * (size_t)s->n_atoms + 3;
-    HIPCHK(s, hipMalloc((void**)&A.red, sizeof(double) * nr));
+    HIPCHK(s, hipMalloc((void**)&A.red, sizeof(Fx) * kFxWords * nr));
   }
   if (backward) {
     for (float** p : {&A.lamU, &A.lamU2, &A.lamP, &A.lamP2, &A.gK, &A.acc})
@@ -973,15 +983,14 @@ int frozen_run(rq_solver* s, const rq_adjoint_spec* sp, bool backward, double* E
   }
   // energy of the final state
   {
-    double* per = A.red + 5 * (size_t)s->n_atoms;
-    double* dip = per + 3 * (size_t)s->n_atoms;
-    zero_doubles(s, per, 3 * (size_t)s->n_atoms + 3);
+    Fx* per = A.red + kFxWords * 5 * (size_t)s->n_atoms;
+    Fx* dip = per + kFxWords * 3 * (size_t)s->n_atoms;
+    zero_fx(s, per, 3 * (size_t)s->n_atoms + 3);
     launch_energy(s->st, T, RQ_ENERGY_S1, s->S, s->h, s->h3, s->U, s->K, s->P, s->lab, s->atoms_d, s->n_atoms, per,
                   dip);
     LAUNCHCHK(s);
     std::vector<double> hv(3 * (size_t)s->n_atoms);
-    HIPCHK(s, hipMemcpyAsync(hv.data(), per, sizeof(double) * hv.size(), hipMemcpyDeviceToHost, s->st));
-    HIPCHK(s, hipStreamSynchronize(s->st));
+    if ((rc = read_fx(s, per, hv.size(), hv.data()))) return rc;
     double e[3] = {0, 0, 0};
     for (int d = 0; d < s->n_atoms; ++d)
       for (int q = 0; q < 3; ++q) e[q] += hv[d * 3 + q];
@@ -1041,12 +1050,13 @@ int frozen_run(rq_solver* s, const rq_adjoint_spec* sp, bool backward, double* E
   launch_init_frozen(s->st, s->S, s->h, s->h2, s->atoms_d, s->n_atoms, s->lab, A.t1, A.t2);
   LAUNCHCHK(s);
   if ((rc = normalize_adj(s, A.t1))) return rc;
-  double* gout = A.red + 2 * (size_t)s->n_atoms;
-  zero_doubles(s, gout, 3 * (size_t)s->n_atoms);
+  Fx* gout = A.red + kFxWords * 2 * (size_t)s->n_atoms;
+  zero_fx(s, gout, 3 * (size_t)s->n_atoms);
   launch_pos_grad(s->st, s->S, s->h, s->h2, A.gK, A.lamU, s->lab, s->atoms_d, s->n_atoms, gout);
   LAUNCHCHK(s);
-  if (dEdR) HIPCHK(s, hipMemcpyAsync(dEdR, gout, sizeof(double) * 3 * s->n_atoms, hipMemcpyDeviceToHost, s->st));
-  HIPCHK(s, hipStreamSynchronize(s->st));
+  std::vector<double> g(3 * (size_t)s->n_atoms);
+  if ((rc = read_fx(s, gout, g.size(), g.data()))) return rc;
+  if (dEdR) std::memcpy(dEdR, g.data(), sizeof(double) * g.size());
   return RQ_OK;
 }
 
@@ -1206,7 +1216,7 @@ int rq_create(const rq_params* pp, const rq_atom* atoms, int32_t n_atoms, rq_sol
     }
   }
   s->red_len = (size_t)n_atoms * 3 + 3;
-  ok = ok && alloc((void**)&s->red, sizeof(double) * s->red_len);
+  ok = ok && alloc((void**)&s->red, sizeof(Fx) * kFxWords * s->red_len);
   if (ok && levels > 0) {
     s->Sl.assign(levels, 0);
     s->e0.assign(levels, nullptr);
@@ -1593,9 +1603,9 @@ int rq_get_tile(const rq_solver* h, int32_t level, int32_t* bx, int32_t* by, int
 int rq_energy(rq_solver* h, double* out6, double* per_domain) {
   if (!h || !out6) return fail(h, RQ_EINVAL, "null argument");
   const int nd = h->n_atoms;
-  double* per = h->red;
-  double* dip = h->red + (size_t)nd * 3;
-  zero_doubles(h, h->red, h->red_len);
+  Fx* per = h->red;
+  Fx* dip = h->red + kFxWords * (size_t)nd * 3;
+  zero_fx(h, h->red, h->red_len);
   {
     Timed t(h, F_ENERGY);
     const float* Pv = (h->p.pother_mode == RQ_POTHER_FRAME) ? h->Pother : h->P;
@@ -1604,8 +1614,7 @@ int rq_energy(rq_solver* h, double* out6, double* per_domain) {
   }
   LAUNCHCHK(h);
   std::vector<double> hostv(h->red_len);
-  HIPCHK(h, hipMemcpyAsync(hostv.data(), h->red, sizeof(double) * h->red_len, hipMemcpyDeviceToHost, h->st));
-  HIPCHK(h, hipStreamSynchronize(h->st));
+  if (int rc = read_fx(h, h->red, h->red_len, hostv.data())) return rc;
   double T = 0, K = 0, E = 0;
   for (int d = 0; d < nd; ++d) {
     T += hostv[d * 3 + 0];
@@ -1625,30 +1634,28 @@ int rq_energy(rq_solver* h, double* out6, double* per_domain) {
 
 int rq_domain_norms(rq_solver* h, double* norms) {
   if (!h || !norms) return fail(h, RQ_EINVAL, "null argument");
-  zero_doubles(h, h->red, h->n_atoms);
+  zero_fx(h, h->red, h->n_atoms);
   launch_norm_sums(h->st, h->tile, h->S, h->h3, h->U, h->lab, h->n_atoms, h->red);
   LAUNCHCHK(h);
-  HIPCHK(h, hipMemcpyAsync(norms, h->red, sizeof(double) * h->n_atoms, hipMemcpyDeviceToHost, h->st));
-  HIPCHK(h, hipStreamSynchronize(h->st));
-  return RQ_OK;
+  return read_fx(h, h->red, h->n_atoms, norms);
 }
 
 int rq_forces_hf(rq_solver* h, int32_t first, int32_t count, double* out3) {
   if (!h || !out3) return fail(h, RQ_EINVAL, "null argument");
   if (first < 0 || count < 0 || first + count > h->n_atoms) return fail(h, RQ_ERANGE, "atom range out of bounds");
   if (count == 0) return RQ_OK;
-  double* buf = nullptr;
-  HIPCHK(h, hipMallocAsync((void**)&buf, sizeof(double) * count * 3, h->st));
-  zero_doubles(h, buf, (size_t)count * 3);
+  Fx* buf = nullptr;
+  HIPCHK(h, hipMallocAsync((void**)&buf, sizeof(Fx) * kFxWords * count * 3, h->st));
+  zero_fx(h, buf, (size_t)count * 3);
   {
     Timed t(h, F_FORCE);
     launch_forces_hf(h->st, h->S, h->h, h->h2, h->h3, h->U, h->atoms_d, first, count, buf);
   }
   LAUNCHCHK(h);
-  HIPCHK(h, hipMemcpyAsync(out3, buf, sizeof(double) * count * 3, hipMemcpyDeviceToHost, h->st));
+  const int rc = read_fx(h, buf, (size_t)count * 3, out3);
   HIPCHK(h, hipFreeAsync(buf, h->st));
   HIPCHK(h, hipStreamSynchronize(h->st));
-  return RQ_OK;
+  return rc;
 }
 
 int rq_forces_gradp(rq_solver* h, int32_t radius, double* out3n) {

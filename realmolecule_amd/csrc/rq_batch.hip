@@ -305,24 +305,21 @@ __global__ __launch_bounds__(1024) void kb_exact(const BLev* __restrict__ L, con
 
 __global__ __launch_bounds__(256) void kb_norm_sums(const BLev* __restrict__ L, const BMol* __restrict__ M, int nm,
                                                     int ntiles, const float* __restrict__ U,
-                                                    const uint16_t* __restrict__ lab, double* __restrict__ norms) {
-  __shared__ double bins[kMaxBatchDom];
+                                                    const uint16_t* __restrict__ lab, Fx* __restrict__ norms) {
+  __shared__ Fx bins[kFxWords * kMaxBatchDom];
   const int nthr = blockDim.x * blockDim.y;
   const int tid = threadIdx.y * blockDim.x + threadIdx.x;
   for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
     const int u = xcd_item(t, ntiles);
     const int m = find_mol(L, nm, u);  // uniform per workgroup
     const int nd = M[m].n_atoms;
-    for (int d = tid; d < nd; d += nthr) bins[d] = 0.0;
+    for (int d = tid; d < kFxWords * nd; d += nthr) bins[d] = 0;
     __syncthreads();
     Col q;
     if (item_col(L, nm, ntiles, t, q))
       norm_sums_col(L[m].S, q.k, q.j, q.i0, q.i1, M[m].h3, U + L[m].off, lab + L[m].off, nd, bins);
     __syncthreads();
-    for (int d = tid; d < nd; d += nthr) {
-      const double b = bins[d];
-      if (b != 0.0) atomicAdd(&norms[M[m].dom0 + d], b);
-    }
+    for (int d = tid; d < nd; d += nthr) fx_add_words(norms + kFxWords * (M[m].dom0 + d), bins + kFxWords * d);
     __syncthreads();
   }
 }
@@ -330,7 +327,7 @@ __global__ __launch_bounds__(256) void kb_norm_sums(const BLev* __restrict__ L, 
 __global__ __launch_bounds__(256) void kb_normalize(const BLev* __restrict__ L, const BMol* __restrict__ M, int nm,
                                                     int ntiles, float* __restrict__ U,
                                                     const uint16_t* __restrict__ lab,
-                                                    const double* __restrict__ norms, const float* __restrict__ Z) {
+                                                    const Fx* __restrict__ norms, const float* __restrict__ Z) {
   __shared__ float scale[kMaxBatchDom];
   const int nthr = blockDim.x * blockDim.y;
   const int tid = threadIdx.y * blockDim.x + threadIdx.x;
@@ -340,7 +337,7 @@ __global__ __launch_bounds__(256) void kb_normalize(const BLev* __restrict__ L, 
     const int nd = M[m].n_atoms, d0 = M[m].dom0;
     __syncthreads();
     for (int d = tid; d < nd; d += nthr) {
-      const double z = (double)Z[d0 + d], nv = norms[d0 + d];
+      const double z = (double)Z[d0 + d], nv = fx_value(norms + kFxWords * (d0 + d));
       scale[d] = (z > 0.0 && nv > 1e-20) ? (float)sqrt(z / nv) : 1.0f;
     }
     __syncthreads();
@@ -352,15 +349,15 @@ __global__ __launch_bounds__(256) void kb_normalize(const BLev* __restrict__ L, 
 __global__ __launch_bounds__(256) void kb_energy(const BLev* __restrict__ L, const BMol* __restrict__ M, int nm,
                                                  int ntiles, const float* __restrict__ U, const float* __restrict__ K,
                                                  const float* __restrict__ P, const uint16_t* __restrict__ lab,
-                                                 double* __restrict__ out) {
-  __shared__ double bins[3 * kMaxBatchDom];
+                                                 Fx* __restrict__ out) {
+  __shared__ Fx bins[kFxWords * 3 * kMaxBatchDom];
   const int nthr = blockDim.x * blockDim.y;
   const int tid = threadIdx.y * blockDim.x + threadIdx.x;
   for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
     const int u = xcd_item(t, ntiles);
     const int m = find_mol(L, nm, u);
     const int nd = M[m].n_atoms;
-    for (int d = tid; d < 3 * nd; d += nthr) bins[d] = 0.0;
+    for (int d = tid; d < kFxWords * 3 * nd; d += nthr) bins[d] = 0;
     __syncthreads();
     Col q;
     if (item_col(L, nm, ntiles, t, q)) {
@@ -368,10 +365,7 @@ __global__ __launch_bounds__(256) void kb_energy(const BLev* __restrict__ L, con
       energy_s1_col(L[m].S, q.k, q.j, q.i0, q.i1, M[m].h, M[m].h3, U + o, K + o, P + o, lab + o, nd, bins);
     }
     __syncthreads();
-    for (int d = tid; d < 3 * nd; d += nthr) {
-      const double b = bins[d];
-      if (b != 0.0) atomicAdd(&out[3 * M[m].dom0 + d], b);
-    }
+    for (int d = tid; d < 3 * nd; d += nthr) fx_add_words(out + kFxWords * (3 * M[m].dom0 + d), bins + kFxWords * d);
     __syncthreads();
   }
 }
@@ -436,7 +430,7 @@ struct rq_batch {
   float *U = nullptr, *U2 = nullptr, *K = nullptr, *P = nullptr, *P2 = nullptr;
   uint16_t* lab = nullptr;
   std::vector<float*> e0, e1, g;  // per level (index 0 unused)
-  double* red = nullptr;          // norms (ndom) + energies (3 ndom)
+  Fx* red = nullptr;              // fixed-point sums: norms (ndom) + energies (3 ndom)
   hipStream_t st = nullptr;
   bool initialized = false, use_graph = false, timing = false;
   std::vector<BatchGraph> graphs;
@@ -652,7 +646,7 @@ int b_vcycle(rq_batch* b, bool fuse_itp, bool* did_itp) {
 int b_normalize(rq_batch* b) {
   const dim3 blk(b->bx, b->by);
   const int nt = b->ntiles[0];
-  launch_fill(b->st, reinterpret_cast<float*>(b->red), 2 * (size_t)b->ndom, 0.0f);
+  launch_fill(b->st, reinterpret_cast<float*>(b->red), 2 * kFxWords * (size_t)b->ndom, 0.0f);
   BTimed t(b, F_NORM);
   hipLaunchKernelGGL(kb_norm_sums, dim3(grid_for(nt)), blk, 0, b->st, LD(b, 0), b->mol_d, b->nm, nt, b->U, b->lab,
                      b->red);
@@ -906,7 +900,7 @@ int rq_batch_create(const rq_params* pp, const rq_molecule* mols, int32_t n_mols
        alloc((void**)&b->Stab, sizeof(double) * Stab.size()) && alloc((void**)&b->Rtab, sizeof(double) * Rtab.size()) &&
        alloc((void**)&b->U, n0 * 4) && alloc((void**)&b->U2, n0 * 4) && alloc((void**)&b->K, n0 * 4) &&
        alloc((void**)&b->P, n0 * 4) && alloc((void**)&b->P2, n0 * 4) && alloc((void**)&b->lab, n0 * 2) &&
-       alloc((void**)&b->red, sizeof(double) * 4 * (size_t)b->ndom);
+       alloc((void**)&b->red, sizeof(Fx) * kFxWords * 4 * (size_t)b->ndom);
   b->e0.assign(L, nullptr);
   b->e1.assign(L, nullptr);
   b->g.assign(L, nullptr);
@@ -1026,8 +1020,8 @@ int rq_batch_sync(rq_batch* b) {
 
 int rq_batch_energy(rq_batch* b, double* out3) {
   if (!b || !out3) return bfail(b, RQ_EINVAL, "null argument");
-  double* per = b->red + b->ndom;
-  launch_fill(b->st, reinterpret_cast<float*>(per), 6 * (size_t)b->ndom, 0.0f);
+  Fx* per = b->red + kFxWords * (size_t)b->ndom;
+  launch_fill(b->st, reinterpret_cast<float*>(per), 6 * kFxWords * (size_t)b->ndom, 0.0f);
   {
     BTimed t(b, F_ENERGY);
     const int nt = b->ntiles[0];
@@ -1035,13 +1029,13 @@ int rq_batch_energy(rq_batch* b, double* out3) {
                        b->U, b->K, b->P, b->lab, per);
   }
   BLAUNCH(b);
-  std::vector<double> hv(3 * (size_t)b->ndom);
-  BCHK(b, hipMemcpyAsync(hv.data(), per, sizeof(double) * hv.size(), hipMemcpyDeviceToHost, b->st));
+  std::vector<Fx> hv(kFxWords * 3 * (size_t)b->ndom);
+  BCHK(b, hipMemcpyAsync(hv.data(), per, sizeof(Fx) * hv.size(), hipMemcpyDeviceToHost, b->st));
   BCHK(b, hipStreamSynchronize(b->st));
   for (int m = 0; m < b->nm; ++m) {
     double e[3] = {0, 0, 0};
     for (int d = 0; d < b->mol[m].n_atoms; ++d)
-      for (int q = 0; q < 3; ++q) e[q] += hv[3 * (size_t)(b->mol[m].dom0 + d) + q];
+      for (int q = 0; q < 3; ++q) e[q] += fx_value(&hv[kFxWords * (3 * (size_t)(b->mol[m].dom0 + d) + q)]);
     for (int q = 0; q < 3; ++q) out3[3 * m + q] = e[q];
   }
   return RQ_OK;

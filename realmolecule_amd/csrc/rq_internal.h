@@ -33,6 +33,38 @@ constexpr int kItpFuseMaxS = 200;
 constexpr int kResResTiledMaxNc = 63;   // 3D-tiled residual+restriction up to 63 coarse interior nodes, march above
 constexpr int kBottomMaxS = 25;         // last two V-cycle levels in one LDS-resident workgroup up to 25^3 nodes
 
+// Order-independent FP64 sums.  A sum is held as two signed 64-bit words in
+// fixed point: w[0] counts units of 2^-30 and w[1] units of 2^-70.  A
+// contribution v (|v| < 2^32) enters as w[0] += rint(v 2^30) and w[1] +=
+// rint((v 2^30 - rint(v 2^30)) 2^40), both parts exact in FP64, so each
+// contribution is represented to within 2^-71.  Integer additions are exact
+// and associative: however the workgroups' and waves' atomic additions
+// interleave, the words -- and the value read from them -- are the same, so
+// every per-domain reduction is bit-reproducible run to run.  The low word
+// takes up to 2^24 contributions per sum before it could overflow.
+typedef unsigned long long Fx;
+constexpr int kFxWords = 2;
+__host__ __device__ inline double fx_value(const Fx* w) {
+  return (double)(long long)w[0] * 0x1p-30 + (double)(long long)w[1] * 0x1p-70;
+}
+#ifdef __HIP__
+// Adds v to the fixed-point sum w; w may live in LDS or in global memory.
+__device__ __forceinline__ void fx_add(Fx* w, double v) {
+  const double s = v * 0x1p30;
+  const double q = rint(s);
+  atomicAdd(w, (Fx)(long long)q);
+  atomicAdd(w + 1, (Fx)(long long)rint((s - q) * 0x1p40));
+}
+// Adds the words of one fixed-point sum to another (exact).
+__device__ __forceinline__ void fx_add_words(Fx* w, const Fx* src) {
+  const Fx a = src[0], b = src[1];
+  if (a | b) {
+    atomicAdd(w, a);
+    atomicAdd(w + 1, b);
+  }
+}
+#endif
+
 // Launch shape of the 2.5D stencil kernels: (bx, by) tile in (k, j), marching
 // ich planes along i.
 struct Tile {
@@ -122,8 +154,8 @@ void launch_smooth_itp(hipStream_t st, const Tile& t, int S, const float* P, flo
                        float dtv);
 void launch_zero(hipStream_t st, float* p, size_t n);
 void launch_norm_sums(hipStream_t st, const Tile& t, int S, float h3, const float* U, const uint16_t* lab,
-                      int n_dom, double* norms);
-void launch_normalize(hipStream_t st, const Tile& t, int S, float* U, const uint16_t* lab, const double* norms,
+                      int n_dom, Fx* norms /* n_dom sums */);
+void launch_normalize(hipStream_t st, const Tile& t, int S, float* U, const uint16_t* lab, const Fx* norms,
                       const float* Z, int n_dom);
 // Score rule.  Up to kScoreScanMaxAtoms atoms: a scan over all atoms.  Above:
 // the argmax over atoms is searched through a uniform cell list (cells of cs
@@ -139,9 +171,9 @@ void launch_levelset(hipStream_t st, const Tile& t, int S, float h, const Atom* 
                      float dt_w, float curv_reg, int rc_flags);
 void launch_energy(hipStream_t st, const Tile& t, int mode, int S, float h, float h3, const float* U,
                    const float* K, const float* P, const uint16_t* lab, const Atom* atoms, int n_dom,
-                   double* per_dom /* n_dom*3 */, double* dip /* 3 */);
+                   Fx* per_dom /* n_dom*3 sums */, Fx* dip /* 3 sums */);
 void launch_forces_hf(hipStream_t st, int S, float h, float h2, float h3, const float* U, const Atom* atoms,
-                      int first, int count, double* out3);
+                      int first, int count, Fx* out3 /* 3 count sums */);
 void launch_forces_gradp(hipStream_t st, int S, float h, const float* P, const Atom* atoms, int n_atoms, int radius,
                          double* out3n);
 void launch_square(hipStream_t st, int S, const float* U, float* rho);
@@ -165,10 +197,10 @@ void launch_correct_adj(hipStream_t st, const Tile& t, int S, const float* lam, 
                         float* acc);
 void launch_axpy_src(hipStream_t st, const Tile& t, int S, const float* U, const float* acc, float* lamU, float coef);
 void launch_domain_dot(hipStream_t st, const Tile& t, int S, float w, const float* A, const float* B,
-                       const uint16_t* lab, int n_dom, double* out);
+                       const uint16_t* lab, int n_dom, Fx* out);
 void launch_normalize_adj(hipStream_t st, const Tile& t, int S, float h3, const float* Upre, float* lam,
-                          const uint16_t* lab, const double* norms, const double* dots, const float* Z, int n_dom);
+                          const uint16_t* lab, const Fx* norms, const Fx* dots, const float* Z, int n_dom);
 void launch_pos_grad(hipStream_t st, int S, float h, float h2, const float* gK, const float* lamU0,
-                     const uint16_t* lab, const Atom* atoms, int n_atoms, double* out);
+                     const uint16_t* lab, const Atom* atoms, int n_atoms, Fx* out /* 3 n_atoms sums */);
 
 }  // namespace rq

@@ -1132,38 +1132,37 @@ __global__ void k_lab32to16(size_t n, const int32_t* __restrict__ in, uint16_t* 
 
 // --------------------------------------------------------------------------
 // Per-domain reductions.  Each thread accumulates a run of equal labels in
-// FP64 registers, flushing to LDS bins (FP64 LDS atomics) when the label
+// FP64 registers, flushing to the workgroup's LDS bins when the label
 // changes; after a barrier each non-zero bin is added to the global result
-// with one FP64 atomic per workgroup.  Labels are clipped to [0, n_dom-1]
+// once per workgroup.  Bins and results are two-word fixed-point sums
+// (fx_add, fx_value), so the totals do not depend on the order in which
+// waves and workgroups add to them.  Labels are clipped to [0, n_dom-1]
 // (cloud/solver.py:427 ``label.clip``).
 // --------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_norm_sums(int S, int ich, float h3, const float* __restrict__ U,
                                                    const uint16_t* __restrict__ lab, int n_dom,
-                                                   double* __restrict__ norms) {
-  extern __shared__ double bins[];
+                                                   Fx* __restrict__ norms) {
+  extern __shared__ Fx fbins[];
   const int nthr = blockDim.x * blockDim.y;
   const int tid = threadIdx.y * blockDim.x + threadIdx.x;
-  for (int t = tid; t < n_dom; t += nthr) bins[t] = 0.0;
+  for (int t = tid; t < kFxWords * n_dom; t += nthr) fbins[t] = 0;
   __syncthreads();
   March m;
-  if (march_coords(S - 2, ich, m)) norm_sums_col(S, m.k, m.j, m.i0, m.i1, h3, U, lab, n_dom, bins);
+  if (march_coords(S - 2, ich, m)) norm_sums_col(S, m.k, m.j, m.i0, m.i1, h3, U, lab, n_dom, fbins);
   __syncthreads();
-  for (int t = tid; t < n_dom; t += nthr) {
-    const double b = bins[t];
-    if (b != 0.0) atomicAdd(&norms[t], b);
-  }
+  for (int t = tid; t < n_dom; t += nthr) fx_add_words(norms + kFxWords * t, fbins + kFxWords * t);
 }
 
 __global__ __launch_bounds__(256) void k_normalize(int S, int ich, float* __restrict__ U,
                                                    const uint16_t* __restrict__ lab,
-                                                   const double* __restrict__ norms, const float* __restrict__ Z,
+                                                   const Fx* __restrict__ norms, const float* __restrict__ Z,
                                                    int n_dom) {
   __shared__ float scale[kMaxDomains];
   const int nthr = blockDim.x * blockDim.y;
   const int tid = threadIdx.y * blockDim.x + threadIdx.x;
   for (int t = tid; t < n_dom; t += nthr) {
     const double z = (double)Z[t];
-    const double nv = norms[t];
+    const double nv = fx_value(norms + kFxWords * t);
     scale[t] = (z > 0.0 && nv > 1e-20) ? (float)sqrt(z / nv) : 1.0f;
   }
   __syncthreads();
@@ -1421,16 +1420,29 @@ __device__ __forceinline__ T wave_sum(T v) {
   return v;
 }
 
+// Energy terms per domain.  The LDS bins cover the domain window [d0, d0 +
+// nw) (3 sums per domain, then the 3 dipole sums, which only the window
+// d0 = 0 adds to the result); the host runs one launch per window when the
+// bins of all domains would not fit the LDS.
 __global__ __launch_bounds__(256) void k_energy(int mode, int S, int ich, float h, float h3, float r_sing,
                                                 const float* __restrict__ U, const float* __restrict__ K,
                                                 const float* __restrict__ P, const uint16_t* __restrict__ lab,
-                                                const Atom* __restrict__ atoms, int n_dom,
-                                                double* __restrict__ per_dom, double* __restrict__ dip) {
-  extern __shared__ double bins[];  // n_dom * 3, then 3 dipole slots
+                                                const Atom* __restrict__ atoms, int n_dom, int d0, int nw,
+                                                Fx* __restrict__ per_dom, Fx* __restrict__ dip) {
+  extern __shared__ Fx fbins[];  // nw * 3 sums, then 3 dipole sums
   const int nthr = blockDim.x * blockDim.y;
   const int tid = threadIdx.y * blockDim.x + threadIdx.x;
-  for (int t = tid; t < n_dom * 3 + 3; t += nthr) bins[t] = 0.0;
+  for (int t = tid; t < kFxWords * (nw * 3 + 3); t += nthr) fbins[t] = 0;
   __syncthreads();
+  // flush of one run of label L into the window's bins
+  auto flush = [&](int L, double aT, double aK, double aE) {
+    const int w = L - d0;
+    if (w >= 0 && w < nw) {
+      fx_add(fbins + kFxWords * (w * 3 + 0), aT);
+      fx_add(fbins + kFxWords * (w * 3 + 1), aK);
+      fx_add(fbins + kFxWords * (w * 3 + 2), aE);
+    }
+  };
   const int n = S - 2;
   March m;
   double dx_acc = 0.0, dy_acc = 0.0, dz_acc = 0.0;
@@ -1470,11 +1482,7 @@ __global__ __launch_bounds__(256) void k_energy(int mode, int S, int ich, float 
       dy_acc += (double)(rho * yj * h3);
       dz_acc += (double)(rho * zk * h3);
       if (L != cur) {
-        if (cur >= 0) {
-          atomicAdd(&bins[cur * 3 + 0], aT);
-          atomicAdd(&bins[cur * 3 + 1], aK);
-          atomicAdd(&bins[cur * 3 + 2], aE);
-        }
+        if (cur >= 0) flush(cur, aT, aK, aE);
         cur = L;
         aT = aK = aE = 0.0;
       }
@@ -1482,41 +1490,34 @@ __global__ __launch_bounds__(256) void k_energy(int mode, int S, int ich, float 
       aK += (double)VeK;
       aE += (double)Vee;
     }
-    if (cur >= 0) {
-      atomicAdd(&bins[cur * 3 + 0], aT);
-      atomicAdd(&bins[cur * 3 + 1], aK);
-      atomicAdd(&bins[cur * 3 + 2], aE);
-    }
+    if (cur >= 0) flush(cur, aT, aK, aE);
   }
   dx_acc = wave_sum(dx_acc);
   dy_acc = wave_sum(dy_acc);
   dz_acc = wave_sum(dz_acc);
   if ((tid & 63) == 0) {
-    atomicAdd(&bins[n_dom * 3 + 0], dx_acc);
-    atomicAdd(&bins[n_dom * 3 + 1], dy_acc);
-    atomicAdd(&bins[n_dom * 3 + 2], dz_acc);
+    fx_add(fbins + kFxWords * (nw * 3 + 0), dx_acc);
+    fx_add(fbins + kFxWords * (nw * 3 + 1), dy_acc);
+    fx_add(fbins + kFxWords * (nw * 3 + 2), dz_acc);
   }
   __syncthreads();
-  for (int t = tid; t < n_dom * 3; t += nthr) {
-    const double b = bins[t];
-    if (b != 0.0) atomicAdd(&per_dom[t], b);
-  }
-  if (tid < 3) atomicAdd(&dip[tid], bins[n_dom * 3 + tid]);
+  for (int t = tid; t < nw * 3; t += nthr) fx_add_words(per_dom + kFxWords * (3 * d0 + t), fbins + kFxWords * t);
+  if (d0 == 0 && tid < 3) fx_add_words(dip + kFxWords * tid, fbins + kFxWords * (nw * 3 + tid));
 }
 
 // --------------------------------------------------------------------------
 // Hellmann-Feynman force (cloud/solver.py:282-315).  Each thread holds a
 // column of `kCol` densities in registers and loops the atoms in chunks of
 // four; per-atom sums are FP64, reduced across the wave with shuffles and
-// across the workgroup through LDS, one FP64 atomic per atom component per
-// workgroup.
+// across the workgroup through LDS in a fixed order, then added to the
+// result (a fixed-point sum, fx_add) once per atom component and workgroup.
 // --------------------------------------------------------------------------
 constexpr int kCol = 16;
 constexpr int kChunk = 4;
 
 __global__ __launch_bounds__(256) void k_forces_hf(int S, float h, float h2, float h3, const float* __restrict__ U,
                                                    const Atom* __restrict__ atoms, int first, int count,
-                                                   double* __restrict__ out) {
+                                                   Fx* __restrict__ out) {
   __shared__ double red[4][kChunk * 3];
   const int n = S - 2;
   const int tid = threadIdx.y * blockDim.x + threadIdx.x;
@@ -1574,7 +1575,7 @@ __global__ __launch_bounds__(256) void k_forces_hf(int S, float h, float h2, flo
       const int a = a0 + tid / 3;
       if (a < count) {
         const double s = red[0][tid] + red[1][tid] + red[2][tid] + red[3][tid];
-        atomicAdd(&out[a * 3 + tid % 3], s);
+        fx_add(out + kFxWords * (a * 3 + tid % 3), s);
       }
     }
     __syncthreads();
@@ -1941,16 +1942,21 @@ void launch_fill(hipStream_t st, float* p, size_t n, float v) {
 }
 
 void launch_norm_sums(hipStream_t st, const Tile& t, int S, float h3, const float* U, const uint16_t* lab, int n_dom,
-                      double* norms) {
+                      Fx* norms) {
   // longer columns than the sweeps' tile: a few thousand workgroups, so a
-  // few thousand FP64 atomics into the per-domain sums instead of ~10^4
+  // few thousand atomics into the per-domain sums instead of ~10^4
   Tile r = t;
   r.ich = std::max(t.ich, kReduceIch);
-  hipLaunchKernelGGL(k_norm_sums, stencil_grid(S - 2, r), dim3(r.bx, r.by), sizeof(double) * n_dom, st, S, r.ich, h3,
-                     U, lab, n_dom, norms);
+  static bool attr_set = false;
+  if (!attr_set) {
+    (void)hipFuncSetAttribute((const void*)k_norm_sums, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    attr_set = true;
+  }
+  hipLaunchKernelGGL(k_norm_sums, stencil_grid(S - 2, r), dim3(r.bx, r.by), sizeof(Fx) * kFxWords * n_dom, st, S,
+                     r.ich, h3, U, lab, n_dom, norms);
 }
 
-void launch_normalize(hipStream_t st, const Tile& t, int S, float* U, const uint16_t* lab, const double* norms,
+void launch_normalize(hipStream_t st, const Tile& t, int S, float* U, const uint16_t* lab, const Fx* norms,
                       const float* Z, int n_dom) {
   hipLaunchKernelGGL(k_normalize, stencil_grid(S - 2, t), dim3(t.bx, t.by), 0, st, S, t.ich, U, lab, norms, Z, n_dom);
 }
@@ -1976,19 +1982,24 @@ void launch_levelset(hipStream_t st, const Tile& t, int S, float h, const Atom* 
 }
 
 void launch_energy(hipStream_t st, const Tile& t, int mode, int S, float h, float h3, const float* U, const float* K,
-                   const float* P, const uint16_t* lab, const Atom* atoms, int n_dom, double* per_dom, double* dip) {
-  const size_t lds = sizeof(double) * ((size_t)n_dom * 3 + 3);
+                   const float* P, const uint16_t* lab, const Atom* atoms, int n_dom, Fx* per_dom, Fx* dip) {
+  // domain windows whose bins (3 sums each + 3 dipole sums) fit 150 KiB of LDS
+  constexpr int kWindow = 3200;
   static bool attr_set = false;
   if (!attr_set) {
     (void)hipFuncSetAttribute((const void*)k_energy, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
     attr_set = true;
   }
-  hipLaunchKernelGGL(k_energy, stencil_grid(S - 2, t), dim3(t.bx, t.by), lds, st, mode, S, t.ich, h, h3, 2.0f * h, U,
-                     K, P, lab, atoms, n_dom, per_dom, dip);
+  for (int d0 = 0; d0 < n_dom; d0 += kWindow) {
+    const int nw = std::min(kWindow, n_dom - d0);
+    const size_t lds = sizeof(Fx) * kFxWords * ((size_t)nw * 3 + 3);
+    hipLaunchKernelGGL(k_energy, stencil_grid(S - 2, t), dim3(t.bx, t.by), lds, st, mode, S, t.ich, h, h3, 2.0f * h,
+                       U, K, P, lab, atoms, n_dom, d0, nw, per_dom, dip);
+  }
 }
 
 void launch_forces_hf(hipStream_t st, int S, float h, float h2, float h3, const float* U, const Atom* atoms,
-                      int first, int count, double* out3) {
+                      int first, int count, Fx* out3) {
   const int n = S - 2;
   dim3 grid((n + 63) / 64, (n + 3) / 4, (n + kCol - 1) / kCol);
   hipLaunchKernelGGL(k_forces_hf, grid, dim3(64, 4), 0, st, S, h, h2, h3, U, atoms, first, count, out3);

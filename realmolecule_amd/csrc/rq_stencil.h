@@ -391,10 +391,11 @@ __device__ __forceinline__ void prolong_correct_col(int Sf, int k, int j, int i0
   }
 }
 
-// run-length FP64 accumulation of (U*U)*h3 per clipped label into LDS bins
+// run-length FP64 accumulation of (U*U)*h3 per clipped label into LDS
+// fixed-point bins (fx_add)
 __device__ __forceinline__ void norm_sums_col(int S, int k, int j, int i0, int i1, float h3,
                                               const float* __restrict__ U, const uint16_t* __restrict__ lab,
-                                              int n_dom, double* bins) {
+                                              int n_dom, Fx* bins) {
   // planes in batches of B: the batch's loads are issued together (indices
   // clamped to the column's last plane, whose repeats are skipped), then the
   // run-length accumulation walks them in order, so the sums are those of a
@@ -418,14 +419,14 @@ __device__ __forceinline__ void norm_sums_col(int S, int k, int j, int i0, int i
       const int L = Lb[q] < n_dom ? Lb[q] : n_dom - 1;
       const float term = (vb[q] * vb[q]) * h3;
       if (L != cur) {
-        if (cur >= 0) atomicAdd(&bins[cur], acc);
+        if (cur >= 0) fx_add(bins + kFxWords * cur, acc);
         cur = L;
         acc = 0.0;
       }
       acc += (double)term;
     }
   }
-  if (cur >= 0) atomicAdd(&bins[cur], acc);
+  if (cur >= 0) fx_add(bins + kFxWords * cur, acc);
 }
 
 // U *= scale[label] for labels inside [0, n_dom)
@@ -443,7 +444,7 @@ __device__ __forceinline__ void normalize_col(int S, int k, int j, int i0, int i
 __device__ __forceinline__ void energy_s1_col(int S, int k, int j, int i0, int i1, float h, float h3,
                                               const float* __restrict__ U, const float* __restrict__ K,
                                               const float* __restrict__ P, const uint16_t* __restrict__ lab,
-                                              int n_dom, double* bins) {
+                                              int n_dom, Fx* bins) {
   const int S2 = S * S;
   int cur = -1;
   double aT = 0.0, aK = 0.0, aE = 0.0;
@@ -461,9 +462,9 @@ __device__ __forceinline__ void energy_s1_col(int S, int k, int j, int i0, int i
     const float Vee = P[id] * rho * h3;
     if (L != cur) {
       if (cur >= 0) {
-        atomicAdd(&bins[cur * 3 + 0], aT);
-        atomicAdd(&bins[cur * 3 + 1], aK);
-        atomicAdd(&bins[cur * 3 + 2], aE);
+        fx_add(bins + kFxWords * (cur * 3 + 0), aT);
+        fx_add(bins + kFxWords * (cur * 3 + 1), aK);
+        fx_add(bins + kFxWords * (cur * 3 + 2), aE);
       }
       cur = L;
       aT = aK = aE = 0.0;
@@ -473,9 +474,9 @@ __device__ __forceinline__ void energy_s1_col(int S, int k, int j, int i0, int i
     aE += (double)Vee;
   }
   if (cur >= 0) {
-    atomicAdd(&bins[cur * 3 + 0], aT);
-    atomicAdd(&bins[cur * 3 + 1], aK);
-    atomicAdd(&bins[cur * 3 + 2], aE);
+    fx_add(bins + kFxWords * (cur * 3 + 0), aT);
+    fx_add(bins + kFxWords * (cur * 3 + 1), aK);
+    fx_add(bins + kFxWords * (cur * 3 + 2), aE);
   }
 }
 

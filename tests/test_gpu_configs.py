@@ -162,7 +162,7 @@ def test_c3_batch_matches_single_solves_at_full_size():
     print(f"C3 {len(mols)} conformers", flush=True)
     b = Batch(mols, mg_levels=4, mg_nu=2, graph=True)
     b.initialize(bootstrap=5)
-    b.step(0, 40)
+    b.step(0, 40, poisson_interval=1)
     E = b.energies()
     assert np.isfinite(E).all()
     sizes = [m[1] for m in mols]
@@ -170,7 +170,7 @@ def test_c3_batch_matches_single_solves_at_full_size():
         rows, NN, screen = mols[m]
         s = make_solver(rows, NN, screen, poisson="multigrid", mg_levels=4, boundary="off", bootstrap_poisson=5)
         s.initialize()
-        s.step(0, 40)
+        s.step(0, 40, poisson_interval=1)  # the batch's default schedule: a V-cycle every step
         s._compute_energy()
         d = ulp_diff(b.read(m, "U"), s.U.get())
         print(f"C3 molecule {m} (NN={NN}): U max ulp {d}; E {E[m]} vs {[s.E_T, s.E_eK, s.E_ee]}", flush=True)
