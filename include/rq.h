@@ -218,6 +218,16 @@ int rq_energy(rq_solver* h, double* out6, double* per_domain);
 int rq_domain_norms(rq_solver* h, double* norms);
 /* Electronic HF force on atoms [first, first+count): Z_n sum rho h^3 d/(d^2+h^2)^1.5 */
 int rq_forces_hf(rq_solver* h, int32_t first, int32_t count, double* out3);
+/* rq_forces_hf without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments and results as rq_forces_hf; its partial sums
+ * live in one library-owned array per GPU (72 KiB, kept until the process
+ * ends) instead of a buffer allocated per call, so at most 1024 atoms per
+ * call (count > 1024: RQ_ERANGE; call it in batches of atoms).  The array is
+ * shared by every handle on that GPU: calls that overlap on one GPU race
+ * (issue them from one thread, or serialise them). */
+int rq_forces_hf_noalloc(rq_solver* h, int32_t first, int32_t count, double* out3);
 /* molecule.js sphere-averaged 2 Z <grad P> over radius `radius` cells (<=0: max(5, round(3/h))) */
 int rq_forces_gradp(rq_solver* h, int32_t radius, double* out3n);
 

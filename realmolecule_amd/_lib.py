@@ -126,6 +126,7 @@ PROTOTYPES = {
     "rq_energy": (C.c_int, [_P, _D, _D]),
     "rq_domain_norms": (C.c_int, [_P, _D]),
     "rq_forces_hf": (C.c_int, [_P, _I, _I, _D]),
+    "rq_forces_hf_noalloc": (C.c_int, [_P, _I, _I, _D]),
     "rq_forces_gradp": (C.c_int, [_P, _I, _D]),
     "rq_read_field": (C.c_int, [_P, _I, _P, C.c_size_t]),
     "rq_write_field": (C.c_int, [_P, _I, _P, C.c_size_t]),

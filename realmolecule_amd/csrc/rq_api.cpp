@@ -1178,6 +1178,12 @@ int rq_create(const rq_params* pp, const rq_atom* atoms, int32_t n_atoms, rq_sol
   s->ptile = tile_for(s->S, 4096);
   s->levels = levels;
 
+  // device address of the allocation-free entry point's scratch array, once
+  // per device, with the handle's device current
+  if (hipError_t se = noalloc_setup(); se != hipSuccess) {
+    delete s;
+    return fail(nullptr, RQ_EHIP, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(se));
+  }
   auto alloc = [&](void** ptr, size_t bytes) -> bool { return hipMalloc(ptr, bytes) == hipSuccess; };
   bool ok = hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking) == hipSuccess;
   const size_t fb = s->S3 * sizeof(float), lb = s->S3 * sizeof(uint16_t);
@@ -1656,6 +1662,30 @@ int rq_forces_hf(rq_solver* h, int32_t first, int32_t count, double* out3) {
   HIPCHK(h, hipFreeAsync(buf, h->st));
   HIPCHK(h, hipStreamSynchronize(h->st));
   return rc;
+}
+
+int rq_forces_hf_noalloc(rq_solver* h, int32_t first, int32_t count, double* out3) {
+  if (!h || !out3) return fail(h, RQ_EINVAL, "null argument");
+  if (first < 0 || count < 0 || first + count > h->n_atoms) return fail(h, RQ_ERANGE, "atom range out of bounds");
+  if (count > kNoAllocMaxAtoms)
+    return fail(h, RQ_ERANGE, "rq_forces_hf_noalloc: count %d above its scratch limit of %d atoms per call", count,
+                kNoAllocMaxAtoms);
+  if (count == 0) return RQ_OK;
+  unsigned char* base = static_cast<unsigned char*>(noalloc_scratch());
+  if (!base) return fail(h, RQ_ESTATE, "rq_forces_hf_noalloc: no scratch address for the current device");
+  Fx* sums = reinterpret_cast<Fx*>(base);
+  double* vals = reinterpret_cast<double*>(base + kNoAllocFxBytes);
+  // the array keeps the previous call's contents: clear the sums first
+  zero_fx(h, sums, (size_t)count * 3);
+  {
+    Timed t(h, F_FORCE);
+    launch_forces_hf(h->st, h->S, h->h, h->h2, h->h3, h->U, h->atoms_d, first, count, sums);
+  }
+  launch_fx_values(h->st, count * 3, sums, vals);
+  LAUNCHCHK(h);
+  HIPCHK(h, hipMemcpyAsync(out3, vals, sizeof(double) * 3 * count, hipMemcpyDeviceToHost, h->st));
+  HIPCHK(h, hipStreamSynchronize(h->st));
+  return RQ_OK;
 }
 
 int rq_forces_gradp(rq_solver* h, int32_t radius, double* out3n) {

@@ -181,6 +181,18 @@ void launch_label_u16_to_i32(hipStream_t st, size_t n, const uint16_t* in, int32
 void launch_label_i32_to_u16(hipStream_t st, size_t n, const int32_t* in, uint16_t* out);
 void launch_fill(hipStream_t st, float* p, size_t n, float v);
 
+// Allocation-free HF forces (rq_forces_hf_noalloc): a file-scope device
+// array in rq_kernels.hip holds kNoAllocMaxAtoms atoms' fixed-point sums and
+// their double values.  noalloc_setup() takes the current device's address
+// of it once (hipGetSymbolAddress); noalloc_scratch() returns that address
+// for the current device (nullptr before setup).
+constexpr int kNoAllocMaxAtoms = 1024;
+constexpr size_t kNoAllocFxBytes = sizeof(Fx) * kFxWords * 3 * (size_t)kNoAllocMaxAtoms;
+constexpr size_t kNoAllocScratchBytes = kNoAllocFxBytes + sizeof(double) * 3 * (size_t)kNoAllocMaxAtoms;
+hipError_t noalloc_setup();
+void* noalloc_scratch();
+void launch_fx_values(hipStream_t st, int n, const Fx* w, double* out);
+
 // ---- adjoint launchers (rq_adjoint.hip) ----
 void launch_init_frozen(hipStream_t st, int S, float h, float h2, const Atom* atoms, int n_atoms, const uint16_t* lab,
                         float* U, float* K);

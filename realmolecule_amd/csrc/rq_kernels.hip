@@ -15,6 +15,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 
 #include "rq_internal.h"
 #include "rq_stencil.h"
@@ -2021,6 +2022,50 @@ void launch_label_u16_to_i32(hipStream_t st, size_t n, const uint16_t* in, int32
 
 void launch_label_i32_to_u16(hipStream_t st, size_t n, const int32_t* in, uint16_t* out) {
   hipLaunchKernelGGL(k_lab32to16, dim3(nblk(n, 256)), dim3(256), 0, st, n, in, out);
+}
+
+// ---- scratch of the allocation-free HF force entry point ----
+// One array per GPU for the whole process: the fixed-point sums of up to
+// kNoAllocMaxAtoms atoms (3 components each), then their values as doubles.
+// In host code `scratch` names a host-side stand-in, not device memory: the
+// device address of each GPU's copy comes from hipGetSymbolAddress, taken
+// once per device (noalloc_setup) and kept in g_scratch_dev.
+__device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+constexpr int kMaxDevicesTracked = 64;
+void* g_scratch_dev[kMaxDevicesTracked] = {};
+std::mutex g_scratch_mu;
+
+__global__ void k_fx_values(int n, const Fx* __restrict__ w, double* __restrict__ out) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) out[t] = fx_value(w + kFxWords * t);
+}
+}  // namespace
+
+hipError_t noalloc_setup() {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= kMaxDevicesTracked) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_dev[dev]) return hipSuccess;
+  void* p = nullptr;
+  e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) return e;
+  g_scratch_dev[dev] = p;
+  return hipSuccess;
+}
+
+void* noalloc_scratch() {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevicesTracked) return nullptr;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  return g_scratch_dev[dev];
+}
+
+void launch_fx_values(hipStream_t st, int n, const Fx* w, double* out) {
+  hipLaunchKernelGGL(k_fx_values, dim3(nblk(n, 256)), dim3(256), 0, st, n, w, out);
 }
 
 }  // namespace rq
