@@ -1,16 +1,23 @@
 #!/usr/bin/env python3
 """Throughput of the RealQM grid-solver step on one MI355X.
 
-Workload (default): 10 H2O, seeded random orientations, O-O >= 2.6 A, in a
-12 A cube; 192^3 FP32 grid (h = 0.1181 bohr), 5-level multigrid V-cycle per
-ITP step, per-domain normalise every 20 steps, nuclear-score boundary rule
-every 100 steps, FP64 reductions.  A "step" is one ITP update plus one
-V-cycle.  Inputs are resident in HBM before the timed region.
+Workload (default): 1000 H2O at 1 g/cm^3 (seed 4, O-O >= 2.5 A) in a
+58.58-bohr cube, 3000 domains; 512^3 FP32 grid (h = 0.1144 bohr), 7-level
+multigrid V(2,2) per ITP step, per-domain normalise every 20 steps,
+nuclear-score boundary rule every 100 steps, FP64 (fixed-point) reductions.
+A "step" is one ITP update plus one V-cycle.  Inputs are resident in HBM
+before the timed region.
 
-Prints ONE JSON line: value = Gvoxel-updates/s = (NN-1)^3 * steps / time,
-plus ``roofline`` for the dominant kernel (algorithmic bytes per launch /
-average launch time from HIP events on the solver's stream) and
-``cpu_baseline`` (the NumPy FP32 restatement on a bounded sample).
+Prints ONE JSON line: value = Gvoxel-updates/s = (NN-1)^3 / ms_per_step,
+where ms_per_step is the timed window's wall time per step with the
+normalisation and the score rule counted once per 20 and per 100 steps
+(their per-call device time, measured in the same run, added for the calls
+the window missed or removed for the ones it over-counts); ``roofline`` for
+the dominant fine kernel (algorithmic bytes per launch / average launch
+time from HIP events on the solver's stream, plus measured HBM traffic from
+rocprofv3 FETCH_SIZE / WRITE_SIZE); ``cpu_baseline`` (the C + OpenMP
+restatement on all host cores, same fields); ``forces`` (adjoint against
+central differences, on the 10-water 192^3 workload by default).
 """
 from __future__ import annotations
 
@@ -42,7 +49,11 @@ BYTES_PER_VOXEL = {
 
 # Fine-level kernel of each family, as rocprofv3 names it, for the HBM
 # traffic counters (FETCH_SIZE / WRITE_SIZE per dispatch).
-PMC_KERNEL = {"smooth": "k_smooth<true>", "smooth_itp": "k_smooth_itp", "itp": "k_itp("}
+# Several kernels run on every level; the fine level's dispatches are the
+# ones within a factor 2 of the family's largest (the next level is 1/8).
+PMC_KERNEL = {"smooth": ("k_smooth<true>", "k_smooth2_march<true"), "smooth_itp": ("k_smooth_itp",),
+              "itp": ("k_itp(",), "restrict": ("k_resres_march<true", "k_resres_tiled<true"),
+              "prolong": ("k_prolong_correct", "k_prolong_smooth")}
 COPY_MIB = 512  # plain device copy of known size that calibrates the counters
 
 
@@ -80,27 +91,6 @@ def make_solver(w, graph=True):
            "poisson": "multigrid", "mg_levels": w["mg_levels"], "mg_nu1": 2, "mg_nu2": 2,
            "bootstrap_poisson": 10, "graph": graph}
     return RealQMSolver(cfg)
-
-
-def cpu_baseline(w, steps):
-    """NumPy FP32 restatement of the same step (oracle), single process."""
-    from oracle import s1, mg
-    NN, screen, rows = w["NN"], w["screen"], w["rows"]
-    st = s1.State(rows, NN, screen, np.float32)
-    g = st.g
-    st.U, st.label, st.K, _ = s1.init_trial(rows, g)
-    st.U, _ = s1.normalize(st.U, st.label, st.Z, g)
-    a, c = 1.0 / 7.0, 7.0
-    t0 = time.perf_counter()
-    for step in range(1, steps + 1):
-        rhs = s1.rho_rhs(st.U, g.two_pi, g.h2, np.float32)
-        st.P = mg.vcycle(st.P, rhs, w["mg_levels"], 2, 2, a, c, np.float32)
-        st.U = s1.itp_update(st.U, st.K, st.P, st.label, g)
-        if step % 20 == 0:
-            st.U, _ = s1.normalize(st.U, st.label, st.Z, g)
-    dt = time.perf_counter() - t0
-    n3 = (NN - 1) ** 3
-    return n3 * steps / dt / 1e9, dt
 
 
 def forces_bench(w, steps, bootstrap, n_batch, fd_delta, log_fn):
@@ -452,11 +442,12 @@ def _pmc_pass(counter, kind, workload_name, outdir, timeout_s=120):
     return out
 
 
-def pmc_traffic(workload_name, kernel_key):
-    """HBM bytes per dispatch of the fine-level kernel ``kernel_key``: FETCH_SIZE
-    and WRITE_SIZE (KiB per dispatch, one rocprofv3 pass each), each scaled by
-    the bytes a plain copy of known size moves over what the same counter
-    reads for it."""
+def pmc_traffic(workload_name, kernel_keys):
+    """HBM bytes per dispatch of each fine-level kernel in ``kernel_keys``
+    ({family: rocprofv3 name fragment}): FETCH_SIZE and WRITE_SIZE (KiB per
+    dispatch, one rocprofv3 pass each over a few eager solver steps), each
+    scaled by the bytes a plain copy of known size moves over what the same
+    counter reads for it.  Returns ({family: bytes}, info) or (None, why)."""
     import shutil
     import tempfile
     if shutil.which("rocprofv3") is None:
@@ -473,32 +464,70 @@ def pmc_traffic(workload_name, kernel_key):
     shutil.rmtree(outdir, ignore_errors=True)
     copy_bytes = COPY_MIB * (1 << 20)
     info = {"copy_MiB": COPY_MIB}
-    total = 0.0
+    total = {}
     for counter in ("FETCH_SIZE", "WRITE_SIZE"):
         copies = [v for k, vs in res[(counter, "copy")].items() if "copy" in k.lower() for v in vs
                   if v * 1024 > 0.25 * copy_bytes]
-        kern = [v for k, vs in res[(counter, "solver")].items() if kernel_key in k for v in vs]
-        if not copies or not kern:
-            return None, f"{counter}: no copy or no {kernel_key} dispatches"
+        if not copies:
+            return None, f"{counter}: no copy dispatches"
         scale = copy_bytes / (np.median(copies) * 1024)
-        per = float(np.mean(kern)) * 1024 * scale
-        info[counter] = {"KiB_per_dispatch": round(float(np.mean(kern)), 1), "dispatches": len(kern),
-                         "copy_KiB": round(float(np.median(copies)), 1), "scale": round(float(scale), 4),
-                         "bytes": per}
-        total += per
-    return total, info
+        info[counter] = {"copy_KiB": round(float(np.median(copies)), 1), "scale": round(float(scale), 4)}
+        for fam, keys in kernel_keys.items():
+            kern = [v for k, vs in res[(counter, "solver")].items() if any(q in k for q in keys) for v in vs]
+            if not kern:
+                continue
+            kern = [v for v in kern if v >= 0.5 * max(kern)]  # the fine level's dispatches
+            per = float(np.mean(kern)) * 1024 * scale
+            info[counter][fam] = {"kernel": "|".join(keys), "KiB_per_dispatch": round(float(np.mean(kern)), 1),
+                                  "dispatches": len(kern), "bytes": round(per)}
+            total[fam] = total.get(fam, 0.0) + per
+    both = {f: v for f, v in total.items()
+            if f in info["FETCH_SIZE"] and f in info["WRITE_SIZE"]}
+    return (both or None), info
+
+
+def cpu_baseline_c(w, fields, steps):
+    """All-core CPU baseline: the C + OpenMP restatement of the same step
+    (oracle/cstep.c) on the GPU's initialised fields (U, P, K, label)."""
+    from oracle import cstep, s1
+    NN = w["NN"]
+    U, P, K, lab = fields
+    g = s1.Grid(NN, w["screen"], w["dv"], np.float32)
+    t0 = time.perf_counter()
+    cstep.run_steps(U, P, K, lab, w["rows"][:, 3], g, w["mg_levels"], 0, steps, poisson_interval=1)
+    dt = time.perf_counter() - t0
+    return (NN - 1) ** 3 * steps / dt / 1e9, dt, cstep.threads()
+
+
+def periodic_correction(first, K, timing, events):
+    """ms to add to a K-step window starting after step ``first`` so that
+    each periodic pass counts K / interval times: events = {family:
+    interval}; timing = per-family {avg_us}.  Returns (ms, detail)."""
+    add = 0.0
+    detail = {}
+    for fam, n in events.items():
+        if fam not in timing:
+            continue
+        in_window = (first + K) // n - first // n
+        per_call_ms = timing[fam]["avg_us"] * 1e-3
+        corr = (K / n - in_window) * per_call_ms
+        add += corr
+        detail[fam] = {"interval": n, "calls_in_window": in_window, "calls_amortised": K / n,
+                       "ms_per_call": round(per_call_ms, 4), "correction_ms": round(corr, 4)}
+    return add, detail
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1000)
-    ap.add_argument("--warmup", type=int, default=50)
-    ap.add_argument("--workload", default="c2", choices=("c1", "c2", "c3", "c4", "c5", "mf", "pd", "sic"))
-    ap.add_argument("--cpu-steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="c5", choices=("c1", "c2", "c3", "c4", "c5", "mf", "pd", "sic"))
+    ap.add_argument("--cpu-steps", type=int, default=12, help="steps of the all-core C baseline")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--timing-steps", type=int, default=100)
+    ap.add_argument("--timing-steps", type=int, default=200)
     ap.add_argument("--no-forces", action="store_true")
+    ap.add_argument("--force-workload", default="c2", choices=("c1", "c2", "c4"))
     ap.add_argument("--force-steps", type=int, default=100)
     ap.add_argument("--force-batch", type=int, default=4)
     ap.add_argument("--no-pmc", action="store_true", help="skip the rocprofv3 FETCH/WRITE_SIZE passes")
@@ -530,7 +559,14 @@ def main():
     n3 = (NN - 1) ** 3
     log(f"[bench] workload {w['desc']}; creating solver")
     s = make_solver(w)
+    w["dv"] = s.dv
     s.initialize()
+    fields = None
+    if not args.no_cpu:
+        # the CPU baseline starts from the same initialised fields
+        S = NN + 1
+        fields = (s.U.get().reshape(S, S, S), s.P.get().reshape(S, S, S), s.K_buf.get().reshape(S, S, S),
+                  s.label.get().reshape(S, S, S))
     sched_pi = 1
     log(f"[bench] warmup {args.warmup} steps")
     s.step(0, args.warmup, poisson_interval=sched_pi)
@@ -541,8 +577,7 @@ def main():
     s.step(args.warmup, args.steps, poisson_interval=sched_pi)
     s.sync()
     dt = time.perf_counter() - t0
-    ms_per_step = dt / args.steps * 1e3
-    value = n3 * args.steps / dt / 1e9
+    ms_window = dt / args.steps * 1e3
     # energy at the next step that is a multiple of 100 (normalise + boundary
     # rule just applied: where cloud/solver.py:594-595 reports it), untimed
     done = args.warmup + args.steps
@@ -550,7 +585,6 @@ def main():
     if extra:
         s.step(done, extra, poisson_interval=sched_pi)
     s._compute_energy()
-    log(f"[bench] {ms_per_step:.4f} ms/step, {value:.2f} Gvoxel-updates/s, E={s.E:.6f}")
 
     # Same steps launched eagerly (no graph replay), for comparison.
     lib = s._lib
@@ -564,29 +598,49 @@ def main():
     log(f"[bench] eager launches: {ms_eager:.4f} ms/step")
     first += args.steps
 
-    # Per-kernel device time: HIP events bracketing each launch on the solver's stream.
+    # Per-kernel device time: HIP events bracketing each launch on the
+    # solver's stream, over a window that is a whole number of boundary
+    # intervals (so the score rule and the normalisation are sampled)
     s._chk(lib.rq_timing_enable(s._h, 1))
-    s.step(first, args.timing_steps, poisson_interval=sched_pi)
+    tsteps = max(100, (args.timing_steps // 100) * 100)
+    s.step(first, tsteps, poisson_interval=sched_pi)
     nf = len(L.FAMILIES)
     ms = (C.c_double * nf)()
     cnt = (C.c_int64 * nf)()
     s._chk(lib.rq_timing_read(s._h, ms, cnt, nf))
     s._chk(lib.rq_timing_enable(s._h, 0))
-    bpv = dict(BYTES_PER_VOXEL)
     kernels = {}
     for f, name in enumerate(L.FAMILIES):
         if cnt[f] == 0:
             continue
-        avg_us = ms[f] / cnt[f] * 1e3
-        ent = {"launches": int(cnt[f]), "avg_us": round(avg_us, 3), "total_ms": round(ms[f], 4)}
+        kernels[name] = {"launches": int(cnt[f]), "avg_us": round(ms[f] / cnt[f] * 1e3, 3),
+                         "total_ms": round(ms[f], 4), "launches_per_step": cnt[f] / tsteps}
+    # algorithmic bytes per launch: per-voxel figure x units per launch.  The
+    # fine smoother family runs either one sweep per launch or a sweep pair
+    # (temporal blocking on grids beyond the MALL); the fine level does
+    # nu1 + nu2 = 4 sweeps per step, one of them inside smooth_itp when that
+    # fused pass runs, so sweeps per launch = (4 - smooth_itp/step) / smooth/step.
+    bpv = dict(BYTES_PER_VOXEL)
+    if "smooth" in kernels:
+        sweeps = 4 - kernels.get("smooth_itp", {}).get("launches_per_step", 0.0)
+        per_launch = sweeps / kernels["smooth"]["launches_per_step"]
+        kernels["smooth"]["sweeps_per_launch"] = round(per_launch, 3)
+        bpv["smooth"] = BYTES_PER_VOXEL["smooth"] * per_launch
+    for name, ent in kernels.items():
         if name in bpv:
-            gbs = bpv[name] * n3 / (avg_us * 1e-6) / 1e9
+            ent["alg_bytes_per_voxel"] = bpv[name]
             ent["alg_bytes_per_launch"] = bpv[name] * n3
-            ent["achieved_GBs"] = round(gbs, 1)
-        kernels[name] = ent
+            ent["achieved_GBs"] = round(bpv[name] * n3 / (ent["avg_us"] * 1e-6) / 1e9, 1)
+    # the score rule and the normalisation every 100 / 20 steps, amortised
+    # into the timed window whatever its length
+    corr_ms, corr = periodic_correction(args.warmup, args.steps, kernels, {"boundary": 100, "normalize": 20})
+    ms_per_step = ms_window + corr_ms / args.steps
+    value = n3 / (ms_per_step * 1e-3) / 1e9
+    log(f"[bench] {ms_window:.4f} ms/step in the window, {ms_per_step:.4f} with the periodic passes amortised: "
+        f"{value:.2f} Gvoxel-updates/s, E={s.E:.6f}")
     fine = {k: v for k, v in kernels.items() if k in bpv}
     dom = max(fine, key=lambda k: fine[k]["total_ms"])
-    step_bytes = sum(bpv[k] * n3 * kernels[k]["launches"] for k in fine) / args.timing_steps
+    step_bytes = sum(bpv[k] * n3 * kernels[k]["launches"] for k in fine) / tsteps
     roof = {
         "kernel": dom,
         "bound": "hbm",
@@ -598,33 +652,41 @@ def main():
         "traffic_unit": "bytes per launch",
         "alg_bytes_per_voxel": bpv[dom],
         "alg_bytes_per_launch": bpv[dom] * n3,
+        "avg_us": fine[dom]["avg_us"],
         "step_alg_bytes_fine": step_bytes,
         "step_frac": round(step_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
     }
-
     s.close()
-    if not args.no_pmc and dom in PMC_KERNEL:
-        log("[bench] HBM traffic of the dominant kernel: rocprofv3 FETCH_SIZE / WRITE_SIZE passes")
-        traffic, info = pmc_traffic(args.workload, PMC_KERNEL[dom])
-        if traffic is not None:
-            roof["traffic"] = round(traffic)
-            roof["traffic_over_alg"] = round(traffic / (bpv[dom] * n3), 4)
+    if not args.no_pmc:
+        keys = {k: PMC_KERNEL[k] for k in fine if k in PMC_KERNEL}
+        log(f"[bench] HBM traffic of the fine kernels {sorted(keys)}: rocprofv3 FETCH_SIZE / WRITE_SIZE passes")
+        traffic, info = pmc_traffic(args.workload, keys)
         roof["traffic_pmc"] = info
+        if traffic:
+            for k, tb in traffic.items():
+                kernels[k]["traffic_bytes_per_launch"] = round(tb)
+                kernels[k]["traffic_over_alg"] = round(tb / (bpv[k] * n3), 4)
+                kernels[k]["hbm_GBs"] = round(tb / (kernels[k]["avg_us"] * 1e-6) / 1e9, 1)
+            if dom in traffic:
+                roof["traffic"] = round(traffic[dom])
+                roof["traffic_over_alg"] = kernels[dom]["traffic_over_alg"]
+                roof["hbm_frac"] = round(kernels[dom]["hbm_GBs"] / HBM_PEAK_GBS, 4)
 
     forces = None
     if not args.no_forces:
-        log(f"[bench] forces: adjoint vs central differences, {args.force_steps} steps per solve")
-        forces = forces_bench(w, args.force_steps, 20, args.force_batch, 1e-2, log)
+        wf = workload(args.force_workload)
+        log(f"[bench] forces ({args.force_workload}): adjoint vs central differences, {args.force_steps} steps per solve")
+        forces = forces_bench(wf, args.force_steps, 20, args.force_batch, 1e-2, log)
 
     cpu = None
-    if not args.no_cpu:
-        log(f"[bench] cpu baseline: {args.cpu_steps} steps of the NumPy FP32 oracle")
-        gv, cdt = cpu_baseline(w, args.cpu_steps)
-        cpu = {"value": round(gv, 6), "unit": "Gvoxel-updates/s", "cores": 1, "kind": "port",
-               "sample": f"{args.cpu_steps} steps (V-cycle + ITP, normalise every 20) of the same "
-                         f"{NN}^3 workload, NumPy FP32 restatement, {cdt:.1f} s; "
-                         f"host cpu_count={os.cpu_count()}"}
-        log(f"[bench] cpu {gv:.5f} Gvoxel-updates/s")
+    if fields is not None:
+        log(f"[bench] cpu baseline: {args.cpu_steps} steps of the C + OpenMP restatement on the same fields")
+        gv, cdt, thr = cpu_baseline_c(w, fields, args.cpu_steps)
+        cpu = {"value": round(gv, 6), "unit": "Gvoxel-updates/s", "cores": thr, "kind": "port",
+               "sample": f"{args.cpu_steps} steps (V-cycle + ITP, normalise every 20) of the same {NN}^3 "
+                         f"workload from the GPU's initialised fields; C + OpenMP restatement (oracle/cstep.c), "
+                         f"{thr} threads, {cdt:.1f} s; host cpu_count={os.cpu_count()}"}
+        log(f"[bench] cpu {gv:.5f} Gvoxel-updates/s on {thr} threads")
 
     out = {
         "metric": METRIC,
@@ -634,6 +696,8 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": round(ms_per_step, 5),
+        "ms_per_step_window": round(ms_window, 5),
+        "periodic_amortisation": corr,
         "ms_per_step_eager": round(ms_eager, 5),
         "higher_is_better": True,
         "scaling": "weak",
