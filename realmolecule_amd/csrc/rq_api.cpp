@@ -641,11 +641,9 @@ int boundary_step(rq_solver* s) {
       const size_t nf = ((size_t)s->S3 + 1) / 2;
       if (!s->best_atom_d) HIPCHK(s, hipMalloc(&s->best_atom_d, nf * sizeof(float)));
       if (s->best_atom_stale) {
-        // every entry 0xFFFF (not yet searched): a fill kernel, not a memset
-        const uint32_t ones = 0xFFFFFFFFu;
-        float v;
-        std::memcpy(&v, &ones, sizeof v);
-        launch_fill(s->st, reinterpret_cast<float*>(s->best_atom_d), nf, v);
+        // the best atom of every interior node, once per geometry: the rule
+        // then evaluates two scores per candidate node and never searches
+        launch_score_map(s->st, s->S, s->h, s->h2, s->atoms_d, s->n_atoms, s->best_atom_d);
         s->best_atom_stale = false;
       }
       launch_boundary_score(s->st, s->tile, s->S, s->h, s->h2, s->atoms_d, s->n_atoms, s->lab, s->lab2,

@@ -166,6 +166,10 @@ void launch_boundary_score(hipStream_t st, const Tile& t, int S, float h, float 
                            const uint16_t* lin, uint16_t* lout, float ratio, int min_foreign, const AtomCells& cells,
                            const int32_t* cell_start, const int32_t* cell_atoms,
                            uint16_t* best_atom);
+// The score rule's best atom of every interior node (Z > 0 atoms, the first
+// strict maximum of Z / sqrt(r^2 + h^2) in index order), the map the rule
+// reads; nodes of blocks with too many candidate atoms are left unknown.
+void launch_score_map(hipStream_t st, int S, float h, float h2, const Atom* atoms, int n_atoms, uint16_t* best_atom);
 void launch_levelset(hipStream_t st, const Tile& t, int S, float h, const Atom* atoms, int n_atoms,
                      const int32_t* lock_group, const uint16_t* lin, uint16_t* lout, const float* U, float* W,
                      float dt_w, float curv_reg, int rc_flags);

@@ -1193,6 +1193,99 @@ __device__ __forceinline__ uint16_t flip_label(uint16_t raw, int cur, int bN, in
   return best > cur_score * ratio ? (uint16_t)bN : raw;
 }
 
+// Best-atom map of the score rule for every interior node, one pass per
+// geometry (the rule's own search then never runs).  A workgroup owns a
+// block of kMapB^3 nodes.  It keeps the atoms that can be the best anywhere
+// in the block -- those whose score bound at the block's nearest point
+// reaches the largest bound of any atom at its farthest point (with 1e-4
+// slack for FP32 rounding) -- in index order; each node then takes the first
+// strict maximum of Z / sqrt(r^2 + h^2) over that list, with the expression
+// of the rule's scan.  An atom left out cannot reach the block maximum, so
+// the result is the scan's.  More than kMapMaxCand candidates: the block is
+// left unknown and the rule searches those nodes itself.
+constexpr int kMapB = 8, kMapMaxCand = 512;
+__global__ __launch_bounds__(256) void k_score_map(int S, float h, float h2, const Atom* __restrict__ atoms,
+                                                   int n_atoms, uint16_t* __restrict__ best_atom) {
+  __shared__ float wred[4];
+  __shared__ int wcnt[4];
+  __shared__ int cand[kMapMaxCand];
+  __shared__ int ncand;
+  const int n = S - 2, nb = (n + kMapB - 1) / kMapB;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int b = blockIdx.x;
+  const int i0 = 1 + (b / (nb * nb)) * kMapB, j0 = 1 + ((b / nb) % nb) * kMapB, k0 = 1 + (b % nb) * kMapB;
+  const int i1 = min(i0 + kMapB, n + 1), j1 = min(j0 + kMapB, n + 1), k1 = min(k0 + kMapB, n + 1);
+  const float xl = (float)i0 * h, xh = (float)(i1 - 1) * h, yl = (float)j0 * h, yh = (float)(j1 - 1) * h;
+  const float zl = (float)k0 * h, zh = (float)(k1 - 1) * h;
+  // largest lower bound (farthest block point) over the atoms
+  float mL = 0.0f;
+  for (int a = tid; a < n_atoms; a += 256) {
+    const Atom at = atoms[a];
+    if (!(at.Z > 0.0f)) continue;
+    const float ax = at.i * h, ay = at.j * h, az = at.k * h;
+    const float dx = fmaxf(fabsf(xl - ax), fabsf(xh - ax)), dy = fmaxf(fabsf(yl - ay), fabsf(yh - ay));
+    const float dz = fmaxf(fabsf(zl - az), fabsf(zh - az));
+    mL = fmaxf(mL, at.Z / sqrtf(dx * dx + dy * dy + dz * dz + h2));
+  }
+  for (int off = 32; off > 0; off >>= 1) mL = fmaxf(mL, __shfl_xor(mL, off, 64));
+  if (lane == 0) wred[wave] = mL;
+  if (tid == 0) ncand = 0;
+  __syncthreads();
+  const float thr = fmaxf(fmaxf(wred[0], wred[1]), fmaxf(wred[2], wred[3])) * 0.9999f;
+  // candidates in index order: 256 atoms per round, appended in order
+  for (int a0 = 0; a0 < n_atoms; a0 += 256) {
+    const int a = a0 + tid;
+    bool keep = false;
+    if (a < n_atoms) {
+      const Atom at = atoms[a];
+      if (at.Z > 0.0f) {
+        const float ax = at.i * h, ay = at.j * h, az = at.k * h;
+        const float dx = fmaxf(0.0f, fmaxf(xl - ax, ax - xh)), dy = fmaxf(0.0f, fmaxf(yl - ay, ay - yh));
+        const float dz = fmaxf(0.0f, fmaxf(zl - az, az - zh));
+        keep = at.Z / sqrtf(dx * dx + dy * dy + dz * dz + h2) >= thr;
+      }
+    }
+    const unsigned long long m = __ballot(keep);
+    if (lane == 0) wcnt[wave] = __popcll(m);
+    __syncthreads();
+    const int base = ncand;
+    int before = 0;
+    for (int w = 0; w < wave; ++w) before += wcnt[w];
+    if (keep) {
+      const int pos = base + before + __popcll(m & ((1ull << lane) - 1ull));
+      if (pos < kMapMaxCand) cand[pos] = a;
+    }
+    __syncthreads();
+    if (tid == 0) ncand = base + wcnt[0] + wcnt[1] + wcnt[2] + wcnt[3];
+    __syncthreads();
+  }
+  const int nc = ncand;
+  const int S2 = S * S;
+  for (int q = tid; q < kMapB * kMapB * kMapB; q += 256) {
+    const int i = i0 + q / (kMapB * kMapB), j = j0 + (q / kMapB) % kMapB, k = k0 + q % kMapB;
+    if (i >= i1 || j >= j1 || k >= k1) continue;
+    int bN = kBestUnknown;
+    if (nc <= kMapMaxCand) {
+      const float xi = (float)i * h, yj = (float)j * h, zk = (float)k * h;
+      float best = 0.0f;
+      bN = kBestNone;
+      for (int c = 0; c < nc; ++c) {
+        const int a = cand[c];
+        const float dx = xi - atoms[a].i * h;
+        const float dy = yj - atoms[a].j * h;
+        const float dz = zk - atoms[a].k * h;
+        const float r2 = dx * dx + dy * dy + dz * dz + h2;
+        const float sc = atoms[a].Z / sqrtf(r2);
+        if (sc > best) {
+          best = sc;
+          bN = a;
+        }
+      }
+    }
+    best_atom[i * S2 + j * S + k] = (uint16_t)bN;
+  }
+}
+
 // Nuclear-score flip (cloud/solver.py:92-147), reading labels from `lin`
 // and writing `lout` (double-buffered).
 // Score rule by a scan over all atoms (cloud/solver.py:92-147; the form for
@@ -1960,6 +2053,11 @@ void launch_norm_sums(hipStream_t st, const Tile& t, int S, float h3, const floa
 void launch_normalize(hipStream_t st, const Tile& t, int S, float* U, const uint16_t* lab, const Fx* norms,
                       const float* Z, int n_dom) {
   hipLaunchKernelGGL(k_normalize, stencil_grid(S - 2, t), dim3(t.bx, t.by), 0, st, S, t.ich, U, lab, norms, Z, n_dom);
+}
+
+void launch_score_map(hipStream_t st, int S, float h, float h2, const Atom* atoms, int n_atoms, uint16_t* best_atom) {
+  const int nb = (S - 2 + kMapB - 1) / kMapB;
+  hipLaunchKernelGGL(k_score_map, dim3((unsigned)nb * nb * nb), dim3(256), 0, st, S, h, h2, atoms, n_atoms, best_atom);
 }
 
 void launch_boundary_score(hipStream_t st, const Tile& t, int S, float h, float h2, const Atom* atoms, int n_atoms,

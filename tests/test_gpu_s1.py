@@ -118,10 +118,12 @@ def test_boundary_score_exact():
     s.close()
 
 
-@pytest.mark.parametrize("n_atoms,NN,h", [(60, 40, 0.35), (150, 48, 0.6), (300, 48, 0.3)])
+@pytest.mark.parametrize("n_atoms,NN,h", [(60, 40, 0.35), (150, 48, 0.6), (300, 48, 0.3), (1500, 24, 0.1)])
 def test_boundary_score_many_atoms_cell_search(n_atoms, NN, h):
-    """Many atoms (mixed Z, one with Z = 0; above 128 atoms the kernel searches a cell list): the argmax
-    search must pick the same atom as the oracle's scan over all atoms,
+    """Many atoms (mixed Z, one with Z = 0): the per-node best atom (from
+    the block-candidate map, or the kernel's own cell-list search on blocks
+    with more than 512 candidates, as 1500 atoms packed in a 2.4-bohr box
+    give) must be the same atom as the oracle's scan over all atoms,
     including boundary voxels far from every nucleus."""
     rng = np.random.default_rng(11 + n_atoms)
     rows = np.zeros((n_atoms, 5))
