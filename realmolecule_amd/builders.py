@@ -299,3 +299,112 @@ def mf_h2o(N=200, bond_au=None, screen=12.0):
               dict(i=N2 - round(hx / h), j=N2 + round(hy / h), k=N2, Z=1, rc=0.0, fixed=True)]
     return {"USER_NN": N, "USER_SCREEN": screen, "USER_STEPS": 100000, "USER_NUCLEI": nuclei,
             "USER_NORM_TARGETS": [3, 1, 1]}
+
+
+def hairpin_solvated():
+    """The bent solvated beta-hairpin of the reference's recorded A100 run
+    (recipe of cloud/run_hairpin.py:21-182; the page hairpin_bent_solvated.html):
+    12 residues on a U-shaped path in the k = N2 plane, backbone N, CA, C with
+    their H and the carbonyl O, a C-terminal OH, and a shell of waters on a
+    2.8 A lattice within 8 A of the protein (none closer than 2.5 A); 300^3
+    cells over an 80-bohr box, positions rounded to whole grid nodes with
+    Python's round().  Returns the RealQMSolver config dict of that run (10000
+    steps, reports every 1000, dynamics on with forceScale 500, fold atoms
+    N-terminus / middle CA / C-terminus)."""
+    NN, screen = 300, 80.0
+    h = screen / NN
+    N2 = NN // 2
+    A = ANGSTROM
+    atoms, xy = [], []
+
+    def put(x, y, el):
+        Z, rc = ELEMENT[el]
+        atoms.append({"i": round(N2 + x / h), "j": round(N2 + y / h), "k": N2, "Z": Z, "el": el, "rc": rc})
+        xy.append((x, y))
+
+    n_res, rise, zig = 12, 3.4 * A, 0.5 * A
+    # strand direction per residue: the two strands 30 degrees apart, blended
+    # through the turn by tanh((r - 5.5) / 2.5)
+    half_open = math.radians(30.0) / 2
+    a_start = -(math.pi - (math.pi / 2 + half_open))
+    a_end = math.pi / 2 - half_open
+    ang = []
+    for r in range(n_res):
+        w = 0.5 * (1.0 + math.tanh((r - 5.5) / 2.5))
+        ang.append(a_start * (1.0 - w) + a_end * w)
+    path = [(0.0, 0.0)]
+    for r in range(1, n_res):
+        mid = 0.5 * (ang[r - 1] + ang[r])
+        path.append((path[-1][0] + rise * math.cos(mid), path[-1][1] + rise * math.sin(mid)))
+    cx0 = sum(p[0] for p in path) / n_res
+    cy0 = sum(p[1] for p in path) / n_res
+    backbone = []
+    for r in range(n_res):
+        px, py = path[r][0] - cx0, path[r][1] - cy0
+        tx, ty = math.cos(ang[r]), math.sin(ang[r])
+        nx, ny = -ty, tx
+        sgn = 1 if r % 2 == 0 else -1
+        lift = 15.0  # the hairpin sits 15 bohr up in j
+        n_at = (px - 0.35 * rise * tx + sgn * zig * 0.6 * nx, py - 0.35 * rise * ty + sgn * zig * 0.6 * ny + lift)
+        ca_at = (px, py - sgn * zig * 0.4 * ny + lift)
+        c_at = (px + 0.35 * rise * tx + sgn * zig * 0.3 * nx, py + 0.35 * rise * ty + sgn * zig * 0.3 * ny + lift)
+        backbone.append((n_at, ca_at, c_at))
+
+    def unit_normal(p, q):
+        dx, dy = q[0] - p[0], q[1] - p[1]
+        L = math.hypot(dx, dy)
+        return -dy / L, dx / L
+
+    d_nh, d_ch, d_co = 1.01 * A, 1.09 * A, 1.24 * A
+    for r, (n_at, ca_at, c_at) in enumerate(backbone):
+        pnx, pny = unit_normal(n_at, ca_at)
+        pcx, pcy = unit_normal(ca_at, c_at)
+        put(n_at[0], n_at[1], "N")
+        s_nh = -1 if r < 6 else 1
+        put(n_at[0] + s_nh * pnx * d_nh, n_at[1] + s_nh * pny * d_nh, "H")
+        if r == 0:  # the N-terminal NH2
+            put(n_at[0] - s_nh * pnx * d_nh, n_at[1] - s_nh * pny * d_nh, "H")
+        put(ca_at[0], ca_at[1], "C")
+        put(ca_at[0] + pcx * d_ch, ca_at[1] + pcy * d_ch, "H")
+        put(ca_at[0] - pcx * d_ch, ca_at[1] - pcy * d_ch, "H")
+        put(c_at[0], c_at[1], "C")
+        even = r % 2 == 0
+        s_co = (1 if even else -1) if r < 6 else (-1 if even else 1)
+        put(c_at[0] + s_co * pcx * d_co, c_at[1] + s_co * pcy * d_co, "O")
+    # C-terminal OH along the last CA -> C bond
+    ca_l, c_l = backbone[-1][1], backbone[-1][2]
+    ux, uy = c_l[0] - ca_l[0], c_l[1] - ca_l[1]
+    L = math.hypot(ux, uy)
+    ux, uy = ux / L, uy / L
+    ox, oy = c_l[0] + ux * 1.34 * A, c_l[1] + uy * 1.34 * A
+    put(ox, oy, "O")
+    put(ox + ux * 0.97 * A, oy + uy * 0.97 * A, "H")
+    n_protein = len(atoms)
+
+    # waters on a lattice: kept if no protein atom is within 2.5 A and one is
+    # within 8 A; orientation from a fixed hash of the lattice point
+    d_oh, half = 0.96 * A, math.radians(52.25)
+    step, near, shell = 2.8 * A, 2.5 * A, 8.0 * A
+    lim = screen / 2 - 3
+    prot = xy[:n_protein]
+    wx = -lim
+    while wx <= lim:
+        wy = -lim
+        while wy <= lim:
+            clash, close = False, False
+            for (px, py) in prot:
+                d2 = (wx - px) ** 2 + (wy - py) ** 2
+                if d2 < near * near:
+                    clash = True
+                    break
+                if d2 < shell * shell:
+                    close = True
+            if close and not clash:
+                phi = (wx * 7.3 + wy * 11.1) % (2 * math.pi)
+                put(wx, wy, "O")
+                put(wx + d_oh * math.cos(phi + half), wy + d_oh * math.sin(phi + half), "H")
+                put(wx + d_oh * math.cos(phi - half), wy + d_oh * math.sin(phi - half), "H")
+            wy += step
+        wx += step
+    return {"gridN": NN, "screen": screen, "atoms": atoms, "steps": 10000, "report_interval": 1000,
+            "dynamics": True, "forceScale": 500.0, "fold_atoms": [0, 45, 83], "nProtein": n_protein}

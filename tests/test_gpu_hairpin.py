@@ -1,0 +1,67 @@
+"""The reference's one recorded solver run, re-run through the drop-in.
+
+cloud/run_hairpin.py ran RealQMSolver on an A100 for the bent solvated
+hairpin (267 atoms, 300^3 cells over 80 bohr, Jacobi Poisson, dynamics on
+with forceScale 500) and stored its snapshots in cloud/hairpin_results.json
+(fixture: tests/golden/hairpin_a100.json, written by
+tests/golden/make_hairpin_fixture.py).  The same config from
+builders.hairpin_solvated() through RealQMSolver.run(10000, 1000) on the
+GPU must stay inside an envelope of that trajectory: every energy component
+of every snapshot within 1e-3 relative (E_KK, which depends on the nuclei
+only, within 1e-4), the fold angle within 0.02 degrees, every nucleus
+within 0.05 grid units at the last snapshot.  The run is not bit-for-bit
+the reference's: the reference sums energies and forces with float atomics
+in arbitrary order, flips labels in place with a race, and its CUDA
+expf/sqrtf round differently; the first snapshot already differs by 2e-4 in
+E_T.  The built-in water positions must equal the fixture's exactly (the
+waters never move)."""
+import json
+import os
+import time
+
+import numpy as np
+import pytest
+
+from realmolecule_amd.builders import hairpin_solvated
+from realmolecule_amd.solver import RealQMSolver
+
+pytestmark = pytest.mark.gpu
+FIX = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "hairpin_a100.json")
+
+
+def test_hairpin_builder_matches_reference_positions():
+    cfg = hairpin_solvated()
+    ref = json.load(open(FIX))
+    pos = np.array([[a["i"], a["j"], a["k"]] for a in cfg["atoms"]], float)
+    first = np.array(ref["nucPos_first"])
+    n_p = cfg["nProtein"]
+    assert len(cfg["atoms"]) == len(first) == 267 and n_p == 87
+    assert np.array_equal(pos[n_p:], first[n_p:])
+    # the protein has moved at most one clipped step pair by the first snapshot
+    assert np.abs(pos[:n_p] - first[:n_p]).max() < 0.05
+
+
+def test_hairpin_trajectory_inside_reference_envelope():
+    cfg = hairpin_solvated()
+    cfg["graph"] = True
+    ref = json.load(open(FIX))
+    s = RealQMSolver(cfg)
+    s.set_fold_atoms(cfg["fold_atoms"])
+    t0 = time.time()
+    snaps = s.run(cfg["steps"], report_interval=cfg["report_interval"])
+    dt = time.time() - t0
+    print(f"hairpin: 10000 steps in {dt:.2f} s ({dt / 10:.3f} ms/step; the A100 run: "
+          f"{ref['snapshots'][-1]['ms_per_step']:.3f} ms/step)", flush=True)
+    assert [a["step"] for a in snaps] == [b["step"] for b in ref["snapshots"]]
+    worst = {}
+    for a, b in zip(snaps, ref["snapshots"]):
+        for k in ("E", "E_T", "E_eK", "E_ee", "E_KK"):
+            rel = abs(a[k] - b[k]) / abs(b[k])
+            worst[k] = max(worst.get(k, 0.0), rel)
+            assert rel <= (1e-4 if k == "E_KK" else 1e-3), (a["step"], k, a[k], b[k])
+        assert abs(a["fold_angle"] - b["fold_angle"]) <= 0.02, (a["step"], a["fold_angle"], b["fold_angle"])
+    dpos = np.abs(np.array(snaps[-1]["nucPos"]) - np.array(ref["nucPos_last"])).max()
+    print("worst relative deviation per component: " + ", ".join(f"{k} {v:.2e}" for k, v in worst.items())
+          + f"; max |dpos| at step 10000 {dpos:.4f} grid units", flush=True)
+    assert dpos <= 0.05
+    s.close()
