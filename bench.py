@@ -168,6 +168,42 @@ def forces_bench(w, steps, bootstrap, n_batch, fd_delta, log_fn):
     }
 
 
+def bench_hairpin():
+    """The reference's recorded run, re-run: the bent solvated hairpin of
+    cloud/run_hairpin.py (267 atoms, 300^3 cells, Jacobi Poisson, dynamics
+    with forceScale 500, energy reports every 1000 steps) through
+    RealQMSolver.run(10000), timed like the reference's ms_per_step (wall time
+    of run() over its steps, host work included), next to the A100's 5.277
+    ms/step for the same run (cloud/hairpin_results.json)."""
+    from realmolecule_amd.builders import hairpin_solvated
+    from realmolecule_amd.solver import RealQMSolver
+    cfg = hairpin_solvated()
+    cfg["graph"] = True
+    s = RealQMSolver(cfg)
+    s.set_fold_atoms(cfg["fold_atoms"])
+    s.initialize()
+    t0 = time.perf_counter()
+    snaps = s.run(cfg["steps"], report_interval=cfg["report_interval"])
+    dt = time.perf_counter() - t0
+    s.close()
+    NN = cfg["gridN"]
+    ms = dt / cfg["steps"] * 1e3
+    a100 = 5.277
+    log(f"[hairpin] {ms:.4f} ms/step against the A100's {a100} ms/step")
+    return {
+        "metric": "Gvoxel-updates/s (reference hairpin run, 300^3, dynamics on)",
+        "value": round((NN - 1) ** 3 / (ms * 1e-3) / 1e9, 3), "unit": "Gvoxel-updates/s", "n_gpus": 1,
+        "steps": cfg["steps"], "warmup": 0, "ms_per_step": round(ms, 5), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": round(a100 / ms, 3), "dtype": "fp32", "data": "synthetic",
+        "config": {"workload": "bent solvated hairpin of cloud/run_hairpin.py: 87 protein atoms + 60 waters, "
+                               "300^3 FP32 (h = 0.267 bohr), Jacobi Poisson (2 sweeps every 2 steps), dynamics every "
+                               "500 steps with forceScale 500, reports every 1000", "parallelism": "single-gpu"},
+        "baseline": {"ms_per_step": a100, "hardware": "NVIDIA A100 (cloud/hairpin_results.json)",
+                     "note": "vs_baseline = A100 ms/step / MI355X ms/step on the same run"},
+        "energy_Ha": snaps[-1]["E"], "fold_angle": snaps[-1]["fold_angle"],
+    }
+
+
 def bench_c3(steps, warmup):
     """Batched geometry scan: 64 seeded conformers (H2O, formamide, ethanol,
     glycine) on grids of 64..160 cells, 4-level V(2,2) + ITP per step.  The
@@ -522,8 +558,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="c5", choices=("c1", "c2", "c3", "c4", "c5", "mf", "pd", "sic"))
-    ap.add_argument("--cpu-steps", type=int, default=12, help="steps of the all-core C baseline")
+    ap.add_argument("--workload", default="c5",
+                    choices=("c1", "c2", "c3", "c4", "c5", "mf", "pd", "sic", "hairpin"))
+    ap.add_argument("--cpu-steps", type=int, default=80, help="steps of the all-core C baseline")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--timing-steps", type=int, default=200)
     ap.add_argument("--no-forces", action="store_true")
@@ -543,6 +580,9 @@ def main():
         return
     if args.workload == "c3":
         print(json.dumps(bench_c3(args.steps, args.warmup)))
+        return
+    if args.workload == "hairpin":
+        print(json.dumps(bench_hairpin()))
         return
     if args.workload == "mf":
         print(json.dumps(bench_mf(args)))
