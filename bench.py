@@ -93,45 +93,88 @@ def make_solver(w, graph=True):
     return RealQMSolver(cfg)
 
 
-def forces_bench(w, steps, bootstrap, n_batch, fd_delta, log_fn):
-    """Forces/sec: plain path = central differences of full frozen-label solves
-    (6N+1 solves per force vector, each molecule in turn) against the adjoint
-    (one forward solve + one reverse sweep per force vector), single and with
-    ``n_batch`` geometries advanced concurrently on their own HIP streams.
-    Both paths evaluate the same E(R) (same steps, same labels), so their
-    energies match exactly."""
+def converge_steps(w, bootstrap, tol=1e-6, chunk=100, max_steps=60000):
+    """Steps of the frozen-label solve (multigrid V-cycle every step) after
+    which the electronic energy changes by <= tol relative over one more
+    chunk of steps.  Returns (steps, relative change, energy)."""
+    from realmolecule_amd.solver import RealQMSolver
+    from realmolecule_amd.builders import to_config_atoms
+    s = RealQMSolver({"gridN": w["NN"], "screen": w["screen"], "atoms": to_config_atoms(w["rows"], w["els"]),
+                      "poisson": "multigrid", "mg_levels": w["mg_levels"], "bootstrap_poisson": bootstrap,
+                      "boundary": "off", "graph": True})
+    s.initialize()
+    prev, d, E, T = None, 1.0, 0.0, 0
+    while T < max_steps:
+        s.step(T, chunk, poisson_interval=1)
+        T += chunk
+        s._compute_energy()
+        E = s.E_T + s.E_eK + s.E_ee
+        if prev is not None:
+            d = abs(E - prev) / abs(E)
+            if d <= tol:
+                break
+        prev = E
+    s.close()
+    return T, d, E
+
+
+def forces_bench(w, bootstrap, n_batch, fd_delta, fd_components, log_fn, conv_tol=1e-6):
+    """Forces/sec at converged energy.  The solve is the frozen-label
+    trajectory (trial at the labels, normalise, bootstrap V-cycles, then the
+    cloud/solver.py loop body with a V-cycle per step) run until its energy
+    changes by <= conv_tol relative over 100 more steps.  Plain path: central
+    differences (delta = fd_delta bohr) of that full solve, 6N+1 solves per
+    force vector, timed on a sample of components; adjoint: one forward solve
+    plus the reverse sweep with checkpointed recomputation; batched: n_batch
+    geometries on concurrent handles.  Both paths evaluate the same E(R)."""
+    import ctypes as C
     import threading
     from realmolecule_amd.adjoint import FrozenSolver
     NN, screen, rows = w["NN"], w["screen"], w["rows"]
+    T, dconv, Econv = converge_steps(w, bootstrap, conv_tol)
+    log_fn(f"[forces] converged: {T} steps, dE/E over the last 100 steps {dconv:.2e}")
     kw = dict(bootstrap=bootstrap, checkpoint=0, poisson_interval=1, norm_interval=20, poisson="multigrid",
               mg_levels=w["mg_levels"])
-    fz = FrozenSolver(rows, NN, screen, steps, **kw)
+    fz = FrozenSolver(rows, NN, screen, T, **kw)
     pos = rows[:, :3] * fz.h
     n = len(rows)
-    # warm both paths on this handle: one energy and one gradient (the
-    # first gradient allocates the recorded trajectory, once per handle)
-    fz.energy(pos)
-    fz.energy_and_gradient(pos)
-    fz.s.sync()
     t0 = time.perf_counter()
     E_adj, g = fz.energy_and_gradient(pos)
     t_adj = time.perf_counter() - t0
-    log_fn(f"[forces] adjoint: {t_adj:.3f} s for {3 * n} components")
+    log_fn(f"[forces] adjoint: {t_adj:.3f} s for {3 * n} components ({T} steps)")
+    # plain path on a sample of components: each is two full solves
     t0 = time.perf_counter()
     E0 = fz.energy(pos)
-    F_fd = fz.forces_fd(pos, delta=fd_delta)
-    t_fd = time.perf_counter() - t0
-    log_fn(f"[forces] central differences: {t_fd:.2f} s for {6 * n + 1} solves")
-    dF = np.abs(-g - F_fd)
-    err = float(dF.max())
-    bare = rows[:, 4] <= 0  # rc = 0: E is smooth in R; rc > 0 atoms have the clamp kink
+    t_e0 = time.perf_counter() - t0
+    # the reference's own force at the converged state: Hellmann-Feynman
+    # (cloud/solver.py:282-315) on the final U of that solve
+    hf = (C.c_double * (3 * n))()
+    fz.s._chk(fz.lib.rq_forces_hf(fz.hdl, 0, n, hf))
+    F_hf = np.frombuffer(hf, np.float64).reshape(n, 3).copy()
+    from realmolecule_amd.solver import nuclear_forces
+    F_el_adj = -(g + nuclear_forces(rows[:, :3].astype(np.float32), rows[:, 3], fz.h))
+    bare = rows[:, 4] <= 0
+    comps = []
+    for a in list(np.nonzero(bare)[0][:1]) + list(np.nonzero(~bare)[0][:1]):
+        for c in range(3):
+            comps.append((int(a), c))
+    comps = comps[:fd_components] if fd_components <= len(comps) else \
+        [(a, c) for a in range(n) for c in range(3)][:fd_components]
+    F_fd = {}
+    t0 = time.perf_counter()
+    for a, c in comps:
+        F_fd[(a, c)] = _fd_component(fz, pos, a, c, fd_delta)
+    t_fd_sample = time.perf_counter() - t0
+    t_solve = (t_fd_sample + t_e0) / (2 * len(comps) + 1)
+    t_fd = (6 * n + 1) * t_solve
+    diffs = [abs(-g[a, c] - F_fd[(a, c)]) for a, c in comps]
+    log_fn(f"[forces] central differences: {2 * len(comps) + 1} solves in {t_fd_sample + t_e0:.2f} s, "
+           f"{t_solve:.3f} s per solve, {t_fd:.1f} s per force vector (6N+1 = {6 * n + 1} solves)")
     fz.close()
     # batched: n_batch geometries (seeded 0.05 bohr jitter) on concurrent handles
     rng = np.random.default_rng(7)
-    solvers = [FrozenSolver(rows, NN, screen, steps, **kw) for _ in range(n_batch)]
+    solvers = [FrozenSolver(rows, NN, screen, T, **kw) for _ in range(n_batch)]
     geos = [pos + rng.normal(0, 0.05, pos.shape) for _ in range(n_batch)]
-    for s_, p_ in zip(solvers, geos):
-        s_.energy_and_gradient(p_)
     out = [None] * n_batch
 
     def run(i):
@@ -147,25 +190,45 @@ def forces_bench(w, steps, bootstrap, n_batch, fd_delta, log_fn):
     for s_ in solvers:
         s_.close()
     log_fn(f"[forces] batched adjoint: {n_batch} force vectors in {t_batch:.3f} s")
+    hf_diff = np.abs(F_el_adj - F_hf)
     return {
-        "workload": f"{w['desc']}; frozen-label solve of {bootstrap} bootstrap V-cycles + {steps} steps; "
-                    "each handle warmed with one energy and one gradient",
+        "workload": f"{w['desc']}; frozen-label solve of {bootstrap} bootstrap V-cycles + {T} steps "
+                    f"(converged: dE/E {dconv:.2e} over the last 100 steps)",
         "n_atoms": n,
+        "steps_to_converge": T,
+        "converged_dE_rel_per_100_steps": dconv,
         "energy_Ha": E0,
         "energy_rel_diff_adjoint_vs_plain": abs(E_adj - E0) / abs(E0),
         "plain_fd_forces_per_s": 1.0 / t_fd,
         "plain_fd_solves": 6 * n + 1,
+        "plain_fd_seconds_per_solve": t_solve,
+        "plain_fd_timed_solves": 2 * len(comps) + 1,
         "adjoint_forces_per_s": 1.0 / t_adj,
         "batched_adjoint_forces_per_s": n_batch / t_batch,
         "batch": n_batch,
         "speedup_adjoint": t_fd / t_adj,
         "speedup_batched_adjoint": (n_batch / t_batch) * t_fd,
-        "max_abs_force_diff_vs_fd_Ha_per_bohr": err,
-        "max_abs_force_diff_rc0_atoms": float(dF[bare].max()) if bare.any() else None,
-        "max_abs_force_diff_rc_atoms": float(dF[~bare].max()) if (~bare).any() else None,
-        "max_abs_force_Ha_per_bohr": float(np.abs(F_fd).max()),
         "fd_delta_bohr": fd_delta,
+        "fd_checked": [{"atom": a, "axis": c, "adjoint": -g[a, c], "fd": F_fd[(a, c)],
+                        "rc": float(rows[a, 4])} for a, c in comps],
+        "max_abs_force_diff_vs_fd_Ha_per_bohr": float(max(diffs)),
+        "max_abs_force_Ha_per_bohr": float(np.abs(g).max()),
+        "hf_electronic_vs_adjoint_rc0_atoms_max_abs": float(hf_diff[bare].max()) if bare.any() else None,
+        "hf_electronic_vs_adjoint_rc0_atoms_rms": float(np.sqrt((hf_diff[bare] ** 2).mean())) if bare.any() else None,
     }
+
+
+def _fd_component(fz, pos, a, c, delta):
+    """Central difference of the full solve for one force component (the
+    step divided by the difference of the stored FP32 grid positions)."""
+    p = np.asarray(pos, np.float64).copy()
+    p[a, c] = pos[a, c] + delta
+    xp = fz.stored_position(p[a, c])
+    ep = fz.energy(p)
+    p[a, c] = pos[a, c] - delta
+    xm = fz.stored_position(p[a, c])
+    em = fz.energy(p)
+    return -(ep - em) / (xp - xm)
 
 
 def bench_hairpin():
@@ -565,7 +628,8 @@ def main():
     ap.add_argument("--timing-steps", type=int, default=200)
     ap.add_argument("--no-forces", action="store_true")
     ap.add_argument("--force-workload", default="c2", choices=("c1", "c2", "c4"))
-    ap.add_argument("--force-steps", type=int, default=100)
+    ap.add_argument("--force-components", type=int, default=4,
+                    help="force components checked (and timed) by central differences")
     ap.add_argument("--force-batch", type=int, default=4)
     ap.add_argument("--no-pmc", action="store_true", help="skip the rocprofv3 FETCH/WRITE_SIZE passes")
     ap.add_argument("--pmc-child", choices=("solver", "copy"), help=argparse.SUPPRESS)
@@ -715,8 +779,8 @@ def main():
     forces = None
     if not args.no_forces:
         wf = workload(args.force_workload)
-        log(f"[bench] forces ({args.force_workload}): adjoint vs central differences, {args.force_steps} steps per solve")
-        forces = forces_bench(wf, args.force_steps, 20, args.force_batch, 1e-2, log)
+        log(f"[bench] forces ({args.force_workload}): adjoint vs central differences at converged energy")
+        forces = forces_bench(wf, 20, args.force_batch, 1e-3, args.force_components, log)
 
     cpu = None
     if fields is not None:
