@@ -245,8 +245,10 @@ def bench_hairpin():
     s = RealQMSolver(cfg)
     s.set_fold_atoms(cfg["fold_atoms"])
     s.initialize()
+    import contextlib
     t0 = time.perf_counter()
-    snaps = s.run(cfg["steps"], report_interval=cfg["report_interval"])
+    with contextlib.redirect_stdout(sys.stderr):  # run() prints a line per report; stdout carries the JSON line
+        snaps = s.run(cfg["steps"], report_interval=cfg["report_interval"])
     dt = time.perf_counter() - t0
     s.close()
     NN = cfg["gridN"]
