@@ -145,6 +145,15 @@ typedef struct rq_params {
  * on its own nodes (scaled by 1/Z for Z > 1).  Needs a coarse level
  * (RQ_POISSON_TWOGRID or RQ_POISSON_MULTIGRID) when sic_interval > 0. */
 #define RQ_POTHER_FRAME 2
+/* DIRECT_SWEEPS: molecule.js's own direct Pother, its default for pages of
+ * at most 5 electrons (molecule.js:111, 624-688, 2833-2858).  For every
+ * domain m with Z != 0 a persistent P_m, set by rq_initialize to
+ * sum_{n != m, Z_n > 0} 0.5 / sqrt(r_n^2 + h^2), takes per Poisson step 10
+ * damped sweeps P' = a P + (sum6 P + h^2 2pi rho_other_m) / c with the
+ * handle's smoother (smooth_a, smooth_c; the page's 0.3333 and 9), and the
+ * frame potential Pother takes P_m on the nodes labelled m.  The ITP update,
+ * the energy and the grad-P forces read Pother.  Any poisson_mode. */
+#define RQ_POTHER_DIRECT_SWEEPS 3
 
 /* Scheduling intervals of the run() loop (cloud/solver.py:550-586). */
 typedef struct rq_schedule {
@@ -172,6 +181,30 @@ int rq_get_atoms(const rq_solver* h, rq_atom* atoms, int32_t n_atoms);
 /* Lock groups of the level-set rule (molecule.js:487-500): group[d] > 0 marks
  * domain d; a boundary between two domains of one group does not evolve. */
 int rq_set_lock_groups(rq_solver* h, const int32_t* group, int32_t n_atoms);
+
+/* The molecule.js per-atom fields beyond rq_atom (struct Atom,
+ * molecule.js:183-191, filled by fillAtomBuf, :1875-1903), used with
+ * init_mode RQ_INIT_S2: the trial init and the K refresh, the grad-P forces
+ * and the level set's sector rule.  Until set, every atom has the defaults
+ * (Z_nuc = Z, init_zeff = Z, init_rcut = 1e6 bohr, no split). */
+#define RQ_SPLIT_SPHERE 0  /* no angular restriction */
+#define RQ_SPLIT_HEMI   2  /* split_idx 0: the half-space axis.d < 0, 1: axis.d >= 0 */
+#define RQ_SPLIT_THIRD  3  /* split_idx 0..2: 120-degree sector about the axis, from split_rot */
+typedef struct rq_atom_s2 {
+  float   Z_nuc;      /* nuclear charge of K, E_KK and forces (molecule.js:22-25); <= 0: Z.
+                         Z = 0 with Z_nuc > 0 is a bare nucleus: in K, no electron domain */
+  float   init_zeff;  /* decay rate of the trial Zeff^2/sqrt(pi) exp(-Zeff r~) (:1889-1891); <= 0: Z */
+  float   init_rcut;  /* trial zero beyond this distance from the nucleus, bohr (:1892); <= 0: 1e6 */
+  int32_t split_type; /* RQ_SPLIT_*: the domain owns only nodes in its sector (molecule.js:194-225) */
+  int32_t split_idx;
+  float   split_axis[3];
+  float   split_rot;  /* angular offset of the THIRD sectors (rad) */
+} rq_atom_s2;
+/* molecule.js USER_Z_NUC, USER_INIT_ZEFF / INIT_ZEFF, USER_INIT_RCUT /
+ * INIT_RCUT and USER_SPLIT_* per atom; takes effect at the next rq_init_trial
+ * / rq_initialize / rq_recompute_k (K, labels, trial) and immediately in the
+ * forces and the level set. */
+int rq_set_atoms_s2(rq_solver* h, const rq_atom_s2* ext, int32_t n_atoms);
 
 /* --- passes --- */
 int rq_init_trial(rq_solver* h);                 /* K, label, U, bestU (and W = 1) */

@@ -2,13 +2,22 @@
 
 Same array conventions as ``oracle.s1``.  molecule.js stores nucleus
 positions as u32 (``molecule.js:1882-1884``), so every position used here is
-``floor(position)``.  Only the parts of the WGSL kernels that the build
-implements are restated: the angular-sector ("split") constraints, bare
-nuclei (Z = 0, Z_nuc > 0), the W-cutoff blend and the frame-stale
-``PotherBuf`` are not.
+``floor(position)``.  The per-atom fields of ``struct Atom`` beyond (i, j, k,
+Z, rc) -- Z_nuc, initZeff, initRcut and the angular split (molecule.js:183-191,
+1875-1903) -- come in as ``ext``, a list of dicts with the keys of
+``rq_atom_s2`` (``resolve_ext`` applies the page's defaults).  The W-cutoff
+blend (off by default) is not restated.
 
 Parity pinning (tests/test_oracle_s2.py):
-  init_s2      scalar brute force per node (all nodes incl. faces)
+  init_s2      scalar brute force per node (all nodes incl. faces), with
+               Z_nuc / initZeff / initRcut / splits too
+  in_split     the hemisphere by the sign of the axis projection, a 120-degree
+               rotation about the axis moves a node to the next third
+  recompute_k_s2  closed form on a single atom; equals init K for rc = 0
+               atoms away from the 2h clamp
+  nuclear_energy_unique  merging co-located nuclei (closed form)
+  pdirect_init / direct_sweeps_step  the start potential is the others'
+               0.5/r sum (scalar loop); sweeps are s1.smooth on rho_other
   rc_mask      scalar brute force
   levelset     explicit-loop restatement + invariants (interior W >= 1,
                flips only to a neighbouring label at voxels with a foreign neighbour)
@@ -37,13 +46,74 @@ def _fl(atoms):
     return a
 
 
-def init_s2(atoms, g: Grid):
-    """gpuInitAccum + gpuInitFinal (molecule.js:1377-1449) with initZeff = Z,
-    initRcut = 1e6, Z_nuc = Z and no splits: K = sum Z/r~ with
-    r~ = max(sqrt(r^2+0.04h^2), rc) (rc > 0) or max(r, 2h); label = first
-    argmax of Z^2/sqrt(pi) exp(-Z r~); U = best trial -- on ALL nodes."""
+SPLIT_CODE = {"sphere": 0, "hemi": 2, "third": 3}
+
+
+def resolve_ext(rows, ext=None):
+    """Per-atom S2 fields with molecule.js's defaults (fillAtomBuf,
+    molecule.js:1875-1903): Zn = Z_nuc || Z, zeff = initZeff || Z, rcut =
+    initRcut || 1e6, split sphere, axis (0, 0, 1)."""
+    rows = np.asarray(rows, np.float64).reshape(-1, 5)
+    out = []
+    for n in range(rows.shape[0]):
+        e = (ext[n] if ext is not None else None) or {}
+        Z = rows[n, 3]
+        st = e.get("split_type", e.get("split", 0))
+        out.append({"Zn": e.get("Z_nuc", 0.0) if e.get("Z_nuc", 0.0) > 0 else Z,
+                    "zeff": e.get("init_zeff", 0.0) if e.get("init_zeff", 0.0) > 0 else Z,
+                    "rcut": e.get("init_rcut", 0.0) if e.get("init_rcut", 0.0) > 0 else 1e6,
+                    "st": SPLIT_CODE[st] if isinstance(st, str) else int(st),
+                    "sidx": int(e.get("split_idx", 0)),
+                    "axis": tuple(e.get("split_axis", (0.0, 0.0, 1.0))),
+                    "rot": float(e.get("split_rot", 0.0))})
+    return out
+
+
+def in_split(dx, dy, dz, r, e, dt):
+    """inSplitAtom (molecule.js:194-225) on arrays of node offsets d (bohr)
+    and |d|, FP32 operation order."""
+    st, sidx = e["st"], e["sidx"]
+    full = np.broadcast_shapes(np.shape(dx), np.shape(dy), np.shape(dz), np.shape(r))
+    if st <= 1:
+        return np.ones(full, bool)
+    ax, ay, az = (dt(v) for v in e["axis"])
+    axL = max(np.sqrt(ax * ax + ay * ay + az * az), dt(1e-12))
+    a0, a1, a2 = ax / axL, ay / axL, az / axL
+    dotp = dx * a0 + dy * a1 + dz * a2
+    small = np.broadcast_to(r < dt(1e-6), full)
+    if st == 2:
+        res = (dotp < 0) if sidx == 0 else (dotp >= 0)
+    elif st == 3:
+        px, py, pz = dx - dotp * a0, dy - dotp * a1, dz - dotp * a2
+        if abs(a2) < dt(0.9):
+            e1x, e1y, e1z = a1, -a0, dt(0.0)
+        else:
+            e1x, e1y, e1z = dt(0.0), a2, -a1
+        e1L = max(np.sqrt(e1x * e1x + e1y * e1y + e1z * e1z), dt(1e-12))
+        e1x, e1y, e1z = e1x / e1L, e1y / e1L, e1z / e1L
+        e2x, e2y, e2z = a1 * e1z - a2 * e1y, a2 * e1x - a0 * e1z, a0 * e1y - a1 * e1x
+        pe1 = px * e1x + py * e1y + pz * e1z
+        pe2 = px * e2x + py * e2y + pz * e2z
+        th = np.arctan2(pe2, pe1).astype(dt) - dt(e["rot"])
+        th = np.where(th < dt(-3.14159265), th + dt(6.28318530), th)
+        th = np.where(th > dt(3.14159265), th - dt(6.28318530), th)
+        sec = np.maximum(np.floor((th + dt(3.14159265)) / dt(2.09439510)), 0).astype(np.int64) % 3
+        res = sec == sidx
+    else:
+        res = np.ones(full, bool)
+    return small | np.broadcast_to(res, full)
+
+
+def init_s2(atoms, g: Grid, ext=None):
+    """gpuInitAccum + gpuInitFinal (molecule.js:1377-1449) on ALL nodes: K =
+    sum Zn/r~ over atoms with Z > 0 or Zn > 0 (Zn = Z_nuc || Z), r~ =
+    max(sqrt(r^2+0.04h^2), rc) (rc > 0) or max(r, 2h); the trial
+    zeff^2/sqrt(pi) exp(-zeff r~) of atoms with Z > 0, zero where the real
+    distance exceeds initRcut or the node lies outside the atom's split
+    sector; label = first argmax of the trial; U = best trial."""
     dt = g.dtype
     S = g.S
+    ex = resolve_ext(atoms, ext)
     atoms = _fl(atoms).astype(dt)
     x = np.arange(S).astype(dt) * g.h
     r_sing = dt(2.0 * g.h64)
@@ -52,22 +122,132 @@ def init_s2(atoms, g: Grid):
     K = np.zeros((S, S, S), dt)
     for a in range(atoms.shape[0]):
         ai, aj, ak, Za, rc = atoms[a]
-        if not Za > 0:
+        e = ex[a]
+        Zn = dt(e["Zn"])
+        if Za <= 0 and Zn <= 0:
             continue
-        dx = (x - ai * g.h)[:, None, None]
-        dy = (x - aj * g.h)[None, :, None]
-        dz = (x - ak * g.h)[None, None, :]
+        dx = np.broadcast_to((x - ai * g.h)[:, None, None], (S, S, S))
+        dy = np.broadcast_to((x - aj * g.h)[None, :, None], (S, S, S))
+        dz = np.broadcast_to((x - ak * g.h)[None, None, :], (S, S, S))
         r2 = dx * dx + dy * dy + dz * dz
         if rc > 0:
             r = np.maximum(np.sqrt(r2 + dt(0.04) * g.h2), rc)
         else:
             r = np.maximum(np.sqrt(r2), r_sing)
-        K = K + Za / r
-        ut = Za * Za * dt(INV_SQRT_PI) * np.exp(-Za * r)
+        K = K + Zn / r
+        Ze = dt(e["zeff"])
+        r_real = np.sqrt(r2)
+        ok = ~(r_real > dt(e["rcut"])) & bool(Za > 0) & in_split(dx, dy, dz, r_real, e, dt)
+        ut = np.where(ok, Ze * Ze * dt(INV_SQRT_PI) * np.exp(-Ze * r), dt(0.0)).astype(dt)
         upd = ut > bU
         bU = np.where(upd, ut, bU)
         best = np.where(upd, np.int32(a), best)
     return bU.copy(), best, K, bU
+
+
+def recompute_k_s2(atoms, g: Grid, ext=None):
+    """recomputeK (molecule.js:1337-1366) on all nodes: offsets (i - posI) h,
+    K = sum Zn / max(r, rc) (rc > 0) or Zn / max(r, 2h)."""
+    dt = g.dtype
+    S = g.S
+    ex = resolve_ext(atoms, ext)
+    atoms = _fl(atoms).astype(dt)
+    idx = np.arange(S).astype(dt)
+    r_sing = dt(2.0 * g.h64)
+    K = np.zeros((S, S, S), dt)
+    for a in range(atoms.shape[0]):
+        ai, aj, ak, Za, rc = atoms[a]
+        Zn = dt(ex[a]["Zn"])
+        if Za <= 0 and Zn <= 0:
+            continue
+        di = ((idx - ai) * g.h)[:, None, None]
+        dj = ((idx - aj) * g.h)[None, :, None]
+        dk = ((idx - ak) * g.h)[None, None, :]
+        r = np.sqrt(di * di + dj * dj + dk * dk)
+        r = np.maximum(r, rc) if rc > 0 else np.maximum(r, r_sing)
+        K = K + Zn / r
+    return K
+
+
+def nuclear_energy_unique(pos, Z, Z_nuc, h):
+    """E_KK of molecule.js (:55-62, 3166-3180): electrons at one position
+    share one nucleus of charge sum(Z_nuc || Z); pairs of distinct nuclei,
+    softened by 0.04 h^2.  FP64."""
+    nuc = {}
+    order = []
+    for e in range(len(Z)):
+        zn = Z_nuc[e] if Z_nuc is not None else 0.0
+        if Z[e] == 0 and zn == 0:
+            continue
+        key = tuple(pos[e])
+        if key not in nuc:
+            nuc[key] = 0.0
+            order.append(key)
+        nuc[key] += zn or Z[e]
+    e = 0.0
+    for a in range(len(order)):
+        for b in range(a + 1, len(order)):
+            d2 = sum(((order[a][c] - order[b][c]) * h) ** 2 for c in range(3))
+            e += nuc[order[a]] * nuc[order[b]] / np.sqrt(d2 + 0.04 * h * h)
+    return e
+
+
+def pdirect_init(rows, g: Grid, m):
+    """initPdirect (molecule.js:624-662): sum over the other atoms with Z > 0
+    of 0.5/sqrt(r^2 + h^2) on the interior, u32 positions, atom order."""
+    dt = g.dtype
+    S = g.S
+    atoms = _fl(rows).astype(dt)
+    x = np.arange(S).astype(dt) * g.h
+    out = np.zeros((S, S, S), dt)
+    I = _interior(S)
+    val = np.zeros((S - 2,) * 3, dt)
+    for n in range(atoms.shape[0]):
+        if atoms[n, 3] <= 0 or n == m:
+            continue
+        dx = (x[1:-1] - atoms[n, 0] * g.h)[:, None, None]
+        dy = (x[1:-1] - atoms[n, 1] * g.h)[None, :, None]
+        dz = (x[1:-1] - atoms[n, 2] * g.h)[None, None, :]
+        val = val + dt(0.5) / np.sqrt(dx * dx + dy * dy + dz * dz + g.h2)
+    out[I] = val
+    return out
+
+
+def direct_sweeps_setup(rows, g: Grid):
+    """The start of molecule.js's direct Pother: (Pdom, Pother) with Pother
+    the sum of every charged domain's P_m (molecule.js:2678-2693)."""
+    Z = np.asarray(rows, np.float64)[:, 3]
+    Pdom = []
+    Pother = np.zeros((g.S,) * 3, g.dtype)
+    for m in range(len(Z)):
+        if Z[m] == 0:
+            Pdom.append(np.zeros((g.S,) * 3, g.dtype))
+            continue
+        Pm = pdirect_init(rows, g, m)
+        Pdom.append(Pm)
+        Pother = Pother + Pm
+    return Pdom, Pother
+
+
+def direct_sweeps_step(U, label, Pdom, Pother, Z, g: Grid, a=0.3333, c=9.0, sweeps=10):
+    """One step of molecule.js's direct Pother (molecule.js:2833-2858): for
+    each domain m with Z != 0, rho_other_m scaled (h^2 2pi) rho, `sweeps`
+    damped sweeps of P_m, and Pother = P_m on the nodes labelled m."""
+    from .s1 import smooth
+    dt = g.dtype
+    I = _interior(g.S)
+    Pdom = [p.copy() for p in Pdom]
+    Pother = Pother.copy()
+    alpha = g.h2 * g.two_pi
+    for m in range(len(Z)):
+        if Z[m] == 0:
+            continue
+        rhs = rho_other(U, label, m, alpha, dt(1.0), dt)
+        for _ in range(sweeps):
+            Pdom[m] = smooth(Pdom[m], rhs, a, c, dt)
+        sel = (label[I] & 0x7FFF) == m
+        Pother[I] = np.where(sel, Pdom[m][I], Pother[I])
+    return Pdom, Pother
 
 
 def rc_mask(label, atoms, g: Grid):
@@ -86,7 +266,28 @@ def rc_mask(label, atoms, g: Grid):
     return (a[..., 4] > 0) & (r < a[..., 4])
 
 
-def levelset(U, label, W, g: Grid, dt_w=0.5, curv_reg=0.15, lock_group=None):
+def _sector_ok(L, atoms_fl, ex, g, dt):
+    """cellSectorOK (molecule.js:438-443) of every interior node for the
+    domain in L (same shape as the interior)."""
+    S = g.S
+    idx = np.arange(1, S - 1).astype(dt)
+    ok = np.ones(L.shape, bool)
+    for n, e in enumerate(ex):
+        if e["st"] <= 1:
+            continue
+        sel = L == n
+        if not sel.any():
+            continue
+        di = ((idx - dt(atoms_fl[n, 0])) * g.h)[:, None, None]
+        dj = ((idx - dt(atoms_fl[n, 1])) * g.h)[None, :, None]
+        dk = ((idx - dt(atoms_fl[n, 2])) * g.h)[None, None, :]
+        di, dj, dk = np.broadcast_arrays(di, dj, dk)
+        r = np.sqrt(di * di + dj * dj + dk * dk)
+        ok = np.where(sel, in_split(di, dj, dk, r, e, dt), ok)
+    return ok
+
+
+def levelset(U, label, W, g: Grid, dt_w=0.5, curv_reg=0.15, lock_group=None, atoms=None, ext=None):
     """One level-set step (molecule.js:426-540): returns (label', W').
 
     Per interior voxel: count same-label neighbours (order +i,-i,+j,-j,+k,-k)
@@ -131,6 +332,20 @@ def levelset(U, label, W, g: Grid, dt_w=0.5, curv_reg=0.15, lock_group=None):
     flip = wn < 0
     wn = np.where(flip, dt(0.1), wn)
     nl = np.where(flip, best_l, L)
+    if ext is not None and any(e["st"] > 1 for e in resolve_ext(atoms, ext)):
+        # split siblings own only the nodes of their sector (molecule.js:521-536)
+        ex = resolve_ext(atoms, ext)
+        afl = _fl(atoms)
+        my_ok = _sector_ok(L, afl, ex, g, dt)
+        new_ok = _sector_ok(nl, afl, ex, g, dt)
+        best_ok = _sector_ok(best_l, afl, ex, g, dt)
+        cancel = ~new_ok
+        rescue = cancel & ~my_ok & (best_l != L) & best_ok
+        release = new_ok & ~my_ok & (nl == L) & (best_l != L) & best_ok
+        nl = np.where(cancel, L, nl)
+        wn = np.where(cancel, np.maximum(wn, dt(0.1)), wn)
+        nl = np.where(rescue | release, best_l, nl)
+        wn = np.where(rescue | release, dt(0.1), wn)
     evolve = ~interior & ~locked
     w_out = np.where(interior, np.maximum(w, dt(1.0)), np.where(locked, np.maximum(w, dt(0.5)), wn))
     out_l = np.where(evolve, nl, L)
@@ -190,7 +405,7 @@ def energy_s2(U, K, P, label, atoms, g: Grid, n_domains=None):
     return per.sum(0), per, np.array(dip)
 
 
-def gradp_forces(P, atoms, g: Grid, R=None):
+def gradp_forces(P, atoms, g: Grid, R=None, ext=None):
     """gradPtotal (molecule.js:1199-1270): F = 2 Z <grad P> over integer cells
     within radius R of the (integer) nucleus, central differences, interior
     cells only; FP64 sums here (the reference sums in FP32 in one thread)."""
@@ -204,7 +419,10 @@ def gradp_forces(P, atoms, g: Grid, R=None):
     rr = np.arange(-R, R + 1)
     DI, DJ, DK = np.meshgrid(rr, rr, rr, indexing="ij")
     ball = DI * DI + DJ * DJ + DK * DK <= R * R
+    ex = resolve_ext(atoms, ext)
     for n, (ai, aj, ak, Z, _rc) in enumerate(atoms):
+        if Z <= 0:  # a bare nucleus feels the force with its nuclear charge (molecule.js:1214)
+            Z = ex[n]["Zn"]
         if not Z > 0:
             continue
         ii = int(ai) + DI[ball]

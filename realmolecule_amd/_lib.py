@@ -35,6 +35,7 @@ RQ_ENERGY_S2 = 1
 RQ_POTHER_TOTAL = 0
 RQ_POTHER_DIRECT = 1
 RQ_POTHER_FRAME = 2
+RQ_POTHER_DIRECT_SWEEPS = 3
 
 FIELDS = {"U": 0, "U2": 1, "K": 2, "P": 3, "P2": 4, "rho": 5, "label": 6, "bestU": 7, "W": 8, "Pother": 9}
 FAMILIES = ["itp", "smooth", "smooth_c", "restrict", "restrict_c", "prolong", "prolong_c", "coarse_solve",
@@ -56,6 +57,15 @@ class RqParams(C.Structure):
         ("energy_mode", C.c_int32), ("rc_dirichlet", C.c_int32), ("pother_mode", C.c_int32),
         ("sic_interval", C.c_int32), ("reserved", C.c_int32 * 6),
     ]
+
+
+class RqAtomS2(C.Structure):
+    _fields_ = [("Z_nuc", C.c_float), ("init_zeff", C.c_float), ("init_rcut", C.c_float),
+                ("split_type", C.c_int32), ("split_idx", C.c_int32), ("split_axis", C.c_float * 3),
+                ("split_rot", C.c_float)]
+
+
+RQ_SPLIT = {"sphere": 0, "hemi": 2, "third": 3}
 
 
 class RqSchedule(C.Structure):
@@ -110,6 +120,7 @@ PROTOTYPES = {
     "rq_set_atoms": (C.c_int, [_P, C.POINTER(RqAtom), _I]),
     "rq_get_atoms": (C.c_int, [_P, C.POINTER(RqAtom), _I]),
     "rq_set_lock_groups": (C.c_int, [_P, C.POINTER(C.c_int32), _I]),
+    "rq_set_atoms_s2": (C.c_int, [_P, C.POINTER(RqAtomS2), _I]),
     "rq_init_trial": (C.c_int, [_P]),
     "rq_initialize": (C.c_int, [_P, _I]),
     "rq_poisson": (C.c_int, [_P, _I]),

@@ -62,6 +62,11 @@ struct rq_solver {
   float h = 0, h2 = 0, h3 = 0, half_dv = 0, dtv = 0, two_pi = 0;
   int n_atoms = 0;
   std::vector<Atom> atoms_h;
+  // molecule.js per-atom fields (rq_set_atoms_s2) as given, and resolved
+  // against the current Z on the device; any_split: some atom has a sector
+  std::vector<rq_atom_s2> ext_h;
+  AtomS2* ext_d = nullptr;
+  bool any_split = false;
   // uniform cell list of the atoms (Z > 0) for the score rule's argmax search
   AtomCells cells;
   int32_t *cell_start_d = nullptr, *cell_atoms_d = nullptr;
@@ -257,6 +262,20 @@ int read_fx(rq_solver* s, const Fx* d, size_t n, double* out) {
 
 int upload_atoms(rq_solver* s) {
   HIPCHK(s, hipMemcpyAsync(s->atoms_d, s->atoms_h.data(), sizeof(Atom) * s->n_atoms, hipMemcpyHostToDevice, s->st));
+  // the S2 fields with their defaults resolved against the current Z
+  // (molecule.js:22-25, 1887-1892: Z_nuc || Z, initZeff || Z, initRcut || 1e6)
+  std::vector<AtomS2> ext(s->n_atoms);
+  s->any_split = false;
+  for (int a = 0; a < s->n_atoms; ++a) {
+    const rq_atom_s2& x = s->ext_h[a];
+    const float Z = s->atoms_h[a].Z;
+    ext[a] = AtomS2{x.Z_nuc > 0.0f ? x.Z_nuc : Z, x.init_zeff > 0.0f ? x.init_zeff : Z,
+                    x.init_rcut > 0.0f ? x.init_rcut : 1e6f, x.split_type, x.split_idx,
+                    x.split_axis[0], x.split_axis[1], x.split_axis[2], x.split_rot};
+    s->any_split |= x.split_type > 1;
+  }
+  HIPCHK(s, hipMemcpyAsync(s->ext_d, ext.data(), sizeof(AtomS2) * s->n_atoms, hipMemcpyHostToDevice, s->st));
+  HIPCHK(s, hipStreamSynchronize(s->st));
   s->best_atom_stale = true;
   std::vector<float> Z(s->n_atoms);
   for (int a = 0; a < s->n_atoms; ++a) Z[a] = s->atoms_h[a].Z;
@@ -600,8 +619,42 @@ int pother_direct_step(rq_solver* s, bool fuse_itp, bool* did_itp) {
   return RQ_OK;
 }
 
+// molecule.js direct Pother, its own scheme (molecule.js:2833-2858): for
+// each domain m with Z != 0, rho_other_m, kDirectSweeps damped sweeps of the
+// persistent P_m (ping-pong with P2, whose faces are zero like P_m's), and
+// Pother = P_m on the nodes labelled m.  The rhs is (h^2 2pi) rho, the
+// page's product order.
+int pother_sweeps_step(rq_solver* s) {
+  const float alpha = s->h2 * s->two_pi;
+  for (int m = 0; m < s->n_atoms; ++m) {
+    if (s->atoms_h[m].Z == 0.0f) continue;
+    launch_rho_other(s->st, s->tile, s->S, s->U, s->lab, m, alpha, 1.0f, s->rhs_dom);
+    float* x = s->Pdom[m];
+    float* y = s->P2;
+    for (int it = 0; it < kDirectSweeps; ++it) {
+      Timed t(s, F_SMOOTH);
+      launch_smooth_coarse(s->st, s->tile, s->S, x, y, s->rhs_dom, s->p.smooth_a, s->p.smooth_c);
+      std::swap(x, y);
+    }
+    if (x != s->Pdom[m]) HIPCHK(s, hipMemcpyAsync(s->Pdom[m], x, s->S3 * sizeof(float), hipMemcpyDeviceToDevice, s->st));
+    launch_copy_for_label(s->st, s->tile, s->S, s->Pdom[m], s->lab, m, s->Pother);
+  }
+  LAUNCHCHK(s);
+  return RQ_OK;
+}
+
+// the potential the ITP update, the energy and the grad-P forces read
+const float* itp_potential(const rq_solver* s) {
+  const int pm = s->p.pother_mode;
+  return (pm == RQ_POTHER_FRAME || pm == RQ_POTHER_DIRECT_SWEEPS) ? s->Pother : s->P;
+}
+const float* gradp_potential(const rq_solver* s) {
+  return s->p.pother_mode == RQ_POTHER_DIRECT_SWEEPS ? s->Pother : s->P;
+}
+
 int poisson_step(rq_solver* s, bool twogrid_due, bool fuse_itp = false, bool* did_itp = nullptr) {
   if (did_itp) *did_itp = false;
+  if (s->p.pother_mode == RQ_POTHER_DIRECT_SWEEPS) return pother_sweeps_step(s);
   if (s->p.pother_mode == RQ_POTHER_DIRECT) return pother_direct_step(s, fuse_itp && s->fused, did_itp);
   if (s->p.poisson_mode == RQ_POISSON_MULTIGRID) return vcycle_fine(s, fuse_itp, did_itp);
   if (int rc = smooth_fine_n(s, s->p.smooth_a, s->p.smooth_c, s->p.jacobi_sweeps)) return rc;
@@ -612,7 +665,7 @@ int poisson_step(rq_solver* s, bool twogrid_due, bool fuse_itp = false, bool* di
 int update_step(rq_solver* s) {
   {
     Timed t(s, F_ITP);
-    const float* Pv = (s->p.pother_mode == RQ_POTHER_FRAME) ? s->Pother : s->P;
+    const float* Pv = itp_potential(s);
     launch_itp(s->st, s->tile, s->S, s->U, s->U2, s->K, Pv, s->lab, s->half_dv, s->dtv);
   }
   LAUNCHCHK(s);
@@ -652,7 +705,8 @@ int boundary_step(rq_solver* s) {
     } else {
       bool any_lock = false;
       for (int32_t g : s->lock_h) any_lock |= (g > 0);
-      launch_levelset(s->st, s->tile, s->S, s->h, s->atoms_d, s->n_atoms, any_lock ? s->lock_d : nullptr, s->lab,
+      launch_levelset(s->st, s->tile, s->S, s->h, s->atoms_d, s->any_split ? s->ext_d : nullptr, s->n_atoms,
+                      any_lock ? s->lock_d : nullptr, s->lab,
                       s->lab2, s->U, s->W, s->p.ls_dt_w, s->p.ls_curv_reg, s->p.rc_dirichlet);
     }
   }
@@ -691,6 +745,7 @@ void free_all(rq_solver* s) {
     if (p) (void)hipFree(p);
   };
   fr(s->atoms_d);
+  fr(s->ext_d);
   fr(s->best_atom_d);
   fr(s->Z_d);
   fr(s->lock_d);
@@ -1120,7 +1175,7 @@ int rq_create(const rq_params* pp, const rq_atom* atoms, int32_t n_atoms, rq_sol
   if (p.boundary_mode < 0 || p.boundary_mode > 2) return fail(nullptr, RQ_EINVAL, "bad boundary_mode");
   if (p.init_mode < 0 || p.init_mode > 1) return fail(nullptr, RQ_EINVAL, "bad init_mode");
   if (p.energy_mode < 0 || p.energy_mode > 1) return fail(nullptr, RQ_EINVAL, "bad energy_mode");
-  if (p.pother_mode < 0 || p.pother_mode > 2) return fail(nullptr, RQ_EINVAL, "bad pother_mode");
+  if (p.pother_mode < 0 || p.pother_mode > 3) return fail(nullptr, RQ_EINVAL, "bad pother_mode");
   if (p.sic_interval < 0) return fail(nullptr, RQ_EINVAL, "negative sic_interval");
   if (p.sic_interval > 0 && p.pother_mode != RQ_POTHER_FRAME)
     return fail(nullptr, RQ_EINVAL, "sic_interval needs pother_mode FRAME");
@@ -1186,7 +1241,13 @@ int rq_create(const rq_params* pp, const rq_atom* atoms, int32_t n_atoms, rq_sol
   bool ok = hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking) == hipSuccess;
   const size_t fb = s->S3 * sizeof(float), lb = s->S3 * sizeof(uint16_t);
   ok = ok && alloc((void**)&s->atoms_d, sizeof(Atom) * n_atoms) && alloc((void**)&s->Z_d, sizeof(float) * n_atoms) &&
-       alloc((void**)&s->lock_d, sizeof(int32_t) * n_atoms);
+       alloc((void**)&s->lock_d, sizeof(int32_t) * n_atoms) && alloc((void**)&s->ext_d, sizeof(AtomS2) * n_atoms);
+  {
+    rq_atom_s2 dflt;
+    std::memset(&dflt, 0, sizeof dflt);
+    dflt.split_axis[2] = 1.0f;  // molecule.js SPLIT_AXIS default [0, 0, 1]
+    s->ext_h.assign(n_atoms, dflt);
+  }
   {
     // cells of about 2 bohr (at least 4 nodes) for the score rule's search
     AtomCells& c = s->cells;
@@ -1255,7 +1316,7 @@ int rq_create(const rq_params* pp, const rq_atom* atoms, int32_t n_atoms, rq_sol
     ok = ok && hipMemcpy(s->dst_S, Sm.data(), sizeof(double) * Sm.size(), hipMemcpyHostToDevice) == hipSuccess &&
          hipMemcpy(s->dst_rden, rd.data(), sizeof(double) * rd.size(), hipMemcpyHostToDevice) == hipSuccess;
   }
-  if (ok && p.pother_mode == RQ_POTHER_DIRECT) {
+  if (ok && (p.pother_mode == RQ_POTHER_DIRECT || p.pother_mode == RQ_POTHER_DIRECT_SWEEPS)) {
     const double bytes = (double)(n_atoms + 1) * (double)s->S3 * sizeof(float);
     if (bytes > 64.0 * (1ull << 30)) {
       free_all(s);
@@ -1273,6 +1334,7 @@ int rq_create(const rq_params* pp, const rq_atom* atoms, int32_t n_atoms, rq_sol
     // concurrent contexts: one per charged domain up to kDomStreams (RQ_DOM_STREAMS overrides; <= 1: sequential)
     int n_ctx = 0;
     for (int a = 0; a < n_atoms; ++a) n_ctx += atoms[a].Z > 0.0f;
+    if (p.pother_mode == RQ_POTHER_DIRECT_SWEEPS) n_ctx = 0;  // sweeps run on the handle's stream
     n_ctx = std::min(n_ctx, kDomStreams);
     if (const char* ds = std::getenv("RQ_DOM_STREAMS")) n_ctx = std::min(n_ctx, std::max(0, std::atoi(ds)));
     if (ok && n_ctx > 1) {
@@ -1294,7 +1356,7 @@ int rq_create(const rq_params* pp, const rq_atom* atoms, int32_t n_atoms, rq_sol
       }
     }
   }
-  if (ok && p.pother_mode == RQ_POTHER_FRAME) {
+  if (ok && (p.pother_mode == RQ_POTHER_FRAME || p.pother_mode == RQ_POTHER_DIRECT_SWEEPS)) {
     ok = alloc((void**)&s->Pother, fb) && hipMemsetAsync(s->Pother, 0, fb, s->st) == hipSuccess;
     if (p.sic_interval > 0)
       for (float** q : {&s->sic_x, &s->sic_x2, &s->sic_g})
@@ -1358,6 +1420,21 @@ int rq_get_atoms(const rq_solver* h, rq_atom* atoms, int32_t n) {
   return RQ_OK;
 }
 
+int rq_set_atoms_s2(rq_solver* h, const rq_atom_s2* ext, int32_t n) {
+  if (!h || !ext) return fail(h, RQ_EINVAL, "null argument");
+  if (n != h->n_atoms) return fail(h, RQ_EINVAL, "n_atoms %d != %d", n, h->n_atoms);
+  for (int a = 0; a < n; ++a) {
+    const rq_atom_s2& x = ext[a];
+    if (x.split_type < 0 || x.split_type > 4)
+      return fail(h, RQ_EINVAL, "atom %d: split_type %d (0/1 sphere, 2 hemi, 3 third, 4 none)", a, x.split_type);
+    if (x.split_idx < 0 || x.split_idx > 2) return fail(h, RQ_EINVAL, "atom %d: split_idx %d", a, x.split_idx);
+    if (!std::isfinite(x.Z_nuc) || !std::isfinite(x.init_zeff) || !std::isfinite(x.init_rcut))
+      return fail(h, RQ_EINVAL, "atom %d: non-finite S2 field", a);
+  }
+  h->ext_h.assign(ext, ext + n);
+  return upload_atoms(h);
+}
+
 int rq_set_lock_groups(rq_solver* h, const int32_t* group, int32_t n) {
   if (!h || !group || n != h->n_atoms) return fail(h, RQ_EINVAL, "lock groups need n_atoms entries");
   h->lock_h.assign(group, group + n);
@@ -1368,7 +1445,7 @@ int rq_set_lock_groups(rq_solver* h, const int32_t* group, int32_t n) {
 
 int rq_init_trial(rq_solver* h) {
   if (!h) return fail(nullptr, RQ_EINVAL, "null handle");
-  launch_init_trial(h->st, h->p.init_mode, h->S, h->h, h->h2, h->atoms_d, h->n_atoms, h->U, h->U2, h->lab, h->K,
+  launch_init_trial(h->st, h->p.init_mode, h->S, h->h, h->h2, h->atoms_d, h->ext_d, h->n_atoms, h->U, h->U2, h->lab, h->K,
                     h->bestU, h->W);
   LAUNCHCHK(h);
   if (h->p.rc_dirichlet) {
@@ -1378,6 +1455,17 @@ int rq_init_trial(rq_solver* h) {
   HIPCHK(h, hipMemcpyAsync(h->lab2, h->lab, h->S3 * sizeof(uint16_t), hipMemcpyDeviceToDevice, h->st));
   HIPCHK(h, hipMemsetAsync(h->P, 0, h->S3 * sizeof(float), h->st));
   HIPCHK(h, hipMemsetAsync(h->P2, 0, h->S3 * sizeof(float), h->st));
+  if (h->p.pother_mode == RQ_POTHER_DIRECT_SWEEPS) {
+    // molecule.js:2678-2693: Pother cleared, then every domain with Z != 0
+    // starts from the others' 0.5/r and adds it into Pother
+    HIPCHK(h, hipMemsetAsync(h->Pother, 0, h->S3 * sizeof(float), h->st));
+    for (int m = 0; m < h->n_atoms; ++m) {
+      if (h->atoms_h[m].Z == 0.0f) continue;
+      launch_pdirect_init(h->st, h->tile, h->S, h->h, h->h2, h->atoms_d, h->n_atoms, m,
+                          h->p.init_mode == RQ_INIT_S2, h->Pdom[m], h->Pother);
+    }
+    LAUNCHCHK(h);
+  }
   h->initialized = true;
   return RQ_OK;
 }
@@ -1423,7 +1511,7 @@ int rq_boundary(rq_solver* h) {
 
 int rq_recompute_k(rq_solver* h) {
   if (!h) return fail(nullptr, RQ_EINVAL, "null handle");
-  launch_recompute_k(h->st, h->p.init_mode, h->S, h->h, h->h2, h->atoms_d, h->n_atoms, h->K);
+  launch_recompute_k(h->st, h->p.init_mode, h->S, h->h, h->h2, h->atoms_d, h->ext_d, h->n_atoms, h->K);
   LAUNCHCHK(h);
   if (h->p.rc_dirichlet) {
     launch_rc_flags(h->st, h->S, h->h, h->atoms_d, h->n_atoms, h->lab);
@@ -1612,7 +1700,7 @@ int rq_energy(rq_solver* h, double* out6, double* per_domain) {
   zero_fx(h, h->red, h->red_len);
   {
     Timed t(h, F_ENERGY);
-    const float* Pv = (h->p.pother_mode == RQ_POTHER_FRAME) ? h->Pother : h->P;
+    const float* Pv = itp_potential(h);
     launch_energy(h->st, h->tile, h->p.energy_mode, h->S, h->h, h->h3, h->U, h->K, Pv, h->lab, h->atoms_d, nd, per,
                   dip);
   }
@@ -1694,7 +1782,7 @@ int rq_forces_gradp(rq_solver* h, int32_t radius, double* out3n) {
   HIPCHK(h, hipMallocAsync((void**)&buf, sizeof(double) * h->n_atoms * 3, h->st));
   {
     Timed t(h, F_FORCE);
-    launch_forces_gradp(h->st, h->S, h->h, h->P, h->atoms_d, h->n_atoms, R, buf);
+    launch_forces_gradp(h->st, h->S, h->h, gradp_potential(h), h->atoms_d, h->ext_d, h->n_atoms, R, buf);
   }
   LAUNCHCHK(h);
   HIPCHK(h, hipMemcpyAsync(out3n, buf, sizeof(double) * h->n_atoms * 3, hipMemcpyDeviceToHost, h->st));

@@ -23,6 +23,7 @@ constexpr int kResResMarchCJ = 2;       // coarse j-columns per residual+restric
 constexpr int kResResMarchCP = 2;       // coarse planes restricted per barrier round of that march
 constexpr int kSmooth2Blocks = 8192;    // target workgroup count of the two-sweep march
 constexpr int kSmooth2MinS = 320;       // smallest level (nodes per axis) that runs sweep pairs as that march
+constexpr int kDirectSweeps = 10;       // damped sweeps per step of each direct potential (molecule.js:2835 JACOBI_DIRECT)
 constexpr int kDomStreams = 4;          // concurrent per-domain V-cycles of the direct potential (one stream each)
 constexpr int kReduceIch = 24;          // planes per thread of the per-domain reductions (fewer global atomics)
 // largest fine grid (nodes per axis) whose last post-sweep carries the ITP
@@ -75,6 +76,15 @@ struct Atom {
   float i, j, k, Z, rc;
 };
 
+// molecule.js per-atom fields of the S2 semantics (rq_atom_s2), defaults
+// resolved on the host: Zn = Z_nuc > 0 ? Z_nuc : Z, zeff = init_zeff > 0 ?
+// init_zeff : Z, rcut = init_rcut > 0 ? init_rcut : 1e6.
+struct AtomS2 {
+  float Zn, zeff, rcut;
+  int st, sidx;
+  float ax, ay, az, rot;
+};
+
 // Uniform cell list over the grid for nearest-atom searches.
 struct AtomCells {
   int cs = 4;          // cell edge in nodes
@@ -91,9 +101,10 @@ enum Family {
 };
 
 // ---- launchers (rq_kernels.hip) ----
-void launch_init_trial(hipStream_t st, int mode, int S, float h, float h2, const Atom* atoms, int n_atoms,
-                       float* U, float* U2, uint16_t* lab, float* K, float* bestU, float* W);
-void launch_recompute_k(hipStream_t st, int mode, int S, float h, float h2, const Atom* atoms, int n_atoms, float* K);
+void launch_init_trial(hipStream_t st, int mode, int S, float h, float h2, const Atom* atoms, const AtomS2* ext,
+                       int n_atoms, float* U, float* U2, uint16_t* lab, float* K, float* bestU, float* W);
+void launch_recompute_k(hipStream_t st, int mode, int S, float h, float h2, const Atom* atoms, const AtomS2* ext,
+                        int n_atoms, float* K);
 void launch_rc_flags(hipStream_t st, int S, float h, const Atom* atoms, int n_atoms, uint16_t* lab);
 void launch_itp(hipStream_t st, const Tile& t, int S, const float* U, float* Un, const float* K, const float* P,
                 const uint16_t* lab, float half_dv, float dtv);
@@ -125,6 +136,12 @@ void launch_down2(hipStream_t st, int S, const float* g, float* e_out, float* gc
 void launch_display_slice(hipStream_t st, int S, int sk, const float* U, const uint16_t* lab, const float* K,
                           const Atom* atoms, int n_atoms, float* out);
 // direct per-domain potential: masked right-hand side and the label gather
+// molecule.js direct Pother (RQ_POTHER_DIRECT_SWEEPS): the start of P_m
+// (also added into Pother) and the copy of P_m onto the nodes labelled m
+void launch_pdirect_init(hipStream_t st, const Tile& t, int S, float h, float h2, const Atom* atoms, int n_atoms,
+                         int m, int floor_pos, float* Pm, float* Pother);
+void launch_copy_for_label(hipStream_t st, const Tile& t, int S, const float* Pm, const uint16_t* lab, int m,
+                           float* Pother);
 void launch_rho_other(hipStream_t st, const Tile& t, int S, const float* U, const uint16_t* lab, int m, float alpha,
                       float beta, float* rhs);
 void launch_rho_self(hipStream_t st, const Tile& t, int S, const float* U, const uint16_t* lab, int m, float alpha,
@@ -170,16 +187,16 @@ void launch_boundary_score(hipStream_t st, const Tile& t, int S, float h, float 
 // strict maximum of Z / sqrt(r^2 + h^2) in index order), the map the rule
 // reads; nodes of blocks with too many candidate atoms are left unknown.
 void launch_score_map(hipStream_t st, int S, float h, float h2, const Atom* atoms, int n_atoms, uint16_t* best_atom);
-void launch_levelset(hipStream_t st, const Tile& t, int S, float h, const Atom* atoms, int n_atoms,
-                     const int32_t* lock_group, const uint16_t* lin, uint16_t* lout, const float* U, float* W,
-                     float dt_w, float curv_reg, int rc_flags);
+void launch_levelset(hipStream_t st, const Tile& t, int S, float h, const Atom* atoms, const AtomS2* ext,
+                     int n_atoms, const int32_t* lock_group, const uint16_t* lin, uint16_t* lout, const float* U,
+                     float* W, float dt_w, float curv_reg, int rc_flags);
 void launch_energy(hipStream_t st, const Tile& t, int mode, int S, float h, float h3, const float* U,
                    const float* K, const float* P, const uint16_t* lab, const Atom* atoms, int n_dom,
                    Fx* per_dom /* n_dom*3 sums */, Fx* dip /* 3 sums */);
 void launch_forces_hf(hipStream_t st, int S, float h, float h2, float h3, const float* U, const Atom* atoms,
                       int first, int count, Fx* out3 /* 3 count sums */);
-void launch_forces_gradp(hipStream_t st, int S, float h, const float* P, const Atom* atoms, int n_atoms, int radius,
-                         double* out3n);
+void launch_forces_gradp(hipStream_t st, int S, float h, const float* P, const Atom* atoms, const AtomS2* ext,
+                         int n_atoms, int radius, double* out3n);
 void launch_square(hipStream_t st, int S, const float* U, float* rho);
 void launch_label_u16_to_i32(hipStream_t st, size_t n, const uint16_t* in, int32_t* out);
 void launch_label_i32_to_u16(hipStream_t st, size_t n, const int32_t* in, uint16_t* out);

@@ -52,14 +52,53 @@ __device__ __forceinline__ bool march_coords(int n, int ich, March& m) {
   return m.k <= n && m.j <= n;
 }
 
+// Angular sector test of a split atom (molecule.js:194-225 inSplitAtom):
+// d = node - nucleus (bohr), r = |d|.  HEMI owns the half-space axis.d < 0
+// (split_idx 0) or >= 0 (1); THIRD owns one 120-degree sector about the
+// axis, measured from the in-plane frame (e1, axis x e1) less split_rot.
+__device__ __forceinline__ bool in_split(float dx, float dy, float dz, float r, const AtomS2& e) {
+  if (e.st <= 1 || r < 1e-6f) return true;
+  const float axL = fmaxf(sqrtf(e.ax * e.ax + e.ay * e.ay + e.az * e.az), 1e-12f);
+  const float a0 = e.ax / axL, a1 = e.ay / axL, a2 = e.az / axL;
+  const float dotp = dx * a0 + dy * a1 + dz * a2;
+  if (e.st == 2) return e.sidx == 0 ? dotp < 0.0f : dotp >= 0.0f;
+  if (e.st == 3) {
+    const float px = dx - dotp * a0, py = dy - dotp * a1, pz = dz - dotp * a2;
+    float e1x, e1y, e1z;
+    if (fabsf(a2) < 0.9f) {
+      e1x = a1;
+      e1y = -a0;
+      e1z = 0.0f;
+    } else {
+      e1x = 0.0f;
+      e1y = a2;
+      e1z = -a1;
+    }
+    const float e1L = fmaxf(sqrtf(e1x * e1x + e1y * e1y + e1z * e1z), 1e-12f);
+    e1x = e1x / e1L;
+    e1y = e1y / e1L;
+    e1z = e1z / e1L;
+    const float e2x = a1 * e1z - a2 * e1y, e2y = a2 * e1x - a0 * e1z, e2z = a0 * e1y - a1 * e1x;
+    const float pe1 = px * e1x + py * e1y + pz * e1z;
+    const float pe2 = px * e2x + py * e2y + pz * e2z;
+    float th = atan2f(pe2, pe1) - e.rot;
+    if (th < -3.14159265f) th = th + 6.28318530f;
+    if (th > 3.14159265f) th = th - 6.28318530f;
+    // a negative quotient converts to sector 0, as WGSL's saturating u32()
+    const unsigned sector = (unsigned)fmaxf(floorf((th + 3.14159265f) / 2.09439510f), 0.0f) % 3u;
+    return sector == (unsigned)e.sidx;
+  }
+  return true;
+}
+
 // --------------------------------------------------------------------------
 // Initialisation: trial functions, labels, K (cloud/solver.py:16-59 and
 // molecule.js:1377-1449).
 // --------------------------------------------------------------------------
 __global__ void k_init_trial(int mode, int S, float h, float h2, float r_sing, const Atom* __restrict__ atoms,
-                             int n_atoms, float* __restrict__ U, float* __restrict__ U2,
-                             uint16_t* __restrict__ lab, float* __restrict__ K, float* __restrict__ bestU,
-                             float* __restrict__ W) {
+                             const AtomS2* __restrict__ ext, int n_atoms, float* __restrict__ U,
+                             float* __restrict__ U2, uint16_t* __restrict__ lab, float* __restrict__ K,
+                             float* __restrict__ bestU, float* __restrict__ W) {
   const int S2 = S * S;
   const int S3 = S2 * S;
   const int id = blockIdx.x * blockDim.x + threadIdx.x;
@@ -83,33 +122,48 @@ __global__ void k_init_trial(int mode, int S, float h, float h2, float r_sing, c
   float bU = 0.0f;
   int bestN = 0;
   float Kv = 0.0f;
-  for (int n = 0; n < n_atoms; ++n) {
-    const float Za = atoms[n].Z;
-    if (!(Za > 0.0f)) continue;
-    const float rc = atoms[n].rc;
-    float ax = atoms[n].i, ay = atoms[n].j, az = atoms[n].k;
-    if (mode == RQ_INIT_S2) {  // molecule.js posI/J/K are u32 (truncated)
-      ax = floorf(ax);
-      ay = floorf(ay);
-      az = floorf(az);
+  if (mode == RQ_INIT_S2) {
+    // molecule.js gpuInitAccum (:1401-1431): u32 positions, K from the
+    // nuclear charge Z_nuc (bare nuclei too), the trial of decay init_zeff
+    // only within init_rcut, on atoms with electrons and inside the atom's
+    // split sector
+    for (int n = 0; n < n_atoms; ++n) {
+      const float Za = atoms[n].Z;
+      const AtomS2 e = ext[n];
+      const float Zn = e.Zn;
+      if (Za <= 0.0f && Zn <= 0.0f) continue;
+      const float rc = atoms[n].rc;
+      const float dx = xi - floorf(atoms[n].i) * h;
+      const float dy = yj - floorf(atoms[n].j) * h;
+      const float dz = zk - floorf(atoms[n].k) * h;
+      const float r2 = dx * dx + dy * dy + dz * dz;
+      const float r = (rc > 0.0f) ? fmaxf(sqrtf(r2 + 0.04f * h2), rc) : fmaxf(sqrtf(r2), r_sing);
+      Kv += Zn / r;
+      const float Ze = e.zeff;
+      const float rReal = sqrtf(r2);
+      const bool ok = !(rReal > e.rcut || Za <= 0.0f) && in_split(dx, dy, dz, rReal, e);
+      const float ut = ok ? Ze * Ze * inv_sqrt_pi * expf(-Ze * r) : 0.0f;
+      if (ut > bU) {
+        bU = ut;
+        bestN = n;
+      }
     }
-    const float dx = xi - ax * h;
-    const float dy = yj - ay * h;
-    const float dz = zk - az * h;
-    const float r2 = dx * dx + dy * dy + dz * dz;
-    float r;
-    if (rc > 0.0f) {
-      r = fmaxf(sqrtf(r2 + 0.04f * h2), rc);
-    } else if (mode == RQ_INIT_S2) {
-      r = fmaxf(sqrtf(r2), r_sing);
-    } else {
-      r = sqrtf(r2 + h2);
-    }
-    Kv += Za / r;
-    const float ut = Za * Za * inv_sqrt_pi * expf(-Za * r);
-    if (ut > bU) {
-      bU = ut;
-      bestN = n;
+  } else {
+    for (int n = 0; n < n_atoms; ++n) {
+      const float Za = atoms[n].Z;
+      if (!(Za > 0.0f)) continue;
+      const float rc = atoms[n].rc;
+      const float dx = xi - atoms[n].i * h;
+      const float dy = yj - atoms[n].j * h;
+      const float dz = zk - atoms[n].k * h;
+      const float r2 = dx * dx + dy * dy + dz * dz;
+      const float r = (rc > 0.0f) ? fmaxf(sqrtf(r2 + 0.04f * h2), rc) : sqrtf(r2 + h2);
+      Kv += Za / r;
+      const float ut = Za * Za * inv_sqrt_pi * expf(-Za * r);
+      if (ut > bU) {
+        bU = ut;
+        bestN = n;
+      }
     }
   }
   K[id] = Kv;
@@ -120,7 +174,7 @@ __global__ void k_init_trial(int mode, int S, float h, float h2, float r_sing, c
 }
 
 __global__ void k_recompute_k(int mode, int S, float h, float h2, float r_sing, const Atom* __restrict__ atoms,
-                              int n_atoms, float* __restrict__ K) {
+                              const AtomS2* __restrict__ ext, int n_atoms, float* __restrict__ K) {
   const int S2 = S * S;
   const int id = blockIdx.x * blockDim.x + threadIdx.x;
   if (id >= S2 * S) return;
@@ -132,31 +186,35 @@ __global__ void k_recompute_k(int mode, int S, float h, float h2, float r_sing, 
     K[id] = 0.0f;
     return;
   }
-  const float xi = (float)i * h, yj = (float)j * h, zk = (float)k * h;
   float Kv = 0.0f;
-  for (int n = 0; n < n_atoms; ++n) {
-    const float Za = atoms[n].Z;
-    if (!(Za > 0.0f)) continue;
-    const float rc = atoms[n].rc;
-    float ax = atoms[n].i, ay = atoms[n].j, az = atoms[n].k;
-    if (mode == RQ_INIT_S2) {
-      ax = floorf(ax);
-      ay = floorf(ay);
-      az = floorf(az);
+  if (mode == RQ_INIT_S2) {
+    // molecule.js recomputeK (:1337-1366): offsets formed in grid units, the
+    // nuclear charge Z_nuc, r~ = max(r, rc) or max(r, 2h) (no 0.04 h^2 term)
+    for (int n = 0; n < n_atoms; ++n) {
+      const float Za = atoms[n].Z;
+      const float Zn = ext[n].Zn;
+      if (Za <= 0.0f && Zn <= 0.0f) continue;
+      const float rc = atoms[n].rc;
+      const float di = ((float)i - floorf(atoms[n].i)) * h;
+      const float dj = ((float)j - floorf(atoms[n].j)) * h;
+      const float dk = ((float)k - floorf(atoms[n].k)) * h;
+      const float r2 = di * di + dj * dj + dk * dk;
+      const float r = (rc > 0.0f) ? fmaxf(sqrtf(r2), rc) : fmaxf(sqrtf(r2), r_sing);
+      Kv += Zn / r;
     }
-    const float dx = xi - ax * h;
-    const float dy = yj - ay * h;
-    const float dz = zk - az * h;
-    const float r2 = dx * dx + dy * dy + dz * dz;
-    float r;
-    if (rc > 0.0f) {
-      r = fmaxf(sqrtf(r2 + 0.04f * h2), rc);
-    } else if (mode == RQ_INIT_S2) {
-      r = fmaxf(sqrtf(r2), r_sing);
-    } else {
-      r = sqrtf(r2 + h2);
+  } else {
+    const float xi = (float)i * h, yj = (float)j * h, zk = (float)k * h;
+    for (int n = 0; n < n_atoms; ++n) {
+      const float Za = atoms[n].Z;
+      if (!(Za > 0.0f)) continue;
+      const float rc = atoms[n].rc;
+      const float dx = xi - atoms[n].i * h;
+      const float dy = yj - atoms[n].j * h;
+      const float dz = zk - atoms[n].k * h;
+      const float r2 = dx * dx + dy * dy + dz * dz;
+      const float r = (rc > 0.0f) ? fmaxf(sqrtf(r2 + 0.04f * h2), rc) : sqrtf(r2 + h2);
+      Kv += Za / r;
     }
-    Kv += Za / r;
   }
   K[id] = Kv;
 }
@@ -972,6 +1030,48 @@ __global__ __launch_bounds__(256) void k_rho_other(int S, int ich, const float* 
   }
 }
 
+// molecule.js initPdirect (:624-662): the start of domain m's persistent
+// direct potential, P_m = sum over the other atoms with Z > 0 of
+// 0.5 / sqrt(r^2 + h^2) on the interior (positions as u32 with floor_pos),
+// also added into the frame potential.
+__global__ __launch_bounds__(256) void k_pdirect_init(int S, int ich, float h, float h2, const Atom* __restrict__ atoms,
+                                                      int n_atoms, int m, int floor_pos, float* __restrict__ Pm,
+                                                      float* __restrict__ Pother) {
+  March q;
+  if (!march_coords(S - 2, ich, q)) return;
+  const int S2 = S * S;
+  const float yj = (float)q.j * h, zk = (float)q.k * h;
+  for (int i = q.i0; i < q.i1; ++i) {
+    const int id = i * S2 + q.j * S + q.k;
+    const float xi = (float)i * h;
+    float val = 0.0f;
+    for (int n = 0; n < n_atoms; ++n) {
+      if (atoms[n].Z <= 0.0f || n == m) continue;
+      const float ai = floor_pos ? floorf(atoms[n].i) : atoms[n].i;
+      const float aj = floor_pos ? floorf(atoms[n].j) : atoms[n].j;
+      const float ak = floor_pos ? floorf(atoms[n].k) : atoms[n].k;
+      const float dx = xi - ai * h, dy = yj - aj * h, dz = zk - ak * h;
+      const float r = sqrtf(dx * dx + dy * dy + dz * dz + h2);
+      val += 0.5f / r;
+    }
+    Pm[id] = val;
+    Pother[id] += val;
+  }
+}
+
+// molecule.js copyPotherForLabel (:664-688): Pother = P_m on the nodes labelled m
+__global__ __launch_bounds__(256) void k_copy_for_label(int S, int ich, const float* __restrict__ Pm,
+                                                        const uint16_t* __restrict__ lab, int m,
+                                                        float* __restrict__ Pother) {
+  March q;
+  if (!march_coords(S - 2, ich, q)) return;
+  const int S2 = S * S;
+  for (int i = q.i0; i < q.i1; ++i) {
+    const int id = i * S2 + q.j * S + q.k;
+    if ((lab[id] & kLabelMask) == m) Pother[id] = Pm[id];
+  }
+}
+
 // rhs of domain m's self-potential: (alpha u^2) beta where label == m, else 0
 // (molecule.js:572-593 computeRhoSelf, in the build's rhs form)
 __global__ __launch_bounds__(256) void k_rho_self(int S, int ich, const float* __restrict__ U,
@@ -1422,7 +1522,20 @@ __global__ __launch_bounds__(256) void k_boundary_score(int S, int ich, float h,
 }
 
 // Level-set rule (molecule.js:426-540; angular-sector constraints omitted).
+// sector test of node (i, j, k) for domain n (molecule.js:438-443 cellSectorOK)
+__device__ __forceinline__ bool cell_sector_ok(int i, int j, int k, int n, float h, const Atom* __restrict__ atoms,
+                                               const AtomS2* __restrict__ ext, int n_atoms) {
+  if (!ext || n >= n_atoms) return true;
+  const AtomS2 e = ext[n];
+  if (e.st <= 1) return true;
+  const float dx = ((float)i - floorf(atoms[n].i)) * h;
+  const float dy = ((float)j - floorf(atoms[n].j)) * h;
+  const float dz = ((float)k - floorf(atoms[n].k)) * h;
+  return in_split(dx, dy, dz, sqrtf(dx * dx + dy * dy + dz * dz), e);
+}
+
 __global__ __launch_bounds__(256) void k_levelset(int S, int ich, float h, const Atom* __restrict__ atoms,
+                                                  const AtomS2* __restrict__ ext,
                                                   int n_atoms, const int32_t* __restrict__ lock_group,
                                                   const uint16_t* __restrict__ lin, uint16_t* __restrict__ lout,
                                                   const float* __restrict__ U, float* __restrict__ W, float dt_w,
@@ -1480,7 +1593,25 @@ __global__ __launch_bounds__(256) void k_levelset(int S, int ich, float h, const
       newL = bestL;
       w = 0.1f;
     }
+    if (ext) {
+      // split siblings own only the nodes of their sector (molecule.js:521-536)
+      const bool myOK = cell_sector_ok(i, m.j, m.k, myL, h, atoms, ext, n_atoms);
+      if (!cell_sector_ok(i, m.j, m.k, newL, h, atoms, ext, n_atoms)) {
+        newL = myL;
+        w = fmaxf(w, 0.1f);
+        if (!myOK && bestL != myL && cell_sector_ok(i, m.j, m.k, bestL, h, atoms, ext, n_atoms)) {
+          newL = bestL;
+          w = 0.1f;
+        }
+      } else if (!myOK && newL == myL) {
+        if (bestL != myL && cell_sector_ok(i, m.j, m.k, bestL, h, atoms, ext, n_atoms)) {
+          newL = bestL;
+          w = 0.1f;
+        }
+      }
+    }
     W[id] = w;
+
     uint16_t out = (uint16_t)newL;
     if (newL == myL) {
       out = raw;
@@ -1680,14 +1811,17 @@ __global__ __launch_bounds__(256) void k_forces_hf(int S, float h, float h2, flo
 // cells around the (integer) nucleus position; central differences.  One
 // workgroup per atom; FP64 sums.
 __global__ __launch_bounds__(256) void k_forces_gradp(int S, float h, const float* __restrict__ P,
-                                                      const Atom* __restrict__ atoms, int n_atoms, int R,
+                                                      const Atom* __restrict__ atoms,
+                                                      const AtomS2* __restrict__ ext, int n_atoms, int R,
                                                       double* __restrict__ out) {
   __shared__ double red[4][4];
   const int a = blockIdx.x;
   if (a >= n_atoms) return;
   const Atom at = atoms[a];
   const int tid = threadIdx.x;
-  if (!(at.Z > 0.0f)) {
+  // the force charge: Z, or Z_nuc for a bare nucleus (molecule.js:1214)
+  const float ZA = (ext && at.Z <= 0.0f) ? ext[a].Zn : at.Z;
+  if (!(ZA > 0.0f)) {
     if (tid < 3) out[a * 3 + tid] = 0.0;
     return;
   }
@@ -1727,7 +1861,7 @@ __global__ __launch_bounds__(256) void k_forces_gradp(int S, float h, const floa
       c += red[w][3];
     }
     if (c > 0.0) s /= c;
-    out[a * 3 + tid] = 2.0 * (double)at.Z * s;
+    out[a * 3 + tid] = 2.0 * (double)ZA * s;
   }
 }
 
@@ -1738,19 +1872,19 @@ __global__ __launch_bounds__(256) void k_forces_gradp(int S, float h, const floa
 // ---------------------------------------------------------------------------
 static inline unsigned nblk(size_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
-void launch_init_trial(hipStream_t st, int mode, int S, float h, float h2, const Atom* atoms, int n_atoms,
-                       float* U, float* U2, uint16_t* lab, float* K, float* bestU, float* W) {
+void launch_init_trial(hipStream_t st, int mode, int S, float h, float h2, const Atom* atoms, const AtomS2* ext,
+                       int n_atoms, float* U, float* U2, uint16_t* lab, float* K, float* bestU, float* W) {
   const size_t S3 = (size_t)S * S * S;
   const float r_sing = 2.0f * h;
-  hipLaunchKernelGGL(k_init_trial, dim3(nblk(S3, 256)), dim3(256), 0, st, mode, S, h, h2, r_sing, atoms, n_atoms, U,
-                     U2, lab, K, bestU, W);
+  hipLaunchKernelGGL(k_init_trial, dim3(nblk(S3, 256)), dim3(256), 0, st, mode, S, h, h2, r_sing, atoms, ext,
+                     n_atoms, U, U2, lab, K, bestU, W);
 }
 
-void launch_recompute_k(hipStream_t st, int mode, int S, float h, float h2, const Atom* atoms, int n_atoms,
-                        float* K) {
+void launch_recompute_k(hipStream_t st, int mode, int S, float h, float h2, const Atom* atoms, const AtomS2* ext,
+                        int n_atoms, float* K) {
   const size_t S3 = (size_t)S * S * S;
-  hipLaunchKernelGGL(k_recompute_k, dim3(nblk(S3, 256)), dim3(256), 0, st, mode, S, h, h2, 2.0f * h, atoms, n_atoms,
-                     K);
+  hipLaunchKernelGGL(k_recompute_k, dim3(nblk(S3, 256)), dim3(256), 0, st, mode, S, h, h2, 2.0f * h, atoms, ext,
+                     n_atoms, K);
 }
 
 void launch_rc_flags(hipStream_t st, int S, float h, const Atom* atoms, int n_atoms, uint16_t* lab) {
@@ -1940,6 +2074,17 @@ static void down2_tiled(hipStream_t st, int S, const float* g, float* e_out, flo
 
 // 256 threads per workgroup on every level; smaller coarse tiles on smaller
 // levels keep a few nodes per thread (these launches are latency-bound)
+void launch_pdirect_init(hipStream_t st, const Tile& t, int S, float h, float h2, const Atom* atoms, int n_atoms,
+                         int m, int floor_pos, float* Pm, float* Pother) {
+  hipLaunchKernelGGL(k_pdirect_init, stencil_grid(S - 2, t), dim3(t.bx, t.by), 0, st, S, t.ich, h, h2, atoms, n_atoms,
+                     m, floor_pos, Pm, Pother);
+}
+
+void launch_copy_for_label(hipStream_t st, const Tile& t, int S, const float* Pm, const uint16_t* lab, int m,
+                           float* Pother) {
+  hipLaunchKernelGGL(k_copy_for_label, stencil_grid(S - 2, t), dim3(t.bx, t.by), 0, st, S, t.ich, Pm, lab, m, Pother);
+}
+
 void launch_rho_other(hipStream_t st, const Tile& t, int S, const float* U, const uint16_t* lab, int m, float alpha,
                       float beta, float* rhs) {
   hipLaunchKernelGGL(k_rho_other, stencil_grid(S - 2, t), dim3(t.bx, t.by), 0, st, S, t.ich, U, lab, m, alpha, beta,
@@ -2073,10 +2218,10 @@ void launch_boundary_score(hipStream_t st, const Tile& t, int S, float h, float 
                      best_atom);
 }
 
-void launch_levelset(hipStream_t st, const Tile& t, int S, float h, const Atom* atoms, int n_atoms,
-                     const int32_t* lock_group, const uint16_t* lin, uint16_t* lout, const float* U, float* W,
-                     float dt_w, float curv_reg, int rc_flags) {
-  hipLaunchKernelGGL(k_levelset, stencil_grid(S - 2, t), dim3(t.bx, t.by), 0, st, S, t.ich, h, atoms, n_atoms,
+void launch_levelset(hipStream_t st, const Tile& t, int S, float h, const Atom* atoms, const AtomS2* ext,
+                     int n_atoms, const int32_t* lock_group, const uint16_t* lin, uint16_t* lout, const float* U,
+                     float* W, float dt_w, float curv_reg, int rc_flags) {
+  hipLaunchKernelGGL(k_levelset, stencil_grid(S - 2, t), dim3(t.bx, t.by), 0, st, S, t.ich, h, atoms, ext, n_atoms,
                      lock_group, lin, lout, U, W, dt_w, curv_reg, rc_flags);
 }
 
@@ -2104,9 +2249,9 @@ void launch_forces_hf(hipStream_t st, int S, float h, float h2, float h3, const 
   hipLaunchKernelGGL(k_forces_hf, grid, dim3(64, 4), 0, st, S, h, h2, h3, U, atoms, first, count, out3);
 }
 
-void launch_forces_gradp(hipStream_t st, int S, float h, const float* P, const Atom* atoms, int n_atoms, int radius,
-                         double* out3n) {
-  hipLaunchKernelGGL(k_forces_gradp, dim3(n_atoms), dim3(256), 0, st, S, h, P, atoms, n_atoms, radius, out3n);
+void launch_forces_gradp(hipStream_t st, int S, float h, const float* P, const Atom* atoms, const AtomS2* ext,
+                         int n_atoms, int radius, double* out3n) {
+  hipLaunchKernelGGL(k_forces_gradp, dim3(n_atoms), dim3(256), 0, st, S, h, P, atoms, ext, n_atoms, radius, out3n);
 }
 
 void launch_square(hipStream_t st, int S, const float* U, float* rho) {

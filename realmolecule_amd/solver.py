@@ -21,9 +21,12 @@ Extra constructor keys (all optional; defaults reproduce cloud/solver.py):
 ``boundary`` ('score' | 'levelset' | 'off'), ``init`` ('s1' | 's2'),
 ``energy`` ('s1' | 's2'), ``rc_dirichlet`` (bool), ``dv``, ``graph`` (bool: replay
 step chunks from HIP graphs), ``bootstrap_poisson`` (Poisson steps in initialize, 100),
-``pother`` ('total' | 'direct' | 'frame': the potential the ITP update and the
-energy see) and ``sic_interval`` (self-interaction correction every n frames
-with 'frame').
+``pother`` ('total' | 'direct' | 'frame' | 'direct_sweeps': the potential the
+ITP update and the energy see) and ``sic_interval`` (self-interaction
+correction every n frames with 'frame').  Atom dicts may carry the
+molecule.js per-atom keys ``Z_nuc``, ``init_zeff``, ``init_rcut``,
+``split`` ('sphere' | 'hemi' | 'third'), ``split_idx``, ``split_axis``,
+``split_rot`` (used with ``init: 's2'``).
 """
 from __future__ import annotations
 
@@ -117,8 +120,8 @@ def make_params(NN, screen, config=None) -> L.RqParams:
     p.rc_dirichlet = int(bool(config.get("rc_dirichlet", False)))
     # 'total': cloud/solver.py's potential of all electrons; 'direct': the
     # molecule.js direct Pother (each domain sees the other domains' charge)
-    p.pother_mode = {"total": L.RQ_POTHER_TOTAL, "direct": L.RQ_POTHER_DIRECT,
-                     "frame": L.RQ_POTHER_FRAME}[config.get("pother", "total")]
+    p.pother_mode = {"total": L.RQ_POTHER_TOTAL, "direct": L.RQ_POTHER_DIRECT, "frame": L.RQ_POTHER_FRAME,
+                     "direct_sweeps": L.RQ_POTHER_DIRECT_SWEEPS}[config.get("pother", "total")]
     p.sic_interval = int(config.get("sic_interval", 0))
     if "flip_ratio" in config:
         p.flip_ratio = float(config["flip_ratio"])
@@ -127,6 +130,28 @@ def make_params(NN, screen, config=None) -> L.RqParams:
     if "ls_curv_reg" in config:
         p.ls_curv_reg = float(config["ls_curv_reg"])
     return p
+
+
+S2_KEYS = ("Z_nuc", "init_zeff", "init_rcut", "split", "split_idx", "split_axis", "split_rot")
+
+
+def atoms_s2_array(atoms):
+    """rq_atom_s2 rows from config atom dicts (molecule.js USER_Z_NUC,
+    USER_INIT_ZEFF, USER_INIT_RCUT, USER_SPLIT_*); missing keys take the
+    library defaults (0: Z_nuc = Z, init_zeff = Z, init_rcut = 1e6)."""
+    arr = (L.RqAtomS2 * len(atoms))()
+    for n, a in enumerate(atoms):
+        arr[n].Z_nuc = float(a.get("Z_nuc", 0.0))
+        arr[n].init_zeff = float(a.get("init_zeff", 0.0))
+        arr[n].init_rcut = float(a.get("init_rcut", 0.0))
+        st = a.get("split", 0)
+        arr[n].split_type = L.RQ_SPLIT[st] if isinstance(st, str) else int(st)
+        arr[n].split_idx = int(a.get("split_idx", 0))
+        ax = a.get("split_axis", (0.0, 0.0, 1.0))
+        for c in range(3):
+            arr[n].split_axis[c] = float(ax[c])
+        arr[n].split_rot = float(a.get("split_rot", 0.0))
+    return arr
 
 
 def atoms_array(atoms):
@@ -181,6 +206,8 @@ class RealQMSolver:
         code = self._lib.rq_create(C.byref(self._params), self._atoms_c(), self.nAtoms, C.byref(h))
         L.check(self._lib, None, code)
         self._h = h
+        if any(k in a for a in atoms for k in S2_KEYS):
+            self._chk(self._lib.rq_set_atoms_s2(self._h, atoms_s2_array(atoms), self.nAtoms))
         self._sched = L.RqSchedule()
         self._lib.rq_default_schedule(C.byref(self._sched))
         if config.get('graph'):

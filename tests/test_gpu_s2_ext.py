@@ -1,0 +1,150 @@
+"""GPU parity of the molecule.js per-atom fields (Z_nuc, initZeff, initRcut,
+angular splits: rq_set_atoms_s2), the S2 K refresh and molecule.js's own
+direct Pother (RQ_POTHER_DIRECT_SWEEPS) against the S2 oracle, same inputs.
+
+Kernels built from + - * / and sqrt are bit-exact against the FP32-faithful
+oracle; the trial's expf differs by at most a few ulp, and a node whose
+sector angle (atan2f) or trial values tie within rounding may take the
+other label, so labels are compared up to a handful of such nodes."""
+import numpy as np
+import pytest
+
+from oracle import s1, s2
+from realmolecule_amd.solver import RealQMSolver
+from tests.helpers import cube, ulp_diff
+
+pytestmark = pytest.mark.gpu
+F32 = np.float32
+
+
+def _rows_ext(NN):
+    c = NN // 2
+    rows = np.array([[c - 5, c, c, 2.0, 0.4],      # O-like, init_zeff 1.6
+                     [c + 3, c + 2, c, 1.0, 0.0],  # H, init_rcut 1.5 bohr
+                     [c + 3, c - 3, c, 0.0, 0.0],  # bare nucleus Z_nuc 1
+                     [c, c + 5, c - 1, 1.0, 0.0],  # hemi pair on one nucleus
+                     [c, c + 5, c - 1, 1.0, 0.0],
+                     [c - 1, c - 5, c + 3, 1.0, 0.0],  # third sectors (two of three)
+                     [c - 1, c - 5, c + 3, 1.0, 0.0]])
+    ext = [{"init_zeff": 1.6}, {"init_rcut": 1.5}, {"Z_nuc": 1.0},
+           {"split": "hemi", "split_idx": 0, "split_axis": (0.0, 1.0, 0.3)},
+           {"split": "hemi", "split_idx": 1, "split_axis": (0.0, 1.0, 0.3)},
+           {"split": "third", "split_idx": 0, "split_axis": (0.2, 0.1, 1.0), "split_rot": 0.3},
+           {"split": "third", "split_idx": 2, "split_axis": (0.2, 0.1, 1.0), "split_rot": 0.3}]
+    return rows, ext
+
+
+def _solver(rows, ext, NN, screen, **cfg):
+    atoms = []
+    for r, e in zip(rows, ext):
+        d = {"i": float(r[0]), "j": float(r[1]), "k": float(r[2]), "Z": float(r[3]), "rc": float(r[4])}
+        d.update(e)
+        atoms.append(d)
+    config = {"gridN": NN, "screen": screen, "atoms": atoms, "init": "s2", "energy": "s2", "rc_dirichlet": False,
+              "poisson": "jacobi", "smooth_a": 0.3333, "smooth_c": 9.0, "boundary": "levelset", "dv": 0.12}
+    config.update(cfg)
+    return RealQMSolver(config)
+
+
+def test_init_s2_with_atom_fields_matches_oracle():
+    NN, screen = 32, 32 * 0.25
+    rows, ext = _rows_ext(NN)
+    s = _solver(rows, ext, NN, screen)
+    s._chk(s._lib.rq_init_trial(s._h))
+    g = s1.Grid(NN, screen, 0.12, F32)
+    U, lab, K, _ = s2.init_s2(rows, g, ext)
+    S = NN + 1
+    Kg, Ug, Lg = cube(s.K_buf.get(), S), cube(s.U.get(), S), cube(s.label.get(), S)
+    print(f"K ulp {ulp_diff(Kg, K)}, U ulp {ulp_diff(Ug, U)}, label mismatches {(Lg != lab).sum()}")
+    assert ulp_diff(Kg, K) == 0
+    same = Lg == lab
+    assert (~same).sum() <= 4
+    assert ulp_diff(Ug[same], U[same]) <= 4
+    # the bare nucleus (atom 2) owns no node; both hemi and both third siblings own some
+    assert (Lg == 2).sum() == 0
+    for a in (3, 4, 5, 6):
+        assert (Lg == a).sum() > 0
+    s.close()
+
+
+def test_recompute_k_s2_bitexact():
+    NN, screen = 32, 32 * 0.25
+    rows, ext = _rows_ext(NN)
+    s = _solver(rows, ext, NN, screen)
+    s.initialize()
+    moved = rows.copy()
+    moved[:3, :3] += np.array([[1.3, 0.2, -0.4], [-0.7, 0.9, 0.1], [0.4, -1.2, 0.6]])
+    from realmolecule_amd.solver import atoms_array
+    s._chk(s._lib.rq_set_atoms(s._h, atoms_array(moved), len(moved)))
+    s._chk(s._lib.rq_recompute_k(s._h))
+    K = s2.recompute_k_s2(moved, s1.Grid(NN, screen, 0.12, F32), ext)
+    assert ulp_diff(cube(s.K_buf.get(), NN + 1), K) == 0
+    s.close()
+
+
+def test_levelset_sector_rule_matches_oracle():
+    NN, screen = 32, 32 * 0.25
+    rows, ext = _rows_ext(NN)
+    s = _solver(rows, ext, NN, screen)
+    s.initialize()
+    s.step(0, 30)
+    g = s1.Grid(NN, screen, 0.12, F32)
+    S = NN + 1
+    U, L, W = cube(s.U.get(), S), cube(s.label.get(), S), cube(s.W.get(), S)
+    s._boundary()
+    Lo, Wo = s2.levelset(U, L, W, g, 0.5, 0.15, atoms=rows, ext=ext)
+    Lg, Wg = cube(s.label.get(), S), cube(s.W.get(), S)
+    print(f"level set with sectors: label mismatches {(Lg != Lo).sum()}, W ulp {ulp_diff(Wg, Wo)}, "
+          f"flips {(Lo != L).sum()}")
+    assert (Lg != Lo).sum() <= 2
+    same = Lg == Lo
+    assert ulp_diff(Wg[same], Wo[same]) == 0
+    s.close()
+
+
+def test_gradp_force_of_a_bare_nucleus():
+    NN, screen = 32, 32 * 0.25
+    rows, ext = _rows_ext(NN)
+    s = _solver(rows, ext, NN, screen)
+    s.initialize()
+    rng = np.random.default_rng(4)
+    S = NN + 1
+    P = np.zeros((S, S, S), F32)
+    P[1:-1, 1:-1, 1:-1] = rng.uniform(0.0, 1.0, (S - 2,) * 3).astype(F32)
+    s.P.set(P)
+    got = s.forces_gradp(4)
+    ref = s2.gradp_forces(P, rows, s1.Grid(NN, screen, 0.12, F32), 4, ext)
+    assert np.abs(got[2]).max() > 0  # the bare nucleus feels a force with Z_nuc
+    assert np.allclose(got, ref, rtol=1e-6, atol=1e-9)
+    s.close()
+
+
+def test_direct_sweeps_pother_matches_oracle():
+    """molecule.js's own direct Pother: the start potentials, then two steps
+    (10 damped sweeps per domain and step, Pother = P_m on domain m) and the
+    ITP update on Pother, bit for bit."""
+    NN, screen = 24, 24 * 0.3
+    c = NN // 2
+    rows = np.array([[c - 4.0, c, c, 1.0, 0.0], [c + 0.0, c + 1, c, 2.0, 0.4], [c + 4.0, c - 1, c + 1, 1.0, 0.0]])
+    ext = [{}, {}, {}]
+    s = _solver(rows, ext, NN, screen, pother="direct_sweeps", boundary="off")
+    s._chk(s._lib.rq_init_trial(s._h))
+    g = s1.Grid(NN, screen, 0.12, F32)
+    S = NN + 1
+    U, lab, K, _ = s2.init_s2(rows, g)
+    Pdom, Pother = s2.direct_sweeps_setup(rows, g)
+    assert ulp_diff(cube(s.Pother.get(), S), Pother) == 0
+    # same start amplitudes on both sides
+    Ug = cube(s.U.get(), S)
+    U = Ug.copy()
+    lab = cube(s.label.get(), S)
+    for step in range(2):
+        s._poisson_step()
+        Pdom, Pother = s2.direct_sweeps_step(U, lab, Pdom, Pother, rows[:, 3], g)
+        Pg = cube(s.Pother.get(), S)
+        print(f"step {step}: Pother ulp {ulp_diff(Pg, Pother)}")
+        assert ulp_diff(Pg, Pother) == 0
+        s._update()
+        U = s1.itp_update(U, K, Pother, lab, g)
+        assert ulp_diff(cube(s.U.get(), S), U) == 0
+    s.close()
