@@ -166,6 +166,7 @@ static napi_value js_create(napi_env env, napi_callback_info info) {
   if (prop_str_eq(env, o, "init", "s2")) p.init_mode = RQ_INIT_S2;
   if (prop_str_eq(env, o, "energy", "s2")) p.energy_mode = RQ_ENERGY_S2;
   if (prop_str_eq(env, o, "pother", "frame")) p.pother_mode = RQ_POTHER_FRAME;
+  if (prop_str_eq(env, o, "pother", "directSweeps")) p.pother_mode = RQ_POTHER_DIRECT_SWEEPS;
   p.sic_interval = (int32_t)prop_num(env, o, "sicInterval", 0);
   p.rc_dirichlet = (int32_t)prop_num(env, o, "rcDirichlet", 0);
   p.ls_dt_w = (float)prop_num(env, o, "dtW", p.ls_dt_w);
@@ -473,6 +474,58 @@ static napi_value js_set_lock_groups(napi_env env, napi_callback_info info) {
   return NULL;
 }
 
+/* setAtomsS2(h, [{Z_nuc, initZeff, initRcut, splitType, splitIdx, splitAxis: [x, y, z],
+ * splitRot}]): the molecule.js per-atom fields (molecule.js:1875-1903); a
+ * missing key keeps the library default (0: Z_nuc = Z, initZeff = Z,
+ * initRcut = 1e6; sphere; axis [0, 0, 1]).  splitType 'sphere' | 'hemi' |
+ * 'third' or its code. */
+static napi_value js_set_atoms_s2(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!get_args(env, info, 2, argv)) return NULL;
+  rq_solver* h = get_handle(env, argv[0]);
+  if (!h) return NULL;
+  bool is_arr = false;
+  napi_is_array(env, argv[1], &is_arr);
+  if (!is_arr) {
+    napi_throw_type_error(env, NULL, "setAtomsS2: expected an array of per-atom objects");
+    return NULL;
+  }
+  uint32_t n = 0;
+  napi_get_array_length(env, argv[1], &n);
+  rq_atom_s2* ext = (rq_atom_s2*)calloc(n ? n : 1, sizeof(rq_atom_s2));
+  for (uint32_t a = 0; a < n; ++a) {
+    napi_value e, v;
+    napi_get_element(env, argv[1], a, &e);
+    ext[a].Z_nuc = (float)prop_num(env, e, "Z_nuc", 0);
+    ext[a].init_zeff = (float)prop_num(env, e, "initZeff", 0);
+    ext[a].init_rcut = (float)prop_num(env, e, "initRcut", 0);
+    if (prop_str_eq(env, e, "splitType", "hemi")) ext[a].split_type = RQ_SPLIT_HEMI;
+    else if (prop_str_eq(env, e, "splitType", "third")) ext[a].split_type = RQ_SPLIT_THIRD;
+    else if (has_prop(env, e, "splitType", &v)) {
+      napi_valuetype vt;
+      napi_typeof(env, v, &vt);
+      if (vt == napi_number) napi_get_value_int32(env, v, &ext[a].split_type);
+    }
+    ext[a].split_idx = (int32_t)prop_num(env, e, "splitIdx", 0);
+    ext[a].split_axis[0] = 0.0f;
+    ext[a].split_axis[1] = 0.0f;
+    ext[a].split_axis[2] = 1.0f;
+    if (has_prop(env, e, "splitAxis", &v)) {
+      for (uint32_t c = 0; c < 3; ++c) {
+        napi_value x;
+        double d = ext[a].split_axis[c];
+        if (napi_get_element(env, v, c, &x) == napi_ok) napi_get_value_double(env, x, &d);
+        ext[a].split_axis[c] = (float)d;
+      }
+    }
+    ext[a].split_rot = (float)prop_num(env, e, "splitRot", 0);
+  }
+  const int rc = rq_set_atoms_s2(h, ext, (int32_t)n);
+  free(ext);
+  if (rc != RQ_OK) return throw_rq(env, h, rc);
+  return NULL;
+}
+
 /* adjointGradient(h, {nSteps, bootstrap, checkpoint, poissonInterval, normInterval})
  *   -> {energy: [T, VeK, Vee], dEdR: Float64Array(3n)} */
 static napi_value js_adjoint(napi_env env, napi_callback_info info) {
@@ -544,6 +597,7 @@ static napi_value init(napi_env env, napi_value exports) {
       {"writeField", 0, js_write_field, 0, 0, 0, napi_enumerable, 0},
       {"setAtoms", 0, js_set_atoms, 0, 0, 0, napi_enumerable, 0},
       {"setLockGroups", 0, js_set_lock_groups, 0, 0, 0, napi_enumerable, 0},
+      {"setAtomsS2", 0, js_set_atoms_s2, 0, 0, 0, napi_enumerable, 0},
       {"adjointGradient", 0, js_adjoint, 0, 0, 0, napi_enumerable, 0},
   };
   NAPI_CALL(env, napi_define_properties(env, exports, sizeof props / sizeof props[0], props));

@@ -453,12 +453,17 @@ class S2State:
     from 1, as the page's draw loop counts them)."""
 
     def __init__(self, rows, NN, screen, dtype=np.float32, dv=0.12, jacobi=2, poisson_interval=50,
-                 curv_reg=0.15, ls_steps=None, pother_frame=False, sic_interval=0):
+                 curv_reg=0.15, ls_steps=None, pother_frame=False, sic_interval=0, direct=False, ext=None):
         from . import s1
         self.rows = np.asarray(rows, np.float64)
+        self.ext = ext
         self.g = s1.Grid(NN, screen, dv, dtype)
-        self.U, self.label, self.K, _ = init_s2(self.rows, self.g)
+        self.U, self.label, self.K, _ = init_s2(self.rows, self.g, ext)
         self.P = np.zeros_like(self.U)
+        # molecule.js's direct Pother (its default up to 5 electrons)
+        self.direct = direct
+        if direct:
+            self.Pdom, self.Pother_direct = direct_sweeps_setup(self.rows, self.g)
         self.W = np.ones_like(self.U)
         self.jacobi = jacobi
         self.poisson_interval = poisson_interval
@@ -475,24 +480,29 @@ class S2State:
         dt = g.dtype
         Z = self.rows[:, 3]
         for s in range(n):
-            rhs = s1.rho_rhs(self.U, g.two_pi, g.h2, dt)
-            for _ in range(self.jacobi):
-                self.P = s1.smooth(self.P, rhs, 0.3333, 9.0, dt)
-            if s > 0 and s % self.poisson_interval == 0:
-                self.P = mg.twogrid_s2(self.P, rhs, dt)
+            if self.direct:
+                self.Pdom, self.Pother_direct = direct_sweeps_step(self.U, self.label, self.Pdom,
+                                                                   self.Pother_direct, Z, g)
+            else:
+                rhs = s1.rho_rhs(self.U, g.two_pi, g.h2, dt)
+                for _ in range(self.jacobi):
+                    self.P = s1.smooth(self.P, rhs, 0.3333, 9.0, dt)
+                if s > 0 and s % self.poisson_interval == 0:
+                    self.P = mg.twogrid_s2(self.P, rhs, dt)
             mask = rc_mask(self.label, self.rows, g)
-            Pv = self.Pother if self.use_frame else self.P
+            Pv = self.Pother_direct if self.direct else (self.Pother if self.use_frame else self.P)
             self.U = s1.itp_update(self.U, self.K, Pv, self.label, g, rc_mask=mask)
             if (s + 1) % 20 == 0 or s == n - 1:
                 self.U, _ = s1.normalize(self.U, self.label, Z, g)
-        if self.use_frame:
+        if self.use_frame and not self.direct:
             self.frame_no += 1
             self.Pother = pother_frame(self.U, self.label, self.P, Z, g, self.frame_no, self.sic_interval,
                                        0.3333, 9.0)
-        Pv = self.Pother if self.use_frame else self.P
+        Pv = self.Pother_direct if self.direct else (self.Pother if self.use_frame else self.P)
         tot, per, dip = energy_s2(self.U, self.K, Pv, self.label, self.rows, g)
         for _ in range(self.ls_steps if self.ls_steps is not None else max(1, n)):
-            self.label, self.W = levelset(self.U, self.label, self.W, g, 0.5, self.curv_reg)
+            self.label, self.W = levelset(self.U, self.label, self.W, g, 0.5, self.curv_reg, atoms=self.rows,
+                                          ext=self.ext)
         return tot, per, dip
 
 

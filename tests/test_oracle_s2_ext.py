@@ -52,7 +52,7 @@ def test_in_split_third_rotates_to_the_next_sector():
 
     s0 = sector(pts)
     s1_ = sector(rot(pts, 2 * math.pi / 3))
-    # away from the sector edges (angle within 1e-3 rad of an edge) the sector advances by one
+    # the sector advances by one (all but points within rounding of an edge)
     assert (s0 >= 0).all() and (s1_ >= 0).all()
     agree = (s1_ - s0) % 3
     assert (agree == 1).mean() > 0.99
