@@ -52,7 +52,9 @@ extern "C" {
 #define RQ_OK         0
 #define RQ_EINVAL    -1  /* bad argument or unsupported parameter combination */
 #define RQ_ENOMEM    -2  /* device allocation failed */
-#define RQ_EHIP      -3  /* HIP runtime error (message has the HIP error string) */
+#define RQ_EHIP      -3  /* HIP runtime error (message has the HIP error string); the handle
+                            then refuses further calls with RQ_EHIP, except rq_destroy and
+                            rq_last_error (molecule.js device.lost, :2258-2261) */
 #define RQ_ESTATE    -4  /* call out of order (e.g. step before initialise) */
 #define RQ_ERANGE    -5  /* size beyond a documented limit */
 #define RQ_EUNSTABLE -6  /* non-finite energy/field detected */
@@ -173,6 +175,10 @@ double      rq_dv_for_electrons(int32_t n_elec);   /* cloud/solver.py:341-351 */
 int rq_create(const rq_params* p, const rq_atom* atoms, int32_t n_atoms, rq_solver** out);
 int rq_destroy(rq_solver* h);
 int rq_sync(rq_solver* h);
+/* Test hook, no counterpart in the reference: the n_checks-th HIP runtime
+ * check after this call reports hipErrorLaunchFailure, as a failing device
+ * would, so callers' lost-device handling can be exercised; 0 disarms. */
+int rq_debug_fail_after(rq_solver* h, int32_t n_checks);
 int rq_get_info(const rq_solver* h, int32_t* NN, int32_t* S, int32_t* n_atoms, double* h_grid, double* dv);
 
 /* --- atoms --- */
@@ -219,7 +225,11 @@ int rq_recompute_k(rq_solver* h);                /* K after rq_set_atoms */
  * the NN/2 grid with the sign-inverted right-hand side, P += 0.5 * prolongation.
  * Needs poisson_mode RQ_POISSON_TWOGRID. */
 int rq_twogrid_correct(rq_solver* h);
-/* Steps first_step+1 .. first_step+n_steps of the run() loop body. */
+/* Steps first_step+1 .. first_step+n_steps of the run() loop body
+ * (cloud/solver.py:561-592).  When a normalisation falls in the range, the
+ * call waits for the stream and returns RQ_EUNSTABLE if the normalisation
+ * met an amplitude that is not finite (molecule.js:3198-3201 stops the same
+ * way on a non-finite energy). */
 int rq_step(rq_solver* h, int64_t first_step, int32_t n_steps, const rq_schedule* s);
 /* on != 0: rq_step replays aligned chunks of norm_interval steps from cached
  * HIP graphs (captured on first use; Jacobi/multigrid modes, timing off,

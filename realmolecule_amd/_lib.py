@@ -116,6 +116,7 @@ PROTOTYPES = {
     "rq_create": (C.c_int, [C.POINTER(RqParams), C.POINTER(RqAtom), _I, C.POINTER(_P)]),
     "rq_destroy": (C.c_int, [_P]),
     "rq_sync": (C.c_int, [_P]),
+    "rq_debug_fail_after": (C.c_int, [_P, _I]),
     "rq_get_info": (C.c_int, [_P, C.POINTER(_I), C.POINTER(_I), C.POINTER(_I), _D, _D]),
     "rq_set_atoms": (C.c_int, [_P, C.POINTER(RqAtom), _I]),
     "rq_get_atoms": (C.c_int, [_P, C.POINTER(RqAtom), _I]),
@@ -215,8 +216,14 @@ class RqError(RuntimeError):
         self.code = code
 
 
+class NumericalInstability(RqError):
+    """RQ_EUNSTABLE: a non-finite amplitude or energy (the reference page
+    stops with "Numerical instability at step N", molecule.js:3198-3201)."""
+
+
 def check(lib, h, code):
     if code != RQ_OK:
         msg = lib.rq_last_error(h)
-        raise RqError(code, msg.decode() if msg else "")
+        cls = NumericalInstability if code == RQ_EUNSTABLE else RqError
+        raise cls(code, msg.decode() if msg else "")
     return code

@@ -136,6 +136,14 @@ struct rq_solver {
   int fuse_max_s = 0;
   int itp_fuse_max_s = kItpFuseMaxS;  // largest grid whose last sweep carries the ITP update (RQ_ITP_FUSE_MAX_S)
   std::vector<GraphEntry> graphs;
+  // set by the normalisation's sums when an amplitude is not finite (4 B,
+  // cleared by rq_init_trial and rq_write_field)
+  int* health = nullptr;
+  // after a HIP runtime error the handle refuses further device work
+  // (molecule.js device.lost, :2258-2261); broken_msg keeps the first error
+  bool broken = false;
+  std::string broken_msg;
+  int32_t fail_after = 0;  // rq_debug_fail_after: the n-th HIP check fails
   std::string err;
 };
 
@@ -149,18 +157,34 @@ int fail(rq_solver* h, int code, const char* fmt, ...) {
   va_end(ap);
   if (h) h->err = buf;
   else g_create_error = buf;
+  if (h && code == RQ_EHIP && !h->broken) {
+    h->broken = true;
+    h->broken_msg = buf;
+  }
   return code;
+}
+
+// the armed rq_debug_fail_after countdown: true on the call it reaches
+bool inject_due(rq_solver* h) { return h && h->fail_after > 0 && --h->fail_after == 0; }
+
+// null handle, or a handle whose device work failed earlier
+int gate(rq_solver* h) {
+  if (!h) return fail(nullptr, RQ_EINVAL, "null handle");
+  if (h->broken) return fail(h, RQ_EHIP, "handle unusable after an earlier HIP error (%s)", h->broken_msg.c_str());
+  return RQ_OK;
 }
 
 #define HIPCHK(h, call)                                                                          \
   do {                                                                                           \
     hipError_t e_ = (call);                                                                      \
+    if (e_ == hipSuccess && inject_due(h)) e_ = hipErrorLaunchFailure;                           \
     if (e_ != hipSuccess) return fail((h), RQ_EHIP, "%s: %s", #call, hipGetErrorString(e_));     \
   } while (0)
 
 #define LAUNCHCHK(h)                                                                             \
   do {                                                                                           \
     hipError_t e_ = hipGetLastError();                                                           \
+    if (e_ == hipSuccess && inject_due(h)) e_ = hipErrorLaunchFailure;                           \
     if (e_ != hipSuccess) return fail((h), RQ_EHIP, "kernel launch: %s", hipGetErrorString(e_)); \
   } while (0)
 
@@ -252,6 +276,13 @@ void zero_fx(rq_solver* s, Fx* p, size_t n) {
 }
 
 // Values of n fixed-point sums copied from the device (synchronises the stream).
+// health word idx (0: normalisation sums, 1: energy sums) after the stream drains
+int read_flag(rq_solver* s, int idx, int* out) {
+  HIPCHK(s, hipMemcpyAsync(out, s->health + idx, sizeof(int), hipMemcpyDeviceToHost, s->st));
+  HIPCHK(s, hipStreamSynchronize(s->st));
+  return RQ_OK;
+}
+
 int read_fx(rq_solver* s, const Fx* d, size_t n, double* out) {
   std::vector<Fx> w(n * kFxWords);
   HIPCHK(s, hipMemcpyAsync(w.data(), d, sizeof(Fx) * w.size(), hipMemcpyDeviceToHost, s->st));
@@ -678,7 +709,7 @@ int normalize_step(rq_solver* s) {
   zero_fx(s, norms, s->n_atoms);
   {
     Timed t(s, F_NORM);
-    launch_norm_sums(s->st, s->tile, s->S, s->h3, s->U, s->lab, s->n_atoms, norms);
+    launch_norm_sums(s->st, s->tile, s->S, s->h3, s->U, s->lab, s->n_atoms, norms, s->health);
     launch_normalize(s->st, s->tile, s->S, s->U, s->lab, norms, s->Z_d, s->n_atoms);
   }
   LAUNCHCHK(s);
@@ -770,6 +801,7 @@ void free_all(rq_solver* s) {
   fr(s->cell_start_d);
   fr(s->cell_atoms_d);
   fr(s->red);
+  fr(s->health);
   fr(s->dst_S);
   fr(s->dst_rden);
   for (size_t l = 0; l < s->e0.size(); ++l) {
@@ -890,7 +922,7 @@ int poisson_into(rq_solver* s, const float* P_in, float* P_out, const float* U, 
 
 int normalize_on(rq_solver* s, float* U) {
   zero_fx(s, s->red, s->n_atoms);
-  launch_norm_sums(s->st, s->tile, s->S, s->h3, U, s->lab, s->n_atoms, s->red);
+  launch_norm_sums(s->st, s->tile, s->S, s->h3, U, s->lab, s->n_atoms, s->red, s->health);
   launch_normalize(s->st, s->tile, s->S, U, s->lab, s->red, s->Z_d, s->n_atoms);
   LAUNCHCHK(s);
   return RQ_OK;
@@ -944,7 +976,7 @@ int normalize_adj(rq_solver* s, const float* Upre) {
   Fx* norms = A.red;
   Fx* dots = A.red + kFxWords * (size_t)s->n_atoms;
   zero_fx(s, A.red, 2 * (size_t)s->n_atoms);
-  launch_norm_sums(s->st, s->tile, s->S, s->h3, Upre, s->lab, s->n_atoms, norms);
+  launch_norm_sums(s->st, s->tile, s->S, s->h3, Upre, s->lab, s->n_atoms, norms, s->health);
   launch_domain_dot(s->st, s->tile, s->S, 1.0f, Upre, A.lamU, s->lab, s->n_atoms, dots);
   launch_normalize_adj(s->st, s->tile, s->S, s->h3, Upre, A.lamU, s->lab, norms, dots, s->Z_d, s->n_atoms);
   LAUNCHCHK(s);
@@ -1039,11 +1071,15 @@ int frozen_run(rq_solver* s, const rq_adjoint_spec* sp, bool backward, double* E
     Fx* per = A.red + kFxWords * 5 * (size_t)s->n_atoms;
     Fx* dip = per + kFxWords * 3 * (size_t)s->n_atoms;
     zero_fx(s, per, 3 * (size_t)s->n_atoms + 3);
+    launch_fill(s->st, reinterpret_cast<float*>(s->health + 1), 1, 0.0f);
     launch_energy(s->st, T, RQ_ENERGY_S1, s->S, s->h, s->h3, s->U, s->K, s->P, s->lab, s->atoms_d, s->n_atoms, per,
-                  dip);
+                  dip, s->health + 1);
     LAUNCHCHK(s);
     std::vector<double> hv(3 * (size_t)s->n_atoms);
     if ((rc = read_fx(s, per, hv.size(), hv.data()))) return rc;
+    int bad = 0;
+    if ((rc = read_flag(s, 1, &bad))) return rc;
+    if (bad) return fail(s, RQ_EUNSTABLE, "non-finite energy in the frozen solve");
     double e[3] = {0, 0, 0};
     for (int d = 0; d < s->n_atoms; ++d)
       for (int q = 0; q < 3; ++q) e[q] += hv[d * 3 + q];
@@ -1282,6 +1318,8 @@ int rq_create(const rq_params* pp, const rq_atom* atoms, int32_t n_atoms, rq_sol
   }
   s->red_len = (size_t)n_atoms * 3 + 3;
   ok = ok && alloc((void**)&s->red, sizeof(Fx) * kFxWords * s->red_len);
+  ok = ok && alloc((void**)&s->health, 256);
+  if (ok) launch_fill(s->st, reinterpret_cast<float*>(s->health), 1, 0.0f);
   if (ok && levels > 0) {
     s->Sl.assign(levels, 0);
     s->e0.assign(levels, nullptr);
@@ -1392,7 +1430,7 @@ int rq_destroy(rq_solver* h) {
 }
 
 int rq_sync(rq_solver* h) {
-  if (!h) return fail(nullptr, RQ_EINVAL, "null handle");
+  if (int g_ = gate(h)) return g_;
   HIPCHK(h, hipStreamSynchronize(h->st));
   return RQ_OK;
 }
@@ -1408,6 +1446,7 @@ int rq_get_info(const rq_solver* h, int32_t* NN, int32_t* S, int32_t* n_atoms, d
 }
 
 int rq_set_atoms(rq_solver* h, const rq_atom* atoms, int32_t n) {
+  if (int g_ = gate(h)) return g_;
   if (!h || !atoms) return fail(h, RQ_EINVAL, "null argument");
   if (n != h->n_atoms) return fail(h, RQ_EINVAL, "n_atoms %d != %d", n, h->n_atoms);
   for (int a = 0; a < n; ++a) h->atoms_h[a] = {atoms[a].i, atoms[a].j, atoms[a].k, atoms[a].Z, atoms[a].rc};
@@ -1421,6 +1460,7 @@ int rq_get_atoms(const rq_solver* h, rq_atom* atoms, int32_t n) {
 }
 
 int rq_set_atoms_s2(rq_solver* h, const rq_atom_s2* ext, int32_t n) {
+  if (int g_ = gate(h)) return g_;
   if (!h || !ext) return fail(h, RQ_EINVAL, "null argument");
   if (n != h->n_atoms) return fail(h, RQ_EINVAL, "n_atoms %d != %d", n, h->n_atoms);
   for (int a = 0; a < n; ++a) {
@@ -1436,6 +1476,7 @@ int rq_set_atoms_s2(rq_solver* h, const rq_atom_s2* ext, int32_t n) {
 }
 
 int rq_set_lock_groups(rq_solver* h, const int32_t* group, int32_t n) {
+  if (int g_ = gate(h)) return g_;
   if (!h || !group || n != h->n_atoms) return fail(h, RQ_EINVAL, "lock groups need n_atoms entries");
   h->lock_h.assign(group, group + n);
   HIPCHK(h, hipMemcpyAsync(h->lock_d, group, sizeof(int32_t) * n, hipMemcpyHostToDevice, h->st));
@@ -1444,7 +1485,7 @@ int rq_set_lock_groups(rq_solver* h, const int32_t* group, int32_t n) {
 }
 
 int rq_init_trial(rq_solver* h) {
-  if (!h) return fail(nullptr, RQ_EINVAL, "null handle");
+  if (int g_ = gate(h)) return g_;
   launch_init_trial(h->st, h->p.init_mode, h->S, h->h, h->h2, h->atoms_d, h->ext_d, h->n_atoms, h->U, h->U2, h->lab, h->K,
                     h->bestU, h->W);
   LAUNCHCHK(h);
@@ -1453,6 +1494,7 @@ int rq_init_trial(rq_solver* h) {
     LAUNCHCHK(h);
   }
   HIPCHK(h, hipMemcpyAsync(h->lab2, h->lab, h->S3 * sizeof(uint16_t), hipMemcpyDeviceToDevice, h->st));
+  launch_fill(h->st, reinterpret_cast<float*>(h->health), 1, 0.0f);
   HIPCHK(h, hipMemsetAsync(h->P, 0, h->S3 * sizeof(float), h->st));
   HIPCHK(h, hipMemsetAsync(h->P2, 0, h->S3 * sizeof(float), h->st));
   if (h->p.pother_mode == RQ_POTHER_DIRECT_SWEEPS) {
@@ -1486,7 +1528,7 @@ int rq_initialize(rq_solver* h, int32_t bootstrap) {
 }
 
 int rq_poisson(rq_solver* h, int32_t n) {
-  if (!h) return fail(nullptr, RQ_EINVAL, "null handle");
+  if (int g_ = gate(h)) return g_;
   for (int i = 0; i < n; ++i) {
     int rc = poisson_step(h, false);
     if (rc) return rc;
@@ -1495,22 +1537,22 @@ int rq_poisson(rq_solver* h, int32_t n) {
 }
 
 int rq_update(rq_solver* h) {
-  if (!h) return fail(nullptr, RQ_EINVAL, "null handle");
+  if (int g_ = gate(h)) return g_;
   return update_step(h);
 }
 
 int rq_normalize(rq_solver* h) {
-  if (!h) return fail(nullptr, RQ_EINVAL, "null handle");
+  if (int g_ = gate(h)) return g_;
   return normalize_step(h);
 }
 
 int rq_boundary(rq_solver* h) {
-  if (!h) return fail(nullptr, RQ_EINVAL, "null handle");
+  if (int g_ = gate(h)) return g_;
   return boundary_step(h);
 }
 
 int rq_recompute_k(rq_solver* h) {
-  if (!h) return fail(nullptr, RQ_EINVAL, "null handle");
+  if (int g_ = gate(h)) return g_;
   launch_recompute_k(h->st, h->p.init_mode, h->S, h->h, h->h2, h->atoms_d, h->ext_d, h->n_atoms, h->K);
   LAUNCHCHK(h);
   if (h->p.rc_dirichlet) {
@@ -1522,7 +1564,7 @@ int rq_recompute_k(rq_solver* h) {
 }
 
 int rq_twogrid_correct(rq_solver* h) {
-  if (!h) return fail(nullptr, RQ_EINVAL, "null handle");
+  if (int g_ = gate(h)) return g_;
   if (h->p.poisson_mode != RQ_POISSON_TWOGRID) return fail(h, RQ_EINVAL, "two-grid correction needs RQ_POISSON_TWOGRID");
   return twogrid_correction(h);
 }
@@ -1608,7 +1650,7 @@ static int graph_chunk(rq_solver* h, int64_t c0, const rq_schedule* sc) {
 }
 
 int rq_step(rq_solver* h, int64_t first, int32_t n, const rq_schedule* sc) {
-  if (!h) return fail(nullptr, RQ_EINVAL, "null handle");
+  if (int g_ = gate(h)) return g_;
   if (!h->initialized) return fail(h, RQ_ESTATE, "rq_step before rq_init_trial/rq_initialize");
   rq_schedule d;
   rq_default_schedule(&d);
@@ -1634,11 +1676,27 @@ int rq_step(rq_solver* h, int64_t first, int32_t n, const rq_schedule* sc) {
     if ((rc = eager_step(h, step, sc, true))) return rc;
     ++step;
   }
+  // a normalisation ran: its sums flag amplitudes that are not finite
+  // (molecule.js:3198-3201 stops on a non-finite energy the same way)
+  if (last / sc->norm_interval > first / sc->norm_interval) {
+    int bad = 0;
+    if (int rc = read_flag(h, 0, &bad)) return rc;
+    if (bad)
+      return fail(h, RQ_EUNSTABLE, "Numerical instability: non-finite amplitude at a normalisation in steps %lld..%lld",
+                  (long long)first + 1, (long long)last);
+  }
+  return RQ_OK;
+}
+
+int rq_debug_fail_after(rq_solver* h, int32_t n_checks) {
+  if (int g_ = gate(h)) return g_;
+  if (n_checks < 0) return fail(h, RQ_EINVAL, "n_checks %d < 0", n_checks);
+  h->fail_after = n_checks;
   return RQ_OK;
 }
 
 int rq_set_fused(rq_solver* h, int32_t on) {
-  if (!h) return fail(nullptr, RQ_EINVAL, "null handle");
+  if (int g_ = gate(h)) return g_;
   h->fused = on != 0;
   h->bottom = on == 3;
   h->legs = on != 2;
@@ -1647,13 +1705,13 @@ int rq_set_fused(rq_solver* h, int32_t on) {
 }
 
 int rq_set_graph(rq_solver* h, int32_t on) {
-  if (!h) return fail(nullptr, RQ_EINVAL, "null handle");
+  if (int g_ = gate(h)) return g_;
   h->use_graph = on != 0;
   return RQ_OK;
 }
 
 int rq_set_tile(rq_solver* h, int32_t level, int32_t bx, int32_t by, int32_t ich) {
-  if (!h) return fail(nullptr, RQ_EINVAL, "null handle");
+  if (int g_ = gate(h)) return g_;
   const int nl = std::max(1, h->levels);
   if (level < -1 || level >= nl) return fail(h, RQ_ERANGE, "level %d outside [-1, %d)", level, nl);
   const bool dflt = bx <= 0;
@@ -1693,20 +1751,25 @@ int rq_get_tile(const rq_solver* h, int32_t level, int32_t* bx, int32_t* by, int
 }
 
 int rq_energy(rq_solver* h, double* out6, double* per_domain) {
+  if (int g_ = gate(h)) return g_;
   if (!h || !out6) return fail(h, RQ_EINVAL, "null argument");
   const int nd = h->n_atoms;
   Fx* per = h->red;
   Fx* dip = h->red + kFxWords * (size_t)nd * 3;
   zero_fx(h, h->red, h->red_len);
+  launch_fill(h->st, reinterpret_cast<float*>(h->health + 1), 1, 0.0f);
   {
     Timed t(h, F_ENERGY);
     const float* Pv = itp_potential(h);
     launch_energy(h->st, h->tile, h->p.energy_mode, h->S, h->h, h->h3, h->U, h->K, Pv, h->lab, h->atoms_d, nd, per,
-                  dip);
+                  dip, h->health + 1);
   }
   LAUNCHCHK(h);
   std::vector<double> hostv(h->red_len);
   if (int rc = read_fx(h, h->red, h->red_len, hostv.data())) return rc;
+  int bad = 0;
+  if (int rc = read_flag(h, 1, &bad)) return rc;
+  if (bad) return fail(h, RQ_EUNSTABLE, "non-finite energy terms (a NaN or Inf amplitude or potential)");
   double T = 0, K = 0, E = 0;
   for (int d = 0; d < nd; ++d) {
     T += hostv[d * 3 + 0];
@@ -1725,14 +1788,16 @@ int rq_energy(rq_solver* h, double* out6, double* per_domain) {
 }
 
 int rq_domain_norms(rq_solver* h, double* norms) {
+  if (int g_ = gate(h)) return g_;
   if (!h || !norms) return fail(h, RQ_EINVAL, "null argument");
   zero_fx(h, h->red, h->n_atoms);
-  launch_norm_sums(h->st, h->tile, h->S, h->h3, h->U, h->lab, h->n_atoms, h->red);
+  launch_norm_sums(h->st, h->tile, h->S, h->h3, h->U, h->lab, h->n_atoms, h->red, h->health);
   LAUNCHCHK(h);
   return read_fx(h, h->red, h->n_atoms, norms);
 }
 
 int rq_forces_hf(rq_solver* h, int32_t first, int32_t count, double* out3) {
+  if (int g_ = gate(h)) return g_;
   if (!h || !out3) return fail(h, RQ_EINVAL, "null argument");
   if (first < 0 || count < 0 || first + count > h->n_atoms) return fail(h, RQ_ERANGE, "atom range out of bounds");
   if (count == 0) return RQ_OK;
@@ -1751,6 +1816,7 @@ int rq_forces_hf(rq_solver* h, int32_t first, int32_t count, double* out3) {
 }
 
 int rq_forces_hf_noalloc(rq_solver* h, int32_t first, int32_t count, double* out3) {
+  if (int g_ = gate(h)) return g_;
   if (!h || !out3) return fail(h, RQ_EINVAL, "null argument");
   if (first < 0 || count < 0 || first + count > h->n_atoms) return fail(h, RQ_ERANGE, "atom range out of bounds");
   if (count > kNoAllocMaxAtoms)
@@ -1775,6 +1841,7 @@ int rq_forces_hf_noalloc(rq_solver* h, int32_t first, int32_t count, double* out
 }
 
 int rq_forces_gradp(rq_solver* h, int32_t radius, double* out3n) {
+  if (int g_ = gate(h)) return g_;
   if (!h || !out3n) return fail(h, RQ_EINVAL, "null argument");
   int R = radius > 0 ? radius : std::max(5, (int)std::lround(3.0 / h->h64));
   if (R > h->NN) return fail(h, RQ_ERANGE, "radius %d beyond grid", R);
@@ -1792,6 +1859,7 @@ int rq_forces_gradp(rq_solver* h, int32_t radius, double* out3n) {
 }
 
 int rq_read_field(rq_solver* h, int32_t f, void* host, size_t bytes) {
+  if (int g_ = gate(h)) return g_;
   if (!h || !host) return fail(h, RQ_EINVAL, "null argument");
   const size_t need = h->S3 * 4;
   if (bytes != need) return fail(h, RQ_EINVAL, "field %d needs %zu bytes, got %zu", f, need, bytes);
@@ -1842,7 +1910,7 @@ static int sic_solve(rq_solver* s, int m) {
 }
 
 int rq_pother_frame(rq_solver* h, int64_t frame) {
-  if (!h) return fail(nullptr, RQ_EINVAL, "null handle");
+  if (int g_ = gate(h)) return g_;
   if (h->p.pother_mode != RQ_POTHER_FRAME) return fail(h, RQ_ESTATE, "rq_pother_frame needs pother_mode FRAME");
   if (!h->initialized) return fail(h, RQ_ESTATE, "rq_pother_frame before initialisation");
   HIPCHK(h, hipMemcpyAsync(h->Pother, h->P, h->S3 * sizeof(float), hipMemcpyDeviceToDevice, h->st));
@@ -1861,6 +1929,7 @@ int rq_pother_frame(rq_solver* h, int64_t frame) {
 }
 
 int rq_display_slice(rq_solver* h, int32_t sk, float* host, size_t n) {
+  if (int g_ = gate(h)) return g_;
   if (!h || !host) return fail(h, RQ_EINVAL, "null argument");
   const size_t S = h->S, need = 3 * S * S + 2 * S;
   if (n != need) return fail(h, RQ_EINVAL, "display slice needs %zu floats, got %zu", need, n);
@@ -1876,6 +1945,7 @@ int rq_display_slice(rq_solver* h, int32_t sk, float* host, size_t n) {
 }
 
 int rq_write_field(rq_solver* h, int32_t f, const void* host, size_t bytes) {
+  if (int g_ = gate(h)) return g_;
   if (!h || !host) return fail(h, RQ_EINVAL, "null argument");
   const size_t need = h->S3 * 4;
   if (bytes != need) return fail(h, RQ_EINVAL, "field %d needs %zu bytes, got %zu", f, need, bytes);
@@ -1900,18 +1970,21 @@ int rq_write_field(rq_solver* h, int32_t f, const void* host, size_t bytes) {
     float* partner = (f == RQ_FIELD_U) ? h->U2 : (f == RQ_FIELD_P) ? h->P2 : nullptr;
     if (partner) HIPCHK(h, hipMemcpyAsync(partner, p, need, hipMemcpyDeviceToDevice, h->st));
   }
+  launch_fill(h->st, reinterpret_cast<float*>(h->health), 1, 0.0f);
+  LAUNCHCHK(h);
   HIPCHK(h, hipStreamSynchronize(h->st));
   h->initialized = true;
   return RQ_OK;
 }
 
 int rq_swap_u(rq_solver* h) {
-  if (!h) return fail(nullptr, RQ_EINVAL, "null handle");
+  if (int g_ = gate(h)) return g_;
   std::swap(h->U, h->U2);
   return RQ_OK;
 }
 
 int rq_field_device_ptr(rq_solver* h, int32_t f, void** dptr) {
+  if (int g_ = gate(h)) return g_;
   if (!h || !dptr) return fail(h, RQ_EINVAL, "null argument");
   if (f == RQ_FIELD_LABEL) {
     *dptr = h->lab;
@@ -1930,17 +2003,17 @@ int rq_stream(rq_solver* h, void** s) {
 }
 
 int rq_solve_frozen(rq_solver* h, const rq_adjoint_spec* spec, double* energy3) {
-  if (!h) return fail(nullptr, RQ_EINVAL, "null handle");
+  if (int g_ = gate(h)) return g_;
   return frozen_run(h, spec, false, energy3, nullptr);
 }
 
 int rq_adjoint_gradient(rq_solver* h, const rq_adjoint_spec* spec, double* energy3, double* dEdR) {
-  if (!h) return fail(nullptr, RQ_EINVAL, "null handle");
+  if (int g_ = gate(h)) return g_;
   return frozen_run(h, spec, true, energy3, dEdR);
 }
 
 int rq_timing_enable(rq_solver* h, int32_t on) {
-  if (!h) return fail(nullptr, RQ_EINVAL, "null handle");
+  if (int g_ = gate(h)) return g_;
   resolve_events(h);
   h->timing = on != 0;
   for (int f = 0; f < F_COUNT; ++f) {
@@ -1951,7 +2024,7 @@ int rq_timing_enable(rq_solver* h, int32_t on) {
 }
 
 int rq_timing_read(rq_solver* h, double* ms, int64_t* launches, int32_t n) {
-  if (!h) return fail(nullptr, RQ_EINVAL, "null handle");
+  if (int g_ = gate(h)) return g_;
   resolve_events(h);
   for (int f = 0; f < n && f < F_COUNT; ++f) {
     if (ms) ms[f] = h->fam_ms[f];

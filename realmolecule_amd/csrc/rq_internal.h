@@ -171,7 +171,7 @@ void launch_smooth_itp(hipStream_t st, const Tile& t, int S, const float* P, flo
                        float dtv);
 void launch_zero(hipStream_t st, float* p, size_t n);
 void launch_norm_sums(hipStream_t st, const Tile& t, int S, float h3, const float* U, const uint16_t* lab,
-                      int n_dom, Fx* norms /* n_dom sums */);
+                      int n_dom, Fx* norms /* n_dom sums */, int* bad /* set to 1 on a non-finite sum */);
 void launch_normalize(hipStream_t st, const Tile& t, int S, float* U, const uint16_t* lab, const Fx* norms,
                       const float* Z, int n_dom);
 // Score rule.  Up to kScoreScanMaxAtoms atoms: a scan over all atoms.  Above:
@@ -192,7 +192,7 @@ void launch_levelset(hipStream_t st, const Tile& t, int S, float h, const Atom* 
                      float* W, float dt_w, float curv_reg, int rc_flags);
 void launch_energy(hipStream_t st, const Tile& t, int mode, int S, float h, float h3, const float* U,
                    const float* K, const float* P, const uint16_t* lab, const Atom* atoms, int n_dom,
-                   Fx* per_dom /* n_dom*3 sums */, Fx* dip /* 3 sums */);
+                   Fx* per_dom /* n_dom*3 sums */, Fx* dip /* 3 sums */, int* bad /* 1 on a non-finite sum */);
 void launch_forces_hf(hipStream_t st, int S, float h, float h2, float h3, const float* U, const Atom* atoms,
                       int first, int count, Fx* out3 /* 3 count sums */);
 void launch_forces_gradp(hipStream_t st, int S, float h, const float* P, const Atom* atoms, const AtomS2* ext,

@@ -1242,14 +1242,14 @@ __global__ void k_lab32to16(size_t n, const int32_t* __restrict__ in, uint16_t* 
 // --------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_norm_sums(int S, int ich, float h3, const float* __restrict__ U,
                                                    const uint16_t* __restrict__ lab, int n_dom,
-                                                   Fx* __restrict__ norms) {
+                                                   Fx* __restrict__ norms, int* __restrict__ bad) {
   extern __shared__ Fx fbins[];
   const int nthr = blockDim.x * blockDim.y;
   const int tid = threadIdx.y * blockDim.x + threadIdx.x;
   for (int t = tid; t < kFxWords * n_dom; t += nthr) fbins[t] = 0;
   __syncthreads();
   March m;
-  if (march_coords(S - 2, ich, m)) norm_sums_col(S, m.k, m.j, m.i0, m.i1, h3, U, lab, n_dom, fbins);
+  if (march_coords(S - 2, ich, m)) norm_sums_col(S, m.k, m.j, m.i0, m.i1, h3, U, lab, n_dom, fbins, bad);
   __syncthreads();
   for (int t = tid; t < n_dom; t += nthr) fx_add_words(norms + kFxWords * t, fbins + kFxWords * t);
 }
@@ -1648,12 +1648,14 @@ __device__ __forceinline__ T wave_sum(T v) {
 // Energy terms per domain.  The LDS bins cover the domain window [d0, d0 +
 // nw) (3 sums per domain, then the 3 dipole sums, which only the window
 // d0 = 0 adds to the result); the host runs one launch per window when the
-// bins of all domains would not fit the LDS.
+// bins of all domains would not fit the LDS.  A run or dipole sum that is
+// not finite sets *bad (the fixed-point sums cannot carry a NaN or Inf).
 __global__ __launch_bounds__(256) void k_energy(int mode, int S, int ich, float h, float h3, float r_sing,
                                                 const float* __restrict__ U, const float* __restrict__ K,
                                                 const float* __restrict__ P, const uint16_t* __restrict__ lab,
                                                 const Atom* __restrict__ atoms, int n_dom, int d0, int nw,
-                                                Fx* __restrict__ per_dom, Fx* __restrict__ dip) {
+                                                Fx* __restrict__ per_dom, Fx* __restrict__ dip,
+                                                int* __restrict__ bad) {
   extern __shared__ Fx fbins[];  // nw * 3 sums, then 3 dipole sums
   const int nthr = blockDim.x * blockDim.y;
   const int tid = threadIdx.y * blockDim.x + threadIdx.x;
@@ -1661,6 +1663,7 @@ __global__ __launch_bounds__(256) void k_energy(int mode, int S, int ich, float 
   __syncthreads();
   // flush of one run of label L into the window's bins
   auto flush = [&](int L, double aT, double aK, double aE) {
+    if (!isfinite(aT + aK + aE)) *bad = 1;
     const int w = L - d0;
     if (w >= 0 && w < nw) {
       fx_add(fbins + kFxWords * (w * 3 + 0), aT);
@@ -1721,6 +1724,7 @@ __global__ __launch_bounds__(256) void k_energy(int mode, int S, int ich, float 
   dy_acc = wave_sum(dy_acc);
   dz_acc = wave_sum(dz_acc);
   if ((tid & 63) == 0) {
+    if (!isfinite(dx_acc + dy_acc + dz_acc)) *bad = 1;
     fx_add(fbins + kFxWords * (nw * 3 + 0), dx_acc);
     fx_add(fbins + kFxWords * (nw * 3 + 1), dy_acc);
     fx_add(fbins + kFxWords * (nw * 3 + 2), dz_acc);
@@ -2181,7 +2185,7 @@ void launch_fill(hipStream_t st, float* p, size_t n, float v) {
 }
 
 void launch_norm_sums(hipStream_t st, const Tile& t, int S, float h3, const float* U, const uint16_t* lab, int n_dom,
-                      Fx* norms) {
+                      Fx* norms, int* bad) {
   // longer columns than the sweeps' tile: a few thousand workgroups, so a
   // few thousand atomics into the per-domain sums instead of ~10^4
   Tile r = t;
@@ -2192,7 +2196,7 @@ void launch_norm_sums(hipStream_t st, const Tile& t, int S, float h3, const floa
     attr_set = true;
   }
   hipLaunchKernelGGL(k_norm_sums, stencil_grid(S - 2, r), dim3(r.bx, r.by), sizeof(Fx) * kFxWords * n_dom, st, S,
-                     r.ich, h3, U, lab, n_dom, norms);
+                     r.ich, h3, U, lab, n_dom, norms, bad);
 }
 
 void launch_normalize(hipStream_t st, const Tile& t, int S, float* U, const uint16_t* lab, const Fx* norms,
@@ -2226,7 +2230,8 @@ void launch_levelset(hipStream_t st, const Tile& t, int S, float h, const Atom* 
 }
 
 void launch_energy(hipStream_t st, const Tile& t, int mode, int S, float h, float h3, const float* U, const float* K,
-                   const float* P, const uint16_t* lab, const Atom* atoms, int n_dom, Fx* per_dom, Fx* dip) {
+                   const float* P, const uint16_t* lab, const Atom* atoms, int n_dom, Fx* per_dom, Fx* dip,
+                   int* bad) {
   // domain windows whose bins (3 sums each + 3 dipole sums) fit 150 KiB of LDS
   constexpr int kWindow = 3200;
   static bool attr_set = false;
@@ -2238,7 +2243,7 @@ void launch_energy(hipStream_t st, const Tile& t, int mode, int S, float h, floa
     const int nw = std::min(kWindow, n_dom - d0);
     const size_t lds = sizeof(Fx) * kFxWords * ((size_t)nw * 3 + 3);
     hipLaunchKernelGGL(k_energy, stencil_grid(S - 2, t), dim3(t.bx, t.by), lds, st, mode, S, t.ich, h, h3, 2.0f * h,
-                       U, K, P, lab, atoms, n_dom, d0, nw, per_dom, dip);
+                       U, K, P, lab, atoms, n_dom, d0, nw, per_dom, dip, bad);
   }
 }
 

@@ -392,10 +392,11 @@ __device__ __forceinline__ void prolong_correct_col(int Sf, int k, int j, int i0
 }
 
 // run-length FP64 accumulation of (U*U)*h3 per clipped label into LDS
-// fixed-point bins (fx_add)
+// fixed-point bins (fx_add); a run whose sum is not finite (a NaN or Inf
+// amplitude) sets *bad when bad is given
 __device__ __forceinline__ void norm_sums_col(int S, int k, int j, int i0, int i1, float h3,
                                               const float* __restrict__ U, const uint16_t* __restrict__ lab,
-                                              int n_dom, Fx* bins) {
+                                              int n_dom, Fx* bins, int* __restrict__ bad = nullptr) {
   // planes in batches of B: the batch's loads are issued together (indices
   // clamped to the column's last plane, whose repeats are skipped), then the
   // run-length accumulation walks them in order, so the sums are those of a
@@ -419,14 +420,20 @@ __device__ __forceinline__ void norm_sums_col(int S, int k, int j, int i0, int i
       const int L = Lb[q] < n_dom ? Lb[q] : n_dom - 1;
       const float term = (vb[q] * vb[q]) * h3;
       if (L != cur) {
-        if (cur >= 0) fx_add(bins + kFxWords * cur, acc);
+        if (cur >= 0) {
+          if (bad && !isfinite(acc)) *bad = 1;
+          fx_add(bins + kFxWords * cur, acc);
+        }
         cur = L;
         acc = 0.0;
       }
       acc += (double)term;
     }
   }
-  if (cur >= 0) fx_add(bins + kFxWords * cur, acc);
+  if (cur >= 0) {
+    if (bad && !isfinite(acc)) *bad = 1;
+    fx_add(bins + kFxWords * cur, acc);
+  }
 }
 
 // U *= scale[label] for labels inside [0, n_dom)

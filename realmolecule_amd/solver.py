@@ -421,7 +421,10 @@ class RealQMSolver:
                 nxt = min(nxt, (step // report_interval + 1) * report_interval)
             if self.dynamics_enabled and force_interval:
                 nxt = min(nxt, (step // force_interval + 1) * force_interval)
-            self._chk(self._lib.rq_step(self._h, step, nxt - step, C.byref(sched)))
+            try:
+                self._chk(self._lib.rq_step(self._h, step, nxt - step, C.byref(sched)))
+            except L.NumericalInstability as e:
+                raise L.NumericalInstability(e.code, f"Numerical instability at step {nxt} ({e})") from None
             step = nxt
             if self.dynamics_enabled and force_interval and step % force_interval == 0:
                 self._compute_forces()

@@ -68,3 +68,25 @@ def test_create_rejects_bad_params_without_gpu(lib):
     p.mg_levels = 4
     p.reserved[0] = 1
     assert lib.rq_create(C.byref(p), atoms, 1, C.byref(h)) == _lib.RQ_EINVAL
+    # an oversize grid is refused before any allocation (the buffer-size
+    # guard of molecule.js:2246-2250)
+    p.reserved[0] = 0
+    p.poisson_mode = _lib.RQ_POISSON_JACOBI
+    p.NN = 2048
+    assert lib.rq_create(C.byref(p), atoms, 1, C.byref(h)) == _lib.RQ_EINVAL
+    assert b"NN=2048" in lib.rq_last_error(None)
+
+
+def test_null_handle_is_refused_without_gpu(lib):
+    """Every entry point that takes a handle refuses a null one before any
+    device call (rq_destroy accepts it as a no-op)."""
+    from realmolecule_amd import _lib
+    sched = _lib.RqSchedule(2, 20, 100)
+    out = (C.c_double * 6)()
+    assert lib.rq_step(None, 0, 1, C.byref(sched)) == _lib.RQ_EINVAL
+    assert lib.rq_init_trial(None) == _lib.RQ_EINVAL
+    assert lib.rq_sync(None) == _lib.RQ_EINVAL
+    assert lib.rq_energy(None, out, None) == _lib.RQ_EINVAL
+    assert lib.rq_debug_fail_after(None, 1) == _lib.RQ_EINVAL
+    assert b"null handle" in lib.rq_last_error(None)
+    assert lib.rq_destroy(None) == _lib.RQ_OK
