@@ -218,6 +218,139 @@ def forces_bench(w, bootstrap, n_batch, fd_delta, fd_components, log_fn, conv_to
     }
 
 
+def bench_full_adjoint(wname, window, converge_max, fd_components, log_fn, tol=1e-6, chunk=100):
+    """One full adjoint force vector on a large workload (all atoms), from a
+    converged start.  The frozen-label solve (trial, normalise, 10 bootstrap
+    V-cycles, then a V-cycle + ITP per step) is run in chunks of ``chunk``
+    steps until the electronic energy changes by <= tol relative over one
+    chunk (or ``converge_max`` steps); that state is stored
+    (rq_adjoint_save_start) and the differentiated map is the ``window``
+    steps that follow it, at any positions (RQ_ADJ_START_SAVED).  Plain path:
+    central differences (delta = 1e-3 bohr) of that map, 6N+1 solves per
+    force vector, timed on ``fd_components`` components; the reference's own
+    force (Hellmann-Feynman, cloud/solver.py:282-315) at the stored state for
+    the rc = 0 atoms."""
+    import ctypes as C
+    from realmolecule_amd import _lib as L
+    from realmolecule_amd.adjoint import FrozenSolver
+    from realmolecule_amd.solver import nuclear_forces
+    w = workload(wname)
+    NN, screen, rows = w["NN"], w["screen"], w["rows"]
+    n = len(rows)
+    fz = FrozenSolver(rows, NN, screen, chunk, bootstrap=10, checkpoint=16, poisson_interval=1, norm_interval=20,
+                      poisson="multigrid", mg_levels=w["mg_levels"])
+    pos = rows[:, :3] * fz.h
+    ekk = fz._ekk(fz.rows)
+    t0 = time.perf_counter()
+    E_prev = fz.warm_start(pos, chunk) - ekk
+    T, d = chunk, 1.0
+    while T < converge_max:
+        E = fz.energy(pos) - ekk
+        fz.s._chk(fz.lib.rq_adjoint_save_start(fz.hdl))
+        T += chunk
+        d = abs(E - E_prev) / abs(E)
+        E_prev = E
+        if d <= tol:
+            break
+    t_conv = time.perf_counter() - t0
+    log_fn(f"[adjoint] {wname}: start after {T} steps ({t_conv:.1f} s), dE/E over the last {chunk} steps {d:.2e}")
+    hf = (C.c_double * (3 * n))()
+    fz.s._chk(fz.lib.rq_forces_hf(fz.hdl, 0, n, hf))
+    F_hf = np.frombuffer(hf, np.float64).reshape(n, 3).copy()
+    fz.spec.n_steps = window
+    t0 = time.perf_counter()
+    E0 = fz.energy(pos)
+    t_e0 = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    E_adj, g = fz.energy_and_gradient(pos)
+    t_adj = time.perf_counter() - t0
+    log_fn(f"[adjoint] {wname}: {3 * n} components in {t_adj:.3f} s ({window}-step window; one solve {t_e0:.3f} s)")
+    F_el_adj = -(g + nuclear_forces(rows[:, :3].astype(np.float32), rows[:, 3], fz.h))
+    bare = rows[:, 4] <= 0
+    comps = [(int(a), c) for a in list(np.nonzero(bare)[0][:1]) + list(np.nonzero(~bare)[0][:1]) for c in range(3)]
+    # FP32 amplitudes put ~|E| 2^-24 of rounding noise on each solve's
+    # energy; beyond ~1e-2 Ha/bohr over 2 delta central differences resolve
+    # nothing (a 1000-water energy of -2e5 Ha: ~6 Ha/bohr)
+    fd_noise = abs(E0) * 2.0 ** -24 / 2e-3
+    if fd_noise > 1e-2:
+        comps = []
+    comps = comps[:fd_components]
+    F_fd = {}
+    t0 = time.perf_counter()
+    for a, c in comps:
+        F_fd[(a, c)] = _fd_component(fz, pos, a, c, 1e-3)
+    t_fd_sample = time.perf_counter() - t0
+    t_solve = (t_fd_sample + t_e0) / (2 * len(comps) + 1)
+    t_fd = (6 * n + 1) * t_solve
+    log_fn(f"[adjoint] {wname}: central differences {t_solve:.3f} s per solve, {t_fd:.0f} s per force vector"
+           f"{'' if comps else f' (not evaluated: FP32 noise ~{fd_noise:.1f} Ha/bohr)'}")
+    fz.close()
+    hf_diff = np.abs(F_el_adj - F_hf)[bare]
+    return {
+        "metric": "full adjoint force vector, one GPU",
+        "workload": f"{w['desc']}; {n} atoms; start converged by {T} frozen-label steps (dE/E {d:.2e} over "
+                    f"the last {chunk}), differentiated map = the {window} steps after it",
+        "n_atoms": n,
+        "components": 3 * n,
+        "start_steps": T,
+        "start_dE_rel_per_chunk": d,
+        "start_seconds": t_conv,
+        "window_steps": window,
+        "energy_Ha": E0,
+        "energy_rel_diff_adjoint_vs_plain": abs(E_adj - E0) / abs(E0),
+        "adjoint_seconds": t_adj,
+        "adjoint_forces_per_s": 1.0 / t_adj,
+        "adjoint_over_one_solve": t_adj / t_e0,
+        "plain_fd_seconds_per_solve": t_solve,
+        "plain_fd_solves": 6 * n + 1,
+        "plain_fd_forces_per_s": 1.0 / t_fd,
+        "speedup_adjoint": t_fd / t_adj,
+        "fd_delta_bohr": 1e-3,
+        "fd_noise_estimate_Ha_per_bohr": fd_noise,
+        "fd_checked": [{"atom": a, "axis": c, "adjoint": -g[a, c], "fd": F_fd[(a, c)], "rc": float(rows[a, 4])}
+                       for a, c in comps],
+        "hf_vs_adjoint_rc0_atoms_max_abs": float(hf_diff.max()) if bare.any() else None,
+        "hf_vs_adjoint_rc0_atoms_rms": float(np.sqrt((hf_diff ** 2).mean())) if bare.any() else None,
+        "max_abs_force_Ha_per_bohr": float(np.abs(g).max()),
+    }
+
+
+def bench_relax(wname, iters, fd_iters, n_steps, log_fn):
+    """Differentiable relaxation: L-BFGS-B (scipy) on adjoint gradients of the
+    frozen-label ``n_steps`` solve for ``iters`` iterations, against the same
+    driver on central-difference gradients (6N+1 solves per gradient,
+    delta = 1e-3 bohr) for ``fd_iters`` iterations; labels re-frozen after
+    each accepted iteration.  Reports wall time per iteration of each and the
+    energies both reach after ``fd_iters`` iterations."""
+    from realmolecule_amd.relax import relax
+    w = workload(wname)
+    NN, screen, rows = w["NN"], w["screen"], w["rows"]
+    kw = dict(bootstrap=10, checkpoint=64, poisson_interval=1, poisson="multigrid", mg_levels=w["mg_levels"])
+    ad_short = relax(rows, NN, screen, n_steps=n_steps, max_iter=fd_iters, gtol=1e-12, **kw)
+    ad = relax(rows, NN, screen, n_steps=n_steps, max_iter=iters, gtol=1e-12, **kw)
+    log_fn(f"[relax] {wname}: adjoint {len(ad['grad_max'])} iterations in {ad['wall_s']:.1f} s, "
+           f"E {ad['energies'][0]:.6f} -> {min(ad['energies']):.6f}")
+    fd = relax(rows, NN, screen, n_steps=n_steps, max_iter=fd_iters, gtol=1e-12, fd=True, fd_delta=1e-3, **kw)
+    log_fn(f"[relax] {wname}: central differences {len(fd['grad_max'])} iterations in {fd['wall_s']:.1f} s")
+    it_ad = max(1, len(ad["grad_max"]))
+    it_fd = max(1, len(fd["grad_max"]))
+    return {
+        "metric": "relaxation iterations per second, adjoint vs central-difference gradients",
+        "workload": f"{w['desc']}; {len(rows)} atoms; frozen-label solve of 10 bootstrap V-cycles + {n_steps} steps "
+                    f"per energy; L-BFGS-B, labels re-frozen per accepted iteration",
+        "adjoint": {"iterations": len(ad["grad_max"]), "evaluations": ad["evaluations"], "wall_s": ad["wall_s"],
+                    "s_per_iteration": ad["wall_s"] / it_ad, "E_first": ad["energies"][0],
+                    "E_min": min(ad["energies"]), "grad_max_first": ad["grad_max"][0] if ad["grad_max"] else None,
+                    "grad_max_last": ad["grad_max"][-1] if ad["grad_max"] else None, "message": ad["message"]},
+        "central_differences": {"iterations": len(fd["grad_max"]), "evaluations": fd["evaluations"],
+                                "wall_s": fd["wall_s"], "s_per_iteration": fd["wall_s"] / it_fd,
+                                "E_first": fd["energies"][0], "E_min": min(fd["energies"]), "fd_delta_bohr": 1e-3},
+        "adjoint_same_iterations": {"iterations": len(ad_short["grad_max"]), "E_min": min(ad_short["energies"]),
+                                    "wall_s": ad_short["wall_s"]},
+        "speedup_per_iteration": (fd["wall_s"] / it_fd) / (ad["wall_s"] / it_ad),
+    }
+
+
 def _fd_component(fz, pos, a, c, delta):
     """Central difference of the full solve for one force component (the
     step divided by the difference of the stored FP32 grid positions)."""
@@ -624,7 +757,13 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="c5",
-                    choices=("c1", "c2", "c3", "c4", "c5", "mf", "pd", "sic", "hairpin"))
+                    choices=("c1", "c2", "c3", "c4", "c5", "mf", "pd", "sic", "hairpin", "c5forces", "c4forces",
+                             "c4relax"))
+    ap.add_argument("--force-window", type=int, default=100, help="steps of the map a full adjoint differentiates")
+    ap.add_argument("--converge-max", type=int, default=10000)
+    ap.add_argument("--relax-iters", type=int, default=50)
+    ap.add_argument("--relax-fd-iters", type=int, default=2)
+    ap.add_argument("--relax-steps", type=int, default=200)
     ap.add_argument("--cpu-steps", type=int, default=80, help="steps of the all-core C baseline")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--timing-steps", type=int, default=200)
@@ -649,6 +788,13 @@ def main():
         return
     if args.workload == "hairpin":
         print(json.dumps(bench_hairpin()))
+        return
+    if args.workload in ("c5forces", "c4forces"):
+        print(json.dumps(bench_full_adjoint(args.workload[:2], args.force_window, args.converge_max,
+                                            args.force_components, log)))
+        return
+    if args.workload == "c4relax":
+        print(json.dumps(bench_relax("c4", args.relax_iters, args.relax_fd_iters, args.relax_steps, log)))
         return
     if args.workload == "mf":
         print(json.dumps(bench_mf(args)))

@@ -309,8 +309,21 @@ typedef struct rq_adjoint_spec {
                                       trajectory when its 2 n_steps + 1 fields fit in a quarter
                                       of the free device memory (no recomputation), else 64 */
   rq_schedule schedule;            /* boundary_interval is ignored (frozen labels) */
-  int32_t     reserved[4];         /* must be zero */
+  int32_t     start;               /* RQ_ADJ_START_TRIAL or RQ_ADJ_START_SAVED */
+  int32_t     reserved[3];         /* must be zero */
 } rq_adjoint_spec;
+/* The map starts from the trial U0(R), normalised, and `bootstrap` Poisson
+ * steps from P = 0 (the reference's initialize, cloud/solver.py:402-419). */
+#define RQ_ADJ_START_TRIAL 0
+/* The map starts from the U and P that rq_adjoint_save_start stored (e.g. a
+ * converged state at a reference geometry); they are constants of the map,
+ * so dE/dR is the derivative of the n_steps that follow.  bootstrap is
+ * ignored.  No counterpart in the reference: it re-converges each FD
+ * geometry from the trial. */
+#define RQ_ADJ_START_SAVED 1
+/* Stores the handle's current U and P as the start of RQ_ADJ_START_SAVED
+ * maps (two fields of device memory, kept until rq_destroy). */
+int rq_adjoint_save_start(rq_solver* h);
 
 /* Forward solve only; energy3 = (T, V_eK, V_ee). */
 int rq_solve_frozen(rq_solver* h, const rq_adjoint_spec* spec, double* energy3);

@@ -526,7 +526,7 @@ static napi_value js_set_atoms_s2(napi_env env, napi_callback_info info) {
   return NULL;
 }
 
-/* adjointGradient(h, {nSteps, bootstrap, checkpoint, poissonInterval, normInterval})
+/* adjointGradient(h, {nSteps, bootstrap, checkpoint, poissonInterval, normInterval, start: 'trial' | 'saved'})
  *   -> {energy: [T, VeK, Vee], dEdR: Float64Array(3n)} */
 static napi_value js_adjoint(napi_env env, napi_callback_info info) {
   napi_value argv[2];
@@ -541,6 +541,7 @@ static napi_value js_adjoint(napi_env env, napi_callback_info info) {
   sp.schedule.poisson_interval = (int32_t)prop_num(env, argv[1], "poissonInterval", 2);
   sp.schedule.norm_interval = (int32_t)prop_num(env, argv[1], "normInterval", 20);
   sp.schedule.boundary_interval = 1 << 30;
+  if (prop_str_eq(env, argv[1], "start", "saved")) sp.start = RQ_ADJ_START_SAVED;
   int32_t na = 0;
   rq_get_info(h, NULL, NULL, &na, NULL, NULL);
   double e3[3];
@@ -556,6 +557,17 @@ static napi_value js_adjoint(napi_env env, napi_callback_info info) {
   napi_set_named_property(env, o, "dEdR", f64array(env, g, 3 * (size_t)na));
   free(g);
   return o;
+}
+
+/* adjointSaveStart(h): the current U and P become the start of
+ * adjointGradient(h, {start: 'saved'}) */
+static napi_value js_adjoint_save_start(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!get_args(env, info, 1, argv)) return NULL;
+  rq_solver* h = get_handle(env, argv[0]);
+  if (!h) return NULL;
+  RQ_CALL(env, h, rq_adjoint_save_start(h));
+  return NULL;
 }
 
 static napi_value js_abi_version(napi_env env, napi_callback_info info) {
@@ -599,6 +611,7 @@ static napi_value init(napi_env env, napi_value exports) {
       {"setLockGroups", 0, js_set_lock_groups, 0, 0, 0, napi_enumerable, 0},
       {"setAtomsS2", 0, js_set_atoms_s2, 0, 0, 0, napi_enumerable, 0},
       {"adjointGradient", 0, js_adjoint, 0, 0, 0, napi_enumerable, 0},
+      {"adjointSaveStart", 0, js_adjoint_save_start, 0, 0, 0, napi_enumerable, 0},
   };
   NAPI_CALL(env, napi_define_properties(env, exports, sizeof props / sizeof props[0], props));
   return exports;

@@ -14,7 +14,9 @@ oracle's passes on the same trajectory (oracle/s1.py init_trial, normalize,
 jacobi, itp_update, energy and oracle/mg.py vcycle), Jacobi and multigrid
 mode, to 1e-12 relative -- so _restrict4, _prolong, _vcycle and
 _exact_solve are checked through it; its autograd gradient against central
-differences of itself (delta = 1e-4 bohr) to 1e-7 Ha/bohr.
+differences of itself (delta = 1e-4 bohr) to 1e-7 Ha/bohr; the saved-start
+map (``start``) by composition with the cold map (bit for bit) and by
+central differences of itself.
 
 Positions R are fractional grid units; gradients returned per bohr.
 The Poisson step is either the cloud/solver.py Jacobi step (``mode='jacobi'``:
@@ -106,9 +108,15 @@ def _vcycle(x, g, levels, nu, a, c):
 
 
 def trajectory_energy(R, rows, label, NN, screen, n_steps, bootstrap=0, poisson_interval=2, norm_interval=20,
-                      dv=None, mode="jacobi", mg_levels=3, mg_nu=2, mg_a=1.0 / 7.0, mg_c=7.0):
+                      dv=None, mode="jacobi", mg_levels=3, mg_nu=2, mg_a=1.0 / 7.0, mg_c=7.0, start=None,
+                      state_out=None):
     """E(R) for positions R (torch, (n,3), grid units) with Z, rc from ``rows``
-    and frozen ``label`` (S^3 int array).  Returns (E, T, VeK, Vee) tensors."""
+    and frozen ``label`` (S^3 int array).  Returns (E, T, VeK, Vee) tensors.
+
+    ``start`` = (U0, P0) (S^3 arrays): the map starts from these constants
+    instead of the normalised trial and the bootstrap Poisson steps (the
+    library's RQ_ADJ_START_SAVED); only K(R) then depends on R.
+    ``state_out`` (a dict) receives the final U and P as numpy arrays."""
     dt = torch.float64
     S = NN + 1
     h = screen / NN
@@ -178,10 +186,14 @@ def trajectory_energy(R, rows, label, NN, screen, n_steps, bootstrap=0, poisson_
             return P
         return _vcycle(P, rhs, mg_levels, mg_nu, mg_a, mg_c)
 
-    U = normalize(_pad(U0in))
-    P = torch.zeros((S,) * 3, dtype=dt)
-    for _ in range(bootstrap):
-        P = poisson(P, U)
+    if start is None:
+        U = normalize(_pad(U0in))
+        P = torch.zeros((S,) * 3, dtype=dt)
+        for _ in range(bootstrap):
+            P = poisson(P, U)
+    else:
+        U = torch.tensor(np.asarray(start[0], np.float64).reshape(S, S, S))
+        P = torch.tensor(np.asarray(start[1], np.float64).reshape(S, S, S))
     for step in range(1, n_steps + 1):
         if step % poisson_interval == 0:
             P = poisson(P, U)
@@ -195,6 +207,9 @@ def trajectory_energy(R, rows, label, NN, screen, n_steps, bootstrap=0, poisson_
     T = 0.5 * h * T.sum()
     VeK = -(K[1:-1, 1:-1, 1:-1] * u * u).sum() * h3
     Vee = (P[1:-1, 1:-1, 1:-1] * u * u).sum() * h3
+    if state_out is not None:
+        state_out["U"] = U.detach().numpy().copy()
+        state_out["P"] = P.detach().numpy().copy()
     return T + VeK + Vee, T, VeK, Vee
 
 

@@ -78,7 +78,11 @@ class RqMolecule(C.Structure):
 
 class RqAdjointSpec(C.Structure):
     _fields_ = [("n_steps", C.c_int64), ("bootstrap", C.c_int32), ("checkpoint_interval", C.c_int32),
-                ("schedule", RqSchedule), ("reserved", C.c_int32 * 4)]
+                ("schedule", RqSchedule), ("start", C.c_int32), ("reserved", C.c_int32 * 3)]
+
+
+RQ_ADJ_START_TRIAL = 0
+RQ_ADJ_START_SAVED = 1
 
 
 RQ_MF_MAX_ELEC = 16
@@ -149,6 +153,7 @@ PROTOTYPES = {
     "rq_pother_frame": (C.c_int, [_P, C.c_int64]),
     "rq_solve_frozen": (C.c_int, [_P, C.POINTER(RqAdjointSpec), _D]),
     "rq_adjoint_gradient": (C.c_int, [_P, C.POINTER(RqAdjointSpec), _D, _D]),
+    "rq_adjoint_save_start": (C.c_int, [_P]),
     "rq_batch_last_error": (C.c_char_p, [_P]),
     "rq_batch_create": (C.c_int, [C.POINTER(RqParams), C.POINTER(RqMolecule), _I, C.POINTER(_P)]),
     "rq_batch_destroy": (C.c_int, [_P]),

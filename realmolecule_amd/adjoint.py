@@ -79,6 +79,21 @@ class FrozenSolver:
     def _ekk(self, rows):
         return nuclear_energy(rows[:, :3].astype(np.float32), rows[:, 3], self.h)
 
+    def warm_start(self, pos_bohr, n_converge, bootstrap=None):
+        """Converge at ``pos_bohr`` (the cold map with ``n_converge`` steps),
+        store that U and P as the start of every later solve
+        (RQ_ADJ_START_SAVED) and return its energy.  Later energies and
+        gradients are those of the spec's n_steps from the stored state, at
+        any positions; the stored state does not move with them."""
+        rows = self._set_positions(pos_bohr)
+        cold = make_spec(n_converge, self.spec.bootstrap if bootstrap is None else bootstrap,
+                         1, self.spec.schedule.poisson_interval, self.spec.schedule.norm_interval)
+        e3 = (C.c_double * 3)()
+        self.s._chk(self.lib.rq_solve_frozen(self.hdl, C.byref(cold), e3))
+        self.s._chk(self.lib.rq_adjoint_save_start(self.hdl))
+        self.spec.start = L.RQ_ADJ_START_SAVED
+        return float(e3[0] + e3[1] + e3[2]) + self._ekk(rows)
+
     def refreeze(self, pos_bohr):
         """Re-freeze the labels at new nuclei: trial argmax at ``pos_bohr``."""
         self._set_positions(pos_bohr)

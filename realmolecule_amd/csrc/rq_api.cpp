@@ -48,6 +48,7 @@ struct GraphEntry {
 struct AdjState {
   float *lamU = nullptr, *lamU2 = nullptr, *lamP = nullptr, *lamP2 = nullptr, *gK = nullptr, *acc = nullptr;
   float *t1 = nullptr, *t2 = nullptr, *t3 = nullptr;
+  float *wU = nullptr, *wP = nullptr;  // saved start of RQ_ADJ_START_SAVED maps
   std::vector<float*> ckU, ckP;    // checkpoints (U after the step, P in use)
   std::vector<float*> segU, segP;  // one segment of recomputed inputs
   Fx* red = nullptr;  // fixed-point sums: 2 n_dom (norms, dots) + 3 n_atoms (dE/dR) + 3 n_dom + 3 (energy)
@@ -815,7 +816,7 @@ void free_all(rq_solver* s) {
   }
   for (auto e : s->ev_free) (void)hipEventDestroy(e);
   AdjState& A = s->adj;
-  for (float* p : {A.lamU, A.lamU2, A.lamP, A.lamP2, A.gK, A.acc, A.t1, A.t2, A.t3}) fr(p);
+  for (float* p : {A.lamU, A.lamU2, A.lamP, A.lamP2, A.gK, A.acc, A.t1, A.t2, A.t3, A.wU, A.wP}) fr(p);
   for (float* p : A.ckU) fr(p);
   for (float* p : A.ckP) fr(p);
   for (float* p : A.segU) fr(p);
@@ -989,6 +990,10 @@ int frozen_run(rq_solver* s, const rq_adjoint_spec* sp, bool backward, double* E
   if (!sp) return fail(s, RQ_EINVAL, "null spec");
   for (int r : sp->reserved)
     if (r) return fail(s, RQ_EINVAL, "rq_adjoint_spec.reserved must be zero");
+  if (sp->start != RQ_ADJ_START_TRIAL && sp->start != RQ_ADJ_START_SAVED)
+    return fail(s, RQ_EINVAL, "rq_adjoint_spec.start %d", sp->start);
+  const bool saved = sp->start == RQ_ADJ_START_SAVED;
+  if (saved && !s->adj.wU) return fail(s, RQ_ESTATE, "RQ_ADJ_START_SAVED before rq_adjoint_save_start");
   const rq_schedule& sc = sp->schedule;
   if (sp->n_steps < 0 || sp->bootstrap < 0 || sc.poisson_interval < 1 || sc.norm_interval < 1)
     return fail(s, RQ_EINVAL, "bad adjoint spec");
@@ -1032,13 +1037,21 @@ int frozen_run(rq_solver* s, const rq_adjoint_spec* sp, bool backward, double* E
   }
   const Tile& T = s->tile;
   // ---- forward ----
-  launch_init_frozen(s->st, s->S, s->h, s->h2, s->atoms_d, s->n_atoms, s->lab, s->U, s->K);
-  LAUNCHCHK(s);
-  if ((rc = normalize_on(s, s->U))) return rc;
-  launch_fill(s->st, s->P, s->S3, 0.0f);
-  for (int b = 0; b < sp->bootstrap; ++b) {
-    if ((rc = poisson_into(s, s->P, s->P2, s->U, A.t1, A.t2))) return rc;
-    std::swap(s->P, s->P2);
+  if (saved) {
+    // K(R) as in the trial pass (its trial amplitudes go to scratch); U and P
+    // are the saved constants
+    launch_init_frozen(s->st, s->S, s->h, s->h2, s->atoms_d, s->n_atoms, s->lab, A.t1, s->K);
+    LAUNCHCHK(s);
+    if ((rc = copy_field(s, s->U, A.wU)) || (rc = copy_field(s, s->P, A.wP))) return rc;
+  } else {
+    launch_init_frozen(s->st, s->S, s->h, s->h2, s->atoms_d, s->n_atoms, s->lab, s->U, s->K);
+    LAUNCHCHK(s);
+    if ((rc = normalize_on(s, s->U))) return rc;
+    launch_fill(s->st, s->P, s->S3, 0.0f);
+    for (int b = 0; b < sp->bootstrap; ++b) {
+      if ((rc = poisson_into(s, s->P, s->P2, s->U, A.t1, A.t2))) return rc;
+      std::swap(s->P, s->P2);
+    }
   }
   if (backward) {
     if ((rc = copy_field(s, A.ckU[0], s->U)) || (rc = copy_field(s, A.ckP[0], s->P))) return rc;
@@ -1132,13 +1145,19 @@ int frozen_run(rq_solver* s, const rq_adjoint_spec* sp, bool backward, double* E
       if (t % sc.poisson_interval == 0 && (rc = poisson_adj(s, Uin))) return rc;
     }
   }
-  // bootstrap Poisson steps (P_in = 0 at the first one; its adjoint is dropped)
-  for (int b = 0; b < sp->bootstrap; ++b)
-    if ((rc = poisson_adj(s, A.ckU[0]))) return rc;
-  // initial normalise: recompute the trial field into t1
-  launch_init_frozen(s->st, s->S, s->h, s->h2, s->atoms_d, s->n_atoms, s->lab, A.t1, A.t2);
-  LAUNCHCHK(s);
-  if ((rc = normalize_adj(s, A.t1))) return rc;
+  if (saved) {
+    // the start does not depend on R: only K(R) carries the gradient
+    launch_fill(s->st, A.lamU, s->S3, 0.0f);
+    LAUNCHCHK(s);
+  } else {
+    // bootstrap Poisson steps (P_in = 0 at the first one; its adjoint is dropped)
+    for (int b = 0; b < sp->bootstrap; ++b)
+      if ((rc = poisson_adj(s, A.ckU[0]))) return rc;
+    // initial normalise: recompute the trial field into t1
+    launch_init_frozen(s->st, s->S, s->h, s->h2, s->atoms_d, s->n_atoms, s->lab, A.t1, A.t2);
+    LAUNCHCHK(s);
+    if ((rc = normalize_adj(s, A.t1))) return rc;
+  }
   Fx* gout = A.red + kFxWords * 2 * (size_t)s->n_atoms;
   zero_fx(s, gout, 3 * (size_t)s->n_atoms);
   launch_pos_grad(s->st, s->S, s->h, s->h2, A.gK, A.lamU, s->lab, s->atoms_d, s->n_atoms, gout);
@@ -1685,6 +1704,17 @@ int rq_step(rq_solver* h, int64_t first, int32_t n, const rq_schedule* sc) {
       return fail(h, RQ_EUNSTABLE, "Numerical instability: non-finite amplitude at a normalisation in steps %lld..%lld",
                   (long long)first + 1, (long long)last);
   }
+  return RQ_OK;
+}
+
+int rq_adjoint_save_start(rq_solver* h) {
+  if (int g_ = gate(h)) return g_;
+  if (!h->initialized) return fail(h, RQ_ESTATE, "rq_adjoint_save_start before the fields are set");
+  if (int rc = adj_alloc(h, &h->adj.wU)) return rc;
+  if (int rc = adj_alloc(h, &h->adj.wP)) return rc;
+  if (int rc = copy_field(h, h->adj.wU, h->U)) return rc;
+  if (int rc = copy_field(h, h->adj.wP, h->P)) return rc;
+  HIPCHK(h, hipStreamSynchronize(h->st));
   return RQ_OK;
 }
 

@@ -135,3 +135,49 @@ def test_relax_h2_on_adjoint_forces():
     assert out["evaluations"] >= 3
     assert min(E) < E[0] - 1e-4
     assert out["grad_max"][-1] < out["grad_max"][0]
+
+
+def test_saved_start_composes_with_the_cold_map_bitexact():
+    """40 cold steps, saved, then 20 steps from the saved state: the energy of
+    60 cold steps, bit for bit (same kernels in the same order)."""
+    NN, screen = 16, 16 * 0.3
+    rows = three_atoms(NN)
+    fz = FrozenSolver(rows, NN, screen, 60, bootstrap=5, checkpoint=16)
+    pos = rows[:, :3] * fz.h
+    E60 = fz.energy(pos)
+    E40 = fz.warm_start(pos, 40)
+    fz.spec.n_steps = 20
+    E_warm = fz.energy(pos)
+    print(f"cold 40 {E40:.12f}, cold 60 {E60:.12f}, saved 40 + 20 {E_warm:.12f}")
+    assert E_warm == E60
+    fz.close()
+
+
+@pytest.mark.parametrize("mode", ["jacobi", "multigrid"])
+def test_saved_start_gradient_matches_fp64_autograd(mode):
+    NN, screen = 16, 16 * 0.3
+    rows = three_atoms(NN)
+    pi = 2 if mode == "jacobi" else 1
+    extra = {} if mode == "jacobi" else {"poisson": "multigrid", "mg_levels": 3}
+    fz = FrozenSolver(rows, NN, screen, 30, bootstrap=5, checkpoint=8, poisson_interval=pi, **extra)
+    pos = rows[:, :3] * fz.h
+    fz.warm_start(pos, 60)
+    U0, P0 = fz.s.U.get().astype(np.float64), fz.s.P.get().astype(np.float64)
+    # a displaced geometry: the saved state stays the one converged at pos
+    pos2 = pos.copy()
+    pos2[0] += np.array([0.05, -0.03, 0.02])
+    E, g = fz.energy_and_gradient(pos2)
+    rows2 = rows.copy()
+    rows2[:, :3] = (pos2 / fz.h).astype(np.float32)
+    lab = fz.s.label.get()
+    okw = dict(poisson_interval=pi, norm_interval=20, start=(U0, P0))
+    if mode == "multigrid":
+        okw.update(mode="multigrid", mg_levels=3)
+    Eo, go = OA.energy_and_gradient(rows2, lab, NN, screen, 30, **okw)
+    go = go - s1.nuclear_forces(rows2[:, :3], rows2[:, 3], fz.h)
+    Eo += s1.nuclear_energy(rows2[:, :3], rows2[:, 3], fz.h)
+    err = np.abs(g - go).max()
+    print(f"saved start, {mode}: E gpu {E:.9f} oracle {Eo:.9f}; max |dE/dR diff| {err:.2e}")
+    assert abs(E - Eo) < 2e-6 * abs(Eo)
+    assert err < 1e-4 + 1e-4 * np.abs(go).max()
+    fz.close()

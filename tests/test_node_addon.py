@@ -17,7 +17,7 @@ needs_node = pytest.mark.skipif(NODE is None or not os.path.exists(ADDON), reaso
 EXPORTS = ["abiVersion", "dvForElectrons", "create", "destroy", "info", "initTrial", "initialize", "poisson",
            "twogridCorrect", "update", "normalize", "boundary", "recomputeK", "step", "sync", "energy", "forcesHF",
            "forcesGradP", "readField", "displaySlice", "potherFrame", "writeField", "setAtoms", "setLockGroups", "adjointGradient",
-           "setAtomsS2"]
+           "setAtomsS2", "adjointSaveStart"]
 
 
 def run_node(code, timeout=60):

@@ -9,7 +9,10 @@ truth of the HIP adjoint tests, so it is checked here against:
     on the same trajectory, in Jacobi and multigrid mode, to 1e-12;
   * its own autograd gradient against central differences, to 1e-7 Ha/bohr.
 It also pins oracle.s1.hf_forces by autograd: the Hellmann-Feynman force is
--dV_eK/dR at fixed U.
+-dV_eK/dR at fixed U.  The saved-start map (``start``) is pinned by
+composition: n1 cold steps, then n2 steps from the state they reach, give
+the energy of n1 + n2 cold steps, bit for bit; its gradient holds the K(R)
+part only (central differences of itself).
 """
 import numpy as np
 import pytest
@@ -131,3 +134,34 @@ def test_hf_forces_are_minus_gradient_of_electron_nucleus_energy():
     F = s1.hf_forces(U, rows, st.g)
     assert np.allclose(F, -dVdR, rtol=1e-10, atol=1e-13), (F, -dVdR)
     assert np.abs(F).max() > 1e-3
+
+
+@pytest.mark.parametrize("mode", ["jacobi", "multigrid"])
+def test_saved_start_composes_with_the_cold_map(mode):
+    NN, screen = 12, 12 * 0.35
+    c = NN / 2
+    rows = np.array([[c - 2.3, c + 0.4, c + 0.2, 1.0, 0.0], [c + 0.3, c - 0.2, c - 0.5, 2.0, 0.0],
+                     [c + 2.6, c + 1.1, c + 0.7, 1.0, 0.0]])
+    lab = A.frozen_labels(rows, NN, screen)
+    kw = dict(mode=mode, mg_levels=2, dv=0.12)
+    R = torch.tensor(rows[:, :3], dtype=torch.float64)
+    st = {}
+    A.trajectory_energy(R, rows, lab, NN, screen, 20, bootstrap=3, state_out=st, **kw)
+    E_cold = float(A.trajectory_energy(R, rows, lab, NN, screen, 34, bootstrap=3, **kw)[0])
+    E_warm = float(A.trajectory_energy(R, rows, lab, NN, screen, 14, start=(st["U"], st["P"]), **kw)[0])
+    assert E_warm == E_cold
+    # the gradient of the saved-start map against central differences of it
+    E, g = A.energy_and_gradient(rows, lab, NN, screen, 14, start=(st["U"], st["P"]), **kw)
+    h = screen / NN
+    delta = 1e-4
+    for a_, d in ((0, 0), (1, 2), (2, 1)):
+        Rp = R.clone()
+        Rm = R.clone()
+        Rp[a_, d] += delta / h
+        Rm[a_, d] -= delta / h
+        Ep = float(A.trajectory_energy(Rp, rows, lab, NN, screen, 14, start=(st["U"], st["P"]), **kw)[0])
+        Em = float(A.trajectory_energy(Rm, rows, lab, NN, screen, 14, start=(st["U"], st["P"]), **kw)[0])
+        assert abs((Ep - Em) / (2 * delta) - g[a_, d]) < 1e-7
+    # and it differs from the cold map's gradient (the start's R dependence is gone)
+    _, gc = A.energy_and_gradient(rows, lab, NN, screen, 34, bootstrap=3, **kw)
+    assert np.abs(gc - g).max() > 1e-4
