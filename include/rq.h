@@ -334,8 +334,9 @@ int rq_adjoint_gradient(rq_solver* h, const rq_adjoint_spec* spec, double* energ
 
 /* --- batches: many molecules with uneven grids in one launch per pass --- */
 /* A batch advances n molecules together: every pass (each V-cycle level,
- * the ITP update, normalise, energy, init) is ONE persistent launch over the
- * concatenated tiles of all molecules, and step chunks replay from HIP
+ * the ITP update, normalise, energy, init) is one launch over the
+ * concatenated tiles of all molecules (one workgroup per tile of every
+ * molecule, not a fixed resident grid), and step chunks replay from HIP
  * graphs.  Semantics per molecule: cloud/solver.py trial init, normalise,
  * multigrid Poisson (rq_params: mg_levels, mg_nu1, mg_nu2, mg_a, mg_c; every
  * NN divisible by 2^(mg_levels-1), coarsest interior <= 19), ITP, normalise

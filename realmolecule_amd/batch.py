@@ -3,8 +3,9 @@
 The Python face of ``rq_batch_*`` (include/rq.h).  Replaces a geometry scan
 that constructs and runs the reference's per-molecule solver once per
 conformer (``cloud/solver.py`` ``RealQMSolver(config).run(...)``): all
-conformers advance together, every pass of the step is one persistent
-launch over all their tiles, and step chunks replay from HIP graphs.
+conformers advance together, every pass of the step is one launch over
+all their tiles (a workgroup per tile), and step chunks replay from HIP
+graphs.
 """
 from __future__ import annotations
 
@@ -25,7 +26,8 @@ class BatchError(RuntimeError):
 class Batch:
     def __init__(self, molecules, mg_levels=4, mg_nu=2, graph=True, dv=None, tiles=None):
         """molecules: list of (rows[n,5] grid units, NN, screen).  ``tiles``:
-        optional (bx, by, {NN: ich}) launch shape, e.g. from the autotuner."""
+        optional (bx, by, {NN: ich}) launch shape, or 'tuned' for the
+        autotuner's cached shapes of these grid sizes (autotune.batch_tiles)."""
         self._lib = L.load()
         self.NN = [int(m[1]) for m in molecules]
         self.screen = [float(m[2]) for m in molecules]
@@ -49,6 +51,9 @@ class Batch:
         self._h = h
         if graph:
             self._chk(self._lib.rq_batch_set_graph(h, 1))
+        if tiles == "tuned":
+            from .autotune import batch_tiles
+            tiles = batch_tiles(self.NN)
         if tiles is not None:
             self.set_tiles(*tiles)
 

@@ -1,8 +1,9 @@
 // Batched solver: many molecules with uneven grids advanced together.
 //
 // Every pass of the step (V-cycle levels, ITP, normalise, energy, init) is
-// ONE persistent launch over the concatenated tile space of all molecules:
-// a fixed grid of workgroups strides over work items, each item decodes its
+// one launch over the concatenated tile space of all molecules: one
+// workgroup per work item (the grid rounded up to a multiple of 8; the
+// grid-stride loop only matters beyond 2^20 items), each item decodes its
 // molecule by binary search over per-level descriptors and runs the same
 // per-column body as the single-grid kernels (rq_stencil.h).  The per-step
 // launch count is that of one molecule, independent of the batch size, and
@@ -84,8 +85,8 @@ __device__ __forceinline__ int find_node(const BMol* __restrict__ M, int nm, int
   return lo;
 }
 
-// Work item t -> XCD-contiguous item (the persistent grid is a multiple of 8,
-// so item t runs on XCD t mod 8).
+// Work item t -> XCD-contiguous item (the grid is a multiple of 8, so item
+// t runs on XCD t mod 8).
 __device__ __forceinline__ int xcd_item(int t, int n) {
   const int per = n >> 3, rem = n & 7;
   const int xcd = t & 7, local = t >> 3;

@@ -20,7 +20,8 @@ Extra constructor keys (all optional; defaults reproduce cloud/solver.py):
 ``mg_nu2``, ``mg_omega``, ``jacobi_sweeps``, ``smooth_a``, ``smooth_c``,
 ``boundary`` ('score' | 'levelset' | 'off'), ``init`` ('s1' | 's2'),
 ``energy`` ('s1' | 's2'), ``rc_dirichlet`` (bool), ``dv``, ``graph`` (bool: replay
-step chunks from HIP graphs), ``bootstrap_poisson`` (Poisson steps in initialize, 100),
+step chunks from HIP graphs), ``tiles`` ('tuned': the autotuner's cached launch
+shape for the grid size), ``bootstrap_poisson`` (Poisson steps in initialize, 100),
 ``pother`` ('total' | 'direct' | 'frame' | 'direct_sweeps': the potential the
 ITP update and the energy see) and ``sic_interval`` (self-interaction
 correction every n frames with 'frame').  Atom dicts may carry the
@@ -212,6 +213,12 @@ class RealQMSolver:
         self._lib.rq_default_schedule(C.byref(self._sched))
         if config.get('graph'):
             self._chk(self._lib.rq_set_graph(self._h, 1))
+        # 'tuned': the autotuner's cached fine-level launch shape for this grid
+        # size, when one exists (results do not depend on the shape)
+        self.tiles_tuned = False
+        if config.get('tiles') == 'tuned':
+            from .autotune import apply_cached
+            self.tiles_tuned = apply_cached(self)
 
         self.E_T = 0.0
         self.E_eK = 0.0
