@@ -45,6 +45,11 @@ BYTES_PER_VOXEL = {
     "restrict": 8.5,     # P, U in; coarse rhs out (1/8 of 4 B)
     "prolong": 8.5,      # P in and out, coarse correction in (1/8)
     "smooth_itp": 22.0,  # last post-sweep + ITP: P, U, K, label in; P', U' out
+    # pre-sweep pair + residual + restriction in one pass: two sweeps (2 x 12)
+    # and the residual + restriction (8.5), each counted as its own unit
+    "smooth_restrict": 32.5,
+    # correction + post-sweep pair + ITP update in one pass: 8.5 + 2 x 12 + 18
+    "post_itp": 50.5,
 }
 
 # Fine-level kernel of each family, as rocprofv3 names it, for the HBM
@@ -52,6 +57,7 @@ BYTES_PER_VOXEL = {
 # Several kernels run on every level; the fine level's dispatches are the
 # ones within a factor 2 of the family's largest (the next level is 1/8).
 PMC_KERNEL = {"smooth": ("k_smooth<true>", "k_smooth2_march<true"), "smooth_itp": ("k_smooth_itp",),
+              "smooth_restrict": ("k_smooth2_resres_march",), "post_itp": ("k_post_march",),
               "itp": ("k_itp(",), "restrict": ("k_resres_march<true", "k_resres_tiled<true"),
               "prolong": ("k_prolong_correct", "k_prolong_smooth")}
 COPY_MIB = 512  # plain device copy of known size that calibrates the counters
@@ -874,11 +880,15 @@ def main():
     # algorithmic bytes per launch: per-voxel figure x units per launch.  The
     # fine smoother family runs either one sweep per launch or a sweep pair
     # (temporal blocking on grids beyond the MALL); the fine level does
-    # nu1 + nu2 = 4 sweeps per step, one of them inside smooth_itp when that
-    # fused pass runs, so sweeps per launch = (4 - smooth_itp/step) / smooth/step.
+    # nu1 + nu2 = 4 sweeps per step, one of them inside smooth_itp and two
+    # inside smooth_restrict or post_itp when those fused passes run, so
+    # sweeps per launch = (4 - smooth_itp/step - 2 smooth_restrict/step
+    # - 2 post_itp/step) / smooth/step.
     bpv = dict(BYTES_PER_VOXEL)
     if "smooth" in kernels:
-        sweeps = 4 - kernels.get("smooth_itp", {}).get("launches_per_step", 0.0)
+        sweeps = (4 - kernels.get("smooth_itp", {}).get("launches_per_step", 0.0)
+                  - 2 * kernels.get("smooth_restrict", {}).get("launches_per_step", 0.0)
+                  - 2 * kernels.get("post_itp", {}).get("launches_per_step", 0.0))
         per_launch = sweeps / kernels["smooth"]["launches_per_step"]
         kernels["smooth"]["sweeps_per_launch"] = round(per_launch, 3)
         bpv["smooth"] = BYTES_PER_VOXEL["smooth"] * per_launch

@@ -375,11 +375,13 @@ int rq_batch_timing_read(rq_batch* b, double* ms, int64_t* launches, int32_t n_f
  * 5 fine prolongation+correction, 6 coarse prolongation+correction,
  * 7 exact coarsest solve, 8 normalise, 9 boundary rule, 10 energy, 11 forces,
  * 12 last fine smoother sweep fused with the ITP update, 13 the last two
- * levels of the V-cycle in one single-workgroup launch.  With fused passes
- * (the default) family 3 is the fused residual+restriction and family 5 the
- * fused prolongation+first post-smoothing sweep.  Events bracket each launch
- * on the handle's stream. */
-#define RQ_TIMING_FAMILIES 14
+ * levels of the V-cycle in one single-workgroup launch, 14 the fine
+ * pre-smoothing pair with the residual and restriction in one launch, 15
+ * the fine correction, post-smoothing pair and ITP update in one launch.  With
+ * fused passes (the default) family 3 is the fused residual+restriction and
+ * family 5 the fused prolongation+first post-smoothing sweep.  Events bracket
+ * each launch on the handle's stream. */
+#define RQ_TIMING_FAMILIES 16
 int rq_timing_enable(rq_solver* h, int32_t on);
 int rq_timing_read(rq_solver* h, double* ms, int64_t* launches, int32_t n_families);
 

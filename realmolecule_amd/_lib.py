@@ -39,7 +39,7 @@ RQ_POTHER_DIRECT_SWEEPS = 3
 
 FIELDS = {"U": 0, "U2": 1, "K": 2, "P": 3, "P2": 4, "rho": 5, "label": 6, "bestU": 7, "W": 8, "Pother": 9}
 FAMILIES = ["itp", "smooth", "smooth_c", "restrict", "restrict_c", "prolong", "prolong_c", "coarse_solve",
-            "normalize", "boundary", "energy", "force", "smooth_itp", "bottom"]
+            "normalize", "boundary", "energy", "force", "smooth_itp", "bottom", "smooth_restrict", "post_itp"]
 
 
 class RqAtom(C.Structure):

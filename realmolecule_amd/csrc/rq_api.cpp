@@ -131,6 +131,15 @@ struct rq_solver {
   // it a level's three fields stay in the 256 MB MALL and two plain sweeps
   // measured faster (RQ_SMOOTH2_MIN_S overrides)
   int smooth2_min_s = kSmooth2MinS;
+  // with smooth2: the fine level's pre-sweep pair, residual and restriction
+  // as one march (RQ_SMOOTH2_RESRES=0 runs the pair and the restriction
+  // march as two launches); s2rr_blocks: its target workgroup count
+  bool s2rr = true;
+  int s2rr_blocks = 0;
+  // with smooth2 and an ITP update due: the fine correction, post-sweep pair
+  // and ITP update as one march (RQ_POST_FUSE=0: separate launches)
+  bool post_fuse = true;
+  int post_blocks = 0;
   // largest level (nodes per axis) that runs the one-pass prolongation+sweep;
   // 0: none -- on gfx950 the separate correction and sweep launches measured
   // faster on every level of the 192^3 workload.  RQ_FUSE_MAX_S overrides it.
@@ -469,23 +478,45 @@ int vcycle_coarse(rq_solver* s, int l) {
 // and the ITP update run as one kernel and *did_itp is set.
 int vcycle_fine(rq_solver* s, bool fuse_itp, bool* did_itp) {
   if (did_itp) *did_itp = false;
+  const bool itp_due = fuse_itp && did_itp;
   fuse_itp = fuse_itp && s->S <= s->itp_fuse_max_s;
   const float a = s->p.mg_a, c = s->p.mg_c;
-  if (int rc = smooth_fine_n(s, a, c, s->p.mg_nu1)) return rc;
   const bool small = s->fused && s->S <= s->fuse_max_s;
-  {
-    Timed t(s, F_RESTRICT);
-    if (s->fused) {
-      launch_residual_restrict(s->st, true, s->S, s->P, s->U, s->two_pi, s->h2, s->g[1], 1.0f);
-    } else {
-      launch_residual_fine(s->st, s->tile, s->S, s->P, s->U, s->two_pi, s->h2, s->P2);
-      launch_restrict4(s->st, s->S, s->P2, s->g[1], 1.0f);
+  if (s->fused && s->smooth2 && s->s2rr && s->S >= s->smooth2_min_s && s->p.mg_nu1 == 2) {
+    {
+      Timed t(s, F_SMOOTH_RESTRICT);
+      launch_smooth2_resres(s->st, s->S, s->P, s->P2, s->U, a, c, s->two_pi, s->h2, s->g[1], s->s2rr_blocks);
     }
+    LAUNCHCHK(s);
+    std::swap(s->P, s->P2);
+  } else {
+    if (int rc = smooth_fine_n(s, a, c, s->p.mg_nu1)) return rc;
+    {
+      Timed t(s, F_RESTRICT);
+      if (s->fused) {
+        launch_residual_restrict(s->st, true, s->S, s->P, s->U, s->two_pi, s->h2, s->g[1], 1.0f);
+      } else {
+        launch_residual_fine(s->st, s->tile, s->S, s->P, s->U, s->two_pi, s->h2, s->P2);
+        launch_restrict4(s->st, s->S, s->P2, s->g[1], 1.0f);
+      }
+    }
+    LAUNCHCHK(s);
   }
-  LAUNCHCHK(s);
   int rc = vcycle_coarse(s, 1);
   if (rc) return rc;
   int post = s->p.mg_nu2;
+  if (itp_due && s->fused && s->smooth2 && s->post_fuse && s->S >= s->smooth2_min_s && post == 2 && !small) {
+    {
+      Timed t(s, F_POST_ITP);
+      launch_post_itp(s->st, s->S, s->P, s->e0[1], s->P2, s->U, s->U2, s->K, s->lab, a, c, s->two_pi, s->h2,
+                      s->half_dv, s->dtv, s->post_blocks);
+    }
+    LAUNCHCHK(s);
+    std::swap(s->P, s->P2);
+    std::swap(s->U, s->U2);
+    *did_itp = true;
+    return RQ_OK;
+  }
   if (small && post >= 1) {
     {
       Timed t(s, F_PROLONG);
@@ -1263,6 +1294,10 @@ int rq_create(const rq_params* pp, const rq_atom* atoms, int32_t n_atoms, rq_sol
   if (const char* fi = std::getenv("RQ_ITP_FUSE_MAX_S")) s->itp_fuse_max_s = std::atoi(fi);
   if (const char* s2 = std::getenv("RQ_SMOOTH2")) s->smooth2 = std::atoi(s2) != 0;
   if (const char* s2m = std::getenv("RQ_SMOOTH2_MIN_S")) s->smooth2_min_s = std::atoi(s2m);
+  if (const char* e = std::getenv("RQ_SMOOTH2_RESRES")) s->s2rr = std::atoi(e) != 0;
+  if (const char* e = std::getenv("RQ_SMOOTH2_RESRES_BLOCKS")) s->s2rr_blocks = std::atoi(e);
+  if (const char* e = std::getenv("RQ_POST_FUSE")) s->post_fuse = std::atoi(e) != 0;
+  if (const char* e = std::getenv("RQ_POST_BLOCKS")) s->post_blocks = std::atoi(e);
   s->NN = p.NN;
   s->S = p.NN + 1;
   s->S3 = (size_t)s->S * s->S * s->S;

@@ -22,6 +22,8 @@ constexpr int kLegsMaxS = 65;           // one down-leg + one up-leg launch per 
 constexpr int kResResMarchCJ = 2;       // coarse j-columns per residual+restriction march workgroup (128 CJ threads)
 constexpr int kResResMarchCP = 2;       // coarse planes restricted per barrier round of that march
 constexpr int kSmooth2Blocks = 8192;    // target workgroup count of the two-sweep march
+constexpr int kSmooth2ResresBlocks = 4096;
+constexpr int kPostBlocks = 8192;  // target workgroup count of the correction + post-sweeps + ITP march  // target workgroup count of the pre-smoothing + restriction march
 constexpr int kSmooth2MinS = 320;       // smallest level (nodes per axis) that runs sweep pairs as that march
 constexpr int kDirectSweeps = 10;       // damped sweeps per step of each direct potential (molecule.js:2835 JACOBI_DIRECT)
 constexpr int kDomStreams = 4;          // concurrent per-domain V-cycles of the direct potential (one stream each)
@@ -97,7 +99,7 @@ struct AtomCells {
 // its own algorithmic bytes.
 enum Family {
   F_ITP = 0, F_SMOOTH, F_SMOOTH_C, F_RESTRICT, F_RESTRICT_C, F_PROLONG, F_PROLONG_C, F_COARSE, F_NORM,
-  F_BOUNDARY, F_ENERGY, F_FORCE, F_SMOOTH_ITP, F_BOTTOM, F_COUNT
+  F_BOUNDARY, F_ENERGY, F_FORCE, F_SMOOTH_ITP, F_BOTTOM, F_SMOOTH_RESTRICT, F_POST_ITP, F_COUNT
 };
 
 // ---- launchers (rq_kernels.hip) ----
@@ -162,6 +164,15 @@ void launch_smooth2_from_zero(hipStream_t st, const Tile& t, int S, const float*
 // two damped-Jacobi sweeps x -> xn in one march (equal to two launch_smooth_* calls)
 void launch_smooth2(hipStream_t st, bool fine, int S, const float* x, float* xn, const float* src, float a, float c,
                     float alpha, float beta);
+// fine level (NN even): two sweeps x -> xn and the coarse rhs 4 FW(residual
+// of xn) into gc, one march (k_smooth2_resres_march); blocks <= 0: default
+void launch_smooth2_resres(hipStream_t st, int Sf, const float* x, float* xn, const float* src, float a, float c,
+                           float alpha, float beta, float* gc, int blocks);
+// fine level: x + P e, two sweeps -> xn, and the ITP update of U with xn ->
+// Un, one march (k_post_march); blocks <= 0: default
+void launch_post_itp(hipStream_t st, int Sf, const float* x, const float* e, float* xn, const float* U, float* Un,
+                     const float* K, const uint16_t* lab, float a, float c, float alpha, float beta, float half_dv,
+                     float dtv, int blocks);
 void launch_residual_restrict(hipStream_t st, bool fine, int Sf, const float* x, const float* src, float alpha,
                               float beta, float* gc, float sign);
 void launch_prolong_smooth(hipStream_t st, const Tile& t, bool fine, int Sf, const float* x, const float* e,

@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cmath>
 #include <mutex>
 
@@ -464,6 +465,344 @@ __global__ __launch_bounds__(256) void k_smooth2_march(int S, int CI, const floa
       if (oid[q] >= 0 && Ih < I1) xn[Ih * S2 + oid[q]] = v;
     }
     lds_barrier();
+  }
+}
+
+// The fine level's pre-smoothing pair, residual and 4 FW restriction as one
+// march along i (temporal blocking of the whole down step).  A workgroup
+// owns CK x CJ coarse interior columns (K0.., J0..) and coarse planes
+// [I0, I1).  Fine regions around the tile, per plane: R0 (2CK+1) x (2CJ+1)
+// nodes whose residuals the tile's restriction reads, R1 = R0 + 1 (second
+// sweep) and R2 = R0 + 2 (first sweep, from x in global memory as
+// k_smooth2_march forms it).  Fine plane p of the march: first sweep at p
+// into an LDS ring (with U, which the later stages read from there), second
+// sweep at p - 1 from it into a second ring (and to xn where the tile owns
+// the node), residual at p - 2 into a third ring, and the coarse plane whose
+// three fine planes are complete is restricted from the third ring.  The
+// global loads of plane p + 1 are issued right after plane p's first sweep,
+// so they are in flight across the later stages and barriers (the barriers
+// wait for LDS traffic only).  Rings of three planes with a barrier after
+// each of the first three stages: a slot is rewritten only after the
+// barriers that end every read of it.  The smooth_col, residual and
+// restrict27 expressions in the same order, so xn and gc equal two k_smooth
+// launches followed by k_resres_march bit for bit.
+template <int CK, int CJ, int NT>
+__global__ __launch_bounds__(NT) void k_smooth2_resres_march(int Sf, int CI, const float* __restrict__ x,
+                                                              float* __restrict__ xn, const float* __restrict__ src,
+                                                              float a, float c, float alpha, float beta,
+                                                              float* __restrict__ gc) {
+  constexpr int RING = 3;
+  constexpr int W0 = 2 * CK + 1, H0 = 2 * CJ + 1, W1 = W0 + 2, H1 = H0 + 2, W2 = W0 + 4, H2 = H0 + 4;
+  constexpr int PL0 = W0 * H0, PL1 = W1 * H1, PL2 = W2 * H2;
+  constexpr int P0 = (PL0 + NT - 1) / NT, P1 = (PL1 + NT - 1) / NT, P2 = (PL2 + NT - 1) / NT;
+  static_assert(CK * CJ <= NT, "one thread per coarse column");
+  __shared__ float ring2[RING * PL2];  // first sweep
+  __shared__ float ringu[RING * PL2];  // U on the first-sweep region
+  __shared__ float ring1[RING * PL1];  // second sweep
+  __shared__ float ring0[RING * PL0];  // residual
+  const int n = Sf - 2, nc = (Sf - 1) / 2 - 1, Sc = nc + 2, S2 = Sf * Sf;
+  const int gx = (nc + CK - 1) / CK, gy = (nc + CJ - 1) / CJ;
+  const int t = xcd_block();
+  const int K0 = 1 + (t % gx) * CK, J0 = 1 + ((t / gx) % gy) * CJ, I0 = 1 + (t / (gx * gy)) * CI;
+  const int I1 = min(I0 + CI, nc + 1);
+  const int tid = threadIdx.x;
+  // fine nodes the tile writes: [2K0 - 1, 2K0 + 2CK - 2] per axis, to n on the last tile
+  const int k_lo = 2 * K0 - 1, k_hi = (K0 + CK - 1 >= nc) ? n : 2 * K0 + 2 * CK - 2;
+  const int j_lo = 2 * J0 - 1, j_hi = (J0 + CJ - 1 >= nc) ? n : 2 * J0 + 2 * CJ - 2;
+  const int i_lo = 2 * I0 - 1, i_hi = (I1 == nc + 1) ? n : 2 * I1 - 2;
+  // first-sweep region nodes of this thread; loads clamped into the grid,
+  // interior nodes read their in-plane neighbours at oj / ok, others keep x
+  int col[P2], oj[P2], ok[P2];
+  bool upd[P2];
+  float xm[P2], xc[P2], xp[P2], nb[P2][4], u[P2];
+  const int p_first = 2 * I0 - 3, p_last = 2 * I1 + 1;
+#pragma unroll
+  for (int q = 0; q < P2; ++q) {
+    const int o = min(tid + q * NT, PL2 - 1);
+    const int j = 2 * J0 - 3 + o / W2, k = 2 * K0 - 3 + o % W2;
+    upd[q] = tid + q * NT < PL2 && j >= 1 && j <= n && k >= 1 && k <= n;
+    oj[q] = upd[q] ? Sf : 0;
+    ok[q] = upd[q] ? 1 : 0;
+    col[q] = min(max(j, 0), Sf - 1) * Sf + min(max(k, 0), Sf - 1);
+    xm[q] = (p_first - 1 >= 0) ? x[(p_first - 1) * S2 + col[q]] : 0.0f;
+    xc[q] = (p_first >= 0) ? x[p_first * S2 + col[q]] : 0.0f;
+  }
+  // inputs of the first sweep at plane p: x at p + 1, the in-plane
+  // neighbours and U at p (only x at p + 1 outside the interior planes)
+  {
+    const int PP = p_first;
+#pragma unroll
+    for (int q = 0; q < P2; ++q) {
+      if (PP >= 1 && PP <= n) {
+        const int id = PP * S2 + col[q];
+        xp[q] = x[id + S2];
+        nb[q][0] = x[id + oj[q]];
+        nb[q][1] = x[id - oj[q]];
+        nb[q][2] = x[id + ok[q]];
+        nb[q][3] = x[id - ok[q]];
+        u[q] = src[id];
+      } else {
+        xp[q] = (PP + 1 >= 0 && PP + 1 <= Sf - 1) ? x[(PP + 1) * S2 + col[q]] : 0.0f;
+      }
+    }
+  }
+
+  // ring slots of planes p, p - 1, p - 2 (p_first >= -1)
+  int s_p = (p_first + RING) % RING;
+  for (int p = p_first; p <= p_last; ++p) {
+    const int s_m1 = (s_p + RING - 1) % RING, s_m2 = (s_p + RING - 2) % RING;
+    // ---- first sweep at fine plane p ----
+    {
+      float* dst = ring2 + s_p * PL2 + tid;
+      float* dstu = ringu + s_p * PL2 + tid;
+      if (p >= 1 && p <= n) {
+#pragma unroll
+        for (int q = 0; q < P2; ++q) {
+          const float s6 = xp[q] + xm[q] + nb[q][0] + nb[q][1] + nb[q][2] + nb[q][3];
+          const float rhs = (alpha * (u[q] * u[q])) * beta;
+          float v = (s6 + rhs) / c;
+          if (a != 0.0f) v = a * xc[q] + v;
+          if (tid + q * NT < PL2) {
+            dst[q * NT] = upd[q] ? v : xc[q];
+            dstu[q * NT] = u[q];
+          }
+        }
+      } else if (p == 0 || p == Sf - 1) {
+        // face planes keep x
+#pragma unroll
+        for (int q = 0; q < P2; ++q)
+          if (tid + q * NT < PL2) dst[q * NT] = xc[q];
+      }
+#pragma unroll
+      for (int q = 0; q < P2; ++q) {
+        xm[q] = xc[q];
+        xc[q] = xp[q];
+      }
+      if (p < p_last) {
+        const int PP = p + 1;
+  #pragma unroll
+      for (int q = 0; q < P2; ++q) {
+        if (PP >= 1 && PP <= n) {
+          const int id = PP * S2 + col[q];
+          xp[q] = x[id + S2];
+          nb[q][0] = x[id + oj[q]];
+          nb[q][1] = x[id - oj[q]];
+          nb[q][2] = x[id + ok[q]];
+          nb[q][3] = x[id - ok[q]];
+          u[q] = src[id];
+        } else {
+          xp[q] = (PP + 1 >= 0 && PP + 1 <= Sf - 1) ? x[(PP + 1) * S2 + col[q]] : 0.0f;
+        }
+      }
+      }
+    }
+    lds_barrier();
+    // ---- second sweep at fine plane p - 1 ----
+    const int f1 = p - 1;
+    if (f1 >= 2 * I0 - 2 && f1 >= 0 && f1 <= Sf - 1) {
+      const bool pin = f1 >= 1 && f1 <= n;
+      // planes f1 + 1 = p, f1, f1 - 1 in the slots of p, p - 1, p - 2
+      const int sp1 = s_p, s0b = s_m1 * PL2, sm1 = s_m2;
+#pragma unroll
+      for (int q = 0; q < P1; ++q) {
+        const int o = tid + q * NT;
+        if (o < PL1) {
+          const int jj = o / W1, kk = o % W1;
+          const int j = 2 * J0 - 2 + jj, k = 2 * K0 - 2 + kk;
+          const int cc = (jj + 1) * W2 + (kk + 1);
+          const int s0 = s0b + cc;
+          float v = ring2[s0];
+          if (pin && j >= 1 && j <= n && k >= 1 && k <= n) {
+            const float s6 = ring2[sp1 * PL2 + cc] + ring2[sm1 * PL2 + cc] + ring2[s0 + W2] + ring2[s0 - W2] +
+                             ring2[s0 + 1] + ring2[s0 - 1];
+            const float uu = ringu[s0];
+            const float rhs = (alpha * (uu * uu)) * beta;
+            float w = (s6 + rhs) / c;
+            if (a != 0.0f) w = a * v + w;
+            v = w;
+            if (f1 >= i_lo && f1 <= i_hi && j >= j_lo && j <= j_hi && k >= k_lo && k <= k_hi)
+              xn[f1 * S2 + j * Sf + k] = v;
+          }
+          ring1[s_m1 * PL1 + o] = v;
+        }
+      }
+    }
+    lds_barrier();
+    // ---- residual at fine plane p - 2 ----
+    const int f0 = p - 2;
+    if (f0 >= 2 * I0 - 1 && f0 <= 2 * I1 - 1) {
+#pragma unroll
+      for (int q = 0; q < P0; ++q) {
+        const int o = tid + q * NT;
+        if (o < PL0) {
+          const int jj = o / W0, kk = o % W0;
+          const int j = 2 * J0 - 1 + jj, k = 2 * K0 - 1 + kk;
+          const int cc = (jj + 1) * W1 + (kk + 1);
+          const int s0 = s_m2 * PL1 + cc;
+          const float s6 = ring1[s_m1 * PL1 + cc] + ring1[s_p * PL1 + cc] + ring1[s0 + W1] + ring1[s0 - W1] +
+                           ring1[s0 + 1] + ring1[s0 - 1];
+          const float uu = ringu[s_m2 * PL2 + (jj + 2) * W2 + (kk + 2)];
+          const float rhs = (alpha * (uu * uu)) * beta;
+          const float r = rhs - (6.0f * ring1[s0] - s6);
+          ring0[s_m2 * PL0 + o] = (j <= n && k <= n) ? r : 0.0f;
+        }
+      }
+    }
+    lds_barrier();
+    // ---- restriction of coarse plane (f0 - 1) / 2 once its three fine planes are in ----
+    if ((f0 & 1) && f0 >= 2 * I0 + 1 && f0 <= 2 * I1 - 1 && tid < CK * CJ) {
+      const int I = (f0 - 1) / 2, tk = tid % CK, tj = tid / CK;
+      const int K = K0 + tk, J = J0 + tj;
+      if (K <= nc && J <= nc) {
+        const int cc = (2 * tj + 1) * W0 + (2 * tk + 1);
+        // planes f0 - 2 = p - 4, f0 - 1 = p - 3 and f0 = p - 2 sit in the
+        // slots of p - 1, p and p - 2 (three-slot rings)
+        gc[I * Sc * Sc + J * Sc + K] = restrict27_planes(ring0, s_m1 * PL0 + cc, s_p * PL0 + cc, s_m2 * PL0 + cc, W0, 1);
+      }
+    }
+    s_p = (s_p + 1) % RING;
+  }
+}
+
+// The fine level's up step and the ITP update as one march along i: the
+// coarse correction (x + P e), two damped-Jacobi post-sweeps and the ITP
+// update on the new potential (temporal blocking of prolongation, sweep
+// pair and update).  A workgroup owns TK x TJ fine interior columns and
+// planes [I0, I1).  Per plane: stage 0 loads, on the region R3 = tile + 2,
+// x with the trilinear coarse correction (PlaneCache, prolong_correct_col's
+// operations), U and the label into LDS rings; stage 1 forms the first sweep
+// on R2 = tile + 1 from them; stage 2 forms the second sweep on the tile
+// (written to xn) and the ITP update of U with it (written to Un).  Stage 0
+// runs at plane s, stage 1 at s - 1, stage 2 at s - 2; rings of three (x,
+// first sweep) and four (U, label) planes with a barrier after each stage.
+// The expressions of prolong_correct_col (scale 1), smooth_col and itp_col
+// in the same order: xn and Un equal k_prolong_correct, two k_smooth and
+// k_itp bit for bit.
+template <int TK, int TJ, int RU>
+__global__ __launch_bounds__(TK * TJ) void k_post_march(int Sf, int CI, const float* __restrict__ x,
+                                                        const float* __restrict__ e, float* __restrict__ xn,
+                                                        const float* __restrict__ U, float* __restrict__ Un,
+                                                        const float* __restrict__ K, const uint16_t* __restrict__ lab,
+                                                        float a, float c, float alpha, float beta, float half_dv,
+                                                        float dtv) {
+  constexpr int NT = TK * TJ;
+  constexpr int W3 = TK + 4, H3 = TJ + 4, W2 = TK + 2, H2 = TJ + 2;
+  constexpr int PL3 = W3 * H3, PL2 = W2 * H2;
+  constexpr int P3 = (PL3 + NT - 1) / NT, P2 = (PL2 + NT - 1) / NT;
+  __shared__ float rx[3 * PL3];       // corrected x
+  static_assert(RU == 4 || RU == 5, "U / label ring of four (barrier after stage 2) or five planes");
+  __shared__ float ru[RU * PL3];      // U
+  __shared__ uint16_t rl[RU * PL3];   // label
+  __shared__ float r1[3 * PL2];       // first sweep
+  const int n = Sf - 2, Sc = (Sf - 1) / 2 + 1, S2 = Sf * Sf;
+  const int gx = (n + TK - 1) / TK, gy = (n + TJ - 1) / TJ;
+  const int t = xcd_block();
+  const int K0 = 1 + (t % gx) * TK, J0 = 1 + ((t / gx) % gy) * TJ, I0 = 1 + (t / (gx * gy)) * CI;
+  const int I1 = min(I0 + CI, n + 1);
+  const int tid = threadIdx.x;
+  // stage-0 nodes of this thread on R3 (loads clamped into the grid)
+  int col[P3], jr[P3], kr[P3];
+  bool cin[P3];
+  PlaneCache pc[P3];
+#pragma unroll
+  for (int q = 0; q < P3; ++q) {
+    const int o = min(tid + q * NT, PL3 - 1);
+    jr[q] = J0 - 2 + o / W3;
+    kr[q] = K0 - 2 + o % W3;
+    cin[q] = tid + q * NT < PL3 && jr[q] >= 1 && jr[q] <= n && kr[q] >= 1 && kr[q] <= n;
+    col[q] = min(max(jr[q], 0), Sf - 1) * Sf + min(max(kr[q], 0), Sf - 1);
+  }
+  // stage-2 node of this thread (one per thread)
+  const int tk = tid % TK, tj = tid / TK;
+  const int jo = J0 + tj, ko = K0 + tk;
+  const bool own = jo <= n && ko <= n;
+  for (int s = I0 - 2; s <= I1 + 1; ++s) {
+    // ---- stage 0: corrected x, U and label at plane s ----
+    if (s >= 0 && s <= Sf - 1) {
+      const int sx = s % 3, su = s % RU;
+      const bool pin = s >= 1 && s <= n;
+      float xv[P3], uv[P3];
+      uint16_t lv[P3];
+#pragma unroll
+      for (int q = 0; q < P3; ++q) {
+        const int id = s * S2 + col[q];
+        xv[q] = x[id];
+        uv[q] = U[id];
+        lv[q] = lab[id];
+      }
+#pragma unroll
+      for (int q = 0; q < P3; ++q) {
+        float v = xv[q];
+        if (pin && cin[q]) v = v + pc[q].corr(e, Sc, s, jr[q], kr[q]);
+        if (tid + q * NT < PL3) {
+          rx[sx * PL3 + tid + q * NT] = v;
+          ru[su * PL3 + tid + q * NT] = uv[q];
+          rl[su * PL3 + tid + q * NT] = lv[q];
+        }
+      }
+    }
+    lds_barrier();
+    // ---- stage 1: first sweep at plane s - 1 on R2 ----
+    const int f1 = s - 1;
+    if (f1 >= I0 - 1 && f1 >= 0 && f1 <= Sf - 1) {
+      const bool pin = f1 >= 1 && f1 <= n;
+      const int bp = ((f1 + 1) % 3) * PL3, b0 = (f1 % 3) * PL3, bm = ((f1 + 2) % 3) * PL3, bu = (f1 % RU) * PL3;
+#pragma unroll
+      for (int q = 0; q < P2; ++q) {
+        const int o = tid + q * NT;
+        if (o < PL2) {
+          const int jj = o / W2, kk = o % W2;
+          const int j = J0 - 1 + jj, k = K0 - 1 + kk;
+          const int cc = (jj + 1) * W3 + (kk + 1);
+          float v = rx[b0 + cc];
+          if (pin && j >= 1 && j <= n && k >= 1 && k <= n) {
+            const float s6 = rx[bp + cc] + rx[bm + cc] + rx[b0 + cc + W3] + rx[b0 + cc - W3] + rx[b0 + cc + 1] +
+                             rx[b0 + cc - 1];
+            const float u = ru[bu + cc];
+            const float rhs = (alpha * (u * u)) * beta;
+            float w = (s6 + rhs) / c;
+            if (a != 0.0f) w = a * v + w;
+            v = w;
+          }
+          r1[(f1 % 3) * PL2 + o] = v;
+        }
+      }
+    }
+    lds_barrier();
+    // ---- stage 2: second sweep and ITP update at plane s - 2 on the tile ----
+    const int f2 = s - 2;
+    if (f2 >= I0 && f2 < I1 && own) {
+      const int bp = ((f2 + 1) % 3) * PL2, b0 = (f2 % 3) * PL2, bm = ((f2 + 2) % 3) * PL2;
+      const int cc = (tj + 1) * W2 + (tk + 1);
+      const float s6 = r1[bp + cc] + r1[bm + cc] + r1[b0 + cc + W2] + r1[b0 + cc - W2] + r1[b0 + cc + 1] +
+                       r1[b0 + cc - 1];
+      const int cu = (tj + 2) * W3 + (tk + 2);
+      const int u0 = (f2 % RU) * PL3 + cu, up_ = ((f2 + 1) % RU) * PL3 + cu, um_ = ((f2 + RU - 1) % RU) * PL3 + cu;
+      const float uc = ru[u0];
+      const float rhs = (alpha * (uc * uc)) * beta;
+      float v = (s6 + rhs) / c;
+      if (a != 0.0f) v = a * r1[b0 + cc] + v;
+      const int id = f2 * S2 + jo * Sf + ko;
+      xn[id] = v;
+      const uint16_t lc = rl[u0];
+      float out;
+      if (lc & kRcFlag) {
+        out = 0.0f;
+      } else {
+        const float t0 = (rl[up_] == lc) ? ru[up_] : uc;
+        const float t1 = (rl[um_] == lc) ? ru[um_] : uc;
+        const float t2 = (rl[u0 + W3] == lc) ? ru[u0 + W3] : uc;
+        const float t3 = (rl[u0 - W3] == lc) ? ru[u0 - W3] : uc;
+        const float t4 = (rl[u0 + 1] == lc) ? ru[u0 + 1] : uc;
+        const float t5 = (rl[u0 - 1] == lc) ? ru[u0 - 1] : uc;
+        const float lap = t0 + t1 + t2 + t3 + t4 + t5 - 6.0f * uc;
+        out = uc + half_dv * lap + dtv * (K[id] - 2.0f * v) * uc;
+      }
+      Un[id] = out;
+    }
+    // with four U / label slots the next stage 0 overwrites plane s - 3,
+    // which this stage read; with five it writes a slot no stage reads now
+    if (RU == 4) lds_barrier();
   }
 }
 
@@ -2041,6 +2380,73 @@ void launch_smooth2(hipStream_t st, bool fine, int S, const float* x, float* xn,
   // 64 x 8 columns, one plane per round: the fastest of the 32/64 x 4/8,
   // 1/2-plane tiles on the 513^3 grid
   smooth2_march<64, 8, 1>(st, fine, S, x, xn, src, a, c, alpha, beta);
+}
+
+namespace {
+template <int CK, int CJ, int NT>
+void smooth2_resres(hipStream_t st, int Sf, const float* x, float* xn, const float* src, float a, float c,
+                    float alpha, float beta, float* gc, int blocks) {
+  const int nc = (Sf - 1) / 2 - 1;
+  const int gx = (nc + CK - 1) / CK, gy = (nc + CJ - 1) / CJ;
+  const int gz = std::max(1, std::min(nc, blocks / (gx * gy)));
+  const int CI = (nc + gz - 1) / gz;
+  const dim3 grid(gx, gy, (nc + CI - 1) / CI);
+  hipLaunchKernelGGL((k_smooth2_resres_march<CK, CJ, NT>), grid, dim3(NT), 0, st, Sf, CI, x, xn, src, a, c, alpha,
+                     beta, gc);
+}
+}  // namespace
+
+void launch_smooth2_resres(hipStream_t st, int Sf, const float* x, float* xn, const float* src, float a, float c,
+                           float alpha, float beta, float* gc, int blocks) {
+  // 32 x 4 coarse columns (64 x 8 fine, as many fine columns per plane as
+  // the two-sweep march), 512 threads, ~4096 workgroups: 852 us on the
+  // 513^3 grid, against 890 for 16 x 8, 931 for 16 x 4 (256 threads) and
+  // 939 for 32 x 8 (1024 threads) (RQ_SMOOTH2_RESRES_SHAPE 1-3)
+  static const int shape = [] {
+    const char* e = std::getenv("RQ_SMOOTH2_RESRES_SHAPE");
+    return e ? std::atoi(e) : 0;
+  }();
+  const int b = blocks > 0 ? blocks : kSmooth2ResresBlocks;
+  switch (shape) {
+    case 1: smooth2_resres<16, 8, 512>(st, Sf, x, xn, src, a, c, alpha, beta, gc, b); break;
+    case 2: smooth2_resres<16, 4, 256>(st, Sf, x, xn, src, a, c, alpha, beta, gc, b); break;
+    case 3: smooth2_resres<32, 8, 1024>(st, Sf, x, xn, src, a, c, alpha, beta, gc, b); break;
+    default: smooth2_resres<32, 4, 512>(st, Sf, x, xn, src, a, c, alpha, beta, gc, b); break;
+  }
+}
+
+namespace {
+template <int TK, int TJ, int RU = 4>
+void post_march(hipStream_t st, int Sf, const float* x, const float* e, float* xn, const float* U, float* Un,
+                const float* K, const uint16_t* lab, float a, float c, float alpha, float beta, float half_dv,
+                float dtv, int blocks) {
+  const int n = Sf - 2;
+  const int gx = (n + TK - 1) / TK, gy = (n + TJ - 1) / TJ;
+  const int gz = std::max(1, std::min(n, blocks / (gx * gy)));
+  const int CI = (n + gz - 1) / gz;
+  const dim3 grid(gx, gy, (n + CI - 1) / CI);
+  hipLaunchKernelGGL((k_post_march<TK, TJ, RU>), grid, dim3(TK * TJ), 0, st, Sf, CI, x, e, xn, U, Un, K, lab, a, c,
+                     alpha, beta, half_dv, dtv);
+}
+}  // namespace
+
+void launch_post_itp(hipStream_t st, int Sf, const float* x, const float* e, float* xn, const float* U, float* Un,
+                     const float* K, const uint16_t* lab, float a, float c, float alpha, float beta, float half_dv,
+                     float dtv, int blocks) {
+  // 32 x 8 columns with five U / label planes (no barrier after the update
+  // stage), ~8192 workgroups: 1042 us on the 513^3 grid, against 1086 with
+  // four planes, 1150 for 64 x 8 and 1125 for 16 x 16 (RQ_POST_SHAPE 1-3)
+  static const int shape = [] {
+    const char* v = std::getenv("RQ_POST_SHAPE");
+    return v ? std::atoi(v) : 0;
+  }();
+  const int b = blocks > 0 ? blocks : kPostBlocks;
+  switch (shape) {
+    case 1: post_march<32, 8, 4>(st, Sf, x, e, xn, U, Un, K, lab, a, c, alpha, beta, half_dv, dtv, b); break;
+    case 2: post_march<64, 8, 5>(st, Sf, x, e, xn, U, Un, K, lab, a, c, alpha, beta, half_dv, dtv, b); break;
+    case 3: post_march<16, 16, 4>(st, Sf, x, e, xn, U, Un, K, lab, a, c, alpha, beta, half_dv, dtv, b); break;
+    default: post_march<32, 8, 5>(st, Sf, x, e, xn, U, Un, K, lab, a, c, alpha, beta, half_dv, dtv, b); break;
+  }
 }
 
 void launch_residual_restrict(hipStream_t st, bool fine, int Sf, const float* x, const float* src, float alpha,

@@ -125,3 +125,74 @@ def test_sweep_pairs_multigrid_bitexact(NN, levels, nu1, nu2, monkeypatch):
         s.close()
     assert ulp_diff(out[0][0], out[1][0]) == 0
     assert ulp_diff(out[0][1], out[1][1]) == 0
+
+
+@pytest.mark.parametrize("NN,levels,blocks", [(80, 3, 0), (136, 4, 0), (136, 4, 1), (192, 4, 0)])
+def test_presmooth_restrict_march_bitexact(NN, levels, blocks, monkeypatch):
+    """The fine pre-smoothing pair, residual and restriction as one march
+    (k_smooth2_resres_march; size gate lowered; ``blocks`` = 1 gives each
+    workgroup every coarse plane) against the unfused sequence and against
+    the pair + restriction march as two launches."""
+    import ctypes as C
+    from realmolecule_amd import _lib as L
+    monkeypatch.setenv("RQ_SMOOTH2_MIN_S", "0")
+    if blocks:
+        monkeypatch.setenv("RQ_SMOOTH2_RESRES_BLOCKS", str(blocks))
+    rows = three_atoms(NN)
+    out = []
+    for mode, s2rr in ((0, "1"), (1, "0"), (1, "1")):
+        monkeypatch.setenv("RQ_SMOOTH2_RESRES", s2rr)
+        s = make_solver(rows, NN, NN * 0.12, poisson="multigrid", mg_levels=levels, bootstrap_poisson=2)
+        s._chk(s._lib.rq_set_fused(s._h, mode))
+        s.initialize()
+        s._chk(s._lib.rq_timing_enable(s._h, 1))
+        s.step(0, 8, poisson_interval=1)
+        nf = len(L.FAMILIES)
+        ms, cnt = (C.c_double * nf)(), (C.c_int64 * nf)()
+        s._chk(s._lib.rq_timing_read(s._h, ms, cnt, nf))
+        s._chk(s._lib.rq_timing_enable(s._h, 0))
+        fused_launches = cnt[L.FAMILIES.index("smooth_restrict")]
+        out.append((s.U.get(), s.P.get(), fused_launches))
+        s.close()
+    assert out[0][2] == 0 and out[1][2] == 0 and out[2][2] == 8
+    for k in (1, 2):
+        assert ulp_diff(out[0][0], out[k][0]) == 0
+        assert ulp_diff(out[0][1], out[k][1]) == 0
+
+
+@pytest.mark.parametrize("NN,levels,blocks,rc,graph", [(80, 3, 0, False, False), (136, 4, 0, True, False),
+                                                       (136, 4, 1, False, True), (192, 4, 0, True, True)])
+def test_post_march_bitexact(NN, levels, blocks, rc, graph, monkeypatch):
+    """The fine correction, post-smoothing pair and ITP update as one march
+    (k_post_march; size gate lowered; ``blocks`` = 1 gives each workgroup
+    every plane; with the rc rule's label flags; with graph replay) against
+    the unfused sequence and against the separate fused launches."""
+    import ctypes as C
+    from realmolecule_amd import _lib as L
+    monkeypatch.setenv("RQ_SMOOTH2_MIN_S", "0")
+    if blocks:
+        monkeypatch.setenv("RQ_POST_BLOCKS", str(blocks))
+    rows = three_atoms(NN)
+    rows[1, 4] = 0.6
+    out = []
+    for mode, post in ((0, "1"), (1, "0"), (1, "1")):
+        monkeypatch.setenv("RQ_POST_FUSE", post)
+        s = make_solver(rows, NN, NN * 0.12, poisson="multigrid", mg_levels=levels, bootstrap_poisson=2,
+                        rc_dirichlet=rc, graph=graph)
+        s._chk(s._lib.rq_set_fused(s._h, mode))
+        s.initialize()
+        s._chk(s._lib.rq_timing_enable(s._h, 1))
+        s.step(0, 8, poisson_interval=1)
+        nf = len(L.FAMILIES)
+        ms, cnt = (C.c_double * nf)(), (C.c_int64 * nf)()
+        s._chk(s._lib.rq_timing_read(s._h, ms, cnt, nf))
+        s._chk(s._lib.rq_timing_enable(s._h, 0))
+        post_launches = cnt[L.FAMILIES.index("post_itp")]
+        if graph:
+            s.step(8, 40, poisson_interval=1)  # replayed chunks
+        out.append((s.U.get(), s.P.get(), post_launches))
+        s.close()
+    assert out[0][2] == 0 and out[1][2] == 0 and out[2][2] == 8
+    for k in (1, 2):
+        assert ulp_diff(out[0][0], out[k][0]) == 0
+        assert ulp_diff(out[0][1], out[k][1]) == 0
