@@ -365,6 +365,20 @@ int rq_batch_sync(rq_batch* b);
 int rq_batch_energy(rq_batch* b, double* out3);
 /* fields U, K, P (float32) or label (int32) of molecule m, (NN_m+1)^3 values */
 int rq_batch_read_field(rq_batch* b, int32_t mol, int32_t field, void* host, size_t bytes);
+/* Positions (and rc) of molecule mol's atoms for the frozen-label map below;
+ * Z must not change.  Replaces re-creating a per-molecule solver for each
+ * geometry of a scan (cloud/solver.py:321-400). */
+int rq_batch_set_atoms(rq_batch* b, int32_t mol, const rq_atom* atoms, int32_t n_atoms);
+/* rq_solve_frozen / rq_adjoint_gradient of every molecule at once, one
+ * launch per pass over all molecules' tiles (forward map and reverse
+ * sweep): labels frozen at those of rq_batch_initialize, start
+ * RQ_ADJ_START_TRIAL, multigrid Poisson.  energy3: (T, V_eK, V_ee) per
+ * molecule (3 n_mols); dEdR: the electronic dE/dR of every atom, molecules
+ * in order (3 per atom, Ha/bohr).  Overwrites the batch's U, P and K (call
+ * rq_batch_initialize before stepping again).  Replaces running the
+ * per-molecule adjoint, or central differences, once per geometry. */
+int rq_batch_solve_frozen(rq_batch* b, const rq_adjoint_spec* spec, double* energy3);
+int rq_batch_adjoint_gradient(rq_batch* b, const rq_adjoint_spec* spec, double* energy3, double* dEdR);
 int rq_batch_timing_enable(rq_batch* b, int32_t on);
 int rq_batch_timing_read(rq_batch* b, double* ms, int64_t* launches, int32_t n_families);
 

@@ -165,6 +165,9 @@ PROTOTYPES = {
     "rq_batch_sync": (C.c_int, [_P]),
     "rq_batch_energy": (C.c_int, [_P, _D]),
     "rq_batch_read_field": (C.c_int, [_P, _I, _I, _P, C.c_size_t]),
+    "rq_batch_set_atoms": (C.c_int, [_P, _I, C.POINTER(RqAtom), _I]),
+    "rq_batch_solve_frozen": (C.c_int, [_P, C.POINTER(RqAdjointSpec), _D]),
+    "rq_batch_adjoint_gradient": (C.c_int, [_P, C.POINTER(RqAdjointSpec), _D, _D]),
     "rq_batch_timing_enable": (C.c_int, [_P, _I]),
     "rq_batch_timing_read": (C.c_int, [_P, _D, C.POINTER(C.c_int64), _I]),
     "rq_timing_enable": (C.c_int, [_P, _I]),

@@ -96,6 +96,32 @@ class Batch:
         self._chk(self._lib.rq_batch_read_field(self._h, int(m), L.FIELDS[field], out.ctypes.data, out.nbytes))
         return out
 
+    def set_atoms(self, m, rows):
+        """Positions (grid units) and rc of molecule m's atoms, for the frozen map."""
+        rows = np.asarray(rows, np.float64)
+        self._chk(self._lib.rq_batch_set_atoms(self._h, int(m), atoms_array(rows), len(rows)))
+
+    def frozen_energies(self, spec):
+        """(n, 3) (T, V_eK, V_ee) of the frozen-label map of every molecule."""
+        e = (C.c_double * (3 * self.n))()
+        self._chk(self._lib.rq_batch_solve_frozen(self._h, C.byref(spec), e))
+        return np.frombuffer(e, np.float64).reshape(self.n, 3).copy()
+
+    def adjoint_gradient(self, spec):
+        """((n, 3) energies, [dE/dR (n_atoms_m, 3) per molecule]) of the
+        frozen-label map, forward map and reverse sweep over all molecules at
+        once (electronic part, Ha/bohr)."""
+        na = sum(len(r) for r in self.rows)
+        e = (C.c_double * (3 * self.n))()
+        g = (C.c_double * (3 * na))()
+        self._chk(self._lib.rq_batch_adjoint_gradient(self._h, C.byref(spec), e, g))
+        flat = np.frombuffer(g, np.float64).reshape(na, 3).copy()
+        out, a0 = [], 0
+        for r in self.rows:
+            out.append(flat[a0:a0 + len(r)])
+            a0 += len(r)
+        return np.frombuffer(e, np.float64).reshape(self.n, 3).copy(), out
+
     def timing(self, enable):
         self._chk(self._lib.rq_batch_timing_enable(self._h, int(enable)))
 

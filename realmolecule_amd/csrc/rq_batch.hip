@@ -21,6 +21,7 @@
 #include <vector>
 
 #include "rq_internal.h"
+#include "rq_adjoint_cols.h"
 #include "rq_stencil.h"
 
 namespace rq {
@@ -383,6 +384,166 @@ __global__ __launch_bounds__(256) void kb_init(const BMol* __restrict__ M, const
   }
 }
 
+// ---- reverse sweep of the frozen-label map over all molecules (the
+// single-grid adjoint's column bodies, rq_adjoint_cols.h) ----
+
+__global__ __launch_bounds__(256) void kb_init_frozen(const BMol* __restrict__ M, const BLev* __restrict__ L, int nm,
+                                                      int nnodes, const Atom* __restrict__ atoms,
+                                                      const uint16_t* __restrict__ lab, float* __restrict__ U,
+                                                      float* __restrict__ K) {
+  const int stride = gridDim.x * blockDim.x;
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nnodes; t += stride) {
+    const int m = find_node(M, nm, t);
+    const long long o = L[m].off;
+    init_frozen_node(L[m].S, t - M[m].node0, M[m].h, M[m].h2, atoms + M[m].atom0, M[m].n_atoms, lab + o, U + o,
+                     K + o);
+  }
+}
+
+__global__ __launch_bounds__(256) void kb_energy_grad(const BLev* __restrict__ L, const BMol* __restrict__ M, int nm,
+                                                      int ntiles, const float* __restrict__ U,
+                                                      const float* __restrict__ K, const float* __restrict__ P,
+                                                      const uint16_t* __restrict__ lab, float* __restrict__ lamU,
+                                                      float* __restrict__ gK, float* __restrict__ lamP) {
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    Col q;
+    if (!item_col(L, nm, ntiles, t, q)) continue;
+    const long long o = L[q.m].off;
+    energy_grad_col(L[q.m].S, q.k, q.j, q.i0, q.i1, M[q.m].h, M[q.m].h3, U + o, K + o, P + o, lab + o, lamU + o,
+                    gK + o, lamP + o);
+  }
+}
+
+__global__ __launch_bounds__(256) void kb_itp_adj(const BLev* __restrict__ L, const BMol* __restrict__ M, int nm,
+                                                  int ntiles, const float* __restrict__ lam,
+                                                  float* __restrict__ lam_out, const float* __restrict__ U,
+                                                  const float* __restrict__ K, const float* __restrict__ P,
+                                                  const uint16_t* __restrict__ lab, float* __restrict__ gK,
+                                                  float* __restrict__ lamP) {
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    Col q;
+    if (!item_col(L, nm, ntiles, t, q)) continue;
+    const long long o = L[q.m].off;
+    itp_adj_col(L[q.m].S, q.k, q.j, q.i0, q.i1, lam + o, lam_out + o, U + o, K + o, P + o, lab + o, M[q.m].half_dv,
+                M[q.m].dtv, gK + o, lamP + o);
+  }
+}
+
+// coef2 of kAdjLast is each molecule's 2 alpha beta = 2 (2 pi) h^2
+template <int MODE>
+__global__ __launch_bounds__(256) void kb_smooth_adj(const BLev* __restrict__ L, const BMol* __restrict__ M, int nm,
+                                                     int ntiles, const float* __restrict__ lam,
+                                                     float* __restrict__ lam_out, const float* __restrict__ src,
+                                                     float* __restrict__ acc, float* __restrict__ lamU, float a,
+                                                     float c, float coef) {
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    Col q;
+    if (!item_col(L, nm, ntiles, t, q)) continue;
+    const long long o = L[q.m].off;
+    const float ab2 = 2.0f * M[q.m].two_pi * M[q.m].h2;
+    smooth_adj_col<MODE>(L[q.m].S, q.k, q.j, q.i0, q.i1, lam + o, lam_out + o, src ? src + o : nullptr, acc + o,
+                         lamU ? lamU + o : nullptr, a, c, coef, ab2);
+  }
+}
+
+__global__ __launch_bounds__(256) void kb_correct_adj(const BLev* __restrict__ L, int nm, int ntiles,
+                                                      const float* __restrict__ lam, const float* __restrict__ z,
+                                                      float* __restrict__ lam_out, float* __restrict__ acc) {
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    Col q;
+    if (!item_col(L, nm, ntiles, t, q)) continue;
+    const long long o = L[q.m].off;
+    correct_adj_col(L[q.m].S, q.k, q.j, q.i0, q.i1, lam + o, z + o, lam_out + o, acc + o);
+  }
+}
+
+__global__ __launch_bounds__(256) void kb_axpy_src(const BLev* __restrict__ L, const BMol* __restrict__ M, int nm,
+                                                   int ntiles, const float* __restrict__ U,
+                                                   const float* __restrict__ acc, float* __restrict__ lamU) {
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    Col q;
+    if (!item_col(L, nm, ntiles, t, q)) continue;
+    const long long o = L[q.m].off;
+    axpy_src_col(L[q.m].S, q.k, q.j, q.i0, q.i1, U + o, acc + o, lamU + o, 2.0f * M[q.m].two_pi * M[q.m].h2);
+  }
+}
+
+__global__ __launch_bounds__(256) void kb_domain_dot(const BLev* __restrict__ L, const BMol* __restrict__ M, int nm,
+                                                     int ntiles, const float* __restrict__ A,
+                                                     const float* __restrict__ B, const uint16_t* __restrict__ lab,
+                                                     Fx* __restrict__ out) {
+  __shared__ Fx bins[kFxWords * kMaxBatchDom];
+  const int nthr = blockDim.x * blockDim.y;
+  const int tid = threadIdx.y * blockDim.x + threadIdx.x;
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const int u = xcd_item(t, ntiles);
+    const int m = find_mol(L, nm, u);  // uniform per workgroup
+    const int nd = M[m].n_atoms;
+    for (int d = tid; d < kFxWords * nd; d += nthr) bins[d] = 0;
+    __syncthreads();
+    Col q;
+    if (item_col(L, nm, ntiles, t, q)) {
+      const long long o = L[m].off;
+      domain_dot_col(L[m].S, q.k, q.j, q.i0, q.i1, 1.0f, A + o, B + o, lab + o, nd, bins);
+    }
+    __syncthreads();
+    for (int d = tid; d < nd; d += nthr) fx_add_words(out + kFxWords * (M[m].dom0 + d), bins + kFxWords * d);
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(256) void kb_normalize_adj(const BLev* __restrict__ L, const BMol* __restrict__ M, int nm,
+                                                        int ntiles, const float* __restrict__ Upre,
+                                                        float* __restrict__ lam, const uint16_t* __restrict__ lab,
+                                                        const Fx* __restrict__ norms, const Fx* __restrict__ dots,
+                                                        const float* __restrict__ Z) {
+  __shared__ float sc[kMaxBatchDom], be[kMaxBatchDom];
+  const int nthr = blockDim.x * blockDim.y;
+  const int tid = threadIdx.y * blockDim.x + threadIdx.x;
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const int u = xcd_item(t, ntiles);
+    const int m = find_mol(L, nm, u);
+    const int nd = M[m].n_atoms, d0 = M[m].dom0;
+    __syncthreads();
+    for (int d = tid; d < nd; d += nthr)
+      normalize_adj_coef(Z[d0 + d], norms + kFxWords * (d0 + d), dots + kFxWords * (d0 + d), M[m].h3, sc[d], be[d]);
+    __syncthreads();
+    Col q;
+    if (item_col(L, nm, ntiles, t, q)) {
+      const long long o = L[m].off;
+      normalize_adj_col(L[m].S, q.k, q.j, q.i0, q.i1, Upre + o, lam + o, lab + o, sc, be, nd);
+    }
+  }
+}
+
+// position gradient: per molecule the single-grid block layout (64 x 4
+// columns, kAdjCol planes), blocks numbered from ptile0[m]
+__global__ __launch_bounds__(256) void kb_pos_grad(const int* __restrict__ ptile0, const BLev* __restrict__ L,
+                                                   const BMol* __restrict__ M, int nm, int nblocks,
+                                                   const float* __restrict__ gK, const float* __restrict__ lamU0,
+                                                   const uint16_t* __restrict__ lab, const Atom* __restrict__ atoms,
+                                                   Fx* __restrict__ out) {
+  __shared__ double red[4][kAdjChunk * 3];
+  for (int t = blockIdx.x; t < nblocks; t += gridDim.x) {
+    int lo = 0, hi = nm - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (ptile0[mid] <= t) lo = mid;
+      else hi = mid - 1;
+    }
+    const int m = lo;
+    const int S = L[m].S, n = S - 2;
+    const int gx = (n + 63) / 64, gy = (n + 3) / 4;
+    const int local = t - ptile0[m];
+    const int k = 1 + (local % gx) * 64 + threadIdx.x;
+    const int j = 1 + ((local / gx) % gy) * 4 + threadIdx.y;
+    const int i0 = 1 + (local / (gx * gy)) * kAdjCol;
+    const long long o = L[m].off;
+    pos_grad_block(S, M[m].h, M[m].h2, k, j, i0, gK + o, lamU0 + o, lab + o, atoms + M[m].atom0, M[m].n_atoms,
+                   out + kFxWords * 3 * (size_t)M[m].atom0, red);
+  }
+}
+
 }  // namespace
 }  // namespace rq
 
@@ -397,6 +558,15 @@ struct BatchPtrs {
   bool operator==(const BatchPtrs& o) const {
     return U == o.U && U2 == o.U2 && P == o.P && P2 == o.P2 && e0 == o.e0 && e1 == o.e1;
   }
+};
+
+struct BAdj {
+  float *lamU = nullptr, *lamU2 = nullptr, *lamP = nullptr, *lamP2 = nullptr, *gK = nullptr, *acc = nullptr;
+  float *t1 = nullptr, *t3 = nullptr, *zero = nullptr;
+  std::vector<float*> ckU, ckP, segU, segP;
+  Fx* red = nullptr;
+  int* ptile0 = nullptr;  // first position-gradient block of each molecule
+  int npg = 0;
 };
 
 struct BatchGraph {
@@ -435,6 +605,8 @@ struct rq_batch {
   hipStream_t st = nullptr;
   bool initialized = false, use_graph = false, timing = false;
   std::vector<BatchGraph> graphs;
+  std::vector<Atom> atoms_h;  // all molecules' atoms, in order
+  BAdj adj;
   std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev;
   double fam_ms[F_COUNT] = {0};
   int64_t fam_n[F_COUNT] = {0};
@@ -751,12 +923,293 @@ void bfree(rq_batch* b) {
   fr(b->P2);
   fr(b->lab);
   fr(b->red);
+  {
+    BAdj& A = b->adj;
+    for (float* q : {A.lamU, A.lamU2, A.lamP, A.lamP2, A.gK, A.acc, A.t1, A.t3, A.zero}) fr(q);
+    for (auto* v : {&A.ckU, &A.ckP, &A.segU, &A.segP})
+      for (float* q : *v) fr(q);
+    fr(A.red);
+    fr(A.ptile0);
+  }
   for (size_t l = 0; l < b->e0.size(); ++l) {
     fr(b->e0[l]);
     fr(b->e1[l]);
     fr(b->g[l]);
   }
   if (b->st) (void)hipStreamDestroy(b->st);
+}
+
+// ---- frozen-label map and its reverse sweep over all molecules ----
+// The rq_adjoint_gradient map of every molecule at once (trial start,
+// multigrid Poisson): K(R) and the trial at the batch's labels, normalise,
+// bootstrap V-cycles, n_steps loop-body steps, E = T + V_eK + V_ee per
+// molecule; the reverse sweep mirrors frozen_run's, one launch per pass
+// over all molecules' tiles.
+
+int b_adj_alloc(rq_batch* b, float** q) {
+  if (*q) return RQ_OK;
+  if (hipMalloc((void**)q, b->len[0] * 4) != hipSuccess) {
+    (void)hipGetLastError();
+    return bfail(b, RQ_ENOMEM, "batch adjoint: allocation of a %zu-node field failed", b->len[0]);
+  }
+  BCHK(b, hipMemsetAsync(*q, 0, b->len[0] * 4, b->st));
+  return RQ_OK;
+}
+
+int b_adj_alloc_vec(rq_batch* b, std::vector<float*>& v, size_t n) {
+  while (v.size() < n) v.push_back(nullptr);
+  for (size_t i = 0; i < n; ++i)
+    if (int rc = b_adj_alloc(b, &v[i])) return rc;
+  return RQ_OK;
+}
+
+int b_copy(rq_batch* b, float* dst, const float* src) {
+  BCHK(b, hipMemcpyAsync(dst, src, b->len[0] * 4, hipMemcpyDeviceToDevice, b->st));
+  return RQ_OK;
+}
+
+// per-domain normalisation of the level-0 field U (the batch's b_normalize on any field)
+int b_normalize_on(rq_batch* b, float* U) {
+  const dim3 blk(b->bx, b->by);
+  const int nt = b->ntiles[0];
+  launch_fill(b->st, reinterpret_cast<float*>(b->red), 2 * kFxWords * (size_t)b->ndom, 0.0f);
+  hipLaunchKernelGGL(kb_norm_sums, dim3(grid_for(nt)), blk, 0, b->st, LD(b, 0), b->mol_d, b->nm, nt, U, b->lab,
+                     b->red);
+  hipLaunchKernelGGL(kb_normalize, dim3(grid_for(nt)), blk, 0, b->st, LD(b, 0), b->mol_d, b->nm, nt, U, b->lab,
+                     b->red, b->Z_d);
+  BLAUNCH(b);
+  return RQ_OK;
+}
+
+// one Poisson step (V-cycle) of b->P on b->U; with fuse the last post-sweep
+// carries the ITP update (then *did and the new U is b->U)
+int b_poisson(rq_batch* b, bool fuse, bool* did) { return b_vcycle(b, fuse, did); }
+
+int b_itp(rq_batch* b, const float* U, float* Un, const float* P) {
+  const int nt = b->ntiles[0];
+  hipLaunchKernelGGL(kb_itp, dim3(grid_for(nt)), dim3(b->bx, b->by), 0, b->st, LD(b, 0), b->mol_d, b->nm, nt, U, Un,
+                     b->K, P, b->lab);
+  BLAUNCH(b);
+  return RQ_OK;
+}
+
+// adjoint of one V-cycle Poisson step on b->adj lamP, accumulating into lamU (U: its input)
+int b_poisson_adj(rq_batch* b, const float* U) {
+  BAdj& A = b->adj;
+  const float a = b->p.mg_a, c = b->p.mg_c;
+  const dim3 blk(b->bx, b->by);
+  const int nt = b->ntiles[0];
+  const dim3 grd(grid_for(nt));
+  if (b->p.mg_nu2 == 0) launch_fill(b->st, A.acc, b->len[0], 0.0f);
+  for (int it = 0; it < b->p.mg_nu2; ++it) {
+    if (it == 0)
+      hipLaunchKernelGGL(kb_smooth_adj<kAdjFirst>, grd, blk, 0, b->st, LD(b, 0), b->mol_d, b->nm, nt, A.lamP, A.lamP2,
+                         nullptr, A.acc, nullptr, a, c, 1.0f / c);
+    else
+      hipLaunchKernelGGL(kb_smooth_adj<kAdjAcc>, grd, blk, 0, b->st, LD(b, 0), b->mol_d, b->nm, nt, A.lamP, A.lamP2,
+                         nullptr, A.acc, nullptr, a, c, 1.0f / c);
+    std::swap(A.lamP, A.lamP2);
+  }
+  // plain 4 FW restriction of lamP: the residual + restriction pass with x = 0
+  const int nct = b->nctiles[0];
+  hipLaunchKernelGGL(kb_resres_tiled<false>, dim3(grid_for(nct)), dim3(kBatchResCK * kBatchResCJ * kBatchResCI), 0,
+                     b->st, LD(b, 0), LD(b, 1), b->mol_d, b->nm, nct, A.zero, A.lamP, b->g[1]);
+  BLAUNCH(b);
+  if (int rc = b_vcycle_coarse(b, 1)) return rc;
+  launch_fill(b->st, A.t3, b->len[0], 0.0f);
+  hipLaunchKernelGGL(kb_prolong_correct, grd, blk, 0, b->st, LD(b, 0), LD(b, 1), b->nm, nt, A.t3, b->e0[1]);
+  hipLaunchKernelGGL(kb_correct_adj, grd, blk, 0, b->st, LD(b, 0), b->nm, nt, A.lamP, A.t3, A.lamP2, A.acc);
+  std::swap(A.lamP, A.lamP2);
+  for (int it = 0; it < b->p.mg_nu1; ++it) {
+    const bool last = it == b->p.mg_nu1 - 1;
+    if (last)
+      hipLaunchKernelGGL(kb_smooth_adj<kAdjLast>, grd, blk, 0, b->st, LD(b, 0), b->mol_d, b->nm, nt, A.lamP, A.lamP2,
+                         U, A.acc, A.lamU, a, c, 1.0f / c);
+    else
+      hipLaunchKernelGGL(kb_smooth_adj<kAdjAcc>, grd, blk, 0, b->st, LD(b, 0), b->mol_d, b->nm, nt, A.lamP, A.lamP2,
+                         nullptr, A.acc, nullptr, a, c, 1.0f / c);
+    std::swap(A.lamP, A.lamP2);
+  }
+  if (b->p.mg_nu1 == 0)
+    hipLaunchKernelGGL(kb_axpy_src, grd, blk, 0, b->st, LD(b, 0), b->mol_d, b->nm, nt, U, A.acc, A.lamU);
+  BLAUNCH(b);
+  return RQ_OK;
+}
+
+// adjoint of normalise on lamU in place, given the pre-normalise field
+int b_normalize_adj(rq_batch* b, const float* Upre) {
+  BAdj& A = b->adj;
+  const dim3 blk(b->bx, b->by);
+  const int nt = b->ntiles[0];
+  Fx* norms = A.red;
+  Fx* dots = A.red + kFxWords * (size_t)b->ndom;
+  launch_fill(b->st, reinterpret_cast<float*>(A.red), 4 * kFxWords * (size_t)b->ndom, 0.0f);
+  hipLaunchKernelGGL(kb_norm_sums, dim3(grid_for(nt)), blk, 0, b->st, LD(b, 0), b->mol_d, b->nm, nt, Upre, b->lab,
+                     norms);
+  hipLaunchKernelGGL(kb_domain_dot, dim3(grid_for(nt)), blk, 0, b->st, LD(b, 0), b->mol_d, b->nm, nt, Upre, A.lamU,
+                     b->lab, dots);
+  hipLaunchKernelGGL(kb_normalize_adj, dim3(grid_for(nt)), blk, 0, b->st, LD(b, 0), b->mol_d, b->nm, nt, Upre, A.lamU,
+                     b->lab, norms, dots, b->Z_d);
+  BLAUNCH(b);
+  return RQ_OK;
+}
+
+int b_frozen_run(rq_batch* b, const rq_adjoint_spec* sp, bool backward, double* E3, double* dEdR) {
+  if (!sp) return bfail(b, RQ_EINVAL, "null spec");
+  for (int r : sp->reserved)
+    if (r) return bfail(b, RQ_EINVAL, "rq_adjoint_spec.reserved must be zero");
+  if (sp->start != RQ_ADJ_START_TRIAL) return bfail(b, RQ_EINVAL, "batches differentiate the trial-start map");
+  const rq_schedule& sc = sp->schedule;
+  if (sp->n_steps < 0 || sp->bootstrap < 0 || sc.poisson_interval < 1 || sc.norm_interval < 1)
+    return bfail(b, RQ_EINVAL, "bad adjoint spec");
+  if (backward && b->p.mg_nu1 != b->p.mg_nu2) return bfail(b, RQ_EINVAL, "adjoint of the V-cycle needs mg_nu1 == mg_nu2");
+  if (!b->initialized) return bfail(b, RQ_ESTATE, "the frozen map needs the labels of rq_batch_initialize");
+  BAdj& A = b->adj;
+  const int64_t Tn = sp->n_steps;
+  const int C = sp->checkpoint_interval > 0 ? sp->checkpoint_interval : 64;
+  const int64_t nck = backward ? (Tn + C - 1) / C + 1 : 0;
+  int rc;
+  // fixed-point sums: norms, dots (ndom each), dE/dR (3 per atom), energies (3 per domain)
+  if (!A.red) BCHK(b, hipMalloc((void**)&A.red, sizeof(Fx) * kFxWords * 8 * (size_t)b->ndom));
+  if (backward) {
+    for (float** q : {&A.lamU, &A.lamU2, &A.lamP, &A.lamP2, &A.gK, &A.acc, &A.t1, &A.t3, &A.zero})
+      if ((rc = b_adj_alloc(b, q))) return rc;
+    if ((rc = b_adj_alloc_vec(b, A.ckU, nck)) || (rc = b_adj_alloc_vec(b, A.ckP, nck))) return rc;
+    const size_t segn = (size_t)std::min<int64_t>(C, std::max<int64_t>(Tn, 1));
+    if ((rc = b_adj_alloc_vec(b, A.segU, segn + 1)) || (rc = b_adj_alloc_vec(b, A.segP, segn))) return rc;
+    if (!A.ptile0) {
+      std::vector<int> p0(b->nm);
+      int tot = 0;
+      for (int m = 0; m < b->nm; ++m) {
+        const int n = b->lev[0][m].S - 2;
+        p0[m] = tot;
+        tot += ((n + 63) / 64) * ((n + 3) / 4) * ((n + kAdjCol - 1) / kAdjCol);
+      }
+      A.npg = tot;
+      BCHK(b, hipMalloc((void**)&A.ptile0, sizeof(int) * b->nm));
+      BCHK(b, hipMemcpyAsync(A.ptile0, p0.data(), sizeof(int) * b->nm, hipMemcpyHostToDevice, b->st));
+    }
+  }
+  const int nt = b->ntiles[0];
+  const dim3 blk(b->bx, b->by), grd(grid_for(nt));
+  // ---- forward ----
+  hipLaunchKernelGGL(kb_init_frozen, dim3(std::min((b->nnodes + 255) / 256, 65535)), dim3(256), 0, b->st, b->mol_d,
+                     LD(b, 0), b->nm, b->nnodes, b->atoms_d, b->lab, b->U, b->K);
+  BLAUNCH(b);
+  if ((rc = b_normalize_on(b, b->U))) return rc;
+  launch_fill(b->st, b->P, b->len[0], 0.0f);
+  for (int i = 0; i < sp->bootstrap; ++i) {
+    bool did = false;
+    if ((rc = b_poisson(b, false, &did))) return rc;
+  }
+  if (backward && ((rc = b_copy(b, A.ckU[0], b->U)) || (rc = b_copy(b, A.ckP[0], b->P)))) return rc;
+  const int64_t nseg = backward ? (Tn + C - 1) / C : 0;
+  const int64_t t0_last = nseg > 0 ? (nseg - 1) * C : Tn;
+  for (int64_t t = 1; t <= Tn; ++t) {
+    const bool rec = backward && t > t0_last;
+    const int idx = rec ? (int)(t - t0_last - 1) : 0;
+    if (rec && idx == 0 && (rc = b_copy(b, A.segU[0], b->U))) return rc;
+    bool did = false;
+    if (t % sc.poisson_interval == 0) {
+      if ((rc = b_poisson(b, b->fuse_itp, &did))) return rc;
+      if (rec && (rc = b_copy(b, A.segP[idx], b->P))) return rc;
+    }
+    if (!did) {
+      if ((rc = b_itp(b, b->U, b->U2, b->P))) return rc;
+      std::swap(b->U, b->U2);
+    }
+    if (t % sc.norm_interval == 0 && (rc = b_normalize_on(b, b->U))) return rc;
+    if (rec && t < Tn && (rc = b_copy(b, A.segU[idx + 1], b->U))) return rc;
+    if (backward && t % C == 0 && t < Tn) {
+      if ((rc = b_copy(b, A.ckU[t / C], b->U)) || (rc = b_copy(b, A.ckP[t / C], b->P))) return rc;
+    }
+  }
+  // energies of the final state, per molecule
+  {
+    Fx* per = A.red + 5 * kFxWords * (size_t)b->ndom;
+    launch_fill(b->st, reinterpret_cast<float*>(per), 6 * kFxWords * (size_t)b->ndom, 0.0f);
+    hipLaunchKernelGGL(kb_energy, grd, blk, 0, b->st, LD(b, 0), b->mol_d, b->nm, nt, b->U, b->K, b->P, b->lab, per);
+    BLAUNCH(b);
+    std::vector<Fx> hv(kFxWords * 3 * (size_t)b->ndom);
+    BCHK(b, hipMemcpyAsync(hv.data(), per, sizeof(Fx) * hv.size(), hipMemcpyDeviceToHost, b->st));
+    BCHK(b, hipStreamSynchronize(b->st));
+    for (int m = 0; m < b->nm; ++m) {
+      double e[3] = {0, 0, 0};
+      for (int d = 0; d < b->mol[m].n_atoms; ++d)
+        for (int q = 0; q < 3; ++q) e[q] += fx_value(&hv[kFxWords * (3 * (size_t)(b->mol[m].dom0 + d) + q)]);
+      if (!std::isfinite(e[0] + e[1] + e[2])) return bfail(b, RQ_EUNSTABLE, "molecule %d: non-finite energy", m);
+      if (E3)
+        for (int q = 0; q < 3; ++q) E3[3 * m + q] = e[q];
+    }
+  }
+  if (!backward) return RQ_OK;
+  // ---- reverse sweep ----
+  hipLaunchKernelGGL(kb_energy_grad, grd, blk, 0, b->st, LD(b, 0), b->mol_d, b->nm, nt, b->U, b->K, b->P, b->lab,
+                     A.lamU, A.gK, A.lamP);
+  BLAUNCH(b);
+  std::vector<const float*> Pused(C);
+  for (int64_t sg = nseg - 1; sg >= 0; --sg) {
+    const int64_t t0 = sg * C, t1 = std::min<int64_t>(Tn, t0 + C);
+    const bool recompute = sg != nseg - 1;
+    if (recompute) {
+      // the segment's inputs, recomputed from its checkpoint with the batch state
+      if ((rc = b_copy(b, b->U, A.ckU[sg])) || (rc = b_copy(b, b->P, A.ckP[sg]))) return rc;
+      for (int64_t t = t0 + 1; t <= t1; ++t) {
+        const int idx = (int)(t - t0 - 1);
+        if ((rc = b_copy(b, A.segU[idx], b->U))) return rc;
+        bool did = false;
+        if (t % sc.poisson_interval == 0) {
+          if ((rc = b_poisson(b, b->fuse_itp && t < t1, &did))) return rc;
+          if ((rc = b_copy(b, A.segP[idx], b->P))) return rc;
+        }
+        if (t < t1) {
+          if (!did) {
+            if ((rc = b_itp(b, b->U, b->U2, b->P))) return rc;
+            std::swap(b->U, b->U2);
+          }
+          if (t % sc.norm_interval == 0 && (rc = b_normalize_on(b, b->U))) return rc;
+        }
+      }
+    }
+    // P in use at each step of the segment (the last Poisson step's output)
+    const float* Pprev = A.ckP[sg];
+    for (int64_t t = t0 + 1; t <= t1; ++t) {
+      const int idx = (int)(t - t0 - 1);
+      Pused[idx] = (t % sc.poisson_interval == 0) ? A.segP[idx] : Pprev;
+      Pprev = Pused[idx];
+    }
+    for (int64_t t = t1; t > t0; --t) {
+      const int idx = (int)(t - t0 - 1);
+      const float* Uin = A.segU[idx];
+      if (t % sc.norm_interval == 0) {
+        if ((rc = b_itp(b, Uin, A.t1, Pused[idx]))) return rc;
+        if ((rc = b_normalize_adj(b, A.t1))) return rc;
+      }
+      hipLaunchKernelGGL(kb_itp_adj, grd, blk, 0, b->st, LD(b, 0), b->mol_d, b->nm, nt, A.lamU, A.lamU2, Uin, b->K,
+                         Pused[idx], b->lab, A.gK, A.lamP);
+      BLAUNCH(b);
+      std::swap(A.lamU, A.lamU2);
+      if (t % sc.poisson_interval == 0 && (rc = b_poisson_adj(b, Uin))) return rc;
+    }
+  }
+  for (int i = 0; i < sp->bootstrap; ++i)
+    if ((rc = b_poisson_adj(b, A.ckU[0]))) return rc;
+  // initial normalise: recompute the trial field into t1 (K is unchanged)
+  hipLaunchKernelGGL(kb_init_frozen, dim3(std::min((b->nnodes + 255) / 256, 65535)), dim3(256), 0, b->st, b->mol_d,
+                     LD(b, 0), b->nm, b->nnodes, b->atoms_d, b->lab, A.t1, b->K);
+  BLAUNCH(b);
+  if ((rc = b_normalize_adj(b, A.t1))) return rc;
+  Fx* gout = A.red + 2 * kFxWords * (size_t)b->ndom;  // 3 per atom (one domain per atom)
+  launch_fill(b->st, reinterpret_cast<float*>(gout), 2 * 3 * kFxWords * (size_t)b->ndom, 0.0f);
+  hipLaunchKernelGGL(kb_pos_grad, dim3(std::min(A.npg, 1 << 20)), dim3(64, 4), 0, b->st, A.ptile0, LD(b, 0), b->mol_d,
+                     b->nm, A.npg, A.gK, A.lamU, b->lab, b->atoms_d, gout);
+  BLAUNCH(b);
+  std::vector<Fx> hg(kFxWords * 3 * (size_t)b->ndom);
+  BCHK(b, hipMemcpyAsync(hg.data(), gout, sizeof(Fx) * hg.size(), hipMemcpyDeviceToHost, b->st));
+  BCHK(b, hipStreamSynchronize(b->st));
+  if (dEdR)
+    for (size_t q = 0; q < 3 * (size_t)b->ndom; ++q) dEdR[q] = fx_value(&hg[kFxWords * q]);
+  return RQ_OK;
 }
 
 int upload_descriptors(rq_batch* b) {
@@ -870,6 +1323,7 @@ int rq_batch_create(const rq_params* pp, const rq_molecule* mols, int32_t n_mols
   }
   b->nnodes = node0;
   b->ndom = (int)atoms.size();
+  b->atoms_h = atoms;
   b->maxS.assign(L, 0);
   for (int l = 0; l < L; ++l)
     for (int m = 0; m < n_mols; ++m) b->maxS[l] = std::max(b->maxS[l], b->lev[l][m].S);
@@ -1011,6 +1465,33 @@ int rq_batch_step(rq_batch* b, int64_t first, int32_t n, const rq_schedule* sc) 
     ++s;
   }
   return RQ_OK;
+}
+
+int rq_batch_set_atoms(rq_batch* b, int32_t m, const rq_atom* atoms, int32_t n) {
+  if (!b || !atoms) return bfail(b, RQ_EINVAL, "null argument");
+  if (m < 0 || m >= b->nm) return bfail(b, RQ_ERANGE, "molecule %d outside [0, %d)", m, b->nm);
+  const BMol& mo = b->mol[m];
+  if (n != mo.n_atoms) return bfail(b, RQ_EINVAL, "molecule %d has %d atoms, got %d", m, mo.n_atoms, n);
+  for (int a = 0; a < n; ++a) {
+    Atom& at = b->atoms_h[mo.atom0 + a];
+    if (atoms[a].Z != at.Z) return bfail(b, RQ_EINVAL, "molecule %d atom %d: Z changes from %g", m, a, at.Z);
+    at = {atoms[a].i, atoms[a].j, atoms[a].k, atoms[a].Z, atoms[a].rc};
+  }
+  BCHK(b, hipMemcpyAsync(b->atoms_d + mo.atom0, b->atoms_h.data() + mo.atom0, sizeof(Atom) * n, hipMemcpyHostToDevice,
+                         b->st));
+  BCHK(b, hipStreamSynchronize(b->st));
+  drop_bgraphs(b);
+  return RQ_OK;
+}
+
+int rq_batch_solve_frozen(rq_batch* b, const rq_adjoint_spec* spec, double* energy3) {
+  if (!b) return bfail(nullptr, RQ_EINVAL, "null batch");
+  return b_frozen_run(b, spec, false, energy3, nullptr);
+}
+
+int rq_batch_adjoint_gradient(rq_batch* b, const rq_adjoint_spec* spec, double* energy3, double* dEdR) {
+  if (!b) return bfail(nullptr, RQ_EINVAL, "null batch");
+  return b_frozen_run(b, spec, true, energy3, dEdR);
 }
 
 int rq_batch_sync(rq_batch* b) {
