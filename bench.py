@@ -469,6 +469,65 @@ def bench_c3(steps, warmup):
     }
 
 
+def bench_c3_forces(n_steps, checkpoint, log_fn):
+    """Batched adjoint forces on the 64-conformer scan: the frozen-label map
+    (trial, normalise, 10 bootstrap V-cycles, n_steps steps of a 4-level
+    V(2,2) + ITP) of every conformer and its reverse sweep in one launch per
+    pass (rq_batch_adjoint_gradient), against the same map and adjoint on one
+    handle per conformer in turn, and against central differences (6N+1
+    solves per conformer, extrapolated from each conformer's timed solve)."""
+    from realmolecule_amd import builders as B
+    from realmolecule_amd.adjoint import FrozenSolver, make_spec
+    from realmolecule_amd.batch import Batch
+    mols = B.c3_conformers()
+    spec = make_spec(n_steps, bootstrap=10, checkpoint=checkpoint, poisson_interval=1, norm_interval=20)
+    b = Batch([(r, nn, sc) for r, nn, sc, _ in mols], mg_levels=4, graph=False)
+    b.initialize(bootstrap=0)
+    b.adjoint_gradient(spec)  # warm: allocations
+    b.sync()
+    t0 = time.perf_counter()
+    E3, grads = b.adjoint_gradient(spec)
+    t_batch = time.perf_counter() - t0
+    b.close()
+    log_fn(f"[c3 forces] batched: {len(mols)} force vectors in {t_batch:.3f} s")
+    t_seq = t_fd = 0.0
+    max_diff = 0.0
+    n_atoms = 0
+    for m, (r, nn, sc, name) in enumerate(mols):
+        fz = FrozenSolver(r, nn, sc, n_steps, bootstrap=10, checkpoint=checkpoint, poisson_interval=1,
+                          poisson="multigrid", mg_levels=4)
+        pos = r[:, :3] * fz.h
+        fz.energy_and_gradient(pos)  # warm
+        fz.s.sync()
+        t0 = time.perf_counter()
+        E, g = fz.energy_and_gradient(pos)
+        t_seq += time.perf_counter() - t0
+        t0 = time.perf_counter()
+        fz.energy(pos)
+        t_solve = time.perf_counter() - t0
+        t_fd += (6 * len(r) + 1) * t_solve
+        from realmolecule_amd.solver import nuclear_forces
+        g_el = g + nuclear_forces(r[:, :3].astype(np.float32), r[:, 3], fz.h)
+        max_diff = max(max_diff, float(np.abs(g_el - grads[m]).max()))
+        n_atoms += len(r)
+        fz.close()
+    log_fn(f"[c3 forces] one handle per conformer: {t_seq:.3f} s; central differences {t_fd:.1f} s")
+    n = len(mols)
+    return {
+        "metric": "batched adjoint force vectors/s (64-conformer scan)",
+        "workload": f"64 conformers (seed 2) of H2O/formamide/ethanol/glycine, grids 64..160 cells, {n_atoms} atoms; "
+                    f"frozen-label map of 10 bootstrap V-cycles + {n_steps} steps (4-level V(2,2) + ITP), "
+                    f"checkpoints every {checkpoint} steps",
+        "batched_adjoint_forces_per_s": n / t_batch,
+        "sequential_adjoint_forces_per_s": n / t_seq,
+        "plain_fd_forces_per_s": n / t_fd,
+        "batch_speedup_over_sequential_adjoint": t_seq / t_batch,
+        "speedup_batched_adjoint_over_fd": t_fd / t_batch,
+        "max_abs_dEdR_diff_batched_vs_single": max_diff,
+        "seconds": {"batched": t_batch, "sequential_adjoint": t_seq, "fd_extrapolated": t_fd},
+    }
+
+
 def bench_mf(args):
     """mol_fast.js water: frames of 500 fused steps (the page's doSteps),
     inputs resident on the device; roofline of the fused update from HIP
@@ -768,7 +827,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="c5",
                     choices=("c1", "c2", "c3", "c4", "c5", "mf", "pd", "sic", "hairpin", "c5forces", "c4forces",
-                             "c4relax"))
+                             "c4relax", "c3forces"))
     ap.add_argument("--force-window", type=int, default=100, help="steps of the map a full adjoint differentiates")
     ap.add_argument("--converge-max", type=int, default=10000)
     ap.add_argument("--relax-iters", type=int, default=50)
@@ -802,6 +861,9 @@ def main():
     if args.workload in ("c5forces", "c4forces"):
         print(json.dumps(bench_full_adjoint(args.workload[:2], args.force_window, args.converge_max,
                                             args.force_components, log)))
+        return
+    if args.workload == "c3forces":
+        print(json.dumps(bench_c3_forces(args.force_window, 10, log)))
         return
     if args.workload == "c4relax":
         print(json.dumps(bench_relax("c4", args.relax_iters, args.relax_fd_iters, args.relax_steps, log)))
