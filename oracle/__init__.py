@@ -16,9 +16,14 @@ Contents
             two-grid correction) and the exact FP64 coarsest-level solve.
 ``s2``      restatement of the ``molecule.js`` WGSL kernels on the hot path:
             level-set boundary update, damped smoother and two-grid correction,
-            exclusion/dipole energy, sphere-averaged grad-P force, trial init.
+            exclusion/dipole energy, sphere-averaged grad-P force, trial init,
+            the per-atom fields (Z_nuc, initZeff, initRcut, angular splits),
+            unique-nucleus E_KK, the frame potential with SIC and the page's
+            direct per-electron Pother.
 ``adjoint`` the S1 fixed-step trajectory energy (frozen labels) in torch
             FP64, whose autograd gives dE/dR with respect to atom positions.
+``cstep``   (``cstep.c`` + ``cstep.py``) the S1 step with the V-cycle in C +
+            OpenMP: the all-core CPU baseline, pinned against ``s1`` + ``mg``.
 ``molfast`` restatement of the ``mol_fast.js`` fused per-electron engine:
             set-up in float64, the WGSL update, reduction, normalisation,
             per-electron two-grid correction, forces, K refresh and extract
