@@ -286,6 +286,10 @@ void zero_fx(rq_solver* s, Fx* p, size_t n) {
 }
 
 // Values of n fixed-point sums copied from the device (synchronises the stream).
+// the normalisation's per-domain factors: n_atoms floats after the n_atoms
+// norm sums at the start of red (red holds 3 n_atoms + 3 sums)
+float* norm_scales(rq_solver* s) { return reinterpret_cast<float*>(s->red + kFxWords * (size_t)s->n_atoms); }
+
 // health word idx (0: normalisation sums, 1: energy sums) after the stream drains
 int read_flag(rq_solver* s, int idx, int* out) {
   HIPCHK(s, hipMemcpyAsync(out, s->health + idx, sizeof(int), hipMemcpyDeviceToHost, s->st));
@@ -742,7 +746,7 @@ int normalize_step(rq_solver* s) {
   {
     Timed t(s, F_NORM);
     launch_norm_sums(s->st, s->tile, s->S, s->h3, s->U, s->lab, s->n_atoms, norms, s->health);
-    launch_normalize(s->st, s->tile, s->S, s->U, s->lab, norms, s->Z_d, s->n_atoms);
+    launch_normalize(s->st, s->tile, s->S, s->U, s->lab, norms, s->Z_d, s->n_atoms, norm_scales(s));
   }
   LAUNCHCHK(s);
   return RQ_OK;
@@ -955,7 +959,7 @@ int poisson_into(rq_solver* s, const float* P_in, float* P_out, const float* U, 
 int normalize_on(rq_solver* s, float* U) {
   zero_fx(s, s->red, s->n_atoms);
   launch_norm_sums(s->st, s->tile, s->S, s->h3, U, s->lab, s->n_atoms, s->red, s->health);
-  launch_normalize(s->st, s->tile, s->S, U, s->lab, s->red, s->Z_d, s->n_atoms);
+  launch_normalize(s->st, s->tile, s->S, U, s->lab, s->red, s->Z_d, s->n_atoms, norm_scales(s));
   LAUNCHCHK(s);
   return RQ_OK;
 }

@@ -183,8 +183,9 @@ void launch_smooth_itp(hipStream_t st, const Tile& t, int S, const float* P, flo
 void launch_zero(hipStream_t st, float* p, size_t n);
 void launch_norm_sums(hipStream_t st, const Tile& t, int S, float h3, const float* U, const uint16_t* lab,
                       int n_dom, Fx* norms /* n_dom sums */, int* bad /* set to 1 on a non-finite sum */);
+// scale: n_dom floats of scratch (the per-domain factors)
 void launch_normalize(hipStream_t st, const Tile& t, int S, float* U, const uint16_t* lab, const Fx* norms,
-                      const float* Z, int n_dom);
+                      const float* Z, int n_dom, float* scale);
 // Score rule.  Up to kScoreScanMaxAtoms atoms: a scan over all atoms.  Above:
 // the argmax over atoms is searched through a uniform cell list (cells of cs
 // nodes, n per axis; cell_start[n^3 + 1], cell_atoms sorted by cell; zmax =

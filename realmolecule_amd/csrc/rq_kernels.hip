@@ -1593,19 +1593,20 @@ __global__ __launch_bounds__(256) void k_norm_sums(int S, int ich, float h3, con
   for (int t = tid; t < n_dom; t += nthr) fx_add_words(norms + kFxWords * t, fbins + kFxWords * t);
 }
 
+// per-domain scale sqrt(Z / n) (1 for empty or uncharged domains), once per
+// normalisation rather than in every workgroup of the scaling pass
+__global__ __launch_bounds__(256) void k_norm_scales(int n_dom, const Fx* __restrict__ norms,
+                                                     const float* __restrict__ Z, float* __restrict__ scale) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n_dom) return;
+  const double z = (double)Z[t];
+  const double nv = fx_value(norms + kFxWords * t);
+  scale[t] = (z > 0.0 && nv > 1e-20) ? (float)sqrt(z / nv) : 1.0f;
+}
+
 __global__ __launch_bounds__(256) void k_normalize(int S, int ich, float* __restrict__ U,
-                                                   const uint16_t* __restrict__ lab,
-                                                   const Fx* __restrict__ norms, const float* __restrict__ Z,
+                                                   const uint16_t* __restrict__ lab, const float* __restrict__ scale,
                                                    int n_dom) {
-  __shared__ float scale[kMaxDomains];
-  const int nthr = blockDim.x * blockDim.y;
-  const int tid = threadIdx.y * blockDim.x + threadIdx.x;
-  for (int t = tid; t < n_dom; t += nthr) {
-    const double z = (double)Z[t];
-    const double nv = fx_value(norms + kFxWords * t);
-    scale[t] = (z > 0.0 && nv > 1e-20) ? (float)sqrt(z / nv) : 1.0f;
-  }
-  __syncthreads();
   March m;
   if (!march_coords(S - 2, ich, m)) return;
   normalize_col(S, m.k, m.j, m.i0, m.i1, U, lab, scale, n_dom);
@@ -2606,8 +2607,9 @@ void launch_norm_sums(hipStream_t st, const Tile& t, int S, float h3, const floa
 }
 
 void launch_normalize(hipStream_t st, const Tile& t, int S, float* U, const uint16_t* lab, const Fx* norms,
-                      const float* Z, int n_dom) {
-  hipLaunchKernelGGL(k_normalize, stencil_grid(S - 2, t), dim3(t.bx, t.by), 0, st, S, t.ich, U, lab, norms, Z, n_dom);
+                      const float* Z, int n_dom, float* scale) {
+  hipLaunchKernelGGL(k_norm_scales, dim3((n_dom + 255) / 256), dim3(256), 0, st, n_dom, norms, Z, scale);
+  hipLaunchKernelGGL(k_normalize, stencil_grid(S - 2, t), dim3(t.bx, t.by), 0, st, S, t.ich, U, lab, scale, n_dom);
 }
 
 void launch_score_map(hipStream_t st, int S, float h, float h2, const Atom* atoms, int n_atoms, uint16_t* best_atom) {
