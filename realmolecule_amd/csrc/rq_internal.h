@@ -22,8 +22,8 @@ constexpr int kLegsMaxS = 65;           // one down-leg + one up-leg launch per 
 constexpr int kResResMarchCJ = 2;       // coarse j-columns per residual+restriction march workgroup (128 CJ threads)
 constexpr int kResResMarchCP = 2;       // coarse planes restricted per barrier round of that march
 constexpr int kSmooth2Blocks = 8192;    // target workgroup count of the two-sweep march
-constexpr int kSmooth2ResresBlocks = 4096;
-constexpr int kPostBlocks = 8192;  // target workgroup count of the correction + post-sweeps + ITP march  // target workgroup count of the pre-smoothing + restriction march
+constexpr int kSmooth2ResresBlocks = 2048;  // target workgroup count of the pre-smoothing + restriction march
+constexpr int kPostBlocks = 8192;  // target workgroup count of the correction + post-sweeps + ITP march
 constexpr int kSmooth2MinS = 320;       // smallest level (nodes per axis) that runs sweep pairs as that march
 constexpr int kDirectSweeps = 10;       // damped sweeps per step of each direct potential (molecule.js:2835 JACOBI_DIRECT)
 constexpr int kDomStreams = 4;          // concurrent per-domain V-cycles of the direct potential (one stream each)

@@ -510,9 +510,12 @@ __global__ __launch_bounds__(NT) void k_smooth2_resres_march(int Sf, int CI, con
   const int k_lo = 2 * K0 - 1, k_hi = (K0 + CK - 1 >= nc) ? n : 2 * K0 + 2 * CK - 2;
   const int j_lo = 2 * J0 - 1, j_hi = (J0 + CJ - 1 >= nc) ? n : 2 * J0 + 2 * CJ - 2;
   const int i_lo = 2 * I0 - 1, i_hi = (I1 == nc + 1) ? n : 2 * I1 - 2;
-  // first-sweep region nodes of this thread; loads clamped into the grid,
-  // interior nodes read their in-plane neighbours at oj / ok, others keep x
-  int col[P2], oj[P2], ok[P2];
+  // first-sweep region nodes of this thread: byte offset of the column
+  // clamped into the grid.  On a plane p in [1, n] every load of the sweep
+  // (x at p +- 1 and the four in-plane neighbours, U at p) stays inside the
+  // grid for a clamped column; the values loaded for a non-interior node are
+  // discarded by the select that keeps its x.
+  uint32_t col4[P2];
   bool upd[P2];
   float xm[P2], xc[P2], xp[P2], nb[P2][4], u[P2];
   const int p_first = 2 * I0 - 3, p_last = 2 * I1 + 1;
@@ -521,29 +524,46 @@ __global__ __launch_bounds__(NT) void k_smooth2_resres_march(int Sf, int CI, con
     const int o = min(tid + q * NT, PL2 - 1);
     const int j = 2 * J0 - 3 + o / W2, k = 2 * K0 - 3 + o % W2;
     upd[q] = tid + q * NT < PL2 && j >= 1 && j <= n && k >= 1 && k <= n;
-    oj[q] = upd[q] ? Sf : 0;
-    ok[q] = upd[q] ? 1 : 0;
-    col[q] = min(max(j, 0), Sf - 1) * Sf + min(max(k, 0), Sf - 1);
-    xm[q] = (p_first - 1 >= 0) ? x[(p_first - 1) * S2 + col[q]] : 0.0f;
-    xc[q] = (p_first >= 0) ? x[p_first * S2 + col[q]] : 0.0f;
+    col4[q] = 4u * static_cast<uint32_t>(min(max(j, 0), Sf - 1) * Sf + min(max(k, 0), Sf - 1));
+    xm[q] = (p_first - 1 >= 0) ? ld_at(x + (p_first - 1) * S2, col4[q]) : 0.0f;
+    xc[q] = (p_first >= 0) ? ld_at(x + p_first * S2, col4[q]) : 0.0f;
+  }
+  // second-sweep nodes: LDS index of the node in a first-sweep slot, and the
+  // byte offset of the store for nodes the tile owns (~0u: no store)
+  int c1[P1];
+  uint32_t st1[P1];
+  bool in1[P1], ok1[P1];
+#pragma unroll
+  for (int q = 0; q < P1; ++q) {
+    const int o = min(tid + q * NT, PL1 - 1);
+    const int jj = o / W1, kk = o % W1;
+    const int j = 2 * J0 - 2 + jj, k = 2 * K0 - 2 + kk;
+    c1[q] = (jj + 1) * W2 + (kk + 1);
+    ok1[q] = tid + q * NT < PL1;
+    in1[q] = j >= 1 && j <= n && k >= 1 && k <= n;
+    st1[q] = (in1[q] && j >= j_lo && j <= j_hi && k >= k_lo && k <= k_hi) ? 4u * static_cast<uint32_t>(j * Sf + k)
+                                                                          : ~0u;
   }
   // inputs of the first sweep at plane p: x at p + 1, the in-plane
   // neighbours and U at p (only x at p + 1 outside the interior planes)
   {
     const int PP = p_first;
+    if (PP >= 1 && PP <= n) {
+      const float* xq = x + PP * S2;
+      const float* uq = src + PP * S2;
 #pragma unroll
-    for (int q = 0; q < P2; ++q) {
-      if (PP >= 1 && PP <= n) {
-        const int id = PP * S2 + col[q];
-        xp[q] = x[id + S2];
-        nb[q][0] = x[id + oj[q]];
-        nb[q][1] = x[id - oj[q]];
-        nb[q][2] = x[id + ok[q]];
-        nb[q][3] = x[id - ok[q]];
-        u[q] = src[id];
-      } else {
-        xp[q] = (PP + 1 >= 0 && PP + 1 <= Sf - 1) ? x[(PP + 1) * S2 + col[q]] : 0.0f;
+      for (int q = 0; q < P2; ++q) {
+        xp[q] = ld_at(xq + S2, col4[q]);
+        nb[q][0] = ld_at(xq + Sf, col4[q]);
+        nb[q][1] = ld_at(xq - Sf, col4[q]);
+        nb[q][2] = ld_at(xq + 1, col4[q]);
+        nb[q][3] = ld_at(xq - 1, col4[q]);
+        u[q] = ld_at(uq, col4[q]);
       }
+    } else {
+      const bool in = PP + 1 >= 0 && PP + 1 <= Sf - 1;
+#pragma unroll
+      for (int q = 0; q < P2; ++q) xp[q] = in ? ld_at(x + (PP + 1) * S2, col4[q]) : 0.0f;
     }
   }
 
@@ -580,20 +600,23 @@ __global__ __launch_bounds__(NT) void k_smooth2_resres_march(int Sf, int CI, con
       }
       if (p < p_last) {
         const int PP = p + 1;
-  #pragma unroll
-      for (int q = 0; q < P2; ++q) {
         if (PP >= 1 && PP <= n) {
-          const int id = PP * S2 + col[q];
-          xp[q] = x[id + S2];
-          nb[q][0] = x[id + oj[q]];
-          nb[q][1] = x[id - oj[q]];
-          nb[q][2] = x[id + ok[q]];
-          nb[q][3] = x[id - ok[q]];
-          u[q] = src[id];
+          const float* xq = x + PP * S2;
+          const float* uq = src + PP * S2;
+#pragma unroll
+          for (int q = 0; q < P2; ++q) {
+            xp[q] = ld_at(xq + S2, col4[q]);
+            nb[q][0] = ld_at(xq + Sf, col4[q]);
+            nb[q][1] = ld_at(xq - Sf, col4[q]);
+            nb[q][2] = ld_at(xq + 1, col4[q]);
+            nb[q][3] = ld_at(xq - 1, col4[q]);
+            u[q] = ld_at(uq, col4[q]);
+          }
         } else {
-          xp[q] = (PP + 1 >= 0 && PP + 1 <= Sf - 1) ? x[(PP + 1) * S2 + col[q]] : 0.0f;
+          const bool in = PP + 1 >= 0 && PP + 1 <= Sf - 1;
+#pragma unroll
+          for (int q = 0; q < P2; ++q) xp[q] = in ? ld_at(x + (PP + 1) * S2, col4[q]) : 0.0f;
         }
-      }
       }
     }
     lds_barrier();
@@ -601,29 +624,28 @@ __global__ __launch_bounds__(NT) void k_smooth2_resres_march(int Sf, int CI, con
     const int f1 = p - 1;
     if (f1 >= 2 * I0 - 2 && f1 >= 0 && f1 <= Sf - 1) {
       const bool pin = f1 >= 1 && f1 <= n;
+      const bool own_plane = f1 >= i_lo && f1 <= i_hi;
+      float* xf = xn + f1 * S2;
       // planes f1 + 1 = p, f1, f1 - 1 in the slots of p, p - 1, p - 2
-      const int sp1 = s_p, s0b = s_m1 * PL2, sm1 = s_m2;
+      const float* rp = ring2 + s_p * PL2;
+      const float* r0 = ring2 + s_m1 * PL2;
+      const float* rm = ring2 + s_m2 * PL2;
+      const float* u0 = ringu + s_m1 * PL2;
 #pragma unroll
       for (int q = 0; q < P1; ++q) {
-        const int o = tid + q * NT;
-        if (o < PL1) {
-          const int jj = o / W1, kk = o % W1;
-          const int j = 2 * J0 - 2 + jj, k = 2 * K0 - 2 + kk;
-          const int cc = (jj + 1) * W2 + (kk + 1);
-          const int s0 = s0b + cc;
-          float v = ring2[s0];
-          if (pin && j >= 1 && j <= n && k >= 1 && k <= n) {
-            const float s6 = ring2[sp1 * PL2 + cc] + ring2[sm1 * PL2 + cc] + ring2[s0 + W2] + ring2[s0 - W2] +
-                             ring2[s0 + 1] + ring2[s0 - 1];
-            const float uu = ringu[s0];
+        if (ok1[q]) {
+          const int cc = c1[q];
+          float v = r0[cc];
+          if (pin && in1[q]) {
+            const float s6 = rp[cc] + rm[cc] + r0[cc + W2] + r0[cc - W2] + r0[cc + 1] + r0[cc - 1];
+            const float uu = u0[cc];
             const float rhs = (alpha * (uu * uu)) * beta;
             float w = (s6 + rhs) / c;
             if (a != 0.0f) w = a * v + w;
             v = w;
-            if (f1 >= i_lo && f1 <= i_hi && j >= j_lo && j <= j_hi && k >= k_lo && k <= k_hi)
-              xn[f1 * S2 + j * Sf + k] = v;
+            if (own_plane && st1[q] != ~0u) st_at(xf, st1[q], v);
           }
-          ring1[s_m1 * PL1 + o] = v;
+          ring1[s_m1 * PL1 + tid + q * NT] = v;
         }
       }
     }
@@ -631,6 +653,10 @@ __global__ __launch_bounds__(NT) void k_smooth2_resres_march(int Sf, int CI, con
     // ---- residual at fine plane p - 2 ----
     const int f0 = p - 2;
     if (f0 >= 2 * I0 - 1 && f0 <= 2 * I1 - 1) {
+      const float* q0 = ring1 + s_m2 * PL1;
+      const float* qa = ring1 + s_m1 * PL1;
+      const float* qb = ring1 + s_p * PL1;
+      const float* uq = ringu + s_m2 * PL2;
 #pragma unroll
       for (int q = 0; q < P0; ++q) {
         const int o = tid + q * NT;
@@ -638,12 +664,10 @@ __global__ __launch_bounds__(NT) void k_smooth2_resres_march(int Sf, int CI, con
           const int jj = o / W0, kk = o % W0;
           const int j = 2 * J0 - 1 + jj, k = 2 * K0 - 1 + kk;
           const int cc = (jj + 1) * W1 + (kk + 1);
-          const int s0 = s_m2 * PL1 + cc;
-          const float s6 = ring1[s_m1 * PL1 + cc] + ring1[s_p * PL1 + cc] + ring1[s0 + W1] + ring1[s0 - W1] +
-                           ring1[s0 + 1] + ring1[s0 - 1];
-          const float uu = ringu[s_m2 * PL2 + (jj + 2) * W2 + (kk + 2)];
+          const float s6 = qa[cc] + qb[cc] + q0[cc + W1] + q0[cc - W1] + q0[cc + 1] + q0[cc - 1];
+          const float uu = uq[(jj + 2) * W2 + (kk + 2)];
           const float rhs = (alpha * (uu * uu)) * beta;
-          const float r = rhs - (6.0f * ring1[s0] - s6);
+          const float r = rhs - (6.0f * q0[cc] - s6);
           ring0[s_m2 * PL0 + o] = (j <= n && k <= n) ? r : 0.0f;
         }
       }
@@ -669,8 +693,9 @@ __global__ __launch_bounds__(NT) void k_smooth2_resres_march(int Sf, int CI, con
 // update on the new potential (temporal blocking of prolongation, sweep
 // pair and update).  A workgroup owns TK x TJ fine interior columns and
 // planes [I0, I1).  Per plane: stage 0 loads, on the region R3 = tile + 2,
-// x with the trilinear coarse correction (PlaneCache, prolong_correct_col's
-// operations), U and the label into LDS rings; stage 1 forms the first sweep
+// x with the trilinear coarse correction (each coarse-plane interpolant
+// formed once, prolong_correct_col's operations), U and the label into LDS
+// rings; stage 1 forms the first sweep
 // on R2 = tile + 1 from them; stage 2 forms the second sweep on the tile
 // (written to xn) and the ITP update of U with it (written to Un).  Stage 0
 // runs at plane s, stage 1 at s - 1, stage 2 at s - 2; rings of three (x,
@@ -694,28 +719,62 @@ __global__ __launch_bounds__(TK * TJ) void k_post_march(int Sf, int CI, const fl
   __shared__ float ru[RU * PL3];      // U
   __shared__ uint16_t rl[RU * PL3];   // label
   __shared__ float r1[3 * PL2];       // first sweep
-  const int n = Sf - 2, Sc = (Sf - 1) / 2 + 1, S2 = Sf * Sf;
+  const int n = Sf - 2, Sc = (Sf - 1) / 2 + 1, S2 = Sf * Sf, Sc2 = Sc * Sc;
   const int gx = (n + TK - 1) / TK, gy = (n + TJ - 1) / TJ;
   const int t = xcd_block();
   const int K0 = 1 + (t % gx) * TK, J0 = 1 + ((t / gx) % gy) * TJ, I0 = 1 + (t / (gx * gy)) * CI;
   const int I1 = min(I0 + CI, n + 1);
   const int tid = threadIdx.x;
-  // stage-0 nodes of this thread on R3 (loads clamped into the grid)
-  int col[P3], jr[P3], kr[P3];
+  // stage-0 nodes of this thread on R3: byte offsets of the column clamped
+  // into the grid (x, U; the label at half of it), and, for interior
+  // columns, of the four coarse nodes of the bilinear in-plane interpolant
+  // (lerp_k along k on rows j >> 1 and (j >> 1) + (j & 1)); other columns
+  // read coarse node 0, which is in the grid, and discard the value
+  uint32_t col4[P3], col2[P3], e00[P3], e01[P3], e10[P3], e11[P3];
   bool cin[P3];
-  PlaneCache pc[P3];
 #pragma unroll
   for (int q = 0; q < P3; ++q) {
     const int o = min(tid + q * NT, PL3 - 1);
-    jr[q] = J0 - 2 + o / W3;
-    kr[q] = K0 - 2 + o % W3;
-    cin[q] = tid + q * NT < PL3 && jr[q] >= 1 && jr[q] <= n && kr[q] >= 1 && kr[q] <= n;
-    col[q] = min(max(jr[q], 0), Sf - 1) * Sf + min(max(kr[q], 0), Sf - 1);
+    const int jr = J0 - 2 + o / W3, kr = K0 - 2 + o % W3;
+    cin[q] = tid + q * NT < PL3 && jr >= 1 && jr <= n && kr >= 1 && kr <= n;
+    const uint32_t col = static_cast<uint32_t>(min(max(jr, 0), Sf - 1) * Sf + min(max(kr, 0), Sf - 1));
+    col4[q] = 4u * col;
+    col2[q] = 2u * col;
+    const uint32_t b = cin[q] ? static_cast<uint32_t>((jr >> 1) * Sc + (kr >> 1)) : 0u;
+    const uint32_t dk = cin[q] ? static_cast<uint32_t>(kr & 1) : 0u;
+    const uint32_t dj = cin[q] ? static_cast<uint32_t>((jr & 1) * Sc) : 0u;
+    e00[q] = 4u * b;
+    e01[q] = 4u * (b + dk);
+    e10[q] = 4u * (b + dj);
+    e11[q] = 4u * (b + dj + dk);
+  }
+  // first-sweep nodes on R2: LDS index in an R3 slot, interior flag
+  int c2[P2];
+  bool in2[P2];
+#pragma unroll
+  for (int q = 0; q < P2; ++q) {
+    const int o = min(tid + q * NT, PL2 - 1);
+    const int jj = o / W2, kk = o % W2;
+    const int j = J0 - 1 + jj, k = K0 - 1 + kk;
+    c2[q] = (jj + 1) * W3 + (kk + 1);
+    in2[q] = j >= 1 && j <= n && k >= 1 && k <= n;
   }
   // stage-2 node of this thread (one per thread)
   const int tk = tid % TK, tj = tid / TK;
   const int jo = J0 + tj, ko = K0 + tk;
   const bool own = jo <= n && ko <= n;
+  const uint32_t own4 = own ? 4u * static_cast<uint32_t>(jo * Sf + ko) : 0u;
+  // coarse-plane interpolants T(ic) of the prolongation, shared by the two
+  // fine planes 2 ic and 2 ic +- 1 that use them: ta = T(ca), tb = T(cb).
+  // The plane index is uniform, so the cache state is too.
+  float ta[P3], tb[P3];
+  int ca = -1, cb = -1;
+  auto tplane = [&](int ic, int q) -> float {
+    const float* ep = e + ic * Sc2;
+    const float l0 = 0.5f * (ld_at(ep, e00[q]) + ld_at(ep, e01[q]));
+    const float l1 = 0.5f * (ld_at(ep, e10[q]) + ld_at(ep, e11[q]));
+    return 0.5f * (l0 + l1);
+  };
   for (int s = I0 - 2; s <= I1 + 1; ++s) {
     // ---- stage 0: corrected x, U and label at plane s ----
     if (s >= 0 && s <= Sf - 1) {
@@ -723,19 +782,43 @@ __global__ __launch_bounds__(TK * TJ) void k_post_march(int Sf, int CI, const fl
       const bool pin = s >= 1 && s <= n;
       float xv[P3], uv[P3];
       uint16_t lv[P3];
+      const float* xs = x + s * S2;
+      const float* us = U + s * S2;
+      const uint16_t* ls = lab + s * S2;
 #pragma unroll
       for (int q = 0; q < P3; ++q) {
-        const int id = s * S2 + col[q];
-        xv[q] = x[id];
-        uv[q] = U[id];
-        lv[q] = lab[id];
+        xv[q] = ld_at(xs, col4[q]);
+        uv[q] = ld_at(us, col4[q]);
+        lv[q] = ld_at16(ls, col2[q]);
+      }
+      if (pin) {
+        // prolong_at(s) = 0.5 (T(s >> 1) + T((s >> 1) + (s & 1)))
+        const int ic = s >> 1;
+        if (ic != ca) {
+          if (ic == cb) {
+#pragma unroll
+            for (int q = 0; q < P3; ++q) ta[q] = tb[q];
+          } else {
+#pragma unroll
+            for (int q = 0; q < P3; ++q) ta[q] = tplane(ic, q);
+          }
+          ca = ic;
+        }
+        if ((s & 1) && cb != ic + 1) {
+#pragma unroll
+          for (int q = 0; q < P3; ++q) tb[q] = tplane(ic + 1, q);
+          cb = ic + 1;
+        }
+#pragma unroll
+        for (int q = 0; q < P3; ++q) {
+          const float t1 = (s & 1) ? tb[q] : ta[q];
+          if (cin[q]) xv[q] = xv[q] + 0.5f * (ta[q] + t1);
+        }
       }
 #pragma unroll
       for (int q = 0; q < P3; ++q) {
-        float v = xv[q];
-        if (pin && cin[q]) v = v + pc[q].corr(e, Sc, s, jr[q], kr[q]);
         if (tid + q * NT < PL3) {
-          rx[sx * PL3 + tid + q * NT] = v;
+          rx[sx * PL3 + tid + q * NT] = xv[q];
           ru[su * PL3 + tid + q * NT] = uv[q];
           rl[su * PL3 + tid + q * NT] = lv[q];
         }
@@ -746,25 +829,24 @@ __global__ __launch_bounds__(TK * TJ) void k_post_march(int Sf, int CI, const fl
     const int f1 = s - 1;
     if (f1 >= I0 - 1 && f1 >= 0 && f1 <= Sf - 1) {
       const bool pin = f1 >= 1 && f1 <= n;
-      const int bp = ((f1 + 1) % 3) * PL3, b0 = (f1 % 3) * PL3, bm = ((f1 + 2) % 3) * PL3, bu = (f1 % RU) * PL3;
+      const float* xp = rx + ((f1 + 1) % 3) * PL3;
+      const float* x0 = rx + (f1 % 3) * PL3;
+      const float* xm = rx + ((f1 + 2) % 3) * PL3;
+      const float* u0 = ru + (f1 % RU) * PL3;
 #pragma unroll
       for (int q = 0; q < P2; ++q) {
-        const int o = tid + q * NT;
-        if (o < PL2) {
-          const int jj = o / W2, kk = o % W2;
-          const int j = J0 - 1 + jj, k = K0 - 1 + kk;
-          const int cc = (jj + 1) * W3 + (kk + 1);
-          float v = rx[b0 + cc];
-          if (pin && j >= 1 && j <= n && k >= 1 && k <= n) {
-            const float s6 = rx[bp + cc] + rx[bm + cc] + rx[b0 + cc + W3] + rx[b0 + cc - W3] + rx[b0 + cc + 1] +
-                             rx[b0 + cc - 1];
-            const float u = ru[bu + cc];
+        if (tid + q * NT < PL2) {
+          const int cc = c2[q];
+          float v = x0[cc];
+          if (pin && in2[q]) {
+            const float s6 = xp[cc] + xm[cc] + x0[cc + W3] + x0[cc - W3] + x0[cc + 1] + x0[cc - 1];
+            const float u = u0[cc];
             const float rhs = (alpha * (u * u)) * beta;
             float w = (s6 + rhs) / c;
             if (a != 0.0f) w = a * v + w;
             v = w;
           }
-          r1[(f1 % 3) * PL2 + o] = v;
+          r1[(f1 % 3) * PL2 + tid + q * NT] = v;
         }
       }
     }
@@ -782,8 +864,7 @@ __global__ __launch_bounds__(TK * TJ) void k_post_march(int Sf, int CI, const fl
       const float rhs = (alpha * (uc * uc)) * beta;
       float v = (s6 + rhs) / c;
       if (a != 0.0f) v = a * r1[b0 + cc] + v;
-      const int id = f2 * S2 + jo * Sf + ko;
-      xn[id] = v;
+      st_at(xn + f2 * S2, own4, v);
       const uint16_t lc = rl[u0];
       float out;
       if (lc & kRcFlag) {
@@ -796,9 +877,9 @@ __global__ __launch_bounds__(TK * TJ) void k_post_march(int Sf, int CI, const fl
         const float t4 = (rl[u0 + 1] == lc) ? ru[u0 + 1] : uc;
         const float t5 = (rl[u0 - 1] == lc) ? ru[u0 - 1] : uc;
         const float lap = t0 + t1 + t2 + t3 + t4 + t5 - 6.0f * uc;
-        out = uc + half_dv * lap + dtv * (K[id] - 2.0f * v) * uc;
+        out = uc + half_dv * lap + dtv * (ld_at(K + f2 * S2, own4) - 2.0f * v) * uc;
       }
-      Un[id] = out;
+      st_at(Un + f2 * S2, own4, out);
     }
     // with four U / label slots the next stage 0 overwrites plane s - 3,
     // which this stage read; with five it writes a slot no stage reads now
@@ -2399,10 +2480,12 @@ void smooth2_resres(hipStream_t st, int Sf, const float* x, float* xn, const flo
 
 void launch_smooth2_resres(hipStream_t st, int Sf, const float* x, float* xn, const float* src, float a, float c,
                            float alpha, float beta, float* gc, int blocks) {
-  // 32 x 4 coarse columns (64 x 8 fine, as many fine columns per plane as
-  // the two-sweep march), 512 threads, ~4096 workgroups: 852 us on the
-  // 513^3 grid, against 890 for 16 x 8, 931 for 16 x 4 (256 threads) and
-  // 939 for 32 x 8 (1024 threads) (RQ_SMOOTH2_RESRES_SHAPE 1-3)
+  // 32 x 8 coarse columns (64 x 16 fine), 768 threads, ~2048 workgroups:
+  // each stage's region fills two rounds of the workgroup (1.5 thread slots
+  // per owned node and stage, against 2 for 32 x 4 with 512 threads).  On
+  // the 513^3 grid: 656 us, against 683 with 4096 workgroups, 777 for
+  // 32 x 4 / 512, 712 for 32 x 8 / 512 and 778 for 16 x 16 / 768
+  // (RQ_SMOOTH2_RESRES_SHAPE 1-6, RQ_SMOOTH2_RESRES_BLOCKS)
   static const int shape = [] {
     const char* e = std::getenv("RQ_SMOOTH2_RESRES_SHAPE");
     return e ? std::atoi(e) : 0;
@@ -2412,7 +2495,10 @@ void launch_smooth2_resres(hipStream_t st, int Sf, const float* x, float* xn, co
     case 1: smooth2_resres<16, 8, 512>(st, Sf, x, xn, src, a, c, alpha, beta, gc, b); break;
     case 2: smooth2_resres<16, 4, 256>(st, Sf, x, xn, src, a, c, alpha, beta, gc, b); break;
     case 3: smooth2_resres<32, 8, 1024>(st, Sf, x, xn, src, a, c, alpha, beta, gc, b); break;
-    default: smooth2_resres<32, 4, 512>(st, Sf, x, xn, src, a, c, alpha, beta, gc, b); break;
+    case 4: smooth2_resres<32, 4, 512>(st, Sf, x, xn, src, a, c, alpha, beta, gc, b); break;
+    case 5: smooth2_resres<32, 8, 512>(st, Sf, x, xn, src, a, c, alpha, beta, gc, b); break;
+    case 6: smooth2_resres<16, 16, 768>(st, Sf, x, xn, src, a, c, alpha, beta, gc, b); break;
+    default: smooth2_resres<32, 8, 768>(st, Sf, x, xn, src, a, c, alpha, beta, gc, b); break;
   }
 }
 

@@ -12,6 +12,25 @@
 
 namespace rq {
 
+// Access through a wave-uniform base pointer plus a per-lane 32-bit byte
+// offset: the base stays in scalar registers (plane and neighbour shifts are
+// scalar additions) and the access takes the offset as is, with no 64-bit
+// address arithmetic per lane.  The empty asm keeps the offset in the
+// access's own block: hoisted out of a loop as a 64-bit zero extension, it
+// would be added per lane again.
+__device__ __forceinline__ float ld_at(const float* __restrict__ base, uint32_t off) {
+  asm volatile("" : "+v"(off));
+  return *reinterpret_cast<const float*>(reinterpret_cast<const char*>(base) + off);
+}
+__device__ __forceinline__ uint16_t ld_at16(const uint16_t* __restrict__ base, uint32_t off) {
+  asm volatile("" : "+v"(off));
+  return *reinterpret_cast<const uint16_t*>(reinterpret_cast<const char*>(base) + off);
+}
+__device__ __forceinline__ void st_at(float* __restrict__ base, uint32_t off, float v) {
+  asm volatile("" : "+v"(off));
+  *reinterpret_cast<float*>(reinterpret_cast<char*>(base) + off) = v;
+}
+
 // label-Neumann ITP update (cloud/solver.py:61-90; rc rule via label bit 15)
 __device__ __forceinline__ void itp_col(int S, int k, int j, int i0, int i1, const float* __restrict__ U,
                                         float* __restrict__ Un, const float* __restrict__ K,

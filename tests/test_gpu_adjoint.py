@@ -118,7 +118,7 @@ def test_autograd_function_backward():
     E = trajectory_energy(pos, fz)
     E.backward()
     E2, g = fz.energy_and_gradient(rows[:, :3] * fz.h)
-    assert float(E) == pytest.approx(E2, rel=1e-12)
+    assert float(E.detach()) == pytest.approx(E2, rel=1e-12)
     assert np.allclose(pos.grad.numpy(), g, rtol=1e-6, atol=1e-9)
     fz.close()
 
