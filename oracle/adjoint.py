@@ -53,7 +53,7 @@ def _restrict4(r):
     """4 * full weighting of a zero-face fine field onto coarse interior nodes."""
     S = r.shape[0]
     NC = (S - 1) // 2
-    w1 = torch.tensor([1.0, 2.0, 1.0], dtype=r.dtype)
+    w1 = torch.tensor([1.0, 2.0, 1.0], dtype=r.dtype, device=r.device)
     w = (w1[:, None, None] * w1[None, :, None] * w1[None, None, :]) / 64.0
     # window J of the stride-2 convolution of r[1:, 1:, 1:] is centred on fine node 2(J+1),
     # i.e. coarse interior node I = J + 1
@@ -78,7 +78,7 @@ def _prolong(ec):
 
 def _exact_solve(gin):
     m = gin.shape[0]
-    p = torch.arange(1, m + 1, dtype=gin.dtype)
+    p = torch.arange(1, m + 1, dtype=gin.dtype, device=gin.device)
     Smat = math.sqrt(2.0 / (m + 1)) * torch.sin(math.pi * p[:, None] * p[None, :] / (m + 1))
     lam = 2.0 - 2.0 * torch.cos(math.pi * p / (m + 1))
     den = lam[:, None, None] + lam[None, :, None] + lam[None, None, :]
@@ -100,7 +100,7 @@ def _vcycle(x, g, levels, nu, a, c):
     r = _pad(g - (6.0 * x[1:-1, 1:-1, 1:-1] - _sum6(x)))
     gc = _restrict4(r)
     Sc = gc.shape[0] + 2
-    ec = _vcycle(torch.zeros((Sc,) * 3, dtype=x.dtype), gc, levels - 1, nu, a, c)
+    ec = _vcycle(torch.zeros((Sc,) * 3, dtype=x.dtype, device=x.device), gc, levels - 1, nu, a, c)
     x = x + _pad(_prolong(ec)[1:-1, 1:-1, 1:-1])
     for _ in range(nu):
         x = _smooth(x, g, a, c)
@@ -109,28 +109,31 @@ def _vcycle(x, g, levels, nu, a, c):
 
 def trajectory_energy(R, rows, label, NN, screen, n_steps, bootstrap=0, poisson_interval=2, norm_interval=20,
                       dv=None, mode="jacobi", mg_levels=3, mg_nu=2, mg_a=1.0 / 7.0, mg_c=7.0, start=None,
-                      state_out=None):
+                      state_out=None, device=None):
     """E(R) for positions R (torch, (n,3), grid units) with Z, rc from ``rows``
     and frozen ``label`` (S^3 int array).  Returns (E, T, VeK, Vee) tensors.
 
     ``start`` = (U0, P0) (S^3 arrays): the map starts from these constants
     instead of the normalised trial and the bootstrap Poisson steps (the
     library's RQ_ADJ_START_SAVED); only K(R) then depends on R.
-    ``state_out`` (a dict) receives the final U and P as numpy arrays."""
+    ``state_out`` (a dict) receives the final U and P as numpy arrays.
+    ``device``: where the tensors live (default: the CPU; the same FP64
+    arithmetic on a GPU makes full-size grids affordable)."""
     dt = torch.float64
+    dev = torch.device("cpu") if device is None else torch.device(device)
     S = NN + 1
     h = screen / NN
     h2, h3 = h * h, h ** 3
-    Z = torch.tensor(np.asarray(rows)[:, 3], dtype=dt)
-    rc = torch.tensor(np.asarray(rows)[:, 4], dtype=dt)
+    Z = torch.tensor(np.asarray(rows)[:, 3], dtype=dt, device=dev)
+    rc = torch.tensor(np.asarray(rows)[:, 4], dtype=dt, device=dev)
     n_elec = int((np.asarray(rows)[:, 3] > 0).sum())
     if dv is None:
         dv = 0.12 if n_elec <= 30 else 0.03
     hd, dtv = 0.5 * dv, dv * h2
-    lab = torch.tensor(np.asarray(label).reshape(S, S, S)[1:-1, 1:-1, 1:-1].astype(np.int64))
-    x = torch.arange(1, S - 1, dtype=dt) * h
-    Kin = torch.zeros((S - 2,) * 3, dtype=dt)
-    U0in = torch.zeros((S - 2,) * 3, dtype=dt)
+    lab = torch.tensor(np.asarray(label).reshape(S, S, S)[1:-1, 1:-1, 1:-1].astype(np.int64), device=dev)
+    x = torch.arange(1, S - 1, dtype=dt, device=dev) * h
+    Kin = torch.zeros((S - 2,) * 3, dtype=dt, device=dev)
+    U0in = torch.zeros((S - 2,) * 3, dtype=dt, device=dev)
     for n in range(len(Z)):
         if not Z[n] > 0:
             continue
@@ -150,14 +153,14 @@ def trajectory_energy(R, rows, label, NN, screen, n_steps, bootstrap=0, poisson_
 
     def normalize(U):
         ui = U[1:-1, 1:-1, 1:-1]
-        norms = torch.zeros(nd, dtype=dt).index_add(0, lab.reshape(-1), (ui * ui * h3).reshape(-1))
+        norms = torch.zeros(nd, dtype=dt, device=dev).index_add(0, lab.reshape(-1), (ui * ui * h3).reshape(-1))
         ok = (Z > 0) & (norms > 1e-20)
         sc = torch.where(ok, torch.sqrt(torch.where(ok, Z / torch.where(ok, norms, torch.ones_like(norms)),
                                                      torch.ones_like(norms))), torch.ones_like(norms))
         return _pad(ui * sc[lab])
 
     def nbr_same(axis, sign):
-        L = torch.tensor(np.asarray(label).reshape(S, S, S).astype(np.int64))
+        L = torch.tensor(np.asarray(label).reshape(S, S, S).astype(np.int64), device=dev)
         sl = [slice(1, -1)] * 3
         sl[axis] = slice(2, None) if sign > 0 else slice(0, -2)
         return L[tuple(sl)] == lab
@@ -188,12 +191,12 @@ def trajectory_energy(R, rows, label, NN, screen, n_steps, bootstrap=0, poisson_
 
     if start is None:
         U = normalize(_pad(U0in))
-        P = torch.zeros((S,) * 3, dtype=dt)
+        P = torch.zeros((S,) * 3, dtype=dt, device=dev)
         for _ in range(bootstrap):
             P = poisson(P, U)
     else:
-        U = torch.tensor(np.asarray(start[0], np.float64).reshape(S, S, S))
-        P = torch.tensor(np.asarray(start[1], np.float64).reshape(S, S, S))
+        U = torch.tensor(np.asarray(start[0], np.float64).reshape(S, S, S), device=dev)
+        P = torch.tensor(np.asarray(start[1], np.float64).reshape(S, S, S), device=dev)
     for step in range(1, n_steps + 1):
         if step % poisson_interval == 0:
             P = poisson(P, U)
@@ -208,18 +211,18 @@ def trajectory_energy(R, rows, label, NN, screen, n_steps, bootstrap=0, poisson_
     VeK = -(K[1:-1, 1:-1, 1:-1] * u * u).sum() * h3
     Vee = (P[1:-1, 1:-1, 1:-1] * u * u).sum() * h3
     if state_out is not None:
-        state_out["U"] = U.detach().numpy().copy()
-        state_out["P"] = P.detach().numpy().copy()
+        state_out["U"] = U.detach().cpu().numpy().copy()
+        state_out["P"] = P.detach().cpu().numpy().copy()
     return T + VeK + Vee, T, VeK, Vee
 
 
 def energy_and_gradient(rows, label, NN, screen, n_steps, **kw):
     """(E, dE/dR per bohr as an (n,3) array) by autograd; R from ``rows``."""
-    R = torch.tensor(np.asarray(rows)[:, :3], dtype=torch.float64, requires_grad=True)
+    R = torch.tensor(np.asarray(rows)[:, :3], dtype=torch.float64, device=kw.get("device"), requires_grad=True)
     E, *_ = trajectory_energy(R, rows, label, NN, screen, n_steps, **kw)
     E.backward()
     h = screen / NN
-    return float(E.detach()), R.grad.numpy() / h
+    return float(E.detach()), R.grad.cpu().numpy() / h
 
 
 def frozen_labels(rows, NN, screen):

@@ -764,56 +764,84 @@ __global__ __launch_bounds__(TK * TJ) void k_post_march(int Sf, int CI, const fl
   const int jo = J0 + tj, ko = K0 + tk;
   const bool own = jo <= n && ko <= n;
   const uint32_t own4 = own ? 4u * static_cast<uint32_t>(jo * Sf + ko) : 0u;
-  // coarse-plane interpolants T(ic) of the prolongation, shared by the two
-  // fine planes 2 ic and 2 ic +- 1 that use them: ta = T(ca), tb = T(cb).
-  // The plane index is uniform, so the cache state is too.
-  float ta[P3], tb[P3];
-  int ca = -1, cb = -1;
-  auto tplane = [&](int ic, int q) -> float {
+  // Coarse-plane interpolants of the prolongation: fine plane s uses
+  // T(s >> 1) and T((s + 1) >> 1) (tA, tB; the same plane for even s), so
+  // each odd plane after the first brings in one new coarse plane.
+  auto tplane_raw = [&](int ic, int q, float* r) {
     const float* ep = e + ic * Sc2;
-    const float l0 = 0.5f * (ld_at(ep, e00[q]) + ld_at(ep, e01[q]));
-    const float l1 = 0.5f * (ld_at(ep, e10[q]) + ld_at(ep, e11[q]));
+    r[0] = ld_at(ep, e00[q]);
+    r[1] = ld_at(ep, e01[q]);
+    r[2] = ld_at(ep, e10[q]);
+    r[3] = ld_at(ep, e11[q]);
+  };
+  auto tcombine = [](const float* r) -> float {
+    const float l0 = 0.5f * (r[0] + r[1]);
+    const float l1 = 0.5f * (r[2] + r[3]);
     return 0.5f * (l0 + l1);
   };
-  for (int s = I0 - 2; s <= I1 + 1; ++s) {
-    // ---- stage 0: corrected x, U and label at plane s ----
+  // Every global input is fetched one iteration ahead, so its latency is
+  // spent behind the current plane's stages and barriers: x, U and the label
+  // of plane s + 1, the raw coarse values of the coarse plane that plane s + 1
+  // brings in, and K at the owned column of the plane stage 2 handles next.
+  float xq[P3] = {}, uq[P3] = {}, tA[P3] = {}, tB[P3] = {}, pend[P3][4] = {};
+  uint16_t lq[P3] = {};
+  float kq = 0.0f;
+  const int s0 = max(I0 - 2, 1);  // first plane of the march that takes the correction
+  if (s0 <= n) {
+#pragma unroll
+    for (int q = 0; q < P3; ++q) {
+      float r[4];
+      tplane_raw(s0 >> 1, q, r);
+      tA[q] = tcombine(r);
+      tplane_raw((s0 + 1) >> 1, q, r);
+      tB[q] = tcombine(r);
+    }
+  }
+  auto fetch = [&](int s) {
     if (s >= 0 && s <= Sf - 1) {
-      const int sx = s % 3, su = s % RU;
-      const bool pin = s >= 1 && s <= n;
-      float xv[P3], uv[P3];
-      uint16_t lv[P3];
       const float* xs = x + s * S2;
       const float* us = U + s * S2;
       const uint16_t* ls = lab + s * S2;
 #pragma unroll
       for (int q = 0; q < P3; ++q) {
-        xv[q] = ld_at(xs, col4[q]);
-        uv[q] = ld_at(us, col4[q]);
-        lv[q] = ld_at16(ls, col2[q]);
+        xq[q] = ld_at(xs, col4[q]);
+        uq[q] = ld_at(us, col4[q]);
+        lq[q] = ld_at16(ls, col2[q]);
       }
-      if (pin) {
-        // prolong_at(s) = 0.5 (T(s >> 1) + T((s >> 1) + (s & 1)))
-        const int ic = s >> 1;
-        if (ic != ca) {
-          if (ic == cb) {
+      if ((s & 1) && s > s0 && s <= n) {
 #pragma unroll
-            for (int q = 0; q < P3; ++q) ta[q] = tb[q];
-          } else {
+        for (int q = 0; q < P3; ++q) tplane_raw((s + 1) >> 1, q, pend[q]);
+      }
+    }
+  };
+  fetch(I0 - 2);
+  for (int s = I0 - 2; s <= I1 + 1; ++s) {
+    float xv[P3], uv[P3];
+    uint16_t lv[P3];
 #pragma unroll
-            for (int q = 0; q < P3; ++q) ta[q] = tplane(ic, q);
-          }
-          ca = ic;
-        }
-        if ((s & 1) && cb != ic + 1) {
+    for (int q = 0; q < P3; ++q) {
+      xv[q] = xq[q];
+      uv[q] = uq[q];
+      lv[q] = lq[q];
+    }
+    const float kv = kq;
+    // ---- coarse interpolants of plane s (before pend is refilled) ----
+    if (s > s0 && s <= n) {
 #pragma unroll
-          for (int q = 0; q < P3; ++q) tb[q] = tplane(ic + 1, q);
-          cb = ic + 1;
-        }
+      for (int q = 0; q < P3; ++q) {
+        tA[q] = tB[q];
+        if (s & 1) tB[q] = tcombine(pend[q]);
+      }
+    }
+    if (s + 1 <= I1 + 1) fetch(s + 1);
+    if (own && s - 1 >= I0 && s - 1 < I1) kq = ld_at(K + (s - 1) * S2, own4);
+    // ---- stage 0: corrected x, U and label at plane s ----
+    if (s >= 0 && s <= Sf - 1) {
+      const int sx = s % 3, su = s % RU;
+      if (s >= 1 && s <= n) {
 #pragma unroll
-        for (int q = 0; q < P3; ++q) {
-          const float t1 = (s & 1) ? tb[q] : ta[q];
-          if (cin[q]) xv[q] = xv[q] + 0.5f * (ta[q] + t1);
-        }
+        for (int q = 0; q < P3; ++q)
+          if (cin[q]) xv[q] = xv[q] + 0.5f * (tA[q] + tB[q]);
       }
 #pragma unroll
       for (int q = 0; q < P3; ++q) {
@@ -870,14 +898,17 @@ __global__ __launch_bounds__(TK * TJ) void k_post_march(int Sf, int CI, const fl
       if (lc & kRcFlag) {
         out = 0.0f;
       } else {
-        const float t0 = (rl[up_] == lc) ? ru[up_] : uc;
-        const float t1 = (rl[um_] == lc) ? ru[um_] : uc;
-        const float t2 = (rl[u0 + W3] == lc) ? ru[u0 + W3] : uc;
-        const float t3 = (rl[u0 - W3] == lc) ? ru[u0 - W3] : uc;
-        const float t4 = (rl[u0 + 1] == lc) ? ru[u0 + 1] : uc;
-        const float t5 = (rl[u0 - 1] == lc) ? ru[u0 - 1] : uc;
+        // all six neighbours read unconditionally, then selected (a read
+        // under the label test would be a branch per neighbour)
+        const float n0 = ru[up_], n1 = ru[um_], n2 = ru[u0 + W3], n3 = ru[u0 - W3], n4 = ru[u0 + 1], n5 = ru[u0 - 1];
+        const float t0 = (rl[up_] == lc) ? n0 : uc;
+        const float t1 = (rl[um_] == lc) ? n1 : uc;
+        const float t2 = (rl[u0 + W3] == lc) ? n2 : uc;
+        const float t3 = (rl[u0 - W3] == lc) ? n3 : uc;
+        const float t4 = (rl[u0 + 1] == lc) ? n4 : uc;
+        const float t5 = (rl[u0 - 1] == lc) ? n5 : uc;
         const float lap = t0 + t1 + t2 + t3 + t4 + t5 - 6.0f * uc;
-        out = uc + half_dv * lap + dtv * (ld_at(K + f2 * S2, own4) - 2.0f * v) * uc;
+        out = uc + half_dv * lap + dtv * (kv - 2.0f * v) * uc;
       }
       st_at(Un + f2 * S2, own4, out);
     }
