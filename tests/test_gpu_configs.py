@@ -3,7 +3,9 @@
 C1 (one H2O, 96^3, h = 0.1, 4-level V-cycle, seeded Gaussian start) and C2
 (10 H2O, 192^3, 5 levels) run the cloud/solver.py schedule with the
 multigrid Poisson step through the C ABI and are compared with the FP32
-oracle (oracle/s1.py passes + oracle/mg.py vcycle) on the same inputs:
+oracle (oracle/s1.py passes + oracle/mg.py vcycle) on the same inputs (C1
+also over its full 2000 steps with a V-cycle in every step, against the C
+restatement of that schedule):
 density relative L2 <= 1e-5, total and per-domain electronic energies
 within 1e-6 relative, identical label maps; the error against the FP64
 oracle is printed per result.  C3 (the 64-conformer batch) is checked
@@ -91,6 +93,43 @@ def test_c1_gaussian_start_multigrid_matches_oracle():
     s.step(0, 200)
     _compare("C1", s, rows, NN, screen, levels, 200, 10, U0=U0)
     s.close()
+
+
+def test_c1_2000_steps_vcycle_per_step_matches_c_oracle():
+    """C1 as the workload states it: the seeded Gaussian start, 10 bootstrap
+    V-cycles, then 2000 steps with a 4-level V(2,2) in every step, normalised
+    every 20th (score rule off, so the labels are the trial's), against the
+    C restatement of the same schedule (oracle/cstep.c, pinned against the
+    NumPy oracle) started from the GPU's initialised fields: density L2
+    within 1e-5 and the electronic energy (the NumPy oracle's energy pass
+    over each side's final fields) within 1e-6 relative."""
+    from oracle import cstep
+    cfg = B.c1_h2o()
+    rows, NN, screen, levels = cfg["rows"], cfg["NN"], cfg["screen"], cfg["mg_levels"]
+    S = NN + 1
+    print(f"C1 2000 steps NN={NN} levels={levels}", flush=True)
+    s = make_solver(rows, NN, screen, poisson="multigrid", mg_levels=levels, bootstrap_poisson=10)
+    s._chk(s._lib.rq_init_trial(s._h))
+    U0 = B.gaussian_init(rows, NN, screen, s.label.get(), seed=0)
+    s.initialize(U0=U0)
+    U = cube(s.U.get(), S).copy()
+    P = cube(s.P.get(), S).copy()
+    K = cube(s.K_buf.get(), S).copy()
+    lab = cube(s.label.get(), S).astype(np.int32)
+    s.step(0, 2000, poisson_interval=1, boundary_interval=10 ** 9)
+    Ug = cube(s.U.get(), S).astype(np.float64)
+    s._compute_energy()
+    Eg = s.E_T + s.E_eK + s.E_ee
+    s.close()
+    g = s1.Grid(NN, screen, s.dv, np.float32)
+    cstep.run_steps(U, P, K, lab, rows[:, 3], g, levels, 0, 2000, poisson_interval=1)
+    tot, _ = s1.energy(U, K, P, lab, g, n_domains=len(rows))
+    Ec = tot.sum()
+    rho_err = rel_l2(Ug ** 2, U.astype(np.float64) ** 2)
+    print(f"C1 2000 steps: rho L2 vs C oracle {rho_err:.2e}; E_el gpu {Eg:.9f} C {Ec:.9f} "
+          f"rel {abs(Eg - Ec) / abs(Ec):.2e}", flush=True)
+    assert rho_err < 1e-5
+    assert abs(Eg - Ec) <= 1e-6 * abs(Ec)
 
 
 def test_c2_multigrid_matches_oracle():
