@@ -3,17 +3,18 @@
 C1 (one H2O, 96^3, h = 0.1, 4-level V-cycle, seeded Gaussian start) and C2
 (10 H2O, 192^3, 5 levels) run the cloud/solver.py schedule with the
 multigrid Poisson step through the C ABI and are compared with the FP32
-oracle (oracle/s1.py passes + oracle/mg.py vcycle) on the same inputs (C1
-also over its full 2000 steps with a V-cycle in every step, against the C
-restatement of that schedule):
+oracle (oracle/s1.py passes + oracle/mg.py vcycle) on the same inputs:
 density relative L2 <= 1e-5, total and per-domain electronic energies
 within 1e-6 relative, identical label maps; the error against the FP64
-oracle is printed per result.  C3 (the 64-conformer batch) is checked
-molecule by molecule against single-grid solves, and C4 / C5 (256^3,
-512^3) at full size through size-independent properties: per-domain norms
-equal Z after a normalisation, finite energies of the right signs, valid
-labels, and (C4) the GPU energy against the oracle's energy pass over the
-GPU's own fields.
+oracle is printed per result.  C1 over its full 2000 steps, and C2, C4
+(256^3) and C5 (512^3) at full size, run with a V-cycle in every step
+against the C restatement of that schedule (oracle/cstep.c, pinned against
+the NumPy oracle) from the GPU's initialised fields, to the same bars.  C3
+(the 64-conformer batch) is checked molecule by molecule against
+single-grid solves, and C4 / C5 also through size-independent properties:
+per-domain norms equal Z after a normalisation, finite energies of the
+right signs, valid labels, and (C4) the GPU energy against the oracle's
+energy pass over the GPU's own fields.
 """
 import numpy as np
 import pytest
@@ -95,6 +96,42 @@ def test_c1_gaussian_start_multigrid_matches_oracle():
     s.close()
 
 
+def _c_oracle_run(name, cfg, n_steps, bootstrap, U0_seed=None, graph=False):
+    """The workload on the GPU (V-cycle every step, normalise every 20th,
+    score rule off) and the C restatement of the same schedule from the
+    GPU's initialised fields; returns (density rel L2, GPU electronic
+    energy, C oracle energy, max relative per-domain energy difference)."""
+    from oracle import cstep
+    rows, NN, screen, levels = cfg["rows"], cfg["NN"], cfg["screen"], cfg["mg_levels"]
+    S = NN + 1
+    print(f"{name}: NN={NN} levels={levels} atoms={len(rows)} steps={n_steps}", flush=True)
+    s = make_solver(rows, NN, screen, poisson="multigrid", mg_levels=levels, bootstrap_poisson=bootstrap, graph=graph)
+    if U0_seed is not None:
+        s._chk(s._lib.rq_init_trial(s._h))
+        s.initialize(U0=B.gaussian_init(rows, NN, screen, s.label.get(), seed=U0_seed))
+    else:
+        s.initialize()
+    U = cube(s.U.get(), S).copy()
+    P = cube(s.P.get(), S).copy()
+    K = cube(s.K_buf.get(), S).copy()
+    lab = cube(s.label.get(), S).astype(np.int32)
+    s.step(0, n_steps, poisson_interval=1, boundary_interval=10 ** 9)
+    Ug = cube(s.U.get(), S).astype(np.float64)
+    s._compute_energy()
+    Eg = s.E_T + s.E_eK + s.E_ee
+    per_g = s.E_domains.copy()
+    g = s1.Grid(NN, screen, s.dv, np.float32)
+    s.close()
+    cstep.run_steps(U, P, K, lab, rows[:, 3], g, levels, 0, n_steps, poisson_interval=1)
+    tot, per = s1.energy(U, K, P, lab, g, n_domains=len(rows))
+    Ec = tot.sum()
+    rho_err = rel_l2(Ug ** 2, U.astype(np.float64) ** 2)
+    per_err = np.abs(per_g - per).max() / np.abs(per).max()
+    print(f"{name}: rho L2 vs C oracle {rho_err:.2e}; E_el gpu {Eg:.9f} C {Ec:.9f} rel {abs(Eg - Ec) / abs(Ec):.2e}; "
+          f"per-domain max rel {per_err:.2e}", flush=True)
+    return rho_err, Eg, Ec, per_err
+
+
 def test_c1_2000_steps_vcycle_per_step_matches_c_oracle():
     """C1 as the workload states it: the seeded Gaussian start, 10 bootstrap
     V-cycles, then 2000 steps with a 4-level V(2,2) in every step, normalised
@@ -103,33 +140,23 @@ def test_c1_2000_steps_vcycle_per_step_matches_c_oracle():
     NumPy oracle) started from the GPU's initialised fields: density L2
     within 1e-5 and the electronic energy (the NumPy oracle's energy pass
     over each side's final fields) within 1e-6 relative."""
-    from oracle import cstep
-    cfg = B.c1_h2o()
-    rows, NN, screen, levels = cfg["rows"], cfg["NN"], cfg["screen"], cfg["mg_levels"]
-    S = NN + 1
-    print(f"C1 2000 steps NN={NN} levels={levels}", flush=True)
-    s = make_solver(rows, NN, screen, poisson="multigrid", mg_levels=levels, bootstrap_poisson=10)
-    s._chk(s._lib.rq_init_trial(s._h))
-    U0 = B.gaussian_init(rows, NN, screen, s.label.get(), seed=0)
-    s.initialize(U0=U0)
-    U = cube(s.U.get(), S).copy()
-    P = cube(s.P.get(), S).copy()
-    K = cube(s.K_buf.get(), S).copy()
-    lab = cube(s.label.get(), S).astype(np.int32)
-    s.step(0, 2000, poisson_interval=1, boundary_interval=10 ** 9)
-    Ug = cube(s.U.get(), S).astype(np.float64)
-    s._compute_energy()
-    Eg = s.E_T + s.E_eK + s.E_ee
-    s.close()
-    g = s1.Grid(NN, screen, s.dv, np.float32)
-    cstep.run_steps(U, P, K, lab, rows[:, 3], g, levels, 0, 2000, poisson_interval=1)
-    tot, _ = s1.energy(U, K, P, lab, g, n_domains=len(rows))
-    Ec = tot.sum()
-    rho_err = rel_l2(Ug ** 2, U.astype(np.float64) ** 2)
-    print(f"C1 2000 steps: rho L2 vs C oracle {rho_err:.2e}; E_el gpu {Eg:.9f} C {Ec:.9f} "
-          f"rel {abs(Eg - Ec) / abs(Ec):.2e}", flush=True)
+    rho_err, Eg, Ec, per_err = _c_oracle_run("C1 2000 steps", B.c1_h2o(), 2000, 10, U0_seed=0)
     assert rho_err < 1e-5
     assert abs(Eg - Ec) <= 1e-6 * abs(Ec)
+    assert per_err <= 1e-6
+
+
+@pytest.mark.parametrize("wl,n_steps", [("c2", 500), ("c4", 100), ("c5", 20)])
+def test_full_size_workloads_match_c_oracle(wl, n_steps):
+    """C2 (192^3, 500 steps), C4 (256^3, 51 atoms, 100 steps) and C5
+    (512^3, 3000 domains, 20 steps) at full size against the C restatement
+    of the schedule from the GPU's initialised fields, graph replay on:
+    the same bars as C1."""
+    cfg = {"c2": B.c2_waters, "c4": B.c4_peptide, "c5": B.c5_waters}[wl]()
+    rho_err, Eg, Ec, per_err = _c_oracle_run(wl.upper(), cfg, n_steps, 5, graph=True)
+    assert rho_err < 1e-5
+    assert abs(Eg - Ec) <= 1e-6 * abs(Ec)
+    assert per_err <= 1e-6
 
 
 def test_c2_multigrid_matches_oracle():
