@@ -493,13 +493,22 @@ __global__ __launch_bounds__(NT) void k_smooth2_resres_march(int Sf, int CI, con
                                                               float* __restrict__ gc) {
   constexpr int RING = 3;
   constexpr int W0 = 2 * CK + 1, H0 = 2 * CJ + 1, W1 = W0 + 2, H1 = H0 + 2, W2 = W0 + 4, H2 = H0 + 4;
-  constexpr int PL0 = W0 * H0, PL1 = W1 * H1, PL2 = W2 * H2;
-  constexpr int P0 = (PL0 + NT - 1) / NT, P1 = (PL1 + NT - 1) / NT, P2 = (PL2 + NT - 1) / NT;
+  // LDS rows: the first-sweep, U and second-sweep rings share the row pitch
+  // W2, and every stage maps its threads onto rows of W2 slots (the last
+  // ones idle on the narrower regions), so a wave's reads and writes are
+  // consecutive words even across row ends.  The residual ring stores each
+  // row's even-k entries, then from R0O (= 16 mod 32) its odd-k ones: the
+  // restriction's stride-2 reads of a row become three unit-stride reads
+  // and the residual's stores fall on disjoint banks.
+  constexpr int E0 = CK + 1, O0 = CK;  // even / odd entries of an R0 row
+  constexpr int R0O = 16 + 32 * ((E0 > 16 ? E0 - 16 + 31 : 0) / 32), R0P = R0O + O0;
+  constexpr int PL0 = R0P * H0, PL1 = W2 * H1, PL2 = W2 * H2, Q0 = W2 * H0;
+  constexpr int P0 = (Q0 + NT - 1) / NT, P1 = (PL1 + NT - 1) / NT, P2 = (PL2 + NT - 1) / NT;
   static_assert(CK * CJ <= NT, "one thread per coarse column");
   __shared__ float ring2[RING * PL2];  // first sweep
   __shared__ float ringu[RING * PL2];  // U on the first-sweep region
   __shared__ float ring1[RING * PL1];  // second sweep
-  __shared__ float ring0[RING * PL0];  // residual
+  __shared__ float ring0[RING * PL0];  // residual (rows split even / odd k)
   const int n = Sf - 2, nc = (Sf - 1) / 2 - 1, Sc = nc + 2, S2 = Sf * Sf;
   const int gx = (nc + CK - 1) / CK, gy = (nc + CJ - 1) / CJ;
   const int t = xcd_block();
@@ -536,10 +545,10 @@ __global__ __launch_bounds__(NT) void k_smooth2_resres_march(int Sf, int CI, con
 #pragma unroll
   for (int q = 0; q < P1; ++q) {
     const int o = min(tid + q * NT, PL1 - 1);
-    const int jj = o / W1, kk = o % W1;
+    const int jj = o / W2, kk = o % W2;
     const int j = 2 * J0 - 2 + jj, k = 2 * K0 - 2 + kk;
-    c1[q] = (jj + 1) * W2 + (kk + 1);
-    ok1[q] = tid + q * NT < PL1;
+    c1[q] = o + W2 + 1;  // (jj + 1) W2 + kk + 1
+    ok1[q] = tid + q * NT < PL1 && kk < W1;
     in1[q] = j >= 1 && j <= n && k >= 1 && k <= n;
     st1[q] = (in1[q] && j >= j_lo && j <= j_hi && k >= k_lo && k <= k_hi) ? 4u * static_cast<uint32_t>(j * Sf + k)
                                                                           : ~0u;
@@ -660,15 +669,15 @@ __global__ __launch_bounds__(NT) void k_smooth2_resres_march(int Sf, int CI, con
 #pragma unroll
       for (int q = 0; q < P0; ++q) {
         const int o = tid + q * NT;
-        if (o < PL0) {
-          const int jj = o / W0, kk = o % W0;
+        const int jj = o / W2, kk = o % W2;
+        if (o < Q0 && kk < W0) {
           const int j = 2 * J0 - 1 + jj, k = 2 * K0 - 1 + kk;
-          const int cc = (jj + 1) * W1 + (kk + 1);
-          const float s6 = qa[cc] + qb[cc] + q0[cc + W1] + q0[cc - W1] + q0[cc + 1] + q0[cc - 1];
-          const float uu = uq[(jj + 2) * W2 + (kk + 2)];
+          const int cc = o + W2 + 1;  // (jj + 1) W2 + kk + 1
+          const float s6 = qa[cc] + qb[cc] + q0[cc + W2] + q0[cc - W2] + q0[cc + 1] + q0[cc - 1];
+          const float uu = uq[cc + W2 + 1];  // (jj + 2) W2 + kk + 2
           const float rhs = (alpha * (uu * uu)) * beta;
           const float r = rhs - (6.0f * q0[cc] - s6);
-          ring0[s_m2 * PL0 + o] = (j <= n && k <= n) ? r : 0.0f;
+          ring0[s_m2 * PL0 + jj * R0P + ((kk & 1) ? R0O + (kk >> 1) : (kk >> 1))] = (j <= n && k <= n) ? r : 0.0f;
         }
       }
     }
@@ -678,10 +687,13 @@ __global__ __launch_bounds__(NT) void k_smooth2_resres_march(int Sf, int CI, con
       const int I = (f0 - 1) / 2, tk = tid % CK, tj = tid / CK;
       const int K = K0 + tk, J = J0 + tj;
       if (K <= nc && J <= nc) {
-        const int cc = (2 * tj + 1) * W0 + (2 * tk + 1);
+        // the centre (row 2 tj + 1, k entry 2 tk + 1: odd entry tk); its k-1 /
+        // k+1 neighbours are the even entries tk and tk + 1
+        const int cc = (2 * tj + 1) * R0P + R0O + tk;
         // planes f0 - 2 = p - 4, f0 - 1 = p - 3 and f0 = p - 2 sit in the
         // slots of p - 1, p and p - 2 (three-slot rings)
-        gc[I * Sc * Sc + J * Sc + K] = restrict27_planes(ring0, s_m1 * PL0 + cc, s_p * PL0 + cc, s_m2 * PL0 + cc, W0, 1);
+        gc[I * Sc * Sc + J * Sc + K] =
+            restrict27_planes_k(ring0, s_m1 * PL0 + cc, s_p * PL0 + cc, s_m2 * PL0 + cc, R0P, 1 - R0O, -R0O);
       }
     }
     s_p = (s_p + 1) % RING;
@@ -712,7 +724,10 @@ __global__ __launch_bounds__(TK * TJ) void k_post_march(int Sf, int CI, const fl
                                                         float dtv) {
   constexpr int NT = TK * TJ;
   constexpr int W3 = TK + 4, H3 = TJ + 4, W2 = TK + 2, H2 = TJ + 2;
-  constexpr int PL3 = W3 * H3, PL2 = W2 * H2;
+  // every ring has the row pitch W3 and stage 1 maps its threads onto rows
+  // of W3 slots (two idle per row), so a wave's LDS accesses are consecutive
+  // words across row ends
+  constexpr int PL3 = W3 * H3, PL2 = W3 * H2;
   constexpr int P3 = (PL3 + NT - 1) / NT, P2 = (PL2 + NT - 1) / NT;
   __shared__ float rx[3 * PL3];       // corrected x
   static_assert(RU == 4 || RU == 5, "U / label ring of four (barrier after stage 2) or five planes");
@@ -748,15 +763,16 @@ __global__ __launch_bounds__(TK * TJ) void k_post_march(int Sf, int CI, const fl
     e10[q] = 4u * (b + dj);
     e11[q] = 4u * (b + dj + dk);
   }
-  // first-sweep nodes on R2: LDS index in an R3 slot, interior flag
+  // first-sweep nodes on R2: LDS index in an R3 slot, slot in use, interior flag
   int c2[P2];
-  bool in2[P2];
+  bool ok2[P2], in2[P2];
 #pragma unroll
   for (int q = 0; q < P2; ++q) {
     const int o = min(tid + q * NT, PL2 - 1);
-    const int jj = o / W2, kk = o % W2;
+    const int jj = o / W3, kk = o % W3;
     const int j = J0 - 1 + jj, k = K0 - 1 + kk;
-    c2[q] = (jj + 1) * W3 + (kk + 1);
+    c2[q] = o + W3 + 1;  // (jj + 1) W3 + kk + 1
+    ok2[q] = tid + q * NT < PL2 && kk < W2;
     in2[q] = j >= 1 && j <= n && k >= 1 && k <= n;
   }
   // stage-2 node of this thread (one per thread)
@@ -863,7 +879,7 @@ __global__ __launch_bounds__(TK * TJ) void k_post_march(int Sf, int CI, const fl
       const float* u0 = ru + (f1 % RU) * PL3;
 #pragma unroll
       for (int q = 0; q < P2; ++q) {
-        if (tid + q * NT < PL2) {
+        if (ok2[q]) {
           const int cc = c2[q];
           float v = x0[cc];
           if (pin && in2[q]) {
@@ -883,8 +899,8 @@ __global__ __launch_bounds__(TK * TJ) void k_post_march(int Sf, int CI, const fl
     const int f2 = s - 2;
     if (f2 >= I0 && f2 < I1 && own) {
       const int bp = ((f2 + 1) % 3) * PL2, b0 = (f2 % 3) * PL2, bm = ((f2 + 2) % 3) * PL2;
-      const int cc = (tj + 1) * W2 + (tk + 1);
-      const float s6 = r1[bp + cc] + r1[bm + cc] + r1[b0 + cc + W2] + r1[b0 + cc - W2] + r1[b0 + cc + 1] +
+      const int cc = (tj + 1) * W3 + (tk + 1);
+      const float s6 = r1[bp + cc] + r1[bm + cc] + r1[b0 + cc + W3] + r1[b0 + cc - W3] + r1[b0 + cc + 1] +
                        r1[b0 + cc - 1];
       const int cu = (tj + 2) * W3 + (tk + 2);
       const int u0 = (f2 % RU) * PL3 + cu, up_ = ((f2 + 1) % RU) * PL3 + cu, um_ = ((f2 + RU - 1) % RU) * PL3 + cu;

@@ -322,37 +322,45 @@ __device__ __forceinline__ void residual_col(int S, int k, int j, int i0, int i1
 }
 
 // 4 FW(r) / 4 = (8 centre + 4 faces + 2 edges + corners) / 16 about r[c0],
-// where r[cm] / r[cp] are its i-1 / i+1 neighbours and dj, dk the in-plane
-// strides; faces, edges and corners each summed in a fixed order
-__device__ __forceinline__ float restrict27_planes(const float* r, int cm, int c0, int cp, int dj, int dk) {
+// where r[cm] / r[cp] are its i-1 / i+1 neighbours, dj the in-plane j
+// stride, and kp / km the offsets of the k+1 / k-1 neighbours (dk and -dk
+// in a plain row; other values for a row stored as its even then its odd
+// entries); faces, edges and corners each summed in a fixed order
+__device__ __forceinline__ float restrict27_planes_k(const float* r, int cm, int c0, int cp, int dj, int kp,
+                                                     int km) {
   float fs = r[cp];
   fs = fs + r[cm];
   fs = fs + r[c0 + dj];
   fs = fs + r[c0 - dj];
-  fs = fs + r[c0 + dk];
-  fs = fs + r[c0 - dk];
+  fs = fs + r[c0 + kp];
+  fs = fs + r[c0 + km];
   float es = r[cp + dj];
   es = es + r[cp - dj];
   es = es + r[cm + dj];
   es = es + r[cm - dj];
-  es = es + r[cp + dk];
-  es = es + r[cp - dk];
-  es = es + r[cm + dk];
-  es = es + r[cm - dk];
-  es = es + r[c0 + dj + dk];
-  es = es + r[c0 + dj - dk];
-  es = es + r[c0 - dj + dk];
-  es = es + r[c0 - dj - dk];
-  float cs = r[cp + dj + dk];
-  cs = cs + r[cp + dj - dk];
-  cs = cs + r[cp - dj + dk];
-  cs = cs + r[cp - dj - dk];
-  cs = cs + r[cm + dj + dk];
-  cs = cs + r[cm + dj - dk];
-  cs = cs + r[cm - dj + dk];
-  cs = cs + r[cm - dj - dk];
+  es = es + r[cp + kp];
+  es = es + r[cp + km];
+  es = es + r[cm + kp];
+  es = es + r[cm + km];
+  es = es + r[c0 + dj + kp];
+  es = es + r[c0 + dj + km];
+  es = es + r[c0 - dj + kp];
+  es = es + r[c0 - dj + km];
+  float cs = r[cp + dj + kp];
+  cs = cs + r[cp + dj + km];
+  cs = cs + r[cp - dj + kp];
+  cs = cs + r[cp - dj + km];
+  cs = cs + r[cm + dj + kp];
+  cs = cs + r[cm + dj + km];
+  cs = cs + r[cm - dj + kp];
+  cs = cs + r[cm - dj + km];
   const float val = 8.0f * r[c0] + 4.0f * fs + 2.0f * es + cs;
   return val * 0.0625f;
+}
+
+// the same in a plain row (k stride dk)
+__device__ __forceinline__ float restrict27_planes(const float* r, int cm, int c0, int cp, int dj, int dk) {
+  return restrict27_planes_k(r, cm, c0, cp, dj, dk, -dk);
 }
 
 // the same about r[c] with i-stride di
