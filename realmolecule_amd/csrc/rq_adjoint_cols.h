@@ -21,6 +21,11 @@ __device__ __forceinline__ T adj_wsum(T v) {
 
 // K(R) and the frozen-label trial U0 = Z_L^2/sqrt(pi) exp(-Z_L r~_L) at node
 // id (cloud/solver.py:36-57 with the argmax replaced by the given label).
+// The map that the adjoint differentiates forms 1/r~ with the hardware
+// reciprocal square root and reciprocal (1 ulp each; Z / max(r, rc) =
+// Z min(1 / r, 1 / rc)): K
+// then carries ~1e-7 relative rounding, the same order as the IEEE form's,
+// at a third of its instructions per node and atom.
 __device__ __forceinline__ void init_frozen_node(int S, int id, float h, float h2, const Atom* __restrict__ atoms,
                                                  int n_atoms, const uint16_t* __restrict__ lab, float* __restrict__ U,
                                                  float* __restrict__ K) {
@@ -39,9 +44,14 @@ __device__ __forceinline__ void init_frozen_node(int S, int id, float h, float h
     const float dy = yj - atoms[n].j * h;
     const float dz = zk - atoms[n].k * h;
     const float r2 = dx * dx + dy * dy + dz * dz;
-    const float r = (rc > 0.0f) ? fmaxf(sqrtf(r2 + 0.04f * h2), rc) : sqrtf(r2 + h2);
-    Kv += Za / r;
-    if (n == L) u0 = Za * Za * 0.5641895835f * expf(-Za * r);
+    const float s2 = r2 + ((rc > 0.0f) ? 0.04f * h2 : h2);
+    const float rs = __builtin_amdgcn_rsqf(s2);
+    const float inv_r = (rc > 0.0f) ? fminf(rs, __builtin_amdgcn_rcpf(rc)) : rs;
+    Kv += Za * inv_r;
+    if (n == L) {
+      const float r = (rc > 0.0f) ? fmaxf(s2 * rs, rc) : s2 * rs;
+      u0 = Za * Za * 0.5641895835f * expf(-Za * r);
+    }
   }
   K[id] = Kv;
   U[id] = u0;
@@ -226,9 +236,13 @@ __device__ __forceinline__ void normalize_adj_col(int S, int k, int j, int i0, i
 // dE/dR (per bohr) of one 64 x 4-column, kAdjCol-plane block from gK = dE/dK
 // and lamU0 = dE/dU0 (pre-normalisation trial):  sum_v gK_v Z d/s^3 +
 // [L_v = n] lamU0_v Z U0_v d / s,  d = x_v - R_n, s = sqrt(d^2 + eps h^2),
-// terms vanish where r~ is clamped at rc.  Per-atom FP64 sums are reduced
-// across the wave and the four waves (red: 4 x kAdjChunk x 3 doubles of LDS)
-// and added to the fixed-point result once per workgroup and component.
+// terms vanish where r~ is clamped at rc.  1/s comes from the hardware
+// reciprocal square root (1 ulp), and each thread sums its kAdjCol planes'
+// terms per atom in FP32 before one conversion: a gradient needs no IEEE
+// division per node and atom, and the 4e11 node-atom terms of a 3000-atom
+// 512^3 grid are the whole cost.  Per-atom FP64 sums are reduced across the
+// wave and the four waves (red: 4 x kAdjChunk x 3 doubles of LDS) and added
+// to the fixed-point result once per workgroup and component.
 constexpr int kAdjCol = 16;
 constexpr int kAdjChunk = 4;
 
@@ -269,23 +283,28 @@ __device__ __forceinline__ void pos_grad_block(int S, float h, float h2, int k, 
       const float eps = (at.rc > 0.0f) ? 0.04f * h2 : h2;
       const float dy = yj - at.j * h;
       const float dz = zk - at.k * h;
+      float fx = 0.0f, fy = 0.0f, fz = 0.0f;
 #pragma unroll
       for (int c = 0; c < kAdjCol; ++c) {
         if (lb[c] < 0) continue;
         const float xi = (float)(i0 + c) * h;
         const float dx = xi - at.i * h;
         const float s2 = dx * dx + dy * dy + dz * dz + eps;
-        const float s = sqrtf(s2);
+        const float rs = __builtin_amdgcn_rsqf(s2);
+        const float s = s2 * rs;
         if (at.rc > 0.0f && s < at.rc) continue;  // clamped: no dependence on R
-        float w = gk[c] * at.Z / (s2 * s);
+        float w = gk[c] * at.Z * (rs * rs * rs);
         if (lb[c] == a) {
           const float u0 = at.Z * at.Z * 0.5641895835f * expf(-at.Z * s);
-          w += lu[c] * at.Z * u0 / s;
+          w += lu[c] * at.Z * u0 * rs;
         }
-        acc[q * 3 + 0] += (double)(w * dx);
-        acc[q * 3 + 1] += (double)(w * dy);
-        acc[q * 3 + 2] += (double)(w * dz);
+        fx += w * dx;
+        fy += w * dy;
+        fz += w * dz;
       }
+      acc[q * 3 + 0] = (double)fx;
+      acc[q * 3 + 1] = (double)fy;
+      acc[q * 3 + 2] = (double)fz;
     }
 #pragma unroll
     for (int q = 0; q < kAdjChunk * 3; ++q) acc[q] = adj_wsum(acc[q]);
