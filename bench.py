@@ -568,10 +568,10 @@ def bench_mf(args):
     host.close()
     if not args.no_pmc:
         log("[bench] mf: HBM traffic of the fused update: rocprofv3 FETCH_SIZE / WRITE_SIZE passes")
-        traffic, info = pmc_traffic("mf", "k_mf_update")
-        if traffic is not None:
-            roof["traffic"] = round(traffic)
-            roof["traffic_over_alg"] = round(traffic / (bpn * n3), 4)
+        traffic, info = pmc_traffic("mf", {"mf_update": ("k_mf_update",)})
+        if traffic and "mf_update" in traffic:
+            roof["traffic"] = round(traffic["mf_update"])
+            roof["traffic_over_alg"] = round(traffic["mf_update"] / (bpn * n3), 4)
         roof["traffic_pmc"] = info
     cpu = None
     if not args.no_cpu:
