@@ -903,7 +903,22 @@ int poisson_into(rq_solver* s, const float* P_in, float* P_out, const float* U, 
   if (s->p.poisson_mode == RQ_POISSON_MULTIGRID) {
     float* x;
     const float a = s->p.mg_a, c = s->p.mg_c;
-    if (s->p.mg_nu1 == 0) {
+    // on the grids where the solver's own step marches (>= smooth2_min_s
+    // nodes per axis, V(2,2)), the same marches: bit-identical values
+    const bool march = s->fused && s->smooth2 && s->s2rr && s->S >= s->smooth2_min_s && s->p.mg_nu1 == 2;
+    if (march) {
+      launch_smooth2_resres(s->st, s->S, P_in, t1, U, a, c, s->two_pi, s->h2, s->g[1], s->s2rr_blocks);
+      LAUNCHCHK(s);
+      if (int rc = vcycle_coarse(s, 1)) return rc;
+      if (U_out && s->post_fuse && s->p.mg_nu2 == 2 && s->S > s->fuse_max_s) {
+        launch_post_itp(s->st, s->S, t1, s->e0[1], P_out, U, U_out, s->K, s->lab, a, c, s->two_pi, s->h2,
+                        s->half_dv, s->dtv, s->post_blocks);
+        LAUNCHCHK(s);
+        if (did_itp) *did_itp = true;
+        return RQ_OK;
+      }
+      x = t1;
+    } else if (s->p.mg_nu1 == 0) {
       if (int rc = copy_field(s, t1, P_in)) return rc;
       x = t1;
     } else {
@@ -916,16 +931,18 @@ int poisson_into(rq_solver* s, const float* P_in, float* P_out, const float* U, 
       }
       x = const_cast<float*>(src);
     }
-    // residual + restriction in one pass (the values of k_residual then
-    // k_restrict4, without the fine residual array)
-    if (s->fused) {
-      launch_residual_restrict(s->st, true, s->S, x, U, s->two_pi, s->h2, s->g[1], 1.0f);
-    } else {
-      launch_residual_fine(s->st, T, s->S, x, U, s->two_pi, s->h2, P_out);
-      launch_restrict4(s->st, s->S, P_out, s->g[1], 1.0f);
+    if (!march) {
+      // residual + restriction in one pass (the values of k_residual then
+      // k_restrict4, without the fine residual array)
+      if (s->fused) {
+        launch_residual_restrict(s->st, true, s->S, x, U, s->two_pi, s->h2, s->g[1], 1.0f);
+      } else {
+        launch_residual_fine(s->st, T, s->S, x, U, s->two_pi, s->h2, P_out);
+        launch_restrict4(s->st, s->S, P_out, s->g[1], 1.0f);
+      }
+      LAUNCHCHK(s);
+      if (int rc = vcycle_coarse(s, 1)) return rc;
     }
-    LAUNCHCHK(s);
-    if (int rc = vcycle_coarse(s, 1)) return rc;
     launch_prolong_correct(s->st, s->ptile, s->S, x, s->e0[1], 1.0f);
     if (s->p.mg_nu2 == 0) return copy_field(s, P_out, x);
     float* src = x;

@@ -181,3 +181,24 @@ def test_saved_start_gradient_matches_fp64_autograd(mode):
     assert abs(E - Eo) < 2e-6 * abs(Eo)
     assert err < 1e-4 + 1e-4 * np.abs(go).max()
     fz.close()
+
+
+@pytest.mark.parametrize("checkpoint", [0, 8])
+def test_adjoint_forward_on_the_fine_marches_is_bitexact(checkpoint, monkeypatch):
+    """With the fine-level marches in the frozen map's Poisson steps (size
+    gate lowered; on by default from 320 nodes per axis), energy and
+    gradient equal those of the separate passes bit for bit, whole
+    trajectory recorded (checkpoint 0) or recomputed from checkpoints."""
+    NN, screen = 96, 96 * 0.12
+    rows = three_atoms(NN)
+    out = []
+    for gate in ("100000", "0"):
+        monkeypatch.setenv("RQ_SMOOTH2_MIN_S", gate)
+        fz = FrozenSolver(rows, NN, screen, 30, bootstrap=3, checkpoint=checkpoint, poisson_interval=1,
+                          poisson="multigrid", mg_levels=4)
+        pos = rows[:, :3] * fz.h
+        out.append(fz.energy_and_gradient(pos))
+        fz.close()
+    print(f"separate passes E {out[0][0]:.12f}, marches E {out[1][0]:.12f}")
+    assert out[0][0] == out[1][0]
+    assert np.array_equal(out[0][1], out[1][1])
