@@ -132,9 +132,9 @@ def forces_bench(w, bootstrap, n_batch, fd_delta, fd_components, log_fn, conv_to
     differences (delta = fd_delta bohr) of that full solve, 6N+1 solves per
     force vector, timed on a sample of components; adjoint: one forward solve
     plus the reverse sweep with checkpointed recomputation; batched: n_batch
-    geometries on concurrent handles.  Both paths evaluate the same E(R)."""
+    geometries in one batch (rq_batch_adjoint_gradient).  Both paths evaluate
+    the same E(R)."""
     import ctypes as C
-    import threading
     from realmolecule_amd.adjoint import FrozenSolver
     NN, screen, rows = w["NN"], w["screen"], w["rows"]
     T, dconv, Econv = converge_steps(w, bootstrap, conv_tol)
@@ -177,24 +177,25 @@ def forces_bench(w, bootstrap, n_batch, fd_delta, fd_components, log_fn, conv_to
     log_fn(f"[forces] central differences: {2 * len(comps) + 1} solves in {t_fd_sample + t_e0:.2f} s, "
            f"{t_solve:.3f} s per solve, {t_fd:.1f} s per force vector (6N+1 = {6 * n + 1} solves)")
     fz.close()
-    # batched: n_batch geometries (seeded 0.05 bohr jitter) on concurrent handles
+    # batched: n_batch geometries (seeded 0.05 bohr jitter, each with its own
+    # trial labels) as one batch: every geometry's frozen map and reverse
+    # sweep in one launch per pass (rq_batch_adjoint_gradient), checkpoints
+    # every 64 steps, allocations included
+    from realmolecule_amd.batch import Batch
+    from realmolecule_amd.adjoint import make_spec
     rng = np.random.default_rng(7)
-    solvers = [FrozenSolver(rows, NN, screen, T, **kw) for _ in range(n_batch)]
-    geos = [pos + rng.normal(0, 0.05, pos.shape) for _ in range(n_batch)]
-    out = [None] * n_batch
-
-    def run(i):
-        out[i] = solvers[i].energy_and_gradient(geos[i])
-
+    h = screen / NN
+    geo_rows = []
+    for _ in range(n_batch):
+        r = rows.copy()
+        r[:, :3] = (pos + rng.normal(0, 0.05, pos.shape)) / h
+        geo_rows.append(r)
+    b = Batch([(r, NN, screen) for r in geo_rows], mg_levels=w["mg_levels"], graph=False)
+    b.initialize(bootstrap=0)
     t0 = time.perf_counter()
-    th = [threading.Thread(target=run, args=(i,)) for i in range(n_batch)]
-    for t in th:
-        t.start()
-    for t in th:
-        t.join()
+    E3b, grads_b = b.adjoint_gradient(make_spec(T, bootstrap, 64, 1, 20))
     t_batch = time.perf_counter() - t0
-    for s_ in solvers:
-        s_.close()
+    b.close()
     log_fn(f"[forces] batched adjoint: {n_batch} force vectors in {t_batch:.3f} s")
     hf_diff = np.abs(F_el_adj - F_hf)
     return {
