@@ -466,11 +466,25 @@ __device__ __forceinline__ void norm_sums_col(int S, int k, int j, int i0, int i
 // U *= scale[label] for labels inside [0, n_dom)
 __device__ __forceinline__ void normalize_col(int S, int k, int j, int i0, int i1, float* __restrict__ U,
                                               const uint16_t* __restrict__ lab, const float* scale, int n_dom) {
-  const int S2 = S * S;
-  int id = i0 * S2 + j * S + k;
-  for (int i = i0; i < i1; ++i, id += S2) {
-    const int L = lab[id] & kLabelMask;
-    if (L < n_dom) U[id] = U[id] * scale[L];
+  // planes in batches of B: labels and amplitudes loaded together (indices
+  // clamped to the column's last plane, whose repeats are not stored), then
+  // the batch's scale gathers, then the products
+  constexpr int B = 8;
+  const int S2 = S * S, col = j * S + k;
+  for (int ib = i0; ib < i1; ib += B) {
+    int Lb[B];
+    float vb[B], sb[B];
+#pragma unroll
+    for (int q = 0; q < B; ++q) {
+      const int id = min(ib + q, i1 - 1) * S2 + col;
+      Lb[q] = lab[id] & kLabelMask;
+      vb[q] = U[id];
+    }
+#pragma unroll
+    for (int q = 0; q < B; ++q) sb[q] = Lb[q] < n_dom ? scale[Lb[q]] : 1.0f;
+#pragma unroll
+    for (int q = 0; q < B; ++q)
+      if (ib + q < i1 && Lb[q] < n_dom) U[(ib + q) * S2 + col] = vb[q] * sb[q];
   }
 }
 
