@@ -99,12 +99,16 @@ __device__ __forceinline__ void itp_adj_col(int S, int k, int j, int i0, int i1,
   for (int i = i0; i < i1; ++i, id += S2) {
     const uint16_t lbp = lab[id + S2];
     const float lvp = lam[id + S2];
+    // the in-plane neighbours loaded unconditionally, then selected (a load
+    // under the label test would be a branch and a dependent wait each)
+    const uint16_t ljp = lab[id + S], ljm = lab[id - S], lkp = lab[id + 1], lkm = lab[id - 1];
+    const float vjp = lam[id + S], vjm = lam[id - S], vkp = lam[id + 1], vkm = lam[id - 1];
     const float t0 = (lbp == lc) ? lvp : lc_v;
     const float t1 = (lbm == lc) ? lvm : lc_v;
-    const float t2 = (lab[id + S] == lc) ? lam[id + S] : lc_v;
-    const float t3 = (lab[id - S] == lc) ? lam[id - S] : lc_v;
-    const float t4 = (lab[id + 1] == lc) ? lam[id + 1] : lc_v;
-    const float t5 = (lab[id - 1] == lc) ? lam[id - 1] : lc_v;
+    const float t2 = (ljp == lc) ? vjp : lc_v;
+    const float t3 = (ljm == lc) ? vjm : lc_v;
+    const float t4 = (lkp == lc) ? vkp : lc_v;
+    const float t5 = (lkm == lc) ? vkm : lc_v;
     const float lap = t0 + t1 + t2 + t3 + t4 + t5 - 6.0f * lc_v;
     const float kk = K[id], pp = P[id];
     lam_out[id] = lc_v + half_dv * lap + dtv * (kk - 2.0f * pp) * lc_v;
