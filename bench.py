@@ -413,10 +413,11 @@ def bench_hairpin():
     }
 
 
-def bench_c3(steps, warmup):
+def bench_c3(steps, warmup, graph=True, sequential=True):
     """Batched geometry scan: 64 seeded conformers (H2O, formamide, ethanol,
     glycine) on grids of 64..160 cells, 4-level V(2,2) + ITP per step.  The
-    batch engine advances all of them with one persistent launch per pass;
+    batch engine advances all of them with one launch per pass over all of
+    their tiles;
     the comparison runs the same molecules one after another, each on its own
     single-grid handle (the per-molecule path a scan would take)."""
     from realmolecule_amd import builders as B
@@ -424,7 +425,7 @@ def bench_c3(steps, warmup):
     from realmolecule_amd.solver import RealQMSolver
     mols = B.c3_conformers()
     vox = sum((nn - 1) ** 3 for _, nn, _, _ in mols)
-    b = Batch([(r, nn, sc) for r, nn, sc, _ in mols], mg_levels=4, graph=True)
+    b = Batch([(r, nn, sc) for r, nn, sc, _ in mols], mg_levels=4, graph=graph)
     b.initialize(bootstrap=10)
     b.step(0, warmup)
     b.sync()
@@ -436,7 +437,7 @@ def bench_c3(steps, warmup):
     log(f"[c3] batch: {dt_b / steps * 1e3:.3f} ms/step for {len(mols)} molecules, {vox * steps / dt_b / 1e9:.2f} Gvox/s")
     seq_steps = max(20, steps // 5)
     dt_s = 0.0
-    for r, nn, sc, name in mols:
+    for r, nn, sc, name in (mols if sequential else []):
         from realmolecule_amd.builders import to_config_atoms
         s = RealQMSolver({"gridN": nn, "screen": sc, "atoms": to_config_atoms(r), "poisson": "multigrid",
                           "mg_levels": 4, "bootstrap_poisson": 10, "graph": True})
@@ -465,8 +466,9 @@ def bench_c3(steps, warmup):
         "config": {"workload": "64 conformers (seed 2) of H2O/formamide/ethanol/glycine, grids 64..160 cells, "
                                "4-level V(2,2) + ITP per step", "molecules": len(mols), "interior_voxels": vox,
                    "parallelism": "single-gpu"},
-        "sequential_ms_per_step": round(dt_s / seq_steps * 1e3, 5),
-        "batch_speedup": round((dt_s / seq_steps) / (dt_b / steps), 3),
+        "batch_graph_replay": graph,
+        "sequential_ms_per_step": round(dt_s / seq_steps * 1e3, 5) if sequential else None,
+        "batch_speedup": round((dt_s / seq_steps) / (dt_b / steps), 3) if sequential else None,
     }
 
 
@@ -843,6 +845,8 @@ def main():
                     help="force components checked (and timed) by central differences")
     ap.add_argument("--force-batch", type=int, default=4)
     ap.add_argument("--no-pmc", action="store_true", help="skip the rocprofv3 FETCH/WRITE_SIZE passes")
+    ap.add_argument("--c3-eager", action="store_true", help="c3: launch the batch's passes without graph replay")
+    ap.add_argument("--c3-batch-only", action="store_true", help="c3: skip the one-handle-per-molecule comparison")
     ap.add_argument("--pmc-child", choices=("solver", "copy"), help=argparse.SUPPRESS)
     ap.add_argument("--pmc-steps", type=int, default=4)
     args = ap.parse_args()
@@ -854,7 +858,7 @@ def main():
         pmc_child(args.pmc_child, workload(args.workload), args.pmc_steps)
         return
     if args.workload == "c3":
-        print(json.dumps(bench_c3(args.steps, args.warmup)))
+        print(json.dumps(bench_c3(args.steps, args.warmup, graph=not args.c3_eager, sequential=not args.c3_batch_only)))
         return
     if args.workload == "hairpin":
         print(json.dumps(bench_hairpin()))
