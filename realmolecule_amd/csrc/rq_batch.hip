@@ -22,6 +22,7 @@
 
 #include "rq_internal.h"
 #include "rq_adjoint_cols.h"
+#include "rq_march.h"
 #include "rq_stencil.h"
 
 namespace rq {
@@ -34,11 +35,18 @@ struct BLev {
   int tile0;     // first tile of this molecule in the level's tile space
   int cnode0;    // first coarse interior node (restriction items from this level)
   int ctile0;    // first coarse tile (kBatchResCK x CJ x CI coarse nodes) of the tiled restriction
+  int dtile0;    // first tile of the down march (kDownCK x kDownCJ coarse columns, down_ci planes)
+  int utile0;    // first tile of the up march (kUpTK x kUpTJ columns, up_ci planes)
   long long off; // element offset of this molecule in the level's arrays
 };
 
 // coarse tile of the batch's residual + restriction (the single-grid tile)
 constexpr int kBatchResCK = 8, kBatchResCJ = 8, kBatchResCI = 4;
+// fine-level marches (the single grid's shapes): down = two pre-sweeps +
+// residual + restriction over kDownCK x kDownCJ coarse columns, up =
+// correction + two post-sweeps + ITP over kUpTK x kUpTJ fine columns
+constexpr int kDownCK = 32, kDownCJ = 8, kDownNT = 768, kUpTK = 32, kUpTJ = 8, kUpRU = 5;
+constexpr int kDownCI = 16, kUpCI = 32;  // default coarse / fine planes per march item
 
 struct BMol {
   int atom0, n_atoms, dom0, node0;
@@ -71,6 +79,26 @@ __device__ __forceinline__ int find_ctile(const BLev* __restrict__ L, int nm, in
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
     if (L[mid].ctile0 <= t) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ int find_dtile(const BLev* __restrict__ L, int nm, int t) {
+  int lo = 0, hi = nm - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (L[mid].dtile0 <= t) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ int find_utile(const BLev* __restrict__ L, int nm, int t) {
+  int lo = 0, hi = nm - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (L[mid].utile0 <= t) lo = mid;
     else hi = mid - 1;
   }
   return lo;
@@ -201,6 +229,44 @@ __global__ __launch_bounds__(kBatchResCK * kBatchResCJ * kBatchResCI) void kb_re
     resres_tile<FINE, CK, CJ, CI>(d.S, 1 + (local % gx) * CK, 1 + ((local / gx) % gy) * CJ,
                                   1 + (local / (gx * gy)) * CI, x + d.off, src + d.off, al, be, gc + Lc[m].off,
                                   1.0f, r);
+    __syncthreads();
+  }
+}
+
+// Fine-level down march on every molecule: each work item is one tile of
+// smooth2_resres_march_tile (the single grid's kernel body), x -> xn after
+// two sweeps and the coarse right-hand side into gc.
+__global__ __launch_bounds__(kDownNT) void kb_down_march(const BLev* __restrict__ Lf, const BLev* __restrict__ Lc,
+                                                         const BMol* __restrict__ M, int nm, int nit, int CI,
+                                                         const float* __restrict__ x, float* __restrict__ xn,
+                                                         const float* __restrict__ src, float a, float c,
+                                                         float* __restrict__ gc) {
+  for (int t = blockIdx.x; t < nit; t += gridDim.x) {
+    const int u = xcd_item(t, nit);
+    const int m = find_dtile(Lf, nm, u);
+    const BLev d = Lf[m];
+    smooth2_resres_march_tile<kDownCK, kDownCJ, kDownNT>(d.S, CI, u - d.dtile0, x + d.off, xn + d.off, src + d.off,
+                                                         a, c, M[m].two_pi, M[m].h2, gc + Lc[m].off);
+    __syncthreads();
+  }
+}
+
+// Fine-level up march on every molecule: post_march_tile per work item
+// (correction from e, two sweeps x -> xn, ITP update U -> Un).
+__global__ __launch_bounds__(kUpTK * kUpTJ) void kb_up_march(const BLev* __restrict__ Lf, const BLev* __restrict__ Lc,
+                                                             const BMol* __restrict__ M, int nm, int nit, int CI,
+                                                             const float* __restrict__ x, const float* __restrict__ e,
+                                                             float* __restrict__ xn, const float* __restrict__ U,
+                                                             float* __restrict__ Un, const float* __restrict__ K,
+                                                             const uint16_t* __restrict__ lab, float a, float c) {
+  for (int t = blockIdx.x; t < nit; t += gridDim.x) {
+    const int u = xcd_item(t, nit);
+    const int m = find_utile(Lf, nm, u);
+    const BLev d = Lf[m];
+    const BMol mm = M[m];
+    post_march_tile<kUpTK, kUpTJ, kUpRU>(d.S, CI, u - d.utile0, x + d.off, e + Lc[m].off, xn + d.off, U + d.off,
+                                         Un + d.off, K + d.off, lab + d.off, a, c, mm.two_pi, mm.h2, mm.half_dv,
+                                         mm.dtv);
     __syncthreads();
   }
 }
@@ -586,6 +652,9 @@ struct rq_batch {
   std::vector<std::vector<BLev>> lev;  // [level][mol]
   std::vector<BMol> mol;
   std::vector<int> ntiles, ncnodes, nctiles;  // per level
+  int ndtiles = 0, nutiles = 0;                // fine-level march items
+  int down_ci = kDownCI, up_ci = kUpCI;        // planes per march item (RQ_BATCH_DOWN_CI / RQ_BATCH_UP_CI)
+  bool march = true;                           // fine-level marches when nu1 = nu2 = 2 (RQ_BATCH_MARCH=0: off)
   std::vector<size_t> len;             // elements per level array
   std::vector<int> maxS;               // largest node count per axis on each level
   int nnodes = 0, ndom = 0;
@@ -759,6 +828,29 @@ int b_vcycle(rq_batch* b, bool fuse_itp, bool* did_itp) {
   const dim3 blk(b->bx, b->by);
   const int nt = b->ntiles[0];
   const dim3 grd(grid_for(nt));
+  // the fine level as two marches (the single grid's fused V-cycle legs):
+  // same values as the separate sweeps, restriction, correction and ITP
+  if (b->march && fuse_itp && b->p.mg_nu1 == 2 && b->p.mg_nu2 == 2 && !(b->fuse_small && b->maxS[0] <= kFuseMaxS)) {
+    {
+      BTimed t(b, F_RESTRICT);
+      hipLaunchKernelGGL(kb_down_march, dim3(grid_for(b->ndtiles)), dim3(kDownNT), 0, b->st, LD(b, 0), LD(b, 1),
+                         b->mol_d, b->nm, b->ndtiles, b->down_ci, b->P, b->P2, b->U, a, c, b->g[1]);
+    }
+    std::swap(b->P, b->P2);
+    BLAUNCH(b);
+    if (int rc = b_vcycle_coarse(b, 1)) return rc;
+    {
+      BTimed t(b, F_SMOOTH_ITP);
+      hipLaunchKernelGGL(kb_up_march, dim3(grid_for(b->nutiles)), dim3(kUpTK * kUpTJ), 0, b->st, LD(b, 0), LD(b, 1),
+                         b->mol_d, b->nm, b->nutiles, b->up_ci, b->P, b->e0[1], b->P2, b->U, b->U2, b->K, b->lab, a,
+                         c);
+    }
+    std::swap(b->P, b->P2);
+    std::swap(b->U, b->U2);
+    *did_itp = true;
+    BLAUNCH(b);
+    return RQ_OK;
+  }
   for (int it = 0; it < b->p.mg_nu1; ++it) {
     {
       BTimed t(b, F_SMOOTH);
@@ -1249,6 +1341,17 @@ void layout(rq_batch* b, const std::vector<int>* ich_fine) {
     b->ncnodes[l] = cnode;
     b->nctiles[l] = ctile;
   }
+  int dt = 0, ut = 0;
+  for (int m = 0; m < b->nm; ++m) {
+    BLev& d = b->lev[0][m];
+    const int n = d.S - 2, nc = (d.S - 1) / 2 - 1;
+    d.dtile0 = dt;
+    d.utile0 = ut;
+    dt += ((nc + kDownCK - 1) / kDownCK) * ((nc + kDownCJ - 1) / kDownCJ) * ((nc + b->down_ci - 1) / b->down_ci);
+    ut += ((n + kUpTK - 1) / kUpTK) * ((n + kUpTJ - 1) / kUpTJ) * ((n + b->up_ci - 1) / b->up_ci);
+  }
+  b->ndtiles = dt;
+  b->nutiles = ut;
 }
 
 }  // namespace
@@ -1269,6 +1372,9 @@ int rq_batch_create(const rq_params* pp, const rq_molecule* mols, int32_t n_mols
   b->p = p;
   if (const char* fs = std::getenv("RQ_BATCH_FUSE_SMALL")) b->fuse_small = std::atoi(fs) != 0;
   if (const char* fi = std::getenv("RQ_BATCH_FUSE_ITP")) b->fuse_itp = std::atoi(fi) != 0;
+  if (const char* mv = std::getenv("RQ_BATCH_MARCH")) b->march = std::atoi(mv) != 0;
+  if (const char* dc = std::getenv("RQ_BATCH_DOWN_CI")) b->down_ci = std::max(1, std::atoi(dc));
+  if (const char* uc = std::getenv("RQ_BATCH_UP_CI")) b->up_ci = std::max(1, std::atoi(uc));
   b->nm = n_mols;
   b->L = L;
   std::vector<Atom> atoms;

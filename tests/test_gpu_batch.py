@@ -48,3 +48,29 @@ def test_c3_batch_runs_and_stays_finite():
     assert np.isfinite(E).all()
     assert (E[:, 1] < 0).all() and (E[:, 0] > 0).all()
     b.close()
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_batch_fine_marches_equal_separate_passes(graph, monkeypatch):
+    """The batch's fine level as two marches (two pre-sweeps + residual +
+    restriction; correction + two post-sweeps + ITP) against the separate
+    passes (RQ_BATCH_MARCH=0), on one conformer of each of the scan's four
+    smallest grids, 80, 96, 112 and 128 cells (ragged tiles: 39, 47 and 55
+    coarse columns against 32-wide march tiles): fields bit for bit after
+    30 steps."""
+    picks = {}
+    for r, NN, sc, _name in c3_conformers():
+        picks.setdefault(NN, (r, NN, sc))
+    mols = [picks[nn] for nn in sorted(picks)][:4]
+    print(f"[batch marches] grids {[m[1] for m in mols]} graph={graph}", flush=True)
+    out = []
+    for march in ("0", "1"):
+        monkeypatch.setenv("RQ_BATCH_MARCH", march)
+        b = Batch(mols, mg_levels=4, mg_nu=2, graph=graph)
+        b.initialize(bootstrap=3)
+        b.step(0, 30)
+        out.append([(b.read(m, "U"), b.read(m, "P")) for m in range(len(mols))])
+        b.close()
+    for m in range(len(mols)):
+        assert np.array_equal(out[0][m][0], out[1][m][0]), m
+        assert np.array_equal(out[0][m][1], out[1][m][1]), m
